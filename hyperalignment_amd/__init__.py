@@ -1,0 +1,398 @@
+"""hyperalignment_amd -- searchlight Procrustes on MI355X (gfx950).
+
+ctypes binding of ``libhyperalignment_amd.so``. The public function
+``searchlight_procrustes`` mirrors the reference wrapper
+``hyperalignment_c.py:112-223`` (same arguments, Fortran-order float64 data,
+int32 flat searchlight arrays, dense float64 output) with one more backend
+name, ``"hip"`` (THaBackend value 4). Everything runs on the GPU; there is no
+CPU fallback, and a missing library or device raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import numpy as np
+
+_here = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_here, "libhyperalignment_amd.so")
+
+_lib = None
+
+# THaBackend values (ha_common.h:70-76 plus kHaBackendHIP = 4)
+BACKENDS = {"cpu64": 0, "c64": 0, "cpu32": 1, "c32": 1, "metal": 2, "cuda": 3, "hip": 4}
+METHODS = {"auto": 0, "jacobi32": 1, "polar": 2, "polar_learned": 3, "jacobi64": 4}
+ERRORS = {0: "success", -1: "allocation failed", -2: "SVD failed", -3: "invalid argument",
+          -4: "internal / GPU error"}
+
+MLP_IN, MLP_HID, MLP_OUT = 12, 64, 2
+MLP_NPARAM = MLP_IN * MLP_HID + MLP_HID + MLP_HID * MLP_HID + MLP_HID + MLP_HID * MLP_OUT + MLP_OUT
+
+
+class HaError(RuntimeError):
+    def __init__(self, fn, rc):
+        super().__init__(f"{fn} failed with error code {rc} ({ERRORS.get(rc, 'unknown')})")
+        self.rc = rc
+
+
+class TMat(ctypes.Structure):
+    _fields_ = [("data", ctypes.POINTER(ctypes.c_double)), ("rows", ctypes.c_int32),
+                ("cols", ctypes.c_int32)]
+
+
+class TSparseCSC(ctypes.Structure):
+    _fields_ = [("data", ctypes.POINTER(ctypes.c_double)), ("indices", ctypes.POINTER(ctypes.c_int32)),
+                ("indptr", ctypes.POINTER(ctypes.c_int32)), ("rows", ctypes.c_int32),
+                ("cols", ctypes.c_int32), ("nnz", ctypes.c_int64)]
+
+
+class TSearchlights(ctypes.Structure):
+    _fields_ = [("indices", ctypes.POINTER(ctypes.POINTER(ctypes.c_int32))),
+                ("dists", ctypes.POINTER(ctypes.POINTER(ctypes.c_double))),
+                ("sizes", ctypes.POINTER(ctypes.c_int32)), ("count", ctypes.c_int32),
+                ("radius", ctypes.c_double)]
+
+
+class HaHipOptions(ctypes.Structure):
+    _fields_ = [("method", ctypes.c_int32), ("fp64_fallback", ctypes.c_int32),
+                ("max_polar_steps", ctypes.c_int32), ("polar_tol", ctypes.c_double),
+                ("plain_ell", ctypes.c_double), ("tile", ctypes.c_int32), ("use_graph", ctypes.c_int32),
+                ("chunk", ctypes.c_int32), ("verbose", ctypes.c_int32),
+                ("learned_margin_scale", ctypes.c_float), ("learned_margin_ell", ctypes.c_float)]
+
+
+class HaHipStats(ctypes.Structure):
+    _fields_ = [("ms_weights", ctypes.c_double), ("ms_crosscov", ctypes.c_double),
+                ("ms_polar", ctypes.c_double), ("ms_fallback", ctypes.c_double),
+                ("ms_scatter", ctypes.c_double), ("ms_total", ctypes.c_double),
+                ("ms_pattern", ctypes.c_double), ("n_fallback", ctypes.c_int64),
+                ("n_unconverged", ctypes.c_int64), ("steps_total", ctypes.c_int64),
+                ("steps_hist", ctypes.c_int32 * 32), ("nnz", ctypes.c_int64),
+                ("gemm_flops", ctypes.c_double), ("kernel_ms_polar_avg", ctypes.c_double),
+                ("polar_launches", ctypes.c_int32)]
+
+    def as_dict(self):
+        d = {name: getattr(self, name) for name, _ in self._fields_}
+        d["steps_hist"] = list(self.steps_hist)
+        return d
+
+
+_P = ctypes.POINTER
+_d, _i32, _f = ctypes.c_double, ctypes.c_int32, ctypes.c_float
+_SIGS = {
+    "ha_hip_init": (ctypes.c_int, []),
+    "ha_hip_cleanup": (None, []),
+    "ha_hip_available": (ctypes.c_bool, []),
+    "ha_hip_get_options": (None, [_P(HaHipOptions)]),
+    "ha_hip_set_options": (ctypes.c_int, [_P(HaHipOptions)]),
+    "ha_hip_set_model": (ctypes.c_int, [_P(_f), _P(_f)]),
+    "ha_set_num_threads": (None, [ctypes.c_int]),
+    "ha_get_num_threads": (ctypes.c_int, []),
+    "ha_searchlight_procrustes_dense": (ctypes.c_int, [_P(_d), _P(_d), _i32, _i32, _P(_i32), _P(_i32), _P(_d),
+                                                       _i32, _d, _P(_d), ctypes.c_bool, ctypes.c_bool,
+                                                       ctypes.c_int, ctypes.c_bool]),
+    "ha_searchlight_procrustes_hip": (ctypes.c_int, [_P(_d), _P(_d), _i32, _i32, _P(_i32), _P(_i32), _P(_d),
+                                                     _i32, _d, _P(_d), ctypes.c_bool, ctypes.c_bool,
+                                                     ctypes.c_bool]),
+    "ha_searchlight_procrustes": (ctypes.c_int, [_P(TMat), _P(TMat), _P(TSearchlights), _P(TSearchlights),
+                                                 _P(TSparseCSC), _P(_P(_d)), ctypes.c_bool, ctypes.c_bool,
+                                                 ctypes.c_int]),
+    "ha_searchlight_weights": (ctypes.c_int, [_P(TSearchlights), _P(_P(_d))]),
+    "ha_sparse_init": (_P(TSparseCSC), [_P(TSearchlights), _i32]),
+    "ha_sparse_free": (None, [_P(TSparseCSC)]),
+    "ha_sparse_scatter_add": (None, [_P(TSparseCSC), _P(_i32), _P(_i32), _i32, _i32, _P(_d), _P(_d)]),
+    "ha_procrustes": (ctypes.c_int, [_P(TMat), _P(TMat), _P(TMat), ctypes.c_bool, ctypes.c_bool]),
+    "ha_hip_plan_create": (ctypes.c_int, [_i32, _i32, _P(_i32), _P(_i32), _P(_d), _i32, _d,
+                                          _P(ctypes.c_void_p)]),
+    "ha_hip_plan_destroy": (None, [ctypes.c_void_p]),
+    "ha_hip_plan_load": (ctypes.c_int, [ctypes.c_void_p, _P(_d), _P(_d), ctypes.c_bool]),
+    "ha_hip_plan_execute": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_bool, ctypes.c_bool]),
+    "ha_hip_plan_nnz": (ctypes.c_int, [ctypes.c_void_p, _P(ctypes.c_int64)]),
+    "ha_hip_plan_download": (ctypes.c_int, [ctypes.c_void_p, _P(_i32), _P(_i32), _P(_d)]),
+    "ha_hip_plan_download_local": (ctypes.c_int, [ctypes.c_void_p, _P(_i32), _i32, _P(_f)]),
+    "ha_hip_plan_stats": (ctypes.c_int, [ctypes.c_void_p, _P(HaHipStats)]),
+}
+
+
+def lib():
+    """Loads the built library (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError(f"{LIB_PATH} not found: run __graft_entry__.build() (or make in "
+                                f"hyperalignment_amd/csrc) first")
+    L = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _ptr(a, ct):
+    return a.ctypes.data_as(ctypes.POINTER(ct)) if a is not None else None
+
+
+def _check(fn, rc):
+    if rc != 0:
+        raise HaError(fn, rc)
+
+
+def init():
+    """ha_hip_init(); raises when no usable GPU is present."""
+    _check("ha_hip_init", lib().ha_hip_init())
+
+
+def available() -> bool:
+    return bool(lib().ha_hip_available())
+
+
+def get_options() -> HaHipOptions:
+    o = HaHipOptions()
+    lib().ha_hip_get_options(ctypes.byref(o))
+    return o
+
+
+def set_options(**kw):
+    """Updates run options, e.g. set_options(method="polar", polar_tol=1e-2)."""
+    o = get_options()
+    for k, v in kw.items():
+        if k == "method" and isinstance(v, str):
+            v = METHODS[v]
+        setattr(o, k, v)
+    _check("ha_hip_set_options", lib().ha_hip_set_options(ctypes.byref(o)))
+    return o
+
+
+def set_model(params, feat_norm):
+    """Installs MLP parameters (MLP_NPARAM floats) and feature normalisation (2*MLP_IN)."""
+    if params is None:
+        _check("ha_hip_set_model", lib().ha_hip_set_model(None, None))
+        return
+    p = np.ascontiguousarray(params, dtype=np.float32).ravel()
+    n = np.ascontiguousarray(feat_norm, dtype=np.float32).ravel()
+    if p.size != MLP_NPARAM or n.size != 2 * MLP_IN:
+        raise ValueError("bad MLP parameter sizes")
+    _check("ha_hip_set_model", lib().ha_hip_set_model(_ptr(p, ctypes.c_float), _ptr(n, ctypes.c_float)))
+
+
+def _flat(sls, dists):
+    sizes = np.array([len(s) for s in sls], dtype=np.int32)
+    offsets = np.zeros(len(sls) + 1, dtype=np.int32)
+    np.cumsum(sizes, out=offsets[1:])
+    idx = np.ascontiguousarray(np.concatenate(sls).astype(np.int32))
+    d = None if dists is None else np.ascontiguousarray(np.concatenate(dists).astype(np.float64))
+    return idx, offsets, d
+
+
+def searchlight_procrustes(X, Y, sls, dists, radius, backend="hip", reflection=True, scaling=False, n_jobs=1,
+                           T=None):
+    """Searchlight Procrustes, dense output (hyperalignment_c.py:112-223).
+
+    X, Y: (n_samples, n_vertices). sls / dists: lists of int / float arrays.
+    Returns T (n_vertices, n_vertices) float64; if T is given it is
+    accumulated in place (the C entry adds into T_out).
+    """
+    if backend not in BACKENDS:
+        raise ValueError(f"Unknown backend '{backend}'. Choose from: {list(BACKENDS)}")
+    idx, offsets, d = _flat(sls, dists)
+    return searchlight_procrustes_flat(X, Y, idx, offsets, d, radius, backend=backend, reflection=reflection,
+                                       scaling=scaling, n_jobs=n_jobs, T=T)
+
+
+def searchlight_procrustes_flat(X, Y, idx, offsets, dists, radius, backend="hip", reflection=True,
+                                scaling=False, n_jobs=1, T=None):
+    L = lib()
+    if n_jobs > 0:
+        L.ha_set_num_threads(n_jobs)
+    X = np.asfortranarray(X, dtype=np.float64)
+    Y = np.asfortranarray(Y, dtype=np.float64)
+    nt, nv = X.shape
+    idx = np.ascontiguousarray(idx, dtype=np.int32)
+    offsets = np.ascontiguousarray(offsets, dtype=np.int32)
+    d = None if dists is None else np.ascontiguousarray(dists, dtype=np.float64)
+    if T is None:
+        T = np.zeros((nv, nv), dtype=np.float64)
+    if T.shape != (nv, nv) or T.dtype != np.float64 or not T.flags.c_contiguous:
+        raise ValueError("T must be a C-contiguous float64 (nv, nv) array")
+    rc = L.ha_searchlight_procrustes_dense(
+        _ptr(X, ctypes.c_double), _ptr(Y, ctypes.c_double), nt, nv, _ptr(idx, ctypes.c_int32),
+        _ptr(offsets, ctypes.c_int32), _ptr(d, ctypes.c_double), len(offsets) - 1, float(radius),
+        _ptr(T, ctypes.c_double), bool(reflection), bool(scaling), BACKENDS[backend], True)
+    _check("ha_searchlight_procrustes_dense", rc)
+    return T
+
+
+class _SlHolder:
+    """Keeps the ctypes arrays of a TSearchlights alive."""
+
+    def __init__(self, sls, dists=None, radius=0.0):
+        self.arrs = [np.ascontiguousarray(s, dtype=np.int32) for s in sls]
+        self.darrs = None if dists is None else [np.ascontiguousarray(x, dtype=np.float64) for x in dists]
+        n = len(self.arrs)
+        self.ip = (ctypes.POINTER(ctypes.c_int32) * n)(*[_ptr(a, ctypes.c_int32) for a in self.arrs])
+        self.dp = None if self.darrs is None else (ctypes.POINTER(ctypes.c_double) * n)(
+            *[_ptr(a, ctypes.c_double) for a in self.darrs])
+        self.sizes = np.array([len(a) for a in self.arrs], dtype=np.int32)
+        self.st = TSearchlights(self.ip, self.dp, _ptr(self.sizes, ctypes.c_int32), n, float(radius))
+
+
+def sparse_init(sls, nv=0):
+    """ha_sparse_init on the GPU; returns a scipy.sparse.csc_matrix of zeros."""
+    import scipy.sparse as sp
+    h = _SlHolder(sls)
+    m = lib().ha_sparse_init(ctypes.byref(h.st), int(nv))
+    if not m:
+        raise HaError("ha_sparse_init", -4)
+    try:
+        c = m.contents
+        n, nnz = c.cols, c.nnz
+        indptr = np.ctypeslib.as_array(c.indptr, shape=(n + 1,)).copy()
+        indices = np.ctypeslib.as_array(c.indices, shape=(max(nnz, 1),))[:nnz].copy()
+        return sp.csc_matrix((np.zeros(nnz), indices, indptr), shape=(c.rows, c.cols))
+    finally:
+        lib().ha_sparse_free(m)
+
+
+def searchlight_weights(sls, dists=None, radius=0.0):
+    """ha_searchlight_weights on the GPU; list of float64 arrays."""
+    h = _SlHolder(sls, dists, radius)
+    n = len(sls)
+    outs = (ctypes.POINTER(ctypes.c_double) * n)()
+    _check("ha_searchlight_weights", lib().ha_searchlight_weights(ctypes.byref(h.st), outs))
+    libc = ctypes.CDLL(None)
+    libc.free.argtypes = [ctypes.c_void_p]
+    res = []
+    for s in range(n):
+        res.append(np.ctypeslib.as_array(outs[s], shape=(len(sls[s]),)).copy())
+        libc.free(ctypes.cast(outs[s], ctypes.c_void_p))
+    return res
+
+
+def searchlight_procrustes_csc(X, Y, sls, mat, sls_Y=None, weights=None, reflection=True, scaling=False):
+    """ha_searchlight_procrustes (CSC API) with backend HIP; mat is accumulated in place.
+
+    mat: scipy.sparse.csc_matrix with sorted indices (e.g. from sparse_init).
+    """
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    Y = np.ascontiguousarray(Y, dtype=np.float64)
+    hx = _SlHolder(sls)
+    hy = _SlHolder(sls_Y) if sls_Y is not None else None
+    data = np.ascontiguousarray(mat.data, dtype=np.float64)
+    indices = np.ascontiguousarray(mat.indices, dtype=np.int32)
+    indptr = np.ascontiguousarray(mat.indptr, dtype=np.int32)
+    csc = TSparseCSC(_ptr(data, ctypes.c_double), _ptr(indices, ctypes.c_int32), _ptr(indptr, ctypes.c_int32),
+                     mat.shape[0], mat.shape[1], data.size)
+    tx = TMat(_ptr(X, ctypes.c_double), X.shape[0], X.shape[1])
+    ty = TMat(_ptr(Y, ctypes.c_double), Y.shape[0], Y.shape[1])
+    wp = None
+    if weights is not None:
+        warr = [np.ascontiguousarray(w, dtype=np.float64) for w in weights]
+        wp = (ctypes.POINTER(ctypes.c_double) * len(warr))(*[_ptr(w, ctypes.c_double) for w in warr])
+    rc = lib().ha_searchlight_procrustes(ctypes.byref(tx), ctypes.byref(ty), ctypes.byref(hx.st),
+                                         ctypes.byref(hy.st) if hy else None, ctypes.byref(csc), wp,
+                                         bool(reflection), bool(scaling), BACKENDS["hip"])
+    _check("ha_searchlight_procrustes", rc)
+    mat.data[:] = data
+    return mat
+
+
+def sparse_scatter_add(mat, rows, cols, local_T, weights=None):
+    """ha_sparse_scatter_add on the GPU (one local transform into a CSC matrix)."""
+    data = np.ascontiguousarray(mat.data, dtype=np.float64)
+    indices = np.ascontiguousarray(mat.indices, dtype=np.int32)
+    indptr = np.ascontiguousarray(mat.indptr, dtype=np.int32)
+    csc = TSparseCSC(_ptr(data, ctypes.c_double), _ptr(indices, ctypes.c_int32), _ptr(indptr, ctypes.c_int32),
+                     mat.shape[0], mat.shape[1], data.size)
+    r = np.ascontiguousarray(rows, dtype=np.int32)
+    c = np.ascontiguousarray(cols, dtype=np.int32)
+    T = np.ascontiguousarray(local_T, dtype=np.float64)
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+    lib().ha_sparse_scatter_add(ctypes.byref(csc), _ptr(r, ctypes.c_int32), _ptr(c, ctypes.c_int32), len(r),
+                                len(c), _ptr(T, ctypes.c_double), _ptr(w, ctypes.c_double))
+    mat.data[:] = data
+    return mat
+
+
+def procrustes(X, Y, reflection=True, scaling=False):
+    """ha_procrustes (one local problem, FP64 Jacobi on the GPU)."""
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    Y = np.ascontiguousarray(Y, dtype=np.float64)
+    T = np.zeros((X.shape[1], X.shape[1]), dtype=np.float64)
+    tx = TMat(_ptr(X, ctypes.c_double), X.shape[0], X.shape[1])
+    ty = TMat(_ptr(Y, ctypes.c_double), Y.shape[0], Y.shape[1])
+    tt = TMat(_ptr(T, ctypes.c_double), T.shape[0], T.shape[1])
+    _check("ha_procrustes", lib().ha_procrustes(ctypes.byref(tx), ctypes.byref(ty), ctypes.byref(tt),
+                                                bool(reflection), bool(scaling)))
+    return T
+
+
+class Plan:
+    """Device-resident searchlight pass (ha_hip_plan_*): searchlights and data in HBM."""
+
+    def __init__(self, nt, nv, idx, offsets, dists, radius):
+        self._idx = np.ascontiguousarray(idx, dtype=np.int32)
+        self._off = np.ascontiguousarray(offsets, dtype=np.int32)
+        self._d = None if dists is None else np.ascontiguousarray(dists, dtype=np.float64)
+        self.nt, self.nv = int(nt), int(nv)
+        self.count = len(self._off) - 1
+        h = ctypes.c_void_p()
+        rc = lib().ha_hip_plan_create(self.nt, self.nv, _ptr(self._idx, ctypes.c_int32),
+                                      _ptr(self._off, ctypes.c_int32), _ptr(self._d, ctypes.c_double),
+                                      self.count, float(radius), ctypes.byref(h))
+        _check("ha_hip_plan_create", rc)
+        self.h = h
+
+    def load(self, X, Y):
+        X = np.asfortranarray(X, dtype=np.float64)
+        Y = np.asfortranarray(Y, dtype=np.float64)
+        _check("ha_hip_plan_load", lib().ha_hip_plan_load(self.h, _ptr(X, ctypes.c_double),
+                                                          _ptr(Y, ctypes.c_double), True))
+
+    def execute(self, reflection=True, scaling=False):
+        _check("ha_hip_plan_execute", lib().ha_hip_plan_execute(self.h, bool(reflection), bool(scaling)))
+
+    def nnz(self):
+        n = ctypes.c_int64()
+        _check("ha_hip_plan_nnz", lib().ha_hip_plan_nnz(self.h, ctypes.byref(n)))
+        return n.value
+
+    def csc(self, with_data=True):
+        nnz = self.nnz()
+        indptr = np.zeros(self.nv + 1, dtype=np.int32)
+        indices = np.zeros(max(nnz, 1), dtype=np.int32)
+        data = np.zeros(max(nnz, 1), dtype=np.float64) if with_data else None
+        _check("ha_hip_plan_download", lib().ha_hip_plan_download(self.h, _ptr(indptr, ctypes.c_int32),
+                                                                  _ptr(indices, ctypes.c_int32),
+                                                                  _ptr(data, ctypes.c_double)))
+        return indptr, indices[:nnz], (data[:nnz] if with_data else None)
+
+    def local(self, which):
+        which = np.ascontiguousarray(which, dtype=np.int32)
+        sizes = np.diff(self._off)[which].astype(np.int64)
+        out = np.zeros(int((sizes ** 2).sum()), dtype=np.float32)
+        _check("ha_hip_plan_download_local", lib().ha_hip_plan_download_local(
+            self.h, _ptr(which, ctypes.c_int32), len(which), _ptr(out, ctypes.c_float)))
+        res, o = [], 0
+        for k in sizes:
+            res.append(out[o:o + k * k].reshape(k, k))
+            o += k * k
+        return res
+
+    def stats(self):
+        s = HaHipStats()
+        _check("ha_hip_plan_stats", lib().ha_hip_plan_stats(self.h, ctypes.byref(s)))
+        return s.as_dict()
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ha_hip_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

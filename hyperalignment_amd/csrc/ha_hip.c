@@ -1,0 +1,822 @@
+/*
+ * ha_hip.c -- HIP backend of searchlight Procrustes: device state, plans and
+ * the pass orchestration. Plain C on the HIP runtime API; all arithmetic runs
+ * in the gfx950 kernels behind hak.h.
+ *
+ * Reference interfaces replaced: ha_cuda.h:38-66 / ha_cuda.cu:47-70, 355-468
+ * (init / cleanup / available / dense entry) and the CPU loop of
+ * ha_searchlight.c:147-202 (CSC entry, which has no GPU branch there).
+ */
+#define _GNU_SOURCE
+#include <hip/hip_runtime_api.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+#include "ha_internal.h"
+#include "hak.h"
+
+/* ------------------------------------------------------------------------ */
+/* global state                                                              */
+/* ------------------------------------------------------------------------ */
+
+static struct {
+	bool ready;
+	hipStream_t stream;
+	int n_cu;
+	HaHipOptions opt;
+	float *d_sched;
+	float *d_mlp;
+	float *d_mlp_norm;
+	bool has_model;
+	int host_threads;
+} g = {
+	.ready = false,
+	.opt = {
+		.method = kHaHipMethodAuto,
+		.fp64_fallback = 1,
+		.max_polar_steps = HA_SCHED_STEPS,
+		.polar_tol = 1e-2,
+		.plain_ell = 1e-7,
+		.tile = 0,
+		.use_graph = 0,
+		.chunk = 0,
+		.verbose = 0,
+		.learned_margin_scale = 0.05f,
+		.learned_margin_ell = 0.5f,
+	},
+	.host_threads = 0,
+};
+
+#define ERR(...) fprintf(stderr, "ha_hip: " __VA_ARGS__)
+
+/* hip call check: on failure print and jump to `fail` with rc set */
+#define HCK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { \
+	ERR("%s failed: %s (%s:%d)\n", #x, hipGetErrorString(e_), __FILE__, __LINE__); \
+	rc = kHaErrorInternal; goto fail; } } while (0)
+/* kernel launcher check */
+#define KCK(x) do { int e_ = (x); if (e_ != 0) { \
+	ERR("%s failed: %s (%s:%d)\n", #x, hipGetErrorString((hipError_t)e_), __FILE__, __LINE__); \
+	rc = kHaErrorInternal; goto fail; } } while (0)
+#define ACK(p, bytes) do { size_t b_ = (size_t)(bytes); if (b_ == 0) b_ = 16; \
+	if (hipMalloc((void **)&(p), b_) != hipSuccess) { \
+	ERR("device allocation of %zu bytes failed (%s:%d)\n", b_, __FILE__, __LINE__); \
+	(void)hipGetLastError(); rc = kHaErrorAlloc; goto fail; } } while (0)
+
+void *ha_hip_stream(void) { return (void *)g.stream; }
+
+int ha_host_threads(void) {
+#ifdef _OPENMP
+	return g.host_threads > 0 ? g.host_threads : omp_get_max_threads();
+#else
+	return 1;
+#endif
+}
+
+void ha_set_num_threads(int n) { g.host_threads = n > 0 ? n : 0; }
+int ha_get_num_threads(void) { return ha_host_threads(); }
+
+static int build_schedules(void) {
+	int rc = kHaSuccess;
+	size_t n = (size_t)HA_SCHED_BUCKETS * HA_SCHED_STEPS * 3;
+	float *h = (float *)malloc(n * sizeof(float));
+	if (!h) return kHaErrorAlloc;
+	for (int b = 0; b < HA_SCHED_BUCKETS; ++b) {
+		double ell = pow(10.0, -(double)b / HA_SCHED_PER_DECADE);
+		if (b == 0) ell = 0.96; /* already converged: quintic Newton-Schulz */
+		if (ha_polar_schedule(ell, HA_SCHED_STEPS, h + (size_t)b * HA_SCHED_STEPS * 3) < 0) {
+			ERR("polar schedule for ell=%g failed\n", ell);
+			free(h);
+			return kHaErrorInternal;
+		}
+	}
+	if (!g.d_sched) ACK(g.d_sched, n * sizeof(float));
+	HCK(hipMemcpy(g.d_sched, h, n * sizeof(float), hipMemcpyHostToDevice));
+fail:
+	free(h);
+	return rc;
+}
+
+int ha_hip_init(void) {
+	int rc = kHaSuccess;
+	if (g.ready) return kHaSuccess;
+	int n = 0;
+	if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+		(void)hipGetLastError();
+		ERR("no HIP device available\n");
+		return kHaErrorInternal;
+	}
+	HCK(hipSetDevice(0));
+	hipDeviceProp_t prop;
+	HCK(hipGetDeviceProperties(&prop, 0));
+	g.n_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+	HCK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+	g.ready = true;
+	rc = build_schedules();
+	if (rc != kHaSuccess) { ha_hip_cleanup(); return rc; }
+	return kHaSuccess;
+fail:
+	return rc;
+}
+
+void ha_hip_cleanup(void) {
+	if (!g.ready) return;
+	hipStreamSynchronize(g.stream);
+	if (g.d_sched) hipFree(g.d_sched);
+	if (g.d_mlp) hipFree(g.d_mlp);
+	if (g.d_mlp_norm) hipFree(g.d_mlp_norm);
+	g.d_sched = g.d_mlp = g.d_mlp_norm = NULL;
+	g.has_model = false;
+	hipStreamDestroy(g.stream);
+	g.ready = false;
+}
+
+bool ha_hip_available(void) { return g.ready; }
+
+int ha_hip_ensure(void) {
+	if (g.ready) return kHaSuccess;
+	return ha_hip_init();
+}
+
+void ha_hip_get_options(HaHipOptions *opt) { if (opt) *opt = g.opt; }
+
+int ha_hip_set_options(const HaHipOptions *opt) {
+	if (!opt) return kHaErrorArg;
+	if (opt->method < kHaHipMethodAuto || opt->method > kHaHipMethodJacobi64) {
+		ERR("unknown method %d\n", opt->method);
+		return kHaErrorArg;
+	}
+	if (opt->tile != 0 && opt->tile != 64 && opt->tile != 128) {
+		ERR("tile must be 0, 64 or 128\n");
+		return kHaErrorArg;
+	}
+	if (opt->max_polar_steps < 1 || opt->max_polar_steps > 64 || !(opt->polar_tol > 0) ||
+	    !(opt->plain_ell > 0 && opt->plain_ell < 1)) {
+		ERR("invalid polar options\n");
+		return kHaErrorArg;
+	}
+	g.opt = *opt;
+	return kHaSuccess;
+}
+
+int ha_hip_set_model(const float *params, const float *feat_norm) {
+	int rc = ha_hip_ensure();
+	if (rc) return rc;
+	if (!params) { g.has_model = false; return kHaSuccess; }
+	if (!feat_norm) return kHaErrorArg;
+	if (!g.d_mlp) ACK(g.d_mlp, HA_HIP_MLP_NPARAM * sizeof(float));
+	if (!g.d_mlp_norm) ACK(g.d_mlp_norm, 2 * HA_HIP_MLP_IN * sizeof(float));
+	HCK(hipMemcpy(g.d_mlp, params, HA_HIP_MLP_NPARAM * sizeof(float), hipMemcpyHostToDevice));
+	HCK(hipMemcpy(g.d_mlp_norm, feat_norm, 2 * HA_HIP_MLP_IN * sizeof(float), hipMemcpyHostToDevice));
+	g.has_model = true;
+	return kHaSuccess;
+fail:
+	return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* plans                                                                     */
+/* ------------------------------------------------------------------------ */
+
+struct HaHipPlan {
+	int32_t nt, nv_x, nv_y, count, ld;
+	int64_t total;
+	double radius;
+	bool has_dists, own_pattern;
+	int32_t kmax, kpmax;
+	/* searchlights */
+	int32_t *slx, *sly, *off, *sl_of;
+	double *dist;
+	int32_t *h_k; /* host copy of sizes */
+	/* inverse index over the Y lists (scatter) and, when they differ, X lists (weights) */
+	int64_t *occ_ptr; int32_t *occ;
+	int64_t *occx_ptr; int32_t *occx;
+	/* weights */
+	double *wsum, *w;
+	bool weights_computed; /* w comes from the GPU weight kernel */
+	bool weights_none;
+	/* CSC */
+	int32_t ncols;
+	int64_t nnz;
+	int64_t *indptr64; int32_t *indptr; int32_t *indices; double *data;
+	/* local transforms */
+	int32_t *qld; int64_t *qoff; float *q; int64_t q_elems;
+	double *q64;
+	float *X, *Y; double *X64, *Y64; double *stage;
+	float *work; int64_t slot_stride; int32_t n_slots;
+	double *work64; int64_t stride64; int32_t n_slots64;
+	int32_t *flag, *steps, *queues, *order;
+	double *sigma_sum, *var_sum, *scale;
+	float *features, *pred;
+	HaHipStats stats;
+	hipEvent_t ev[8];
+	bool events;
+	bool loaded;
+};
+
+static void plan_free(HaHipPlan *p) {
+	if (!p) return;
+	void *ptrs[] = {p->slx, p->sly == p->slx ? NULL : p->sly, p->off, p->sl_of, p->dist, p->occ_ptr, p->occ,
+	                p->occx_ptr, p->occx, p->wsum, p->w, p->indptr64, p->indptr, p->indices, p->data,
+	                p->qld, p->qoff, p->q, p->q64, p->X, p->Y, p->X64, p->Y64, p->stage, p->work, p->work64,
+	                p->flag, p->steps, p->queues, p->order, p->sigma_sum, p->var_sum, p->scale,
+	                p->features, p->pred};
+	for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i)
+		if (ptrs[i]) hipFree(ptrs[i]);
+	if (p->events)
+		for (int i = 0; i < 8; ++i) hipEventDestroy(p->ev[i]);
+	free(p->h_k);
+	free(p);
+}
+
+void ha_hip_plan_destroy(HaHipPlan *p) {
+	if (p && g.ready) hipStreamSynchronize(g.stream);
+	plan_free(p);
+}
+
+static int cmp_k_desc(const void *a, const void *b, void *ctx) {
+	const int32_t *k = (const int32_t *)ctx;
+	int32_t x = *(const int32_t *)a, y = *(const int32_t *)b;
+	if (k[x] != k[y]) return k[x] > k[y] ? -1 : 1;
+	return x < y ? -1 : (x > y);
+}
+
+/* vertex -> occurrence index (sorted by flat position) over list idx */
+static int build_occ(HaHipPlan *p, const int32_t *idx, int32_t nv, int64_t **ptr_out, int32_t **occ_out) {
+	int rc = kHaSuccess;
+	hipStream_t st = g.stream;
+	int32_t *cnt = NULL, *err = NULL;
+	int64_t *ptr = NULL;
+	int32_t *occ = NULL;
+	ACK(cnt, (size_t)nv * sizeof(int32_t));
+	ACK(err, sizeof(int32_t));
+	ACK(ptr, (size_t)(nv + 1) * sizeof(int64_t));
+	ACK(occ, (size_t)p->total * sizeof(int32_t));
+	KCK(hak_fill_i32(cnt, nv, 0, st));
+	KCK(hak_fill_i32(err, 1, 0, st));
+	KCK(hak_count_occ(idx, p->total, cnt, st));
+	KCK(hak_scan_i32_i64(cnt, nv, ptr, st));
+	KCK(hak_fill_i32(cnt, nv, 0, st));
+	KCK(hak_fill_occ(idx, p->total, ptr, cnt, occ, st));
+	KCK(hak_sort_occ(ptr, nv, p->sl_of, occ, err, st));
+	int32_t h_err = 0;
+	HCK(hipMemcpyAsync(&h_err, err, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+	HCK(hipStreamSynchronize(st));
+	if (h_err == 1) {
+		ERR("a vertex belongs to more than 8192 searchlights (unsupported)\n");
+		rc = kHaErrorArg;
+		goto fail;
+	}
+	if (h_err == 2) {
+		ERR("a searchlight lists the same vertex twice\n");
+		rc = kHaErrorArg;
+		goto fail;
+	}
+	hipFree(cnt);
+	hipFree(err);
+	*ptr_out = ptr;
+	*occ_out = occ;
+	return kHaSuccess;
+fail:
+	if (cnt) hipFree(cnt);
+	if (err) hipFree(err);
+	if (ptr) hipFree(ptr);
+	if (occ) hipFree(occ);
+	return rc;
+}
+
+static int pick_tile(int kpmax) {
+	if (g.opt.tile) return g.opt.tile;
+	return kpmax <= 64 ? 64 : 64;
+}
+
+int ha_plan_create_ex(int32_t nt, const HaPlanSetup *su, const int32_t *slx_idx, const int32_t *sly_idx,
+                      const int32_t *sl_off, const double *sl_dists, int32_t count, double radius,
+                      HaHipPlan **out) {
+	int rc = ha_hip_ensure();
+	if (rc) return rc;
+	if (!out || !slx_idx || !sl_off || count <= 0 || nt <= 0 || su->nv_x <= 0 || su->nv_y <= 0) {
+		ERR("invalid plan arguments\n");
+		return kHaErrorArg;
+	}
+	if (sl_off[0] != 0) { ERR("sl_offsets[0] must be 0\n"); return kHaErrorArg; }
+	for (int32_t s = 0; s < count; ++s)
+		if (sl_off[s + 1] <= sl_off[s]) { ERR("searchlight %d is empty or offsets decrease\n", s); return kHaErrorArg; }
+	const int64_t total = sl_off[count];
+	for (int64_t k = 0; k < total; ++k) {
+		if (slx_idx[k] < 0 || slx_idx[k] >= su->nv_x) { ERR("vertex index %d out of range\n", slx_idx[k]); return kHaErrorArg; }
+		if (sly_idx && (sly_idx[k] < 0 || sly_idx[k] >= su->nv_y)) { ERR("vertex index %d out of range\n", sly_idx[k]); return kHaErrorArg; }
+	}
+	HaHipPlan *p = (HaHipPlan *)calloc(1, sizeof(HaHipPlan));
+	if (!p) return kHaErrorAlloc;
+	hipStream_t st = g.stream;
+	p->nt = nt; p->nv_x = su->nv_x; p->nv_y = su->nv_y; p->count = count; p->total = total;
+	p->radius = radius; p->has_dists = sl_dists != NULL;
+	p->ld = (nt + 31) / 32 * 32;
+	p->h_k = (int32_t *)malloc((size_t)count * sizeof(int32_t));
+	int32_t *h_qld = (int32_t *)malloc((size_t)count * sizeof(int32_t));
+	int64_t *h_qoff = (int64_t *)malloc((size_t)count * sizeof(int64_t));
+	int32_t *h_order = (int32_t *)malloc((size_t)count * sizeof(int32_t));
+	if (!p->h_k || !h_qld || !h_qoff || !h_order) { rc = kHaErrorAlloc; goto fail; }
+	int64_t qe = 0;
+	for (int32_t s = 0; s < count; ++s) {
+		int32_t k = sl_off[s + 1] - sl_off[s];
+		p->h_k[s] = k;
+		if (k > p->kmax) p->kmax = k;
+		h_qld[s] = (k + 31) / 32 * 32;
+		h_qoff[s] = qe;
+		qe += (int64_t)h_qld[s] * h_qld[s];
+		h_order[s] = s;
+	}
+	p->kpmax = (p->kmax + 31) / 32 * 32;
+	p->q_elems = qe;
+	qsort_r(h_order, (size_t)count, sizeof(int32_t), cmp_k_desc, p->h_k);
+
+	for (int i = 0; i < 8; ++i) HCK(hipEventCreate(&p->ev[i]));
+	p->events = true;
+
+	ACK(p->slx, total * sizeof(int32_t));
+	HCK(hipMemcpyAsync(p->slx, slx_idx, total * sizeof(int32_t), hipMemcpyHostToDevice, st));
+	if (sly_idx && sly_idx != slx_idx) {
+		ACK(p->sly, total * sizeof(int32_t));
+		HCK(hipMemcpyAsync(p->sly, sly_idx, total * sizeof(int32_t), hipMemcpyHostToDevice, st));
+	} else {
+		p->sly = p->slx;
+	}
+	ACK(p->off, (size_t)(count + 1) * sizeof(int32_t));
+	HCK(hipMemcpyAsync(p->off, sl_off, (size_t)(count + 1) * sizeof(int32_t), hipMemcpyHostToDevice, st));
+	if (sl_dists) {
+		ACK(p->dist, total * sizeof(double));
+		HCK(hipMemcpyAsync(p->dist, sl_dists, total * sizeof(double), hipMemcpyHostToDevice, st));
+	}
+	ACK(p->sl_of, total * sizeof(int32_t));
+	KCK(hak_sl_of(p->off, count, p->sl_of, st));
+
+	/* scatter index over the Y lists; weights index over the X lists */
+	rc = build_occ(p, p->sly, p->nv_y, &p->occ_ptr, &p->occ);
+	if (rc) goto fail;
+	if (su->build_weights_index) {
+		if (p->sly == p->slx && p->nv_x == p->nv_y) {
+			p->occx_ptr = NULL; p->occx = NULL;
+		} else {
+			rc = build_occ(p, p->slx, p->nv_x, &p->occx_ptr, &p->occx);
+			if (rc) goto fail;
+		}
+		ACK(p->wsum, (size_t)p->nv_x * sizeof(double));
+		ACK(p->w, total * sizeof(double));
+		p->weights_computed = true;
+	}
+	p->weights_none = !su->build_weights_index;
+
+	if (su->build_pattern) {
+		if (p->sly != p->slx || p->nv_x != p->nv_y) {
+			ERR("the union pattern needs identical X and Y searchlights\n");
+			rc = kHaErrorArg;
+			goto fail;
+		}
+		hipEventRecord(p->ev[0], st);
+		int32_t *slmin = NULL, *slmax = NULL, *colcnt = NULL;
+		ACK(slmin, (size_t)count * sizeof(int32_t));
+		ACK(slmax, (size_t)count * sizeof(int32_t));
+		ACK(colcnt, (size_t)p->nv_x * sizeof(int32_t));
+		ACK(p->indptr64, (size_t)(p->nv_x + 1) * sizeof(int64_t));
+		KCK(hak_sl_minmax(p->slx, p->off, count, slmin, slmax, st));
+		KCK(hak_pattern_count(p->slx, p->off, p->sl_of, p->occ_ptr, p->occ, slmin, slmax, p->nv_x, colcnt, st));
+		KCK(hak_scan_i32_i64(colcnt, p->nv_x, p->indptr64, st));
+		int64_t nnz = 0;
+		HCK(hipMemcpyAsync(&nnz, p->indptr64 + p->nv_x, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+		HCK(hipStreamSynchronize(st));
+		if (nnz >= ((int64_t)1 << 31)) {
+			ERR("pattern has %lld nonzeros; int32 indptr cannot hold it\n", (long long)nnz);
+			hipFree(slmin); hipFree(slmax); hipFree(colcnt);
+			rc = kHaErrorArg;
+			goto fail;
+		}
+		p->nnz = nnz;
+		p->ncols = p->nv_x;
+		ACK(p->indptr, (size_t)(p->nv_x + 1) * sizeof(int32_t));
+		ACK(p->indices, (size_t)nnz * sizeof(int32_t));
+		ACK(p->data, (size_t)nnz * sizeof(double));
+		KCK(hak_i64_to_i32(p->indptr64, p->nv_x + 1, p->indptr, st));
+		KCK(hak_pattern_fill(p->slx, p->off, p->sl_of, p->occ_ptr, p->occ, slmin, slmax, p->nv_x, p->indptr64,
+		                     p->indices, st));
+		hipEventRecord(p->ev[1], st);
+		HCK(hipStreamSynchronize(st));
+		float ms = 0;
+		hipEventElapsedTime(&ms, p->ev[0], p->ev[1]);
+		p->stats.ms_pattern = ms;
+		hipFree(slmin); hipFree(slmax); hipFree(colcnt);
+		p->own_pattern = true;
+	}
+
+	if (su->index_only) {
+		HCK(hipStreamSynchronize(st));
+		free(h_qld); free(h_qoff); free(h_order);
+		*out = p;
+		return kHaSuccess;
+	}
+	/* local transform storage */
+	ACK(p->qld, (size_t)count * sizeof(int32_t));
+	ACK(p->qoff, (size_t)count * sizeof(int64_t));
+	ACK(p->order, (size_t)count * sizeof(int32_t));
+	HCK(hipMemcpyAsync(p->qld, h_qld, (size_t)count * sizeof(int32_t), hipMemcpyHostToDevice, st));
+	HCK(hipMemcpyAsync(p->qoff, h_qoff, (size_t)count * sizeof(int64_t), hipMemcpyHostToDevice, st));
+	HCK(hipMemcpyAsync(p->order, h_order, (size_t)count * sizeof(int32_t), hipMemcpyHostToDevice, st));
+	ACK(p->q, (size_t)p->q_elems * sizeof(float));
+	ACK(p->flag, (size_t)count * sizeof(int32_t));
+	ACK(p->steps, (size_t)count * sizeof(int32_t));
+	ACK(p->queues, 8 * sizeof(int32_t));
+	ACK(p->sigma_sum, (size_t)count * sizeof(double));
+	ACK(p->var_sum, (size_t)count * sizeof(double));
+	ACK(p->scale, (size_t)count * sizeof(double));
+	p->slot_stride = (int64_t)p->kpmax * p->kpmax;
+	p->n_slots = 2 * g.n_cu;
+	if (p->n_slots > count) p->n_slots = count;
+	ACK(p->work, (size_t)p->n_slots * 5 * p->slot_stride * sizeof(float));
+	p->stride64 = (int64_t)p->kpmax * p->kpmax;
+	p->n_slots64 = g.n_cu;
+	if (p->n_slots64 > count) p->n_slots64 = count;
+	if (p->n_slots64 > 128) p->n_slots64 = 128;
+	ACK(p->work64, (size_t)p->n_slots64 * 4 * p->stride64 * sizeof(double));
+	ACK(p->X, (size_t)p->ld * p->nv_x * sizeof(float));
+	ACK(p->Y, (size_t)p->ld * p->nv_y * sizeof(float));
+	ACK(p->X64, (size_t)nt * p->nv_x * sizeof(double));
+	ACK(p->Y64, (size_t)nt * p->nv_y * sizeof(double));
+	HCK(hipStreamSynchronize(st));
+	free(h_qld); free(h_qoff); free(h_order);
+	*out = p;
+	return kHaSuccess;
+fail:
+	hipStreamSynchronize(st);
+	free(h_qld); free(h_qoff); free(h_order);
+	plan_free(p);
+	return rc;
+}
+
+int ha_hip_plan_create(int32_t nt, int32_t nv, const int32_t *sl_indices, const int32_t *sl_offsets,
+                       const double *sl_dists, int32_t count, double radius, HaHipPlan **plan_out) {
+	HaPlanSetup su = {.build_pattern = true, .build_weights_index = true, .nv_x = nv, .nv_y = nv};
+	return ha_plan_create_ex(nt, &su, sl_indices, NULL, sl_offsets, sl_dists, count, radius, plan_out);
+}
+
+int ha_plan_set_pattern(HaHipPlan *p, int32_t ncols, const int32_t *indptr, const int32_t *indices, int64_t nnz,
+                        const double *data) {
+	int rc = kHaSuccess;
+	hipStream_t st = g.stream;
+	if (p->indptr) { hipFree(p->indptr); p->indptr = NULL; }
+	if (p->indptr64) { hipFree(p->indptr64); p->indptr64 = NULL; }
+	if (p->indices) { hipFree(p->indices); p->indices = NULL; }
+	if (p->data) { hipFree(p->data); p->data = NULL; }
+	if (ncols > p->nv_y) { ERR("pattern has more columns than Y vertices\n"); return kHaErrorArg; }
+	p->ncols = ncols;
+	p->nnz = nnz;
+	int64_t *h64 = (int64_t *)malloc((size_t)(ncols + 1) * sizeof(int64_t));
+	if (!h64) return kHaErrorAlloc;
+	for (int32_t c = 0; c <= ncols; ++c) h64[c] = indptr[c];
+	ACK(p->indptr, (size_t)(ncols + 1) * sizeof(int32_t));
+	ACK(p->indptr64, (size_t)(ncols + 1) * sizeof(int64_t));
+	ACK(p->indices, (size_t)nnz * sizeof(int32_t));
+	ACK(p->data, (size_t)nnz * sizeof(double));
+	HCK(hipMemcpyAsync(p->indptr, indptr, (size_t)(ncols + 1) * sizeof(int32_t), hipMemcpyHostToDevice, st));
+	HCK(hipMemcpyAsync(p->indptr64, h64, (size_t)(ncols + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
+	if (nnz > 0) {
+		HCK(hipMemcpyAsync(p->indices, indices, (size_t)nnz * sizeof(int32_t), hipMemcpyHostToDevice, st));
+		if (data)
+			HCK(hipMemcpyAsync(p->data, data, (size_t)nnz * sizeof(double), hipMemcpyHostToDevice, st));
+		else
+			KCK(hak_fill_f64(p->data, nnz, 0.0, st));
+	}
+	HCK(hipStreamSynchronize(st));
+	free(h64);
+	return kHaSuccess;
+fail:
+	free(h64);
+	return rc;
+}
+
+int ha_plan_set_weights(HaHipPlan *p, const double *w_flat, bool use_computed) {
+	int rc = kHaSuccess;
+	if (use_computed) {
+		if (!p->weights_computed) { ERR("plan has no weight index\n"); return kHaErrorArg; }
+		p->weights_none = false;
+		return kHaSuccess;
+	}
+	p->weights_computed = false;
+	if (!w_flat) { p->weights_none = true; return kHaSuccess; }
+	p->weights_none = false;
+	if (!p->w) ACK(p->w, (size_t)p->total * sizeof(double));
+	HCK(hipMemcpy(p->w, w_flat, (size_t)p->total * sizeof(double), hipMemcpyHostToDevice));
+	return kHaSuccess;
+fail:
+	return rc;
+}
+
+static int upload_matrix(HaHipPlan *p, const double *M, int32_t nv, bool col_major, float *d32, double *d64) {
+	int rc = kHaSuccess;
+	hipStream_t st = g.stream;
+	size_t bytes = (size_t)p->nt * nv * sizeof(double);
+	if (!p->stage) {
+		int32_t nvm = p->nv_x > p->nv_y ? p->nv_x : p->nv_y;
+		ACK(p->stage, (size_t)p->nt * nvm * sizeof(double));
+	}
+	HCK(hipMemcpyAsync(p->stage, M, bytes, hipMemcpyHostToDevice, st));
+	KCK(hak_convert_data(p->stage, p->nt, nv, col_major ? 1 : 0, d32, p->ld, d64, st));
+	HCK(hipStreamSynchronize(st));
+	return kHaSuccess;
+fail:
+	return rc;
+}
+
+int ha_plan_load_xy(HaHipPlan *p, const double *X, const double *Y, bool col_major) {
+	if (!p || !X || !Y) return kHaErrorArg;
+	int rc = upload_matrix(p, X, p->nv_x, col_major, p->X, p->X64);
+	if (rc) return rc;
+	rc = upload_matrix(p, Y, p->nv_y, col_major, p->Y, p->Y64);
+	if (rc) return rc;
+	p->loaded = true;
+	return kHaSuccess;
+}
+
+int ha_hip_plan_load(HaHipPlan *plan, const double *X, const double *Y, bool col_major) {
+	return ha_plan_load_xy(plan, X, Y, col_major);
+}
+
+static HakLocalArgs local_args(HaHipPlan *p) {
+	HakLocalArgs a;
+	memset(&a, 0, sizeof(a));
+	a.X = p->X; a.Y = p->Y; a.X64 = p->X64; a.Y64 = p->Y64;
+	a.nt = p->nt; a.ld = p->ld;
+	a.slx_idx = p->slx; a.sly_idx = p->sly; a.sl_off = p->off;
+	a.order = p->order; a.n_order = p->count;
+	a.q = p->q; a.qoff = p->qoff; a.qld = p->qld;
+	a.flag = p->flag; a.steps = p->steps;
+	a.work = p->work; a.slot_stride = p->slot_stride; a.n_slots = p->n_slots;
+	a.sched = g.d_sched;
+	a.n_buckets = HA_SCHED_BUCKETS; a.max_sched = HA_SCHED_STEPS;
+	a.bucket_per_decade = HA_SCHED_PER_DECADE;
+	a.max_steps = g.opt.max_polar_steps;
+	int b = (int)ceil(-log10(g.opt.plain_ell) * HA_SCHED_PER_DECADE);
+	a.default_bucket = b < 0 ? 0 : (b >= HA_SCHED_BUCKETS ? HA_SCHED_BUCKETS - 1 : b);
+	a.tol = (float)g.opt.polar_tol;
+	a.tile = pick_tile(p->kpmax);
+	a.fallback_enabled = g.opt.fp64_fallback;
+	a.rank_check_nt = g.opt.fp64_fallback;
+	a.mlp_margin_scale = g.opt.learned_margin_scale;
+	a.mlp_margin_ell = g.opt.learned_margin_ell;
+	return a;
+}
+
+static int resolve_method(const HaHipPlan *p, int method) {
+	if (method < 0) method = g.opt.method;
+	if (method == kHaHipMethodAuto) {
+		if (p->kmax <= 32) return kHaHipMethodJacobi64;
+		return g.has_model ? kHaHipMethodPolarLearned : kHaHipMethodPolar;
+	}
+	if (method == kHaHipMethodPolarLearned && !g.has_model) {
+		ERR("learned polar scaling requested but no model is loaded\n");
+		return -1;
+	}
+	return method;
+}
+
+static void stats_update(HaHipPlan *p, int method) {
+	int32_t *st = (int32_t *)malloc((size_t)p->count * sizeof(int32_t));
+	int32_t *fl = (int32_t *)malloc((size_t)p->count * sizeof(int32_t));
+	if (!st || !fl) { free(st); free(fl); return; }
+	hipMemcpy(st, p->steps, (size_t)p->count * sizeof(int32_t), hipMemcpyDeviceToHost);
+	hipMemcpy(fl, p->flag, (size_t)p->count * sizeof(int32_t), hipMemcpyDeviceToHost);
+	HaHipStats *S = &p->stats;
+	memset(S->steps_hist, 0, sizeof(S->steps_hist));
+	S->n_fallback = 0; S->n_unconverged = 0; S->steps_total = 0;
+	double flops = 0;
+	for (int32_t s = 0; s < p->count; ++s) {
+		double k = p->h_k[s];
+		int32_t v = st[s];
+		if (fl[s]) S->n_fallback++;
+		flops += 2.0 * p->nt * k * k; /* cross-covariance */
+		if (method == kHaHipMethodPolar || method == kHaHipMethodPolarLearned) {
+			if (v >= 1000) v = 0;      /* solved by the FP64 path */
+			if (v < 0) { S->n_unconverged++; v = -v; }
+			S->steps_total += v;
+			S->steps_hist[v < 31 ? v : 31]++;
+			if (v > 0) flops += v * 4.0 * k * k * k + (v - 1) * 2.0 * k * k * k;
+		} else {
+			if (v >= 1000) v -= 1000;
+			if (v < 0) { S->n_unconverged++; v = -v; }
+			S->steps_total += v;
+			S->steps_hist[v < 31 ? v : 31]++;
+		}
+	}
+	S->gemm_flops = flops;
+	S->nnz = p->nnz;
+	free(st);
+	free(fl);
+}
+
+int ha_plan_run(HaHipPlan *p, bool isReflection, bool isScaling, bool zero_data, int method_in) {
+	int rc = kHaSuccess;
+	if (!p || !p->loaded) { ERR("plan has no data loaded\n"); return kHaErrorArg; }
+	if (!p->data || !p->indptr64) { ERR("plan has no CSC pattern\n"); return kHaErrorArg; }
+	const int method = resolve_method(p, method_in);
+	if (method < 0) return kHaErrorArg;
+	hipStream_t st = g.stream;
+	HakLocalArgs a = local_args(p);
+	if (isScaling) a.sigma_sum = p->sigma_sum;
+	if (method == kHaHipMethodPolarLearned) {
+		a.mlp = g.d_mlp;
+		a.mlp_norm = g.d_mlp_norm;
+	}
+	HCK(hipEventRecord(p->ev[0], st));
+	KCK(hak_fill_i32(p->queues, 8, 0, st));
+	KCK(hak_fill_i32(p->flag, p->count, 0, st));
+	KCK(hak_fill_i32(p->steps, p->count, 0, st));
+	if (zero_data) KCK(hak_fill_f64(p->data, p->nnz, 0.0, st));
+	/* weights (ha_searchlight.c:970-1001) */
+	if (p->weights_computed && !p->weights_none) {
+		const int64_t *optr = p->occx_ptr ? p->occx_ptr : p->occ_ptr;
+		const int32_t *occ = p->occx ? p->occx : p->occ;
+		KCK(hak_weights(p->slx, p->dist, p->radius, p->total, optr, occ, p->nv_x, p->wsum, p->w, st));
+	}
+	HCK(hipEventRecord(p->ev[1], st));
+	/* local transforms */
+	const bool det_fix = !isReflection;
+	if (det_fix || method == kHaHipMethodJacobi64) {
+		a.flag = NULL;
+		a.queue = p->queues + 1;
+		KCK(hak_jacobi64(&a, det_fix ? 1 : 0, p->work64, p->stride64, p->n_slots64, p->count, st));
+		a.flag = p->flag;
+	} else if (method == kHaHipMethodJacobi32) {
+		a.queue = p->queues + 0;
+		KCK(hak_jacobi32(&a, st));
+	} else {
+		a.queue = p->queues + 0;
+		KCK(hak_polar(&a, st));
+	}
+	HCK(hipEventRecord(p->ev[2], st));
+	if (!det_fix && method != kHaHipMethodJacobi64 && g.opt.fp64_fallback) {
+		HakLocalArgs b = a;
+		b.queue = p->queues + 2;
+		KCK(hak_jacobi64(&b, 0, p->work64, p->stride64, p->n_slots64, p->count, st));
+	}
+	HCK(hipEventRecord(p->ev[3], st));
+	if (isScaling) {
+		KCK(hak_var_sum(p->X64, p->nt, p->slx, p->off, p->count, p->var_sum, st));
+		KCK(hak_scale(p->sigma_sum, p->var_sum, p->nt, p->count, p->scale, st));
+	}
+	HCK(hipEventRecord(p->ev[4], st));
+	KCK(hak_scatter(p->slx, p->off, p->sl_of, p->occ_ptr, p->occ, p->ncols, p->indptr64, p->indices, p->data,
+	                p->q, p->qoff, p->qld, isScaling ? p->scale : NULL, p->weights_none ? NULL : p->w, 0,
+	                p->count, st));
+	HCK(hipEventRecord(p->ev[5], st));
+	HCK(hipStreamSynchronize(st));
+	{
+		float t[5];
+		for (int i = 0; i < 5; ++i) hipEventElapsedTime(&t[i], p->ev[i], p->ev[i + 1]);
+		float tot = 0;
+		hipEventElapsedTime(&tot, p->ev[0], p->ev[5]);
+		p->stats.ms_weights = t[0];
+		p->stats.ms_polar = t[1];
+		p->stats.ms_fallback = t[2];
+		p->stats.ms_crosscov = 0; /* fused into the local-transform kernel */
+		p->stats.ms_scatter = t[4] + t[3];
+		p->stats.ms_total = tot;
+		p->stats.kernel_ms_polar_avg = t[1];
+		p->stats.polar_launches = 1;
+	}
+	stats_update(p, method);
+	if (g.opt.verbose)
+		fprintf(stderr, "ha_hip: pass %.3f ms (weights %.3f, local %.3f, fp64 %.3f, scatter %.3f); fallback %lld\n",
+		        p->stats.ms_total, p->stats.ms_weights, p->stats.ms_polar, p->stats.ms_fallback,
+		        p->stats.ms_scatter, (long long)p->stats.n_fallback);
+	return kHaSuccess;
+fail:
+	hipStreamSynchronize(st);
+	return rc;
+}
+
+int ha_hip_plan_execute(HaHipPlan *plan, bool isReflection, bool isScaling) {
+	return ha_plan_run(plan, isReflection, isScaling, true, -1);
+}
+
+int ha_hip_plan_nnz(const HaHipPlan *p, int64_t *nnz) {
+	if (!p || !nnz) return kHaErrorArg;
+	*nnz = p->nnz;
+	return kHaSuccess;
+}
+
+int ha_hip_plan_download(const HaHipPlan *p, int32_t *indptr, int32_t *indices, double *data) {
+	if (!p) return kHaErrorArg;
+	int rc = kHaSuccess;
+	if (indptr) HCK(hipMemcpy(indptr, p->indptr, (size_t)(p->ncols + 1) * sizeof(int32_t), hipMemcpyDeviceToHost));
+	if (indices && p->nnz) HCK(hipMemcpy(indices, p->indices, (size_t)p->nnz * sizeof(int32_t), hipMemcpyDeviceToHost));
+	if (data && p->nnz) HCK(hipMemcpy(data, p->data, (size_t)p->nnz * sizeof(double), hipMemcpyDeviceToHost));
+	return kHaSuccess;
+fail:
+	return rc;
+}
+
+int ha_plan_download_data(const HaHipPlan *p, double *data) { return ha_hip_plan_download(p, NULL, NULL, data); }
+
+int ha_hip_plan_download_local(const HaHipPlan *p, const int32_t *which, int32_t n_which, float *out) {
+	if (!p || (!which && n_which) || !out) return kHaErrorArg;
+	int rc = kHaSuccess;
+	int64_t *qoff = (int64_t *)malloc((size_t)p->count * sizeof(int64_t));
+	float *buf = (float *)malloc((size_t)p->kpmax * p->kpmax * sizeof(float));
+	if (!qoff || !buf) { free(qoff); free(buf); return kHaErrorAlloc; }
+	HCK(hipMemcpy(qoff, p->qoff, (size_t)p->count * sizeof(int64_t), hipMemcpyDeviceToHost));
+	int64_t o = 0;
+	for (int32_t i = 0; i < n_which; ++i) {
+		int32_t s = which[i];
+		if (s < 0 || s >= p->count) { rc = kHaErrorArg; goto fail; }
+		int32_t k = p->h_k[s], kp = (k + 31) / 32 * 32;
+		HCK(hipMemcpy(buf, p->q + qoff[s], (size_t)kp * kp * sizeof(float), hipMemcpyDeviceToHost));
+		for (int32_t r = 0; r < k; ++r)
+			for (int32_t c = 0; c < k; ++c) out[o + (int64_t)r * k + c] = buf[r + (int64_t)c * kp];
+		o += (int64_t)k * k;
+	}
+fail:
+	free(qoff);
+	free(buf);
+	return rc;
+}
+
+int ha_hip_plan_stats(const HaHipPlan *p, HaHipStats *s) {
+	if (!p || !s) return kHaErrorArg;
+	*s = p->stats;
+	return kHaSuccess;
+}
+
+int ha_plan_weights(HaHipPlan *p, double *w_flat) {
+	int rc = kHaSuccess;
+	hipStream_t st = g.stream;
+	if (!p->weights_computed) return kHaErrorArg;
+	const int64_t *optr = p->occx_ptr ? p->occx_ptr : p->occ_ptr;
+	const int32_t *occ = p->occx ? p->occx : p->occ;
+	KCK(hak_weights(p->slx, p->dist, p->radius, p->total, optr, occ, p->nv_x, p->wsum, p->w, st));
+	HCK(hipMemcpyAsync(w_flat, p->w, (size_t)p->total * sizeof(double), hipMemcpyDeviceToHost, st));
+	HCK(hipStreamSynchronize(st));
+	return kHaSuccess;
+fail:
+	return rc;
+}
+
+int ha_plan_pattern(HaHipPlan *p, int32_t **indptr, int32_t **indices, int64_t *nnz) {
+	int rc = kHaSuccess;
+	*indptr = (int32_t *)malloc((size_t)(p->ncols + 1) * sizeof(int32_t));
+	*indices = (int32_t *)malloc((size_t)(p->nnz > 0 ? p->nnz : 1) * sizeof(int32_t));
+	if (!*indptr || !*indices) { free(*indptr); free(*indices); return kHaErrorAlloc; }
+	rc = ha_hip_plan_download(p, *indptr, *indices, NULL);
+	*nnz = p->nnz;
+	if (rc) { free(*indptr); free(*indices); *indptr = NULL; *indices = NULL; }
+	return rc;
+}
+
+/* FP64 local transforms for every searchlight (row-major k x k each). */
+int ha_plan_local64(HaHipPlan *p, bool isReflection, bool isScaling, double *out) {
+	int rc = kHaSuccess;
+	hipStream_t st = g.stream;
+	double *q64 = NULL;
+	double *h = NULL;
+	int64_t *qoff = NULL;
+	if (!p->loaded) return kHaErrorArg;
+	HakLocalArgs a = local_args(p);
+	a.flag = NULL;
+	a.queue = p->queues + 3;
+	a.sigma_sum = isScaling ? p->sigma_sum : NULL;
+	ACK(q64, (size_t)p->q_elems * sizeof(double));
+	a.q64 = q64;
+	KCK(hak_fill_i32(p->queues, 8, 0, st));
+	KCK(hak_jacobi64(&a, isReflection ? 0 : 1, p->work64, p->stride64, p->n_slots64, p->count, st));
+	if (isScaling) {
+		KCK(hak_var_sum(p->X64, p->nt, p->slx, p->off, p->count, p->var_sum, st));
+		KCK(hak_scale(p->sigma_sum, p->var_sum, p->nt, p->count, p->scale, st));
+	}
+	HCK(hipStreamSynchronize(st));
+	h = (double *)malloc((size_t)p->q_elems * sizeof(double));
+	qoff = (int64_t *)malloc((size_t)p->count * sizeof(int64_t));
+	double *scale = (double *)malloc((size_t)p->count * sizeof(double));
+	if (!h || !qoff || !scale) { free(scale); rc = kHaErrorAlloc; goto fail; }
+	HCK(hipMemcpy(h, q64, (size_t)p->q_elems * sizeof(double), hipMemcpyDeviceToHost));
+	HCK(hipMemcpy(qoff, p->qoff, (size_t)p->count * sizeof(int64_t), hipMemcpyDeviceToHost));
+	if (isScaling) HCK(hipMemcpy(scale, p->scale, (size_t)p->count * sizeof(double), hipMemcpyDeviceToHost));
+	int64_t o = 0;
+	for (int32_t s = 0; s < p->count; ++s) {
+		int32_t k = p->h_k[s], kp = (k + 31) / 32 * 32;
+		double sc = isScaling ? scale[s] : 1.0;
+		for (int32_t r = 0; r < k; ++r)
+			for (int32_t c = 0; c < k; ++c) {
+				double v = h[qoff[s] + r + (int64_t)c * kp];
+				out[o + (int64_t)r * k + c] = isScaling ? v * sc : v;
+			}
+		o += (int64_t)k * k;
+	}
+	free(scale);
+fail:
+	if (q64) hipFree(q64);
+	free(h);
+	free(qoff);
+	return rc;
+}

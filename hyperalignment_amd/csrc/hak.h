@@ -1,0 +1,135 @@
+/*
+ * hak.h -- thin C ABI between the C host layer (ha_hip.c) and the gfx950
+ * kernels (*.hip). Every pointer is a device pointer unless named h_*;
+ * `stream` is a hipStream_t passed as void*. Each launcher returns 0 or the
+ * hipError_t of the launch.
+ */
+#ifndef HAK_H
+#define HAK_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- fills (used instead of hipMemsetAsync so passes can be graph-captured) ---- */
+int hak_fill_i32(int32_t *p, int64_t n, int32_t v, void *stream);
+int hak_fill_f64(double *p, int64_t n, double v, void *stream);
+
+/* ---- searchlight index (k_index.hip) ---- */
+/* sl_of[k] = searchlight owning flat position k. */
+int hak_sl_of(const int32_t *sl_off, int32_t count, int32_t *sl_of, void *stream);
+/* per-vertex occurrence counts (cnt zeroed by caller). */
+int hak_count_occ(const int32_t *sl_idx, int64_t total, int32_t *cnt, void *stream);
+/* exclusive scan of n int32 counts into n+1 int64 prefixes (out[n] = total). */
+int hak_scan_i32_i64(const int32_t *in, int64_t n, int64_t *out, void *stream);
+/* int64 prefix -> int32 copy (caller checked the total fits). */
+int hak_i64_to_i32(const int64_t *in, int64_t n, int32_t *out, void *stream);
+/* fill occ[ptr[v] ..] with the flat positions of vertex v (unordered). */
+int hak_fill_occ(const int32_t *sl_idx, int64_t total, const int64_t *occ_ptr,
+                 int32_t *cursor, int32_t *occ, void *stream);
+/* sort each vertex's occurrence list ascending; *err set to 1 if a list is
+ * longer than the in-LDS limit (8192), to 2 if a searchlight lists a vertex twice. */
+int hak_sort_occ(const int64_t *occ_ptr, int32_t nv, const int32_t *sl_of, int32_t *occ,
+                 int32_t *err, void *stream);
+/* weights exactly as ha_searchlight.c:970-1001 (dists may be NULL). */
+int hak_weights(const int32_t *sl_idx, const double *dists, double radius, int64_t total,
+                const int64_t *occ_ptr, const int32_t *occ, int32_t nv,
+                double *wsum, double *w, void *stream);
+/* min/max vertex id of each searchlight. */
+int hak_sl_minmax(const int32_t *sl_idx, const int32_t *sl_off, int32_t count,
+                  int32_t *slmin, int32_t *slmax, void *stream);
+/* CSC pattern: counts per column, then sorted row ids. Columns are the
+ * vertices; the rows of column c are the union of the searchlights that
+ * contain c. */
+int hak_pattern_count(const int32_t *sl_idx, const int32_t *sl_off, const int32_t *sl_of,
+                      const int64_t *occ_ptr, const int32_t *occ,
+                      const int32_t *slmin, const int32_t *slmax, int32_t nv,
+                      int32_t *colcnt, void *stream);
+int hak_pattern_fill(const int32_t *sl_idx, const int32_t *sl_off, const int32_t *sl_of,
+                     const int64_t *occ_ptr, const int32_t *occ,
+                     const int32_t *slmin, const int32_t *slmax, int32_t nv,
+                     const int64_t *indptr, int32_t *indices, void *stream);
+
+/* Deterministic scatter: for every column c (a Y-vertex) and, in increasing
+ * searchlight order, every searchlight s in [s_begin, s_end) that holds c at
+ * local position j: data[pos(slx_s[i], c)] += (double)Q_s[i,j] * scale_s * w_s[i].
+ * Q_s is k_s x k_s at q + qoff[s], column-major with leading dim qld[s].
+ * scale may be NULL (1.0), w may be NULL (1.0). Pairs outside the pattern are
+ * skipped. occ lists come from the Y-side searchlight lists. */
+int hak_scatter(const int32_t *slx_idx, const int32_t *sl_off, const int32_t *sl_of,
+                const int64_t *occ_ptr, const int32_t *occ, int32_t ncols,
+                const int64_t *indptr, const int32_t *indices, double *data,
+                const float *q, const int64_t *qoff, const int32_t *qld,
+                const double *scale, const double *w,
+                int32_t s_begin, int32_t s_end, void *stream);
+
+/* Single local transform scatter (ha_sparse_scatter_add on the GPU). */
+int hak_scatter_one(const int32_t *indptr, const int32_t *indices, double *data,
+                    const int32_t *rows, const int32_t *cols, int32_t nr, int32_t nc,
+                    const double *T, const double *w, void *stream);
+
+/* ---- data layout (k_gemm.hip) ---- */
+/* FP64 nt x nv (col- or row-major) -> FP32 col-major with leading dim ld
+ * (rows nt..ld-1 zero) and an FP64 col-major copy with leading dim nt. */
+int hak_convert_data(const double *src, int32_t nt, int32_t nv, int col_major,
+                     float *dst32, int32_t ld, double *dst64, void *stream);
+
+/* ---- local transforms (k_local.hip) ---- */
+#define HAK_NFEAT 12
+typedef struct {
+	/* inputs */
+	const float *X;  const float *Y;     /* FP32 col-major, leading dim ld (rows >= nt are zero) */
+	const double *X64; const double *Y64;/* FP64 col-major, leading dim nt */
+	int32_t nt, ld;
+	const int32_t *slx_idx; const int32_t *sly_idx; const int32_t *sl_off;
+	const int32_t *order;                /* processing order (NULL = 0..n_order-1) */
+	int32_t n_order;
+	/* outputs */
+	float *q; const int64_t *qoff; const int32_t *qld;  /* Q_s: qld[s] x qld[s], col-major */
+	double *q64;                         /* optional FP64 copy of Q (FP64 path only) */
+	double *sigma_sum;                   /* per searchlight trace(Q^T A) (may be NULL) */
+	int32_t *flag;                       /* per searchlight: 0 ok, 1 needs the FP64 path */
+	int32_t *steps;                      /* per searchlight polar steps / Jacobi sweeps (may be NULL) */
+	/* work: per slot 5 buffers of slot_stride floats */
+	float *work; int64_t slot_stride; int32_t n_slots;
+	int32_t *queue;                      /* atomic work counter, zeroed before the launch */
+	/* polar schedule: sched[n_buckets][max_sched][3]; bucket b assumes
+	 * sigma_min / scale >= 10^(-b / bucket_per_decade) */
+	const float *sched;
+	int32_t n_buckets, max_sched, max_steps, default_bucket;
+	float bucket_per_decade;
+	float tol;                           /* finish when ||X^T X - I||_F < tol */
+	const float *sl_scale;               /* per searchlight scale (NULL = ||A||_F) */
+	const int32_t *sl_bucket;            /* per searchlight bucket (NULL = default_bucket) */
+	/* learned scaling (NULL mlp = off) */
+	const float *mlp; const float *mlp_norm;
+	float mlp_margin_scale, mlp_margin_ell;
+	float *pred_out;                     /* per searchlight 2 predictions (may be NULL) */
+	float *features;                     /* per searchlight HAK_NFEAT features (may be NULL) */
+	int32_t tile;
+	int32_t fallback_enabled;
+	int32_t rank_check_nt;               /* flag k >= nt as rank deficient */
+} HakLocalArgs;
+
+int hak_polar(const HakLocalArgs *a, void *stream);
+int hak_jacobi32(const HakLocalArgs *a, void *stream);
+/* FP64 cross-covariance + one-sided Jacobi + Q = U V^T for every searchlight
+ * s < count with flag[s] != 0 (all of them when a->flag is NULL); det-sign
+ * correction when want_det_fix. work64: n_slots * 4 * stride64 doubles. */
+int hak_jacobi64(const HakLocalArgs *a, int want_det_fix, double *work64, int64_t stride64,
+                 int32_t n_slots, int32_t count, void *stream);
+
+/* Per-searchlight sum of population variances of X columns (FP64), for scaling. */
+int hak_var_sum(const double *X64, int32_t nt, const int32_t *slx_idx, const int32_t *sl_off,
+                int32_t count, double *var_sum, void *stream);
+/* scale[s] = sigma_sum[s] / (var_sum[s] * nt) (ha_searchlight.c:635-656). */
+int hak_scale(const double *sigma_sum, const double *var_sum, int32_t nt, int32_t count,
+              double *scale, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HAK_H */

@@ -1,0 +1,940 @@
+// k_local.hip -- local transforms of searchlight Procrustes on gfx950.
+//
+// For every searchlight s with vertex lists slx, sly (k entries each):
+//   A_s = X[:, slx]^T Y[:, sly]             (k x k, inner dimension nt)
+//   Q_s = polar factor of A_s = U V^T       (A = U S V^T)
+// Reference: csrc/ha_searchlight.c:578-659 (A = X^T Y, dgesdd, T = U Vt),
+// csrc/ha_procrustes.c:17-127.
+//
+// Three solvers share the persistent one-workgroup-per-searchlight layout:
+//  * hak_polar   : bf16x3 MFMA cross-covariance, then the odd-polynomial polar
+//                  iteration X <- X (a I + b G + c G^2), G = X^T X, every GEMM in
+//                  bf16x3 (hi*hi + lo*hi + hi*lo, FP32 accumulation) on
+//                  v_mfma_f32_32x32x8_bf16, finishing with one Newton-Schulz
+//                  step X (3I - G)/2 once ||G - I||_F < tol.
+//  * hak_jacobi32: the plain FP32 one-sided (Hestenes) Jacobi SVD, Q = U V^T.
+//  * hak_jacobi64: FP64 one-sided Jacobi from an FP64 cross-covariance, used as
+//                  fallback for rank-deficient / ill-conditioned / unconverged
+//                  searchlights, and for the det-sign (reflection) correction.
+//
+// Matrices live in per-workgroup slot buffers in HBM, column-major with leading
+// dimension kp = roundup(k, 32); rows/cols k..kp-1 are kept exactly zero.
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <math.h>
+#include "hak.h"
+#include "k_mlp.h"
+
+#define LAUNCH_RET() return (int)hipGetLastError()
+
+typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+#define NT 256   // threads per workgroup (4 waves)
+#define KC 32    // K chunk staged in LDS
+
+// ---------------------------------------------------------------------------
+// bf16 split: x = hi + lo + O(2^-17 |x|), hi and lo rounded to nearest even
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint16_t f2bf(float x) {
+  uint32_t u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
+
+struct Split4 { uint2 hi, lo; };
+__device__ __forceinline__ Split4 split4(f32x4_t v) {
+  uint16_t h0 = f2bf(v[0]), h1 = f2bf(v[1]), h2 = f2bf(v[2]), h3 = f2bf(v[3]);
+  uint16_t l0 = f2bf(v[0] - bf2f(h0)), l1 = f2bf(v[1] - bf2f(h1));
+  uint16_t l2 = f2bf(v[2] - bf2f(h2)), l3 = f2bf(v[3] - bf2f(h3));
+  Split4 r;
+  r.hi = make_uint2((uint32_t)h0 | ((uint32_t)h1 << 16), (uint32_t)h2 | ((uint32_t)h3 << 16));
+  r.lo = make_uint2((uint32_t)l0 | ((uint32_t)l1 << 16), (uint32_t)l2 | ((uint32_t)l3 << 16));
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// Workgroup GEMM tile engine
+// ---------------------------------------------------------------------------
+// Operand element (m, k):
+//   k-contiguous : base[col(m) * ld + k], col(m) = cols ? cols[m] : m
+//   m-contiguous : base[m + k * ld]
+// m >= mlimit reads as zero. K is a multiple of KC and every K index < K is
+// in bounds (padding rows are stored zeros).
+struct Opnd {
+  const float *base;
+  const int32_t *cols;
+  int32_t ld;
+  int32_t mlimit;
+  int32_t mcontig;
+};
+
+template <int TILE>
+struct TileSmem {
+  uint2 ah[KC / 4][TILE], al[KC / 4][TILE];
+  uint2 bh[KC / 4][TILE], bl[KC / 4][TILE];
+};
+
+template <int TILE>
+__device__ __forceinline__ void load_operand(uint2 (*sh)[TILE], uint2 (*sl)[TILE], const Opnd &o,
+                                             int m0, int kb) {
+  const int tid = threadIdx.x;
+  if (!o.mcontig) {
+    // TILE columns x (KC/4) float4 each
+    constexpr int NV = TILE * (KC / 4);
+#pragma unroll
+    for (int it = 0; it < NV / NT; ++it) {
+      int e = tid + it * NT;
+      int m = e / (KC / 4), kg = e % (KC / 4);
+      int gm = m0 + m;
+      f32x4_t v = {0.f, 0.f, 0.f, 0.f};
+      if (gm < o.mlimit) {
+        int64_t col = o.cols ? (int64_t)o.cols[gm] : (int64_t)gm;
+        v = *(const f32x4_t *)(o.base + col * o.ld + kb + kg * 4);
+      }
+      Split4 s = split4(v);
+      sh[kg][m] = s.hi;
+      sl[kg][m] = s.lo;
+    }
+  } else {
+    // KC rows of TILE contiguous floats
+    constexpr int NV = KC * (TILE / 4);
+#pragma unroll
+    for (int it = 0; it < NV / NT; ++it) {
+      int e = tid + it * NT;
+      int kk = e / (TILE / 4), mg = e % (TILE / 4);
+      int gm = m0 + mg * 4;
+      f32x4_t v = {0.f, 0.f, 0.f, 0.f};
+      if (gm + 3 < o.mlimit) {
+        v = *(const f32x4_t *)(o.base + gm + (int64_t)(kb + kk) * o.ld);
+      } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if (gm + q < o.mlimit) v[q] = o.base[gm + q + (int64_t)(kb + kk) * o.ld];
+      }
+      Split4 s = split4(v);
+      uint16_t *ph = (uint16_t *)&sh[kk >> 2][mg * 4];
+      uint16_t *pl = (uint16_t *)&sl[kk >> 2][mg * 4];
+      const int kq = kk & 3;
+      ph[0 * 4 + kq] = (uint16_t)(s.hi.x & 0xFFFF);
+      ph[1 * 4 + kq] = (uint16_t)(s.hi.x >> 16);
+      ph[2 * 4 + kq] = (uint16_t)(s.hi.y & 0xFFFF);
+      ph[3 * 4 + kq] = (uint16_t)(s.hi.y >> 16);
+      pl[0 * 4 + kq] = (uint16_t)(s.lo.x & 0xFFFF);
+      pl[1 * 4 + kq] = (uint16_t)(s.lo.x >> 16);
+      pl[2 * 4 + kq] = (uint16_t)(s.lo.y & 0xFFFF);
+      pl[3 * 4 + kq] = (uint16_t)(s.lo.y >> 16);
+    }
+  }
+}
+
+// acc[bm][bn] holds C(m, n) for m = tm0 + wm0 + bm*32 + lane%32,
+// n = tn0 + wn0 + bn*32 + (r/4)*8 + (lane/32)*4 + r%4  (r = element index).
+template <int TILE>
+struct Acc {
+  static constexpr int W = TILE / 64;
+  f32x16_t v[W][W];
+};
+
+template <int TILE>
+__device__ __forceinline__ void tile_mma(TileSmem<TILE> &sm, const Opnd &A, const Opnd &B, int tm0, int tn0,
+                                         int K, Acc<TILE> &acc) {
+  constexpr int W = TILE / 64;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm0 = (wave >> 1) * (TILE / 2), wn0 = (wave & 1) * (TILE / 2);
+#pragma unroll
+  for (int i = 0; i < W; ++i)
+#pragma unroll
+    for (int j = 0; j < W; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc.v[i][j][r] = 0.f;
+  for (int kb = 0; kb < K; kb += KC) {
+    load_operand<TILE>(sm.ah, sm.al, A, tm0, kb);
+    load_operand<TILE>(sm.bh, sm.bl, B, tn0, kb);
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < KC / 8; ++ks) {
+      const int g = ks * 2 + (lane >> 5);
+      bf16x4_t ahf[W], alf[W], bhf[W], blf[W];
+#pragma unroll
+      for (int i = 0; i < W; ++i) {
+        uint2 h = sm.ah[g][wm0 + i * 32 + (lane & 31)];
+        uint2 l = sm.al[g][wm0 + i * 32 + (lane & 31)];
+        ahf[i] = *(bf16x4_t *)&h;
+        alf[i] = *(bf16x4_t *)&l;
+      }
+#pragma unroll
+      for (int j = 0; j < W; ++j) {
+        uint2 h = sm.bh[g][wn0 + j * 32 + (lane & 31)];
+        uint2 l = sm.bl[g][wn0 + j * 32 + (lane & 31)];
+        bhf[j] = *(bf16x4_t *)&h;
+        blf[j] = *(bf16x4_t *)&l;
+      }
+#pragma unroll
+      for (int i = 0; i < W; ++i)
+#pragma unroll
+        for (int j = 0; j < W; ++j) {
+          acc.v[i][j] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(bhf[j], ahf[i], acc.v[i][j], 0, 0, 0);
+          acc.v[i][j] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(blf[j], ahf[i], acc.v[i][j], 0, 0, 0);
+          acc.v[i][j] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(bhf[j], alf[i], acc.v[i][j], 0, 0, 0);
+        }
+    }
+    __syncthreads();
+  }
+}
+
+// Visits every accumulator element with its (m, n) coordinates.
+template <int TILE, class F>
+__device__ __forceinline__ void acc_foreach(const Acc<TILE> &acc, int tm0, int tn0, F f) {
+  constexpr int W = TILE / 64;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm0 = (wave >> 1) * (TILE / 2), wn0 = (wave & 1) * (TILE / 2);
+#pragma unroll
+  for (int i = 0; i < W; ++i)
+#pragma unroll
+    for (int j = 0; j < W; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        int m = tm0 + wm0 + i * 32 + (lane & 31);
+        int n = tn0 + wn0 + j * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
+        f(m, n, acc.v[i][j][r]);
+      }
+}
+
+__device__ __forceinline__ double wg_sum_f64(double v, double *red) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  double r = red[0] + red[1] + red[2] + red[3];
+  __syncthreads();
+  return r;
+}
+
+// Full-size GEMM over kp x kp output tiles. sym: output is symmetric, only
+// tiles with I <= J are computed and the transposed tile is mirrored.
+template <int TILE, class Epi>
+__device__ void wg_gemm(TileSmem<TILE> &sm, const Opnd &A, const Opnd &B, int kp, int K, bool sym, Epi epi) {
+  const int nt = (kp + TILE - 1) / TILE;
+  for (int I = 0; I < nt; ++I)
+    for (int J = sym ? I : 0; J < nt; ++J) {
+      Acc<TILE> acc;
+      tile_mma<TILE>(sm, A, B, I * TILE, J * TILE, K, acc);
+      const bool mirror = sym && (I != J);
+      acc_foreach<TILE>(acc, I * TILE, J * TILE, [&](int m, int n, float v) {
+        if (m < kp && n < kp) epi(m, n, v, mirror);
+      });
+    }
+}
+
+// ---------------------------------------------------------------------------
+// polar iteration
+// ---------------------------------------------------------------------------
+
+struct WorkSlots {
+  float *A, *X, *X2, *G, *P;
+};
+
+__device__ __forceinline__ WorkSlots slot_buffers(const HakLocalArgs &a, int slot) {
+  WorkSlots w;
+  float *b = a.work + (int64_t)slot * 5 * a.slot_stride;
+  w.A = b;
+  w.X = b + a.slot_stride;
+  w.X2 = b + 2 * a.slot_stride;
+  w.G = b + 3 * a.slot_stride;
+  w.P = b + 4 * a.slot_stride;
+  return w;
+}
+
+__device__ __forceinline__ int next_item(int32_t *queue, int32_t *sh) {
+  __syncthreads();
+  if (threadIdx.x == 0) *sh = atomicAdd(queue, 1);
+  __syncthreads();
+  return *sh;
+}
+
+// cross-covariance A = X[:, slx]^T Y[:, sly] into buffer Ab (kp x kp)
+template <int TILE>
+__device__ void crosscov(TileSmem<TILE> &sm, const HakLocalArgs &a, int s, int k, int kp, float *Ab) {
+  const int o = a.sl_off[s];
+  Opnd oa = {a.X, a.slx_idx + o, a.ld, k, 0};
+  Opnd ob = {a.Y, a.sly_idx + o, a.ld, k, 0};
+  wg_gemm<TILE>(sm, oa, ob, kp, a.ld, false, [&](int m, int n, float v, bool) {
+    Ab[m + (int64_t)n * kp] = v;
+  });
+}
+
+// Learned-scaling features of A (12 per searchlight), written when a.features != NULL.
+__device__ void features_of(const HakLocalArgs &a, int s, int k, int kp, const float *Ab, float *Pv,
+                            float *feat_sh, double *red) {
+  // Frobenius, trace, diagonal statistics
+  double fro2 = 0.0;
+  for (int64_t e = threadIdx.x; e < (int64_t)kp * kp; e += NT) fro2 += (double)Ab[e] * Ab[e];
+  fro2 = wg_sum_f64(fro2, red);
+  double tr = 0.0, dabs = 0.0, d2 = 0.0;
+  float dmin = 3.0e38f, dmax = 0.f;
+  for (int i = threadIdx.x; i < k; i += NT) {
+    float d = Ab[i + (int64_t)i * kp];
+    tr += d;
+    dabs += fabsf(d);
+    d2 += (double)d * d;
+  }
+  tr = wg_sum_f64(tr, red);
+  dabs = wg_sum_f64(dabs, red);
+  d2 = wg_sum_f64(d2, red);
+  // min / max |diag| via the same reduction scratch (sequential over waves)
+  for (int i = threadIdx.x; i < k; i += NT) {
+    float d = fabsf(Ab[i + (int64_t)i * kp]);
+    dmin = fminf(dmin, d);
+    dmax = fmaxf(dmax, d);
+  }
+  for (int o2 = 32; o2 > 0; o2 >>= 1) {
+    dmin = fminf(dmin, __shfl_xor(dmin, o2));
+    dmax = fmaxf(dmax, __shfl_xor(dmax, o2));
+  }
+  __shared__ float mm_sh[8];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) { mm_sh[wave] = dmin; mm_sh[4 + wave] = dmax; }
+  __syncthreads();
+  dmin = fminf(fminf(mm_sh[0], mm_sh[1]), fminf(mm_sh[2], mm_sh[3]));
+  dmax = fmaxf(fmaxf(mm_sh[4], mm_sh[5]), fmaxf(mm_sh[6], mm_sh[7]));
+  __syncthreads();
+  // power iteration on A^T A with v0 = ones: estimates after 2, 4, 8 steps
+  float *v = Pv, *u = Pv + kp;
+  for (int i = threadIdx.x; i < kp; i += NT) v[i] = (i < k) ? 1.0f : 0.f;
+  __syncthreads();
+  double est[3] = {0, 0, 0};
+  int ei = 0;
+  for (int it = 1; it <= 8; ++it) {
+    // u = A v
+    for (int i = threadIdx.x; i < k; i += NT) {
+      float acc = 0.f;
+      for (int j = 0; j < k; ++j) acc += Ab[i + (int64_t)j * kp] * v[j];
+      u[i] = acc;
+    }
+    __syncthreads();
+    // v = A^T u
+    double nv = 0.0;
+    for (int j = threadIdx.x; j < k; j += NT) {
+      float acc = 0.f;
+      const float *col = Ab + (int64_t)j * kp;
+      for (int i = 0; i < k; ++i) acc += col[i] * u[i];
+      v[j] = acc;
+      nv += (double)acc * acc;
+    }
+    nv = wg_sum_f64(nv, red);
+    double nu = 0.0;
+    for (int i = threadIdx.x; i < k; i += NT) nu += (double)u[i] * u[i];
+    nu = wg_sum_f64(nu, red);
+    // Rayleigh estimate of sigma_max^2 = |A^T u|^2 / |u|^2 ... normalised: |A^T A v| / |A v|
+    double sig = (nu > 0) ? sqrt(nv / nu) : 0.0;
+    if (it == 2 || it == 4 || it == 8) est[ei++] = sig;
+    double inv = (nv > 0) ? 1.0 / sqrt(nv) : 0.0;
+    for (int j = threadIdx.x; j < k; j += NT) v[j] = (float)(v[j] * inv);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double fro = sqrt(fro2) + 1e-30;
+    double sk = fro / sqrt((double)k);
+    feat_sh[0] = (float)log((double)k);
+    feat_sh[1] = (float)log(fro);
+    feat_sh[2] = (float)(tr / fro);
+    feat_sh[3] = (float)(dabs / k / sk);
+    feat_sh[4] = (float)(dmin / sk);
+    feat_sh[5] = (float)(dmax / sk);
+    double dm = dabs / k;
+    double dv = d2 / k - dm * dm;
+    feat_sh[6] = (float)(sqrt(dv > 0 ? dv : 0) / sk);
+    feat_sh[7] = (float)log(est[0] / fro + 1e-30);
+    feat_sh[8] = (float)log(est[1] / fro + 1e-30);
+    feat_sh[9] = (float)log(est[2] / fro + 1e-30);
+    feat_sh[10] = (float)log((double)a.nt);
+    feat_sh[11] = (float)(d2 / (fro2 + 1e-30));
+  }
+  __syncthreads();
+}
+
+template <int TILE>
+__global__ __launch_bounds__(NT) void k_polar(HakLocalArgs a) {
+  __shared__ TileSmem<TILE> sm;
+  __shared__ double red[4];
+  __shared__ int32_t item_sh;
+  __shared__ float feat_sh[HAK_NFEAT];
+  __shared__ float mlp_sh[2 * 64 + 2];
+  const int slot = blockIdx.x;
+  WorkSlots w = slot_buffers(a, slot);
+  for (;;) {
+    const int it = next_item(a.queue, &item_sh);
+    if (it >= a.n_order) break;
+    const int s = a.order ? a.order[it] : it;
+    const int k = a.sl_off[s + 1] - a.sl_off[s];
+    const int kp = a.qld[s];
+    float *Q = a.q + a.qoff[s];
+    if (a.rank_check_nt && k >= a.nt) {
+      if (threadIdx.x == 0) {
+        a.flag[s] = 1;
+        if (a.steps) a.steps[s] = 0;
+      }
+      continue;
+    }
+    crosscov<TILE>(sm, a, s, k, kp, w.A);
+    __syncthreads();
+    __threadfence_block();
+    // scale and schedule bucket
+    double fro2 = 0.0;
+    for (int64_t e = threadIdx.x; e < (int64_t)kp * kp; e += NT) fro2 += (double)w.A[e] * w.A[e];
+    fro2 = wg_sum_f64(fro2, red);
+    float scale = (float)sqrt(fro2);
+    int bucket = a.sl_bucket ? a.sl_bucket[s] : a.default_bucket;
+    if (a.features || a.mlp) {
+      features_of(a, s, k, kp, w.A, w.P, feat_sh, red);
+      if (a.features && threadIdx.x < HAK_NFEAT) a.features[(int64_t)s * HAK_NFEAT + threadIdx.x] = feat_sh[threadIdx.x];
+      if (a.mlp) {
+        float pred[2];
+        hak_mlp_forward_wg(a.mlp, a.mlp_norm, feat_sh, mlp_sh, pred);
+        // pred[0] = log(sigma_max / ||A||_F), pred[1] = log10(sigma_min / sigma_max)
+        // scale = predicted sigma_max inflated by the margin (never above ||A||_F);
+        // ell = predicted sigma_min / scale, deflated by its margin
+        float lf = fminf(pred[0] + a.mlp_margin_scale, 0.f);
+        float smax = scale * expf(lf);
+        float ell = powf(10.f, pred[1] - a.mlp_margin_ell) * expf(pred[0] - lf);
+        scale = smax;
+        int b = (int)ceilf(-log10f(fmaxf(ell, 1e-30f)) * a.bucket_per_decade);
+        bucket = b < 0 ? 0 : (b >= a.n_buckets ? a.n_buckets - 1 : b);
+        if (a.pred_out && threadIdx.x == 0) {
+          a.pred_out[2 * s] = pred[0];
+          a.pred_out[2 * s + 1] = pred[1];
+        }
+      }
+    } else if (a.sl_scale) {
+      scale = a.sl_scale[s];
+    }
+    if (!(scale > 0.f) || !isfinite(scale)) {
+      if (threadIdx.x == 0) { a.flag[s] = 1; if (a.steps) a.steps[s] = 0; }
+      continue;
+    }
+    const float inv = 1.0f / scale;
+    for (int64_t e = threadIdx.x; e < (int64_t)kp * kp; e += NT) w.X[e] = w.A[e] * inv;
+    __syncthreads();
+    float *X = w.X, *Xn = w.X2;
+    int step = 0;
+    bool done = false;
+    for (step = 0; step < a.max_steps; ++step) {
+      // G = X^T X and r = ||G - I||_F over the k x k block
+      double r2 = 0.0;
+      {
+        Opnd oa = {X, nullptr, kp, kp, 0};
+        float *G = w.G;
+        wg_gemm<TILE>(sm, oa, oa, kp, kp, true, [&](int m, int n, float v, bool mirror) {
+          G[m + (int64_t)n * kp] = v;
+          if (mirror) G[n + (int64_t)m * kp] = v;
+          if (m < k && n < k) {
+            float d = v - (m == n ? 1.f : 0.f);
+            r2 += (mirror ? 2.0 : 1.0) * (double)d * d;
+          }
+        });
+      }
+      r2 = wg_sum_f64(r2, red);
+      __threadfence_block();
+      const double r = sqrt(r2);
+      if (!isfinite(r)) break;
+      const bool finish = (r < a.tol);
+      float ca, cb, cc;
+      if (finish) {
+        ca = 1.5f; cb = -0.5f; cc = 0.f;
+      } else {
+        if (step < a.max_sched) {
+          const float *co = a.sched + ((int64_t)bucket * a.max_sched + step) * 3;
+          ca = co[0]; cb = co[1]; cc = co[2];
+        } else {
+          ca = 1.875f; cb = -1.25f; cc = 0.375f;  // quintic Newton-Schulz
+        }
+      }
+      // P = ca I + cb G + cc G^2  (G^2 only when cc != 0)
+      float *P = w.P;
+      const float *G = w.G;
+      if (cc != 0.f) {
+        Opnd og = {G, nullptr, kp, kp, 0};
+        wg_gemm<TILE>(sm, og, og, kp, kp, true, [&](int m, int n, float v, bool mirror) {
+          float pv = 0.f;
+          if (m < k && n < k) pv = (m == n ? ca : 0.f) + cb * G[m + (int64_t)n * kp] + cc * v;
+          P[m + (int64_t)n * kp] = pv;
+          if (mirror) P[n + (int64_t)m * kp] = pv;
+        });
+      } else {
+        for (int64_t e = threadIdx.x; e < (int64_t)kp * kp; e += NT) {
+          int m = (int)(e % kp), n = (int)(e / kp);
+          float pv = 0.f;
+          if (m < k && n < k) pv = (m == n ? ca : 0.f) + cb * G[e];
+          P[e] = pv;
+        }
+      }
+      __syncthreads();
+      __threadfence_block();
+      // Xn = X P (the final product goes straight to the Q store)
+      float *dst = finish ? Q : Xn;
+      {
+        Opnd ox = {X, nullptr, kp, kp, 1};
+        Opnd op = {P, nullptr, kp, kp, 0};
+        wg_gemm<TILE>(sm, ox, op, kp, kp, false, [&](int m, int n, float v, bool) {
+          dst[m + (int64_t)n * kp] = v;
+        });
+      }
+      __syncthreads();
+      __threadfence_block();
+      if (finish) { done = true; ++step; break; }
+      float *t = X; X = Xn; Xn = t;
+    }
+    if (!done) {
+      // budget exhausted or non-finite: FP64 Jacobi takes over (Q written later)
+      if (threadIdx.x == 0) a.flag[s] = 1;
+    } else if (a.sigma_sum) {
+      double tr = 0.0;
+      for (int64_t e = threadIdx.x; e < (int64_t)kp * kp; e += NT) tr += (double)Q[e] * w.A[e];
+      tr = wg_sum_f64(tr, red);
+      if (threadIdx.x == 0) a.sigma_sum[s] = tr;
+    }
+    if (threadIdx.x == 0 && a.steps) a.steps[s] = done ? step : -step;
+    __syncthreads();
+  }
+}
+
+extern "C" int hak_polar(const HakLocalArgs *a, void *stream) {
+  if (a->n_order <= 0) return 0;
+  if (a->tile == 64)
+    hipLaunchKernelGGL(k_polar<64>, dim3(a->n_slots), dim3(NT), 0, (hipStream_t)stream, *a);
+  else
+    hipLaunchKernelGGL(k_polar<128>, dim3(a->n_slots), dim3(NT), 0, (hipStream_t)stream, *a);
+  LAUNCH_RET();
+}
+
+// ---------------------------------------------------------------------------
+// One-sided Jacobi (Hestenes), FP32 and FP64
+// ---------------------------------------------------------------------------
+// Pairs follow the round-robin tournament on n = k (+1 if odd) players: in
+// round r, pair i is (p(i), p(n-1-i)) with p(0) = 0 and
+// p(j) = ((j - 1 + r) mod (n - 1)) + 1. Columns >= k are virtual (skipped).
+// 16 lanes cooperate on one pair; 16 pairs run concurrently per workgroup.
+
+__device__ __forceinline__ int rr_player(int j, int r, int n) {
+  return j == 0 ? 0 : ((j - 1 + r) % (n - 1)) + 1;
+}
+
+template <typename T>
+__device__ __forceinline__ T grp16_sum(T v) {
+  for (int o = 8; o > 0; o >>= 1) v += __shfl_xor(v, o, 16);
+  return v;
+}
+
+template <typename T>
+struct JacobiTol;
+template <>
+struct JacobiTol<float> { static constexpr float tol = 1.2e-7f; static constexpr int sweeps = 30; };
+template <>
+struct JacobiTol<double> { static constexpr double tol = 2.3e-16; static constexpr int sweeps = 40; };
+
+// Orthogonalises the k columns of Am (col-major, ld kp) while accumulating the
+// rotations in Vm (starts as identity). Returns the number of sweeps used
+// (negative if not converged).
+template <typename T>
+__device__ int jacobi_sweeps(T *Am, T *Vm, int k, int kp, int32_t *flag_sh) {
+  const int n = k + (k & 1);
+  const int grp = threadIdx.x >> 4, gl = threadIdx.x & 15;
+  int sweep;
+  for (sweep = 0; sweep < JacobiTol<T>::sweeps; ++sweep) {
+    if (threadIdx.x == 0) *flag_sh = 0;
+    __syncthreads();
+    for (int r = 0; r < n - 1; ++r) {
+      for (int i0 = 0; i0 < n / 2; i0 += NT / 16) {
+        int i = i0 + grp;
+        if (i < n / 2) {
+          int p = rr_player(i, r, n), q = rr_player(n - 1 - i, r, n);
+          if (p > q) { int t = p; p = q; q = t; }
+          if (q < k) {
+            T *ap = Am + (int64_t)p * kp, *aq = Am + (int64_t)q * kp;
+            T al = 0, be = 0, ga = 0;
+            for (int t = gl; t < k; t += 16) {
+              T x = ap[t], y = aq[t];
+              al += x * x; be += y * y; ga += x * y;
+            }
+            al = grp16_sum(al); be = grp16_sum(be); ga = grp16_sum(ga);
+            if (fabs((double)ga) > (double)JacobiTol<T>::tol * sqrt((double)al * (double)be) && ga != (T)0) {
+              T zeta = (be - al) / (2 * ga);
+              T tt = (zeta >= 0 ? (T)1 : (T)-1) / (fabs(zeta) + sqrt((T)1 + zeta * zeta));
+              T c = (T)1 / sqrt((T)1 + tt * tt), sn = c * tt;
+              for (int t = gl; t < k; t += 16) {
+                T x = ap[t], y = aq[t];
+                ap[t] = c * x - sn * y;
+                aq[t] = sn * x + c * y;
+              }
+              T *vp = Vm + (int64_t)p * kp, *vq = Vm + (int64_t)q * kp;
+              for (int t = gl; t < k; t += 16) {
+                T x = vp[t], y = vq[t];
+                vp[t] = c * x - sn * y;
+                vq[t] = sn * x + c * y;
+              }
+              if (gl == 0) *flag_sh = 1;
+            }
+          }
+        }
+      }
+      __syncthreads();
+    }
+    __syncthreads();
+    const int rotated = *flag_sh;
+    __syncthreads();
+    if (!rotated) return sweep + 1;
+  }
+  return -sweep;
+}
+
+// Q = sum_j (a_j / |a_j|) v_j^T over columns with |a_j| > cut; returns sum |a_j|.
+// Columns with |a_j| <= cut are reported through *rank_def.
+template <typename T, typename TO>
+__device__ double jacobi_compose(T *Am, const T *Vm, int k, int kp, TO *Q, int qld, T *sig, int32_t *rank_sh,
+                                 double *red) {
+  // singular values and normalised U columns
+  if (threadIdx.x == 0) *rank_sh = 0;
+  __syncthreads();
+  T smax = 0;
+  for (int j = threadIdx.x; j < k; j += NT) {
+    T s2 = 0;
+    const T *col = Am + (int64_t)j * kp;
+    for (int t = 0; t < k; ++t) s2 += col[t] * col[t];
+    sig[j] = sqrt(s2);
+  }
+  __syncthreads();
+  for (int j = 0; j < k; ++j) smax = sig[j] > smax ? sig[j] : smax;
+  const T cut = smax * (T)k * (sizeof(T) == 8 ? (T)2.3e-16 : (T)1.2e-7);
+  for (int j = threadIdx.x; j < k; j += NT) {
+    if (sig[j] <= cut) atomicExch(rank_sh, 1);
+  }
+  __syncthreads();
+  for (int64_t e = threadIdx.x; e < (int64_t)k * kp; e += NT) {
+    int j = (int)(e / kp);
+    T sj = sig[j];
+    Am[e] = (sj > cut) ? Am[e] / sj : (T)0;
+  }
+  __syncthreads();
+  // Q(m, n) = sum_j U(m, j) V(n, j)
+  for (int64_t e = threadIdx.x; e < (int64_t)qld * qld; e += NT) {
+    int m = (int)(e % qld), n = (int)(e / qld);
+    T acc = 0;
+    if (m < k && n < k)
+      for (int j = 0; j < k; ++j) acc += Am[m + (int64_t)j * kp] * Vm[n + (int64_t)j * kp];
+    Q[e] = (TO)acc;
+  }
+  __syncthreads();
+  double ssum = 0.0;
+  for (int j = threadIdx.x; j < k; j += NT) ssum += (double)sig[j];
+  return wg_sum_f64(ssum, red);
+}
+
+template <int TILE>
+__global__ __launch_bounds__(NT) void k_jacobi32(HakLocalArgs a) {
+  __shared__ TileSmem<TILE> sm;
+  __shared__ double red[4];
+  __shared__ int32_t item_sh, flag_sh, rank_sh;
+  const int slot = blockIdx.x;
+  WorkSlots w = slot_buffers(a, slot);
+  for (;;) {
+    const int it = next_item(a.queue, &item_sh);
+    if (it >= a.n_order) break;
+    const int s = a.order ? a.order[it] : it;
+    const int k = a.sl_off[s + 1] - a.sl_off[s];
+    const int kp = a.qld[s];
+    if (a.rank_check_nt && k >= a.nt) {
+      if (threadIdx.x == 0) { a.flag[s] = 1; if (a.steps) a.steps[s] = 0; }
+      continue;
+    }
+    // cross-covariance in bf16x3 MFMA (same kernel as the polar path), then Jacobi in FP32
+    crosscov<TILE>(sm, a, s, k, kp, w.A);
+    __syncthreads();
+    if (a.sigma_sum)  // keep A for trace(Q^T A)
+      for (int64_t e = threadIdx.x; e < (int64_t)kp * kp; e += NT) w.X2[e] = w.A[e];
+    for (int64_t e = threadIdx.x; e < (int64_t)kp * kp; e += NT) {
+      int m = (int)(e % kp), n = (int)(e / kp);
+      w.G[e] = (m == n && m < k) ? 1.f : 0.f;
+    }
+    __syncthreads();
+    int sweeps = jacobi_sweeps<float>(w.A, w.G, k, kp, &flag_sh);
+    float *Q = a.q + a.qoff[s];
+    jacobi_compose<float, float>(w.A, w.G, k, kp, Q, kp, w.P, &rank_sh, red);
+    if (a.sigma_sum) {
+      double tr = 0.0;
+      for (int64_t e = threadIdx.x; e < (int64_t)kp * kp; e += NT) tr += (double)Q[e] * w.X2[e];
+      tr = wg_sum_f64(tr, red);
+      if (threadIdx.x == 0) a.sigma_sum[s] = tr;
+    }
+    if (threadIdx.x == 0) {
+      if (a.steps) a.steps[s] = sweeps;
+      if ((rank_sh || sweeps < 0) && a.fallback_enabled) a.flag[s] = 1;
+    }
+    __syncthreads();
+  }
+}
+
+extern "C" int hak_jacobi32(const HakLocalArgs *a, void *stream) {
+  if (a->n_order <= 0) return 0;
+  if (a->tile == 64)
+    hipLaunchKernelGGL(k_jacobi32<64>, dim3(a->n_slots), dim3(NT), 0, (hipStream_t)stream, *a);
+  else
+    hipLaunchKernelGGL(k_jacobi32<128>, dim3(a->n_slots), dim3(NT), 0, (hipStream_t)stream, *a);
+  LAUNCH_RET();
+}
+
+// FP64 path: one workgroup per flagged searchlight (all of them when
+// a.flag == NULL). Work per slot: A, V, LU scratch (3 * slot_stride64 doubles)
+// and sig (kp doubles, inside the LU scratch tail).
+__global__ __launch_bounds__(NT) void k_jacobi64(HakLocalArgs a, int want_det_fix, double *work64,
+                                                 int64_t stride64, int32_t count) {
+  __shared__ double red[4];
+  __shared__ int32_t item_sh, flag_sh, rank_sh, sign_sh, piv_sh;
+  __shared__ double pmax_sh[4];
+  __shared__ int32_t pidx_sh[4];
+  double *Am = work64 + (int64_t)blockIdx.x * 4 * stride64;
+  double *Vm = Am + stride64;
+  double *Lm = Am + 2 * stride64;
+  double *sig = Am + 3 * stride64;
+  for (;;) {
+    const int s = next_item(a.queue, &item_sh);
+    if (s >= count) break;
+    if (a.flag && a.flag[s] == 0) continue;
+    const int o = a.sl_off[s];
+    const int k = a.sl_off[s + 1] - o;
+    const int kp = a.qld[s];
+    // A = X64[:, slx]^T Y64[:, sly] in FP64
+    for (int64_t e = threadIdx.x; e < (int64_t)kp * kp; e += NT) {
+      int m = (int)(e % kp), n = (int)(e / kp);
+      double acc = 0.0;
+      if (m < k && n < k) {
+        const double *xc = a.X64 + (int64_t)a.slx_idx[o + m] * a.nt;
+        const double *yc = a.Y64 + (int64_t)a.sly_idx[o + n] * a.nt;
+        for (int t = 0; t < a.nt; ++t) acc += xc[t] * yc[t];
+      }
+      Am[e] = acc;
+      Vm[e] = (m == n && m < k) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    if (want_det_fix) {
+      // sign(det Q) = sign(det A) for nonsingular A: LU with partial pivoting on a copy
+      for (int64_t e = threadIdx.x; e < (int64_t)kp * kp; e += NT) Lm[e] = Am[e];
+      if (threadIdx.x == 0) sign_sh = 1;
+      __syncthreads();
+      for (int c = 0; c < k; ++c) {
+        // pivot search in column c, rows c..k-1
+        double best = -1.0;
+        int bi = c;
+        for (int r = c + threadIdx.x; r < k; r += NT) {
+          double v = fabs(Lm[r + (int64_t)c * kp]);
+          if (v > best) { best = v; bi = r; }
+        }
+        for (int o2 = 32; o2 > 0; o2 >>= 1) {
+          double ob = __shfl_xor(best, o2);
+          int oi = __shfl_xor(bi, o2);
+          if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+        }
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        if (lane == 0) { pmax_sh[wave] = best; pidx_sh[wave] = bi; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+          double b0 = pmax_sh[0];
+          int i0 = pidx_sh[0];
+          for (int q = 1; q < 4; ++q)
+            if (pmax_sh[q] > b0 || (pmax_sh[q] == b0 && pidx_sh[q] < i0)) { b0 = pmax_sh[q]; i0 = pidx_sh[q]; }
+          piv_sh = i0;
+          if (i0 != c) sign_sh = -sign_sh;
+        }
+        __syncthreads();
+        const int pr = piv_sh;
+        if (pr != c)
+          for (int j = threadIdx.x; j < k; j += NT) {
+            double t = Lm[c + (int64_t)j * kp];
+            Lm[c + (int64_t)j * kp] = Lm[pr + (int64_t)j * kp];
+            Lm[pr + (int64_t)j * kp] = t;
+          }
+        __syncthreads();
+        const double d = Lm[c + (int64_t)c * kp];
+        if (threadIdx.x == 0 && d < 0.0) sign_sh = -sign_sh;
+        if (d != 0.0) {
+          for (int64_t e = threadIdx.x; e < (int64_t)(k - c - 1) * (k - c - 1); e += NT) {
+            int r = c + 1 + (int)(e % (k - c - 1)), j = c + 1 + (int)(e / (k - c - 1));
+            Lm[r + (int64_t)j * kp] -= Lm[r + (int64_t)c * kp] / d * Lm[c + (int64_t)j * kp];
+          }
+        }
+        __syncthreads();
+      }
+    }
+    int sweeps = jacobi_sweeps<double>(Am, Vm, k, kp, &flag_sh);
+    // U columns, rank completion for zero singular values
+    if (threadIdx.x == 0) rank_sh = 0;
+    __syncthreads();
+    for (int j = threadIdx.x; j < k; j += NT) {
+      double s2 = 0;
+      const double *col = Am + (int64_t)j * kp;
+      for (int t = 0; t < k; ++t) s2 += col[t] * col[t];
+      sig[j] = sqrt(s2);
+    }
+    __syncthreads();
+    double smax = 0;
+    for (int j = 0; j < k; ++j) smax = fmax(smax, sig[j]);
+    const double cut = smax * k * 2.3e-16;
+    for (int64_t e = threadIdx.x; e < (int64_t)k * kp; e += NT) {
+      int j = (int)(e / kp);
+      Am[e] = (sig[j] > cut) ? Am[e] / sig[j] : 0.0;
+    }
+    __syncthreads();
+    // complete U for deficient columns: Gram-Schmidt of unit vectors against
+    // all current columns (sequential over deficient columns, two passes)
+    for (int j = 0; j < k; ++j) {
+      if (sig[j] > cut) continue;
+      double *uj = Am + (int64_t)j * kp;
+      for (int cand = 0; cand < k; ++cand) {
+        for (int t = threadIdx.x; t < k; t += NT) uj[t] = (t == cand) ? 1.0 : 0.0;
+        __syncthreads();
+        for (int pass = 0; pass < 2; ++pass)
+          for (int i = 0; i < k; ++i) {
+            if (i == j) continue;
+            const double *ui = Am + (int64_t)i * kp;
+            double dot = 0.0;
+            for (int t = threadIdx.x; t < k; t += NT) dot += ui[t] * uj[t];
+            dot = wg_sum_f64(dot, red);
+            for (int t = threadIdx.x; t < k; t += NT) uj[t] -= dot * ui[t];
+            __syncthreads();
+          }
+        double nn = 0.0;
+        for (int t = threadIdx.x; t < k; t += NT) nn += uj[t] * uj[t];
+        nn = wg_sum_f64(nn, red);
+        if (nn > 1e-6) {
+          const double is = 1.0 / sqrt(nn);
+          for (int t = threadIdx.x; t < k; t += NT) uj[t] *= is;
+          __syncthreads();
+          break;
+        }
+      }
+      if (threadIdx.x == 0) sig[j] = 0.0;
+      __syncthreads();
+    }
+    // reflection fix: if det(Q) < 0, flip the pair of the smallest singular value
+    int jmin = 0;
+    if (want_det_fix) {
+      for (int j = 1; j < k; ++j)
+        if (sig[j] <= sig[jmin]) jmin = j;
+    }
+    const bool flip = want_det_fix && sign_sh < 0;
+    float *Q = a.q + a.qoff[s];
+    double *Q64 = a.q64 ? a.q64 + a.qoff[s] : nullptr;
+    for (int64_t e = threadIdx.x; e < (int64_t)kp * kp; e += NT) {
+      int m = (int)(e % kp), n = (int)(e / kp);
+      double acc = 0.0;
+      if (m < k && n < k) {
+        for (int j = 0; j < k; ++j) acc += Am[m + (int64_t)j * kp] * Vm[n + (int64_t)j * kp];
+        if (flip) acc -= 2.0 * Am[m + (int64_t)jmin * kp] * Vm[n + (int64_t)jmin * kp];
+      }
+      Q[e] = (float)acc;
+      if (Q64) Q64[e] = acc;
+    }
+    __syncthreads();
+    if (a.sigma_sum) {
+      double ss = 0.0;
+      for (int j = threadIdx.x; j < k; j += NT) ss += sig[j];
+      ss = wg_sum_f64(ss, red);
+      if (threadIdx.x == 0) a.sigma_sum[s] = flip ? ss - 2.0 * sig[jmin] : ss;
+    }
+    if (threadIdx.x == 0) {
+      if (a.steps) a.steps[s] = 1000 + (sweeps > 0 ? sweeps : -sweeps);
+    }
+    __syncthreads();
+  }
+}
+
+extern "C" int hak_jacobi64(const HakLocalArgs *a, int want_det_fix, double *work64, int64_t stride64,
+                            int32_t n_slots, int32_t count, void *stream) {
+  if (count <= 0) return 0;
+  hipLaunchKernelGGL(k_jacobi64, dim3(n_slots), dim3(NT), 0, (hipStream_t)stream, *a, want_det_fix, work64,
+                     stride64, count);
+  LAUNCH_RET();
+}
+
+// ---------------------------------------------------------------------------
+// data layout and scaling helpers
+// ---------------------------------------------------------------------------
+
+__global__ void k_convert(const double *__restrict__ src, int32_t nt, int32_t nv, int col_major,
+                          float *__restrict__ d32, int32_t ld, double *__restrict__ d64) {
+  const int64_t n = (int64_t)ld * nv;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    int64_t v = e / ld;
+    int32_t t = (int32_t)(e % ld);
+    double x = 0.0;
+    if (t < nt) {
+      x = col_major ? src[t + v * nt] : src[(int64_t)t * nv + v];
+      if (d64) d64[t + v * nt] = x;
+    }
+    d32[e] = (float)x;
+  }
+}
+extern "C" int hak_convert_data(const double *src, int32_t nt, int32_t nv, int col_major, float *dst32,
+                                int32_t ld, double *dst64, void *stream) {
+  int64_t n = (int64_t)ld * nv;
+  int64_t g = (n + 255) / 256;
+  if (g > 65536) g = 65536;
+  hipLaunchKernelGGL(k_convert, dim3((int)g), dim3(256), 0, (hipStream_t)stream, src, nt, nv, col_major, dst32,
+                     ld, dst64);
+  LAUNCH_RET();
+}
+
+// sum over the searchlight's X columns of the population variance, in column
+// order (ha_searchlight.c:641-653)
+__global__ __launch_bounds__(NT) void k_var_sum(const double *__restrict__ X64, int32_t nt,
+                                                const int32_t *__restrict__ idx, const int32_t *__restrict__ off,
+                                                int32_t count, double *__restrict__ var_sum) {
+  __shared__ double vs[1024];
+  for (int s = blockIdx.x; s < count; s += gridDim.x) {
+    const int o = off[s], k = off[s + 1] - o;
+    double total = 0.0;
+    for (int j0 = 0; j0 < k; j0 += 1024) {
+      for (int j = j0 + threadIdx.x; j < k && j < j0 + 1024; j += NT) {
+        const double *col = X64 + (int64_t)idx[o + j] * nt;
+        double mean = 0.0;
+        for (int t = 0; t < nt; ++t) mean += col[t];
+        mean /= nt;
+        double var = 0.0;
+        for (int t = 0; t < nt; ++t) {
+          double d = col[t] - mean;
+          var += d * d;
+        }
+        vs[j - j0] = var / nt;
+      }
+      __syncthreads();
+      if (threadIdx.x == 0)
+        for (int j = j0; j < k && j < j0 + 1024; ++j) total += vs[j - j0];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) var_sum[s] = total;
+  }
+}
+extern "C" int hak_var_sum(const double *X64, int32_t nt, const int32_t *slx_idx, const int32_t *sl_off,
+                           int32_t count, double *var_sum, void *stream) {
+  if (count <= 0) return 0;
+  hipLaunchKernelGGL(k_var_sum, dim3(count < 65536 ? count : 65536), dim3(NT), 0, (hipStream_t)stream, X64, nt,
+                     slx_idx, sl_off, count, var_sum);
+  LAUNCH_RET();
+}
+
+__global__ void k_scale(const double *__restrict__ ss, const double *__restrict__ vs, int32_t nt, int32_t count,
+                        double *__restrict__ scale) {
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < count; s += gridDim.x * blockDim.x)
+    scale[s] = ss[s] / (vs[s] * nt);
+}
+extern "C" int hak_scale(const double *sigma_sum, const double *var_sum, int32_t nt, int32_t count,
+                         double *scale, void *stream) {
+  if (count <= 0) return 0;
+  hipLaunchKernelGGL(k_scale, dim3((count + 255) / 256), dim3(256), 0, (hipStream_t)stream, sigma_sum, var_sum,
+                     nt, count, scale);
+  LAUNCH_RET();
+}

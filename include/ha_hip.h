@@ -1,0 +1,162 @@
+/*
+ * ha_hip.h -- MI355X (gfx950) HIP backend for searchlight Procrustes.
+ *
+ * This is the plug-in module a reference build would include next to
+ * csrc/ha_cuda.h. Its first four entry points have the shape of
+ * ha_cuda.h:56-66 (init / cleanup / available / dense searchlight Procrustes);
+ * the rest add what the reference's CUDA plug-in lacks: a CSC variant (the
+ * reference has no GPU branch in ha_searchlight_procrustes,
+ * ha_searchlight.c:204-207), a device-resident plan for repeated passes over
+ * data already in HBM, run options, and per-pass statistics.
+ *
+ * All functions return kHaSuccess (0) or a negative error code from
+ * ha_common.h and print a one-line reason to stderr on failure. Nothing falls
+ * back to a CPU implementation: without a usable GPU every call fails with
+ * kHaErrorInternal. GPU state is process-global and not thread-safe, as in the
+ * reference (ha_cuda.cu:39-41): one call at a time per process.
+ */
+#ifndef HA_HIP_H
+#define HA_HIP_H
+
+#include "ha_common.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- plug-in lifecycle (ha_cuda.h:38-50) ---- */
+
+/* Selects device 0 and creates the library's stream. Idempotent. */
+int ha_hip_init(void);
+/* Releases the stream and cached device buffers. Safe without init. */
+void ha_hip_cleanup(void);
+/* True after a successful ha_hip_init. */
+bool ha_hip_available(void);
+
+/* ---- run options (new; the reference has no equivalent) ---- */
+
+enum {
+	kHaHipMethodAuto         = 0, /* polar (learned if a model is loaded), Jacobi-64 for k<=32 */
+	kHaHipMethodJacobi32     = 1, /* plain FP32 one-sided Jacobi SVD, Q = U V^T */
+	kHaHipMethodPolar        = 2, /* bf16x3 MFMA polar iteration, default scaling */
+	kHaHipMethodPolarLearned = 3, /* polar iteration with MLP-predicted scaling/schedule */
+	kHaHipMethodJacobi64     = 4, /* FP64 one-sided Jacobi SVD for every searchlight */
+};
+
+typedef struct {
+	int32_t method;          /* kHaHipMethod* */
+	int32_t fp64_fallback;   /* 1: rank-deficient (k >= nt), unconverged or
+	                            sigma_min/sigma_max < 1e-6 searchlights use FP64 Jacobi */
+	int32_t max_polar_steps; /* iteration budget per searchlight (default 24) */
+	double  polar_tol;       /* stop when ||X^T X - I||_F < polar_tol (default 2e-2,
+	                            followed by one Newton-Schulz finishing step) */
+	double  plain_ell;       /* sigma_min/scale assumed by the plain schedule (default 1e-7) */
+	int32_t tile;            /* GEMM tile edge 64 or 128; 0 = per-bucket autotuned */
+	int32_t use_graph;       /* 1: capture a plan's pass in a hipGraph and replay it */
+	int32_t chunk;           /* searchlights per polar chunk (0 = auto) */
+	int32_t verbose;         /* 1: timing lines on stderr */
+	float   learned_margin_scale; /* log-margin added to the predicted sigma_max (default 0.05) */
+	float   learned_margin_ell;   /* decades subtracted from the predicted sigma_min/sigma_max (default 0.5) */
+} HaHipOptions;
+
+void ha_hip_get_options(HaHipOptions *opt);
+int  ha_hip_set_options(const HaHipOptions *opt);
+
+/* ---- drop-in entry points ---- */
+
+/* Same contract as ha_searchlight_procrustes_cuda (ha_cuda.h:56-66) with two
+ * differences that follow the reference CPU path instead of the CUDA one:
+ * T_out is accumulated (+=), not overwritten (ha_cuda.cu:445,458), and
+ * isReflection=false / isScaling=true are honoured (ha_cuda.cu:364 ignores
+ * them). col_major selects the layout of X_data / Y_data (nt x nv). */
+int ha_searchlight_procrustes_hip(
+    const double *X_data, const double *Y_data,
+    int32_t nt, int32_t nv,
+    const int32_t *sl_indices,
+    const int32_t *sl_offsets,
+    const double *sl_dists,
+    int32_t count,
+    double radius,
+    double *T_out,
+    bool isReflection, bool isScaling,
+    bool col_major);
+
+/* CSC variant with the arguments of ha_searchlight_procrustes
+ * (ha_searchlight.h:46-52) minus the backend. */
+int ha_searchlight_procrustes_hip_csc(const TMat *X, const TMat *Y,
+                                      const TSearchlights *sls_X,
+                                      const TSearchlights *sls_Y,
+                                      TSparseCSC *mat,
+                                      double **weights,
+                                      bool isReflection, bool isScaling);
+
+/* ---- device-resident plan: searchlights uploaded once, passes over data in HBM ---- */
+
+typedef struct HaHipPlan HaHipPlan;
+
+typedef struct {
+	double  ms_weights;       /* last pass, HIP events on the plan's stream */
+	double  ms_crosscov;
+	double  ms_polar;          /* local transforms (polar or Jacobi) */
+	double  ms_fallback;       /* FP64 Jacobi fallback */
+	double  ms_scatter;
+	double  ms_total;
+	double  ms_pattern;        /* pattern build at plan creation */
+	int64_t n_fallback;        /* searchlights solved by FP64 Jacobi */
+	int64_t n_unconverged;     /* polar searchlights that hit the step budget */
+	int64_t steps_total;       /* sum of polar steps over searchlights */
+	int32_t steps_hist[32];    /* histogram of polar steps (last bin = overflow) */
+	int64_t nnz;
+	double  gemm_flops;        /* algorithmic MFMA flops of the last pass */
+	double  kernel_ms_polar_avg; /* average duration of one polar launch */
+	int32_t polar_launches;
+} HaHipStats;
+
+/* Uploads the flat searchlight arrays (layout of hyperalignment_c.py:161-169)
+ * and builds, on the GPU, the vertex->searchlight index and the CSC pattern.
+ * sl_dists may be NULL (uniform weights). nt is the number of samples of the
+ * data passes that will follow. */
+int ha_hip_plan_create(int32_t nt, int32_t nv,
+                       const int32_t *sl_indices, const int32_t *sl_offsets,
+                       const double *sl_dists, int32_t count, double radius,
+                       HaHipPlan **plan_out);
+void ha_hip_plan_destroy(HaHipPlan *plan);
+
+/* Host -> HBM upload of X and Y (nt x nv, FP64, col- or row-major). */
+int ha_hip_plan_load(HaHipPlan *plan, const double *X, const double *Y, bool col_major);
+
+/* One full pass on data already in HBM: weights, cross-covariance, local
+ * transforms, deterministic scatter into the plan's CSC data (zeroed first).
+ * Synchronous. */
+int ha_hip_plan_execute(HaHipPlan *plan, bool isReflection, bool isScaling);
+
+int ha_hip_plan_nnz(const HaHipPlan *plan, int64_t *nnz);
+/* Downloads the CSC pattern and values; any pointer may be NULL. */
+int ha_hip_plan_download(const HaHipPlan *plan, int32_t *indptr, int32_t *indices, double *data);
+/* Downloads local transforms of chosen searchlights (k_s x k_s row-major each,
+ * concatenated in the order given) for diagnostics and parity tests. */
+int ha_hip_plan_download_local(const HaHipPlan *plan, const int32_t *which, int32_t n_which,
+                               float *out);
+int ha_hip_plan_stats(const HaHipPlan *plan, HaHipStats *stats);
+
+/* ---- learned polar scaling ---- */
+
+/* Number of features per searchlight and of MLP parameters (12 -> 64 -> 64 -> 2). */
+#define HA_HIP_MLP_IN 12
+#define HA_HIP_MLP_HID 64
+#define HA_HIP_MLP_OUT 2
+#define HA_HIP_MLP_NPARAM (HA_HIP_MLP_IN*HA_HIP_MLP_HID + HA_HIP_MLP_HID + \
+                           HA_HIP_MLP_HID*HA_HIP_MLP_HID + HA_HIP_MLP_HID + \
+                           HA_HIP_MLP_HID*HA_HIP_MLP_OUT + HA_HIP_MLP_OUT)
+
+/* Installs MLP parameters, FP32, concatenated in the order
+ * W1[IN][HID], b1[HID], W2[HID][HID], b2[HID], W3[HID][OUT], b3[OUT]
+ * (row index = input unit), plus the feature normalisation: mean[IN] then
+ * std[IN]. NULL params unloads the model. */
+int ha_hip_set_model(const float *params, const float *feat_norm);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HA_HIP_H */

@@ -1,0 +1,225 @@
+"""GPU parity of the HIP backend against the CPU FP64 oracle, through the C-ABI.
+
+Tolerances: integer work (pattern, index lists) and the FP64 weights are
+bit-exact; the FP64 Jacobi path is within 1e-6 of LAPACK (its local transforms
+are stored in FP32); FP32 / bf16x3 paths are within max-abs 1e-4 of the
+oracle on T, the bar the reference's GPU target states.
+"""
+
+import numpy as np
+import pytest
+
+import hyperalignment_amd as ha
+from hyperalignment_amd import synth
+from oracle import ha_oracle as O
+from tests import refcases
+
+pytestmark = pytest.mark.gpu
+
+TOL_T = 1e-4     # max |T - T_fp64| for FP32 / bf16x3 local transforms
+TOL_64 = 1e-6    # FP64 Jacobi (FP32 storage of local transforms)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def hip():
+    ha.init()
+    yield
+    ha.set_options(method="auto")
+
+
+@pytest.fixture(scope="module")
+def c0():
+    w = synth.make_workload("c0")
+    s = w.sls
+    T = O.searchlight_procrustes_dense(w.X, w.Y, s.indices, s.offsets, s.dists, s.radius)
+    return w, T
+
+
+def run_dense(w, method, reflection=True, scaling=False):
+    ha.set_options(method=method)
+    s = w.sls
+    return ha.searchlight_procrustes_flat(w.X, w.Y, s.indices, s.offsets, s.dists, s.radius,
+                                          reflection=reflection, scaling=scaling)
+
+
+# ---- the reference's known-answer tests, on the GPU ----
+
+def test_ref_procrustes_identity():
+    X = refcases.identity_case()
+    T = ha.procrustes(X, X)
+    assert np.abs(np.abs(T) - np.eye(20)).max() < 1e-8
+
+
+def test_ref_procrustes_known_rotation():
+    X, Y, R = refcases.rotation_case()
+    T = ha.procrustes(X, Y)
+    assert np.abs(X @ T - Y).max() < 1e-8
+
+
+def test_ref_procrustes_no_reflection():
+    X, Y = refcases.no_reflection_case()
+    T = ha.procrustes(X, Y, reflection=False)
+    assert np.linalg.det(T) > 0
+    assert np.abs(T - O.procrustes(X, Y, False, False)).max() < 1e-8
+
+
+def test_ref_procrustes_scaling():
+    rng = np.random.default_rng(3)
+    X = rng.standard_normal((80, 9))
+    Y = rng.standard_normal((80, 9))
+    assert np.abs(ha.procrustes(X, Y, scaling=True) - O.procrustes(X, Y, True, True)).max() < 1e-8
+
+
+def test_ref_sparse_init_toy():
+    m = ha.sparse_init(refcases.SPARSE_TOY, 5)
+    assert m.nnz == 21 and m.shape == (5, 5)
+    p, i = O.sparse_init_pairs(refcases.SPARSE_TOY, 5)
+    assert np.array_equal(m.indptr, p) and np.array_equal(m.indices, i)
+    assert np.all(m.data == 0)
+
+
+def test_ref_scatter_add_identity():
+    m = ha.sparse_init([[0, 1, 2]], 3)
+    ha.sparse_scatter_add(m, [0, 1, 2], [0, 1, 2], np.eye(3))
+    assert np.array_equal(m.toarray(), np.eye(3))
+
+
+def test_ref_weights_uniform_toy():
+    w = ha.searchlight_weights(refcases.WEIGHTS_TOY)
+    sums = np.zeros(6)
+    for l, x in zip(refcases.WEIGHTS_TOY, w):
+        np.add.at(sums, l, x)
+    assert np.abs(sums - 1).max() < 1e-12
+
+
+# ---- bit-exact integer and FP64 work ----
+
+@pytest.mark.parametrize("uniform", [True, False])
+def test_weights_bitexact_c0(uniform):
+    w = synth.make_workload("c0")
+    s = w.sls
+    d = None if uniform else s.dist_lists()
+    got = np.concatenate(ha.searchlight_weights(s.lists(), d, s.radius))
+    ref = O.searchlight_weights_flat(s.indices, s.offsets, None if uniform else s.dists, s.radius, s.nv)
+    assert np.array_equal(got.view(np.uint64), ref.view(np.uint64))
+
+
+@pytest.mark.parametrize("name", ["c0", "c1"])
+def test_pattern_bitexact(name):
+    w = synth.make_workload(name)
+    s = w.sls
+    m = ha.sparse_init(s.lists(), s.nv)
+    p, i = O.sparse_init(s.indices, s.offsets, s.nv)
+    assert np.array_equal(m.indptr.astype(np.int64), p)
+    assert np.array_equal(m.indices, i)
+
+
+# ---- dense entry point vs the oracle ----
+
+@pytest.mark.parametrize("method,tol", [("jacobi64", TOL_64), ("jacobi32", TOL_T), ("polar", TOL_T)])
+def test_dense_c0(c0, method, tol):
+    w, Tref = c0
+    T = run_dense(w, method)
+    err = np.abs(T - Tref).max()
+    assert err < tol, (method, err)
+    assert np.array_equal(T != 0, Tref != 0)
+
+
+@pytest.mark.parametrize("method", ["jacobi32", "polar"])
+def test_identity_invariant(method):
+    w = synth.make_workload("c0")
+    s = w.sls
+    ha.set_options(method=method)
+    T = ha.searchlight_procrustes_flat(w.X, w.X, s.indices, s.offsets, s.dists, s.radius)
+    assert np.abs(T - np.eye(s.nv)).max() < 1e-4
+
+
+def test_dense_accumulates(c0):
+    w, Tref = c0
+    s = w.sls
+    ha.set_options(method="jacobi64")
+    T0 = np.full((s.nv, s.nv), 0.25)
+    T = ha.searchlight_procrustes_flat(w.X, w.Y, s.indices, s.offsets, s.dists, s.radius, T=T0.copy())
+    ref = O.searchlight_procrustes_dense(w.X, w.Y, s.indices, s.offsets, s.dists, s.radius, T_out=T0.copy())
+    assert np.abs(T - ref).max() < TOL_64
+
+
+def test_reflection_false_c0():
+    w = synth.make_workload("c0")
+    s = w.sls
+    ha.set_options(method="polar")
+    T = ha.searchlight_procrustes_flat(w.X, w.Y, s.indices, s.offsets, s.dists, s.radius, reflection=False)
+    ref = O.searchlight_procrustes_dense(w.X, w.Y, s.indices, s.offsets, s.dists, s.radius, reflection=False)
+    assert np.abs(T - ref).max() < TOL_64
+
+
+@pytest.mark.parametrize("method", ["jacobi32", "polar"])
+def test_scaling_c0(method):
+    w = synth.make_workload("c0")
+    s = w.sls
+    T = run_dense(w, method, scaling=True)
+    ref = O.searchlight_procrustes_dense(w.X, w.Y, s.indices, s.offsets, s.dists, s.radius, scaling=True)
+    assert np.abs(T - ref).max() < TOL_T
+
+
+def test_deterministic(c0):
+    w, _ = c0
+    a = run_dense(w, "polar")
+    b = run_dense(w, "polar")
+    assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
+
+
+# ---- CSC entry point ----
+
+def test_csc_api_c0(c0):
+    w, _ = c0
+    s = w.sls
+    ha.set_options(method="jacobi32")
+    m = ha.sparse_init(s.lists(), s.nv)
+    wts = O.searchlight_weights_flat(s.indices, s.offsets, s.dists, s.radius, s.nv)
+    wl = [wts[s.offsets[i]:s.offsets[i + 1]] for i in range(s.count)]
+    m.data[:] = 0.5
+    ha.searchlight_procrustes_csc(w.X.T.copy().T, w.Y, s.lists(), m, weights=wl)
+    ref = O.searchlight_procrustes_csc(w.X, w.Y, s.indices, s.offsets, m.indptr, m.indices,
+                                       data0=np.full(m.nnz, 0.5), weights=wts)
+    assert np.abs(m.data - ref).max() < TOL_T
+
+
+def test_csc_api_unweighted_separate_y_lists():
+    rng = np.random.default_rng(7)
+    nv, nt = 30, 40
+    X = rng.standard_normal((nt, nv))
+    Y = rng.standard_normal((nt, nv))
+    lx = [np.sort(rng.choice(nv, 6, replace=False)) for _ in range(12)]
+    ly = [np.sort(rng.choice(nv, 6, replace=False)) for _ in range(12)]
+    m = ha.sparse_init(lx, nv)
+    ha.set_options(method="jacobi64")
+    ha.searchlight_procrustes_csc(X, Y, lx, m, sls_Y=ly)
+    sx = synth.toy_searchlights(lx, nv=nv)
+    sy = synth.toy_searchlights(ly, nv=nv)
+    ref = O.searchlight_procrustes_csc(X, Y, sx.indices, sx.offsets, m.indptr, m.indices, idx_y=sy.indices)
+    assert np.abs(m.data - ref).max() < TOL_64
+
+
+# ---- larger config: sampled columns ----
+
+def test_c1_polar_columns():
+    w = synth.make_workload("c1")
+    s = w.sls
+    plan = ha.Plan(w.X.shape[0], s.nv, s.indices, s.offsets, s.dists, s.radius)
+    plan.load(w.X, w.Y)
+    ha.set_options(method="polar")
+    plan.execute()
+    indptr, indices, data = plan.csc()
+    p, i = O.sparse_init(s.indices, s.offsets, s.nv)
+    assert np.array_equal(indptr.astype(np.int64), p) and np.array_equal(indices, i)
+    cols = [0, 17, 5000, 10241]
+    wts = O.searchlight_weights_flat(s.indices, s.offsets, s.dists, s.radius, s.nv)
+    ref = O.searchlight_procrustes_csc(w.X, w.Y, s.indices, s.offsets, indptr, indices, weights=wts,
+                                       which_cols=cols, n_jobs=8)
+    for c in cols:
+        sl = slice(indptr[c], indptr[c + 1])
+        assert np.abs(data[sl] - ref[sl]).max() < TOL_T
+    st = plan.stats()
+    assert st["n_unconverged"] == 0
+    plan.close()
