@@ -210,7 +210,7 @@ struct HaHipPlan {
 	double *work64; int64_t stride64; int32_t n_slots64;
 	int32_t *flag, *steps, *queues, *order;
 	double *sigma_sum, *var_sum, *scale;
-	float *features, *pred;
+	float *features, *pred, *trace;
 	HaHipStats stats;
 	hipEvent_t ev[8];
 	bool events;
@@ -223,7 +223,7 @@ static void plan_free(HaHipPlan *p) {
 	                p->occx_ptr, p->occx, p->wsum, p->w, p->indptr64, p->indptr, p->indices, p->data,
 	                p->qld, p->qoff, p->q, p->q64, p->X, p->Y, p->X64, p->Y64, p->stage, p->work, p->work64,
 	                p->flag, p->steps, p->queues, p->order, p->sigma_sum, p->var_sum, p->scale,
-	                p->features, p->pred};
+	                p->features, p->pred, p->trace};
 	for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i)
 		if (ptrs[i]) hipFree(ptrs[i]);
 	if (p->events)
@@ -653,7 +653,27 @@ int ha_plan_run(HaHipPlan *p, bool isReflection, bool isScaling, bool zero_data,
 		KCK(hak_jacobi32(&a, st));
 	} else {
 		a.queue = p->queues + 0;
+		if (getenv("HA_HIP_TRACE")) {
+			if (!p->trace) ACK(p->trace, (size_t)p->count * a.max_steps * sizeof(float));
+			KCK(hak_fill_i32((int32_t *)p->trace, (int64_t)p->count * a.max_steps, 0, st));
+			a.trace = p->trace;
+		}
 		KCK(hak_polar(&a, st));
+		if (a.trace) {
+			int ns = atoi(getenv("HA_HIP_TRACE"));
+			if (ns > p->count) ns = p->count;
+			float *h = (float *)malloc((size_t)ns * a.max_steps * sizeof(float));
+			if (h) {
+				HCK(hipMemcpyAsync(h, p->trace, (size_t)ns * a.max_steps * sizeof(float), hipMemcpyDeviceToHost, st));
+				HCK(hipStreamSynchronize(st));
+				for (int i = 0; i < ns; ++i) {
+					fprintf(stderr, "ha_hip trace s=%d k=%d:", i, p->h_k[i]);
+					for (int t = 0; t < a.max_steps; ++t) fprintf(stderr, " %.3g", h[(size_t)i * a.max_steps + t]);
+					fprintf(stderr, "\n");
+				}
+				free(h);
+			}
+		}
 	}
 	HCK(hipEventRecord(p->ev[2], st));
 	if (!det_fix && method != kHaHipMethodJacobi64 && g.opt.fp64_fallback) {

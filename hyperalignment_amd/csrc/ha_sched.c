@@ -12,6 +12,14 @@
  * construction of Chen & Chow (2014, degree 3) and Amsel et al. (2025,
  * "The Polar Express", degree 5).
  *
+ * Floor: for l -> 0 the minimax error E -> 1, i.e. the interior minimum of
+ * the step, (1 - E)/(1 + E), approaches 0 and singular values sitting there
+ * would be crushed to ~0 (losing their precision, or their sign to rounding).
+ * Each step is therefore designed on [max(l, ELL_FLOOR), 1], where E <= 0.79
+ * and the minimum image is >= 0.12; below the floor p is increasing, so the
+ * new lower bound is min(p(l), (1 - E)/(1 + E)). The small-value gain
+ * a / (1 + E) is ~4.1 either way.
+ *
  * Safety: each step is designed on [l / (1 + SAFE), 1] and applied as
  * p(x / (1 + SAFE)), so singular values up to 1 + SAFE (from rounding, or
  * a scale slightly below sigma_max) still converge. Once 1 - l < 0.05 the
@@ -22,6 +30,7 @@
 #include "ha_internal.h"
 
 #define SAFE 0.01
+#define ELL_FLOOR 0.03
 
 static int solve4(double M[4][5]) {
 	for (int c = 0; c < 4; ++c) {
@@ -88,13 +97,14 @@ int ha_polar_schedule(double ell0, int max_steps, float *out) {
 			double x = l;
 			l = (15 * x - 10 * x * x * x + 3 * x * x * x * x * x) / 8;
 		} else {
-			E = remez5(l / (1 + SAFE), &a, &b, &c);
+			const double ld = l > ELL_FLOOR ? l : ELL_FLOOR;
+			E = remez5(ld / (1 + SAFE), &a, &b, &c);
 			if (E < 0) {
 				if (!have) return -1;
 				a = la; b = lb; c = lc;
 				E = 0;
 				for (int i = 0; i <= 4000; ++i) {
-					double x = l / (1 + SAFE) + (1 - l / (1 + SAFE)) * i / 4000.0;
+					double x = ld / (1 + SAFE) + (1 - ld / (1 + SAFE)) * i / 4000.0;
 					double e = fabs(1 - (a * x + b * x * x * x + c * x * x * x * x * x));
 					if (e > E) E = e;
 				}
@@ -105,7 +115,9 @@ int ha_polar_schedule(double ell0, int max_steps, float *out) {
 			a = a * s * sf;
 			b = b * s * sf * sf * sf;
 			c = c * s * sf * sf * sf * sf * sf;
-			l = (1 - E) / (1 + E);
+			const double lo = (1 - E) / (1 + E);
+			const double pl = a * l + b * l * l * l + c * l * l * l * l * l;
+			l = pl < lo ? pl : lo;
 		}
 		out[3 * t + 0] = (float)a;
 		out[3 * t + 1] = (float)b;

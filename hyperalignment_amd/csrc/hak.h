@@ -108,6 +108,7 @@ typedef struct {
 	float mlp_margin_scale, mlp_margin_ell;
 	float *pred_out;                     /* per searchlight 2 predictions (may be NULL) */
 	float *features;                     /* per searchlight HAK_NFEAT features (may be NULL) */
+	float *trace;                        /* per searchlight max_steps residuals ||G - I||_F (may be NULL) */
 	int32_t tile;
 	int32_t fallback_enabled;
 	int32_t rank_check_nt;               /* flag k >= nt as rank deficient */

@@ -441,6 +441,7 @@ __global__ __launch_bounds__(NT) void k_polar(HakLocalArgs a) {
       r2 = wg_sum_f64(r2, red);
       __threadfence_block();
       const double r = sqrt(r2);
+      if (a.trace && threadIdx.x == 0) a.trace[(int64_t)s * a.max_steps + step] = (float)r;
       if (!isfinite(r)) break;
       const bool finish = (r < a.tol);
       float ca, cb, cc;
