@@ -37,25 +37,30 @@ WORKLOADS = {
 }
 
 
-def cpu_baseline(w, sample, threads):
-    """Oracle (numpy/LAPACK FP64, one searchlight per thread) on a bounded sample."""
+def cpu_baseline(w, seconds, threads):
+    """Oracle (numpy/LAPACK FP64, one searchlight per thread) on a bounded sample:
+    random searchlights in batches of `threads` until `seconds` of wall time."""
     from concurrent.futures import ThreadPoolExecutor
     from oracle import ha_oracle as O
 
     s = w.sls
     lists = s.lists()
     rng = np.random.default_rng(0)
-    which = np.sort(rng.choice(s.count, min(sample, s.count), replace=False))
+    perm = rng.permutation(s.count)
 
     def one(i):
         sl = lists[i]
         return O.procrustes(w.X[:, sl], w.Y[:, sl])
 
+    done, Ts = [], []
     t0 = time.perf_counter()
     with ThreadPoolExecutor(threads) as ex:
-        Ts = list(ex.map(one, which))
+        while time.perf_counter() - t0 < seconds and len(done) < s.count:
+            batch = perm[len(done):len(done) + threads]
+            Ts.extend(ex.map(one, batch))
+            done.extend(batch)
     dt = time.perf_counter() - t0
-    return len(which) / dt, which, Ts
+    return len(done) / dt, np.array(done), Ts
 
 
 def main():
@@ -65,7 +70,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--method", default="auto", choices=sorted(ha.METHODS))
-    ap.add_argument("--cpu-sample", type=int, default=96)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile", type=int, default=0)
     args = ap.parse_args()
@@ -117,11 +122,11 @@ def main():
     parity = None
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
-        rate, which, Ts = cpu_baseline(w, args.cpu_sample, threads)
+        rate, which, Ts = cpu_baseline(w, args.cpu_seconds, threads)
         cpu = {"value": rate, "unit": "searchlights/s", "cores": threads, "kind": "port",
-               "sample": f"{len(which)} random searchlights of {args.config}, numpy/LAPACK FP64 dgesdd "
-                         f"(oracle), one searchlight per thread, hemisphere estimate "
-                         f"{s.count / rate:.1f} s"}
+               "sample": f"{len(which)} random searchlights of {args.config} (about {args.cpu_seconds:.0f} s), "
+                         f"numpy/LAPACK FP64 dgesdd (oracle), one searchlight per thread; hemisphere "
+                         f"estimate {s.count / rate:.1f} s"}
         loc = plan.local(which)
         errs = [float(np.abs(a - b).max()) for a, b in zip(loc, Ts)]
         parity = {"local_maxabs": max(errs), "local_median": float(np.median(errs))}

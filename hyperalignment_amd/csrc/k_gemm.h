@@ -1,0 +1,329 @@
+// k_gemm.h -- workgroup-level bf16x3 MFMA GEMM engine for gfx950.
+//
+// One 256-thread workgroup computes a full kp x kp product C = A B (or only
+// the tiles I <= J of a symmetric product) tile by tile, with
+//   * TILE x TILE output tiles, each wave owning a (TILE/2)^2 quarter made of
+//     32x32 blocks of v_mfma_f32_32x32x16_bf16 (FP32 accumulation);
+//   * bf16x3 products: C += Bhi*Ahi + Blo*Ahi + Bhi*Alo (the Blo*Alo term,
+//     2^-16 relative, is dropped), so FP32-level accuracy at bf16 MFMA rate;
+//   * K chunks of 32 staged in double-buffered LDS: the next chunk's global
+//     loads are in flight while the current chunk's MFMAs run;
+//   * 32x32 blocks that lie entirely outside the valid rows/columns skip their
+//     MFMAs (edge tiles of kp that is not a multiple of TILE).
+//
+// Operand kinds (element (m, k) of the operand):
+//   OP_F32_GATHER  f32[col(m) * ld + k], col(m) = cols[m]: FP32 data, split
+//                  into bf16 hi/lo while staging (the cross-covariance)
+//   OP_SPLIT_K     hi/lo[m * ld + k]: pre-split planes, k contiguous
+//   OP_SPLIT_M     hi/lo[m + k * ld]: pre-split planes, m contiguous
+// Rows m >= mlimit read as zero; every k < K is in bounds.
+//
+// MFMA operand order is swapped (the B fragment feeds the instruction's A
+// slot), so lane % 32 indexes the output row m and output columns are
+// contiguous per register: column-major stores coalesce.
+#ifndef K_GEMM_H
+#define K_GEMM_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+#define NT 256  // threads per workgroup (4 waves)
+#define KC 32   // K chunk staged in LDS
+
+enum { OP_F32_GATHER = 0, OP_SPLIT_K = 1, OP_SPLIT_M = 2 };
+
+// ---- bf16 helpers (round to nearest even) ----
+__device__ __forceinline__ uint16_t f2bf(float x) {
+  uint32_t u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
+
+struct SplitM {
+  uint16_t *hi, *lo;
+};
+// a split matrix occupies the bytes of a kp x kp FP32 buffer: hi plane then lo plane
+__device__ __forceinline__ SplitM as_split(float *buf, int kp) {
+  SplitM s;
+  s.hi = (uint16_t *)buf;
+  s.lo = s.hi + (int64_t)kp * kp;
+  return s;
+}
+__device__ __forceinline__ void put_split(const SplitM &s, int64_t i, float v) {
+  uint16_t h = f2bf(v);
+  s.hi[i] = h;
+  s.lo[i] = f2bf(v - bf2f(h));
+}
+__device__ __forceinline__ float get_split(const SplitM &s, int64_t i) { return bf2f(s.hi[i]) + bf2f(s.lo[i]); }
+
+struct Opnd {
+  int32_t kind;
+  const float *f32;
+  const int32_t *cols;
+  const uint16_t *hi, *lo;
+  int32_t ld;
+  int32_t mlimit;
+};
+
+__device__ __forceinline__ Opnd opnd_gather(const float *base, const int32_t *cols, int ld, int mlimit) {
+  Opnd o;
+  o.kind = OP_F32_GATHER; o.f32 = base; o.cols = cols; o.hi = nullptr; o.lo = nullptr; o.ld = ld; o.mlimit = mlimit;
+  return o;
+}
+__device__ __forceinline__ Opnd opnd_split(const SplitM &s, int kind, int ld, int mlimit) {
+  Opnd o;
+  o.kind = kind; o.f32 = nullptr; o.cols = nullptr; o.hi = s.hi; o.lo = s.lo; o.ld = ld; o.mlimit = mlimit;
+  return o;
+}
+
+template <int TILE>
+struct GemmSmem {
+  union {
+    struct {
+      uint4 ah[2][KC / 8][TILE], al[2][KC / 8][TILE];
+      uint4 bh[2][KC / 8][TILE], bl[2][KC / 8][TILE];
+    };
+    float ctile[TILE * (TILE + 1)];  // finished tile, column-major, padded leading dim
+  };
+};
+
+template <int TILE>
+struct Stage {
+  static constexpr int N = TILE * KC / (4 * NT);
+  uint4 r[N];
+};
+
+template <int TILE, int KIND>
+__device__ __forceinline__ void fetch(const Opnd &o, int m0, int kb, Stage<TILE> &st) {
+  constexpr int N = Stage<TILE>::N;
+  const int tid = threadIdx.x;
+  if constexpr (KIND == OP_F32_GATHER) {
+#pragma unroll
+    for (int it = 0; it < N; ++it) {
+      const int e = tid + it * NT;
+      const int m = e / (KC / 4), kg = e % (KC / 4);
+      const int gm = m0 + m;
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (gm < o.mlimit) v = *(const uint4 *)(o.f32 + (int64_t)o.cols[gm] * o.ld + kb + kg * 4);
+      st.r[it] = v;
+    }
+  } else if constexpr (KIND == OP_SPLIT_K) {
+#pragma unroll
+    for (int it = 0; it < N / 2; ++it) {
+      const int e = tid + it * NT;
+      const int m = e / (KC / 8), kg = e % (KC / 8);
+      const int gm = m0 + m;
+      uint4 h = make_uint4(0, 0, 0, 0), l = h;
+      if (gm < o.mlimit) {
+        const int64_t ofs = (int64_t)gm * o.ld + kb + kg * 8;
+        h = *(const uint4 *)(o.hi + ofs);
+        l = *(const uint4 *)(o.lo + ofs);
+      }
+      st.r[it] = h;
+      st.r[N / 2 + it] = l;
+    }
+  } else {
+#pragma unroll
+    for (int it = 0; it < N / 2; ++it) {
+      const int e = tid + it * NT;
+      const int kk = e / (TILE / 8), mg = e % (TILE / 8);
+      const int gm = m0 + mg * 8;
+      uint4 h = make_uint4(0, 0, 0, 0), l = h;
+      if (gm < o.mlimit) {  // mlimit is a multiple of 8 here
+        const int64_t ofs = gm + (int64_t)(kb + kk) * o.ld;
+        h = *(const uint4 *)(o.hi + ofs);
+        l = *(const uint4 *)(o.lo + ofs);
+      }
+      st.r[it] = h;
+      st.r[N / 2 + it] = l;
+    }
+  }
+}
+
+__device__ __forceinline__ uint32_t pack2(uint16_t a, uint16_t b) { return (uint32_t)a | ((uint32_t)b << 16); }
+
+template <int TILE, int KIND>
+__device__ __forceinline__ void stash(const Stage<TILE> &st, uint4 (*sh)[TILE], uint4 (*sl)[TILE]) {
+  constexpr int N = Stage<TILE>::N;
+  const int tid = threadIdx.x;
+  if constexpr (KIND == OP_F32_GATHER) {
+#pragma unroll
+    for (int it = 0; it < N; ++it) {
+      const int e = tid + it * NT;
+      const int m = e / (KC / 4), kg = e % (KC / 4);
+      const uint4 u = st.r[it];
+      const float v0 = __uint_as_float(u.x), v1 = __uint_as_float(u.y);
+      const float v2 = __uint_as_float(u.z), v3 = __uint_as_float(u.w);
+      const uint16_t h0 = f2bf(v0), h1 = f2bf(v1), h2 = f2bf(v2), h3 = f2bf(v3);
+      const uint16_t l0 = f2bf(v0 - bf2f(h0)), l1 = f2bf(v1 - bf2f(h1));
+      const uint16_t l2 = f2bf(v2 - bf2f(h2)), l3 = f2bf(v3 - bf2f(h3));
+      ((uint2 *)&sh[kg >> 1][m])[kg & 1] = make_uint2(pack2(h0, h1), pack2(h2, h3));
+      ((uint2 *)&sl[kg >> 1][m])[kg & 1] = make_uint2(pack2(l0, l1), pack2(l2, l3));
+    }
+  } else if constexpr (KIND == OP_SPLIT_K) {
+#pragma unroll
+    for (int it = 0; it < N / 2; ++it) {
+      const int e = tid + it * NT;
+      const int m = e / (KC / 8), kg = e % (KC / 8);
+      sh[kg][m] = st.r[it];
+      sl[kg][m] = st.r[N / 2 + it];
+    }
+  } else {
+#pragma unroll
+    for (int it = 0; it < N / 2; ++it) {
+      const int e = tid + it * NT;
+      const int kk = e / (TILE / 8), mg = e % (TILE / 8);
+      const uint4 h = st.r[it], l = st.r[N / 2 + it];
+      const uint32_t hw[4] = {h.x, h.y, h.z, h.w}, lw[4] = {l.x, l.y, l.z, l.w};
+      const int g = kk >> 3, q = kk & 7;
+#pragma unroll
+      for (int p = 0; p < 4; ++p) {
+        ((uint16_t *)&sh[g][mg * 8 + 2 * p])[q] = (uint16_t)(hw[p] & 0xFFFF);
+        ((uint16_t *)&sh[g][mg * 8 + 2 * p + 1])[q] = (uint16_t)(hw[p] >> 16);
+        ((uint16_t *)&sl[g][mg * 8 + 2 * p])[q] = (uint16_t)(lw[p] & 0xFFFF);
+        ((uint16_t *)&sl[g][mg * 8 + 2 * p + 1])[q] = (uint16_t)(lw[p] >> 16);
+      }
+    }
+  }
+}
+
+// acc.v[i][j] holds C(m, n) for m = tm0 + wm0 + i*32 + lane%32 and
+// n = tn0 + wn0 + j*32 + (r/4)*8 + (lane/32)*4 + r%4 (r = element index).
+template <int TILE>
+struct Acc {
+  static constexpr int W = TILE / 64;
+  f32x16_t v[W][W];
+};
+
+template <int TILE>
+__device__ __forceinline__ void mma_chunk(const GemmSmem<TILE> &sm, int buf, Acc<TILE> &acc, const bool *bm,
+                                          const bool *bn) {
+  constexpr int W = TILE / 64;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm0 = (wave >> 1) * (TILE / 2), wn0 = (wave & 1) * (TILE / 2);
+#pragma unroll
+  for (int ks = 0; ks < KC / 16; ++ks) {
+    const int g = ks * 2 + (lane >> 5);
+    bf16x8_t ahf[W], alf[W], bhf[W], blf[W];
+#pragma unroll
+    for (int i = 0; i < W; ++i) {
+      uint4 h = sm.ah[buf][g][wm0 + i * 32 + (lane & 31)];
+      uint4 l = sm.al[buf][g][wm0 + i * 32 + (lane & 31)];
+      ahf[i] = __builtin_bit_cast(bf16x8_t, h);
+      alf[i] = __builtin_bit_cast(bf16x8_t, l);
+    }
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+      uint4 h = sm.bh[buf][g][wn0 + j * 32 + (lane & 31)];
+      uint4 l = sm.bl[buf][g][wn0 + j * 32 + (lane & 31)];
+      bhf[j] = __builtin_bit_cast(bf16x8_t, h);
+      blf[j] = __builtin_bit_cast(bf16x8_t, l);
+    }
+#pragma unroll
+    for (int i = 0; i < W; ++i)
+#pragma unroll
+      for (int j = 0; j < W; ++j) {
+        if (!(bm[i] && bn[j])) continue;
+        acc.v[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bhf[j], ahf[i], acc.v[i][j], 0, 0, 0);
+        acc.v[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(blf[j], ahf[i], acc.v[i][j], 0, 0, 0);
+        acc.v[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bhf[j], alf[i], acc.v[i][j], 0, 0, 0);
+      }
+  }
+}
+
+template <int TILE, int KA, int KB>
+__device__ __forceinline__ void tile_gemm(GemmSmem<TILE> &sm, const Opnd &A, const Opnd &B, int tm0, int tn0,
+                                          int K, Acc<TILE> &acc) {
+  constexpr int W = TILE / 64;
+  const int wave = threadIdx.x >> 6;
+  const int wm0 = (wave >> 1) * (TILE / 2), wn0 = (wave & 1) * (TILE / 2);
+  bool bm[W], bn[W];
+#pragma unroll
+  for (int i = 0; i < W; ++i) {
+    bm[i] = tm0 + wm0 + i * 32 < A.mlimit;
+    bn[i] = tn0 + wn0 + i * 32 < B.mlimit;
+  }
+#pragma unroll
+  for (int i = 0; i < W; ++i)
+#pragma unroll
+    for (int j = 0; j < W; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc.v[i][j][r] = 0.f;
+  const int nc = K / KC;
+  Stage<TILE> sa, sb;
+  fetch<TILE, KA>(A, tm0, 0, sa);
+  fetch<TILE, KB>(B, tn0, 0, sb);
+  stash<TILE, KA>(sa, sm.ah[0], sm.al[0]);
+  stash<TILE, KB>(sb, sm.bh[0], sm.bl[0]);
+  __syncthreads();
+  for (int c = 0; c < nc; ++c) {
+    const int cur = c & 1;
+    const bool more = c + 1 < nc;
+    if (more) {
+      fetch<TILE, KA>(A, tm0, (c + 1) * KC, sa);
+      fetch<TILE, KB>(B, tn0, (c + 1) * KC, sb);
+    }
+    mma_chunk<TILE>(sm, cur, acc, bm, bn);
+    if (more) {
+      stash<TILE, KA>(sa, sm.ah[cur ^ 1], sm.al[cur ^ 1]);
+      stash<TILE, KB>(sb, sm.bh[cur ^ 1], sm.bl[cur ^ 1]);
+    }
+    __syncthreads();
+  }
+}
+
+// Writes the accumulators into sm.ctile: element (ml, nl) of the tile at ctile[nl * (TILE + 1) + ml].
+template <int TILE>
+__device__ __forceinline__ void acc_to_lds(const Acc<TILE> &acc, float *ctile) {
+  constexpr int W = TILE / 64;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm0 = (wave >> 1) * (TILE / 2), wn0 = (wave & 1) * (TILE / 2);
+#pragma unroll
+  for (int i = 0; i < W; ++i)
+#pragma unroll
+    for (int j = 0; j < W; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int ml = wm0 + i * 32 + (lane & 31);
+        const int nl = wn0 + j * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
+        ctile[nl * (TILE + 1) + ml] = acc.v[i][j][r];
+      }
+}
+
+// Full product over kp x kp. Each finished tile is staged in LDS and handed to
+// epi(m, n, v) element by element with consecutive threads on consecutive rows
+// m (coalesced column-major writes). sym: only tiles I <= J are computed and
+// every off-diagonal tile is also delivered transposed, so epi still sees
+// every (m, n) of the full product exactly once.
+template <int TILE, int KA, int KB, class Epi>
+__device__ void wg_gemm(GemmSmem<TILE> &sm, const Opnd &A, const Opnd &B, int kp, int K, bool sym, Epi epi) {
+  const int nt = (kp + TILE - 1) / TILE;
+  constexpr int LD = TILE + 1;
+  for (int I = 0; I < nt; ++I)
+    for (int J = sym ? I : 0; J < nt; ++J) {
+      Acc<TILE> acc;
+      tile_gemm<TILE, KA, KB>(sm, A, B, I * TILE, J * TILE, K, acc);
+      acc_to_lds<TILE>(acc, sm.ctile);
+      __syncthreads();
+      const int mmax = min(TILE, kp - I * TILE), nmax = min(TILE, kp - J * TILE);
+      for (int e = threadIdx.x; e < TILE * TILE; e += NT) {
+        const int ml = e % TILE, nl = e / TILE;
+        if (ml < mmax && nl < nmax) epi(I * TILE + ml, J * TILE + nl, sm.ctile[nl * LD + ml]);
+      }
+      if (sym && I != J) {
+        // transposed delivery: output (J*TILE + a, I*TILE + b) = tile element (b, a)
+        for (int e = threadIdx.x; e < TILE * TILE; e += NT) {
+          const int al = e % TILE, bl = e / TILE;
+          if (al < nmax && bl < mmax) epi(J * TILE + al, I * TILE + bl, sm.ctile[al * LD + bl]);
+        }
+      }
+      __syncthreads();
+    }
+}
+
+#endif  // K_GEMM_H

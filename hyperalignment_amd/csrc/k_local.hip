@@ -28,181 +28,7 @@
 
 #define LAUNCH_RET() return (int)hipGetLastError()
 
-typedef short bf16x4_t __attribute__((ext_vector_type(4)));
-typedef float f32x16_t __attribute__((ext_vector_type(16)));
-typedef float f32x4_t __attribute__((ext_vector_type(4)));
-
-#define NT 256   // threads per workgroup (4 waves)
-#define KC 32    // K chunk staged in LDS
-
-// ---------------------------------------------------------------------------
-// bf16 split: x = hi + lo + O(2^-17 |x|), hi and lo rounded to nearest even
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ uint16_t f2bf(float x) {
-  uint32_t u = __float_as_uint(x);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
-__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
-
-struct Split4 { uint2 hi, lo; };
-__device__ __forceinline__ Split4 split4(f32x4_t v) {
-  uint16_t h0 = f2bf(v[0]), h1 = f2bf(v[1]), h2 = f2bf(v[2]), h3 = f2bf(v[3]);
-  uint16_t l0 = f2bf(v[0] - bf2f(h0)), l1 = f2bf(v[1] - bf2f(h1));
-  uint16_t l2 = f2bf(v[2] - bf2f(h2)), l3 = f2bf(v[3] - bf2f(h3));
-  Split4 r;
-  r.hi = make_uint2((uint32_t)h0 | ((uint32_t)h1 << 16), (uint32_t)h2 | ((uint32_t)h3 << 16));
-  r.lo = make_uint2((uint32_t)l0 | ((uint32_t)l1 << 16), (uint32_t)l2 | ((uint32_t)l3 << 16));
-  return r;
-}
-
-// ---------------------------------------------------------------------------
-// Workgroup GEMM tile engine
-// ---------------------------------------------------------------------------
-// Operand element (m, k):
-//   k-contiguous : base[col(m) * ld + k], col(m) = cols ? cols[m] : m
-//   m-contiguous : base[m + k * ld]
-// m >= mlimit reads as zero. K is a multiple of KC and every K index < K is
-// in bounds (padding rows are stored zeros).
-struct Opnd {
-  const float *base;
-  const int32_t *cols;
-  int32_t ld;
-  int32_t mlimit;
-  int32_t mcontig;
-};
-
-template <int TILE>
-struct TileSmem {
-  uint2 ah[KC / 4][TILE], al[KC / 4][TILE];
-  uint2 bh[KC / 4][TILE], bl[KC / 4][TILE];
-};
-
-template <int TILE>
-__device__ __forceinline__ void load_operand(uint2 (*sh)[TILE], uint2 (*sl)[TILE], const Opnd &o,
-                                             int m0, int kb) {
-  const int tid = threadIdx.x;
-  if (!o.mcontig) {
-    // TILE columns x (KC/4) float4 each
-    constexpr int NV = TILE * (KC / 4);
-#pragma unroll
-    for (int it = 0; it < NV / NT; ++it) {
-      int e = tid + it * NT;
-      int m = e / (KC / 4), kg = e % (KC / 4);
-      int gm = m0 + m;
-      f32x4_t v = {0.f, 0.f, 0.f, 0.f};
-      if (gm < o.mlimit) {
-        int64_t col = o.cols ? (int64_t)o.cols[gm] : (int64_t)gm;
-        v = *(const f32x4_t *)(o.base + col * o.ld + kb + kg * 4);
-      }
-      Split4 s = split4(v);
-      sh[kg][m] = s.hi;
-      sl[kg][m] = s.lo;
-    }
-  } else {
-    // KC rows of TILE contiguous floats
-    constexpr int NV = KC * (TILE / 4);
-#pragma unroll
-    for (int it = 0; it < NV / NT; ++it) {
-      int e = tid + it * NT;
-      int kk = e / (TILE / 4), mg = e % (TILE / 4);
-      int gm = m0 + mg * 4;
-      f32x4_t v = {0.f, 0.f, 0.f, 0.f};
-      if (gm + 3 < o.mlimit) {
-        v = *(const f32x4_t *)(o.base + gm + (int64_t)(kb + kk) * o.ld);
-      } else {
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-          if (gm + q < o.mlimit) v[q] = o.base[gm + q + (int64_t)(kb + kk) * o.ld];
-      }
-      Split4 s = split4(v);
-      uint16_t *ph = (uint16_t *)&sh[kk >> 2][mg * 4];
-      uint16_t *pl = (uint16_t *)&sl[kk >> 2][mg * 4];
-      const int kq = kk & 3;
-      ph[0 * 4 + kq] = (uint16_t)(s.hi.x & 0xFFFF);
-      ph[1 * 4 + kq] = (uint16_t)(s.hi.x >> 16);
-      ph[2 * 4 + kq] = (uint16_t)(s.hi.y & 0xFFFF);
-      ph[3 * 4 + kq] = (uint16_t)(s.hi.y >> 16);
-      pl[0 * 4 + kq] = (uint16_t)(s.lo.x & 0xFFFF);
-      pl[1 * 4 + kq] = (uint16_t)(s.lo.x >> 16);
-      pl[2 * 4 + kq] = (uint16_t)(s.lo.y & 0xFFFF);
-      pl[3 * 4 + kq] = (uint16_t)(s.lo.y >> 16);
-    }
-  }
-}
-
-// acc[bm][bn] holds C(m, n) for m = tm0 + wm0 + bm*32 + lane%32,
-// n = tn0 + wn0 + bn*32 + (r/4)*8 + (lane/32)*4 + r%4  (r = element index).
-template <int TILE>
-struct Acc {
-  static constexpr int W = TILE / 64;
-  f32x16_t v[W][W];
-};
-
-template <int TILE>
-__device__ __forceinline__ void tile_mma(TileSmem<TILE> &sm, const Opnd &A, const Opnd &B, int tm0, int tn0,
-                                         int K, Acc<TILE> &acc) {
-  constexpr int W = TILE / 64;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int wm0 = (wave >> 1) * (TILE / 2), wn0 = (wave & 1) * (TILE / 2);
-#pragma unroll
-  for (int i = 0; i < W; ++i)
-#pragma unroll
-    for (int j = 0; j < W; ++j)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) acc.v[i][j][r] = 0.f;
-  for (int kb = 0; kb < K; kb += KC) {
-    load_operand<TILE>(sm.ah, sm.al, A, tm0, kb);
-    load_operand<TILE>(sm.bh, sm.bl, B, tn0, kb);
-    __syncthreads();
-#pragma unroll
-    for (int ks = 0; ks < KC / 8; ++ks) {
-      const int g = ks * 2 + (lane >> 5);
-      bf16x4_t ahf[W], alf[W], bhf[W], blf[W];
-#pragma unroll
-      for (int i = 0; i < W; ++i) {
-        uint2 h = sm.ah[g][wm0 + i * 32 + (lane & 31)];
-        uint2 l = sm.al[g][wm0 + i * 32 + (lane & 31)];
-        ahf[i] = *(bf16x4_t *)&h;
-        alf[i] = *(bf16x4_t *)&l;
-      }
-#pragma unroll
-      for (int j = 0; j < W; ++j) {
-        uint2 h = sm.bh[g][wn0 + j * 32 + (lane & 31)];
-        uint2 l = sm.bl[g][wn0 + j * 32 + (lane & 31)];
-        bhf[j] = *(bf16x4_t *)&h;
-        blf[j] = *(bf16x4_t *)&l;
-      }
-#pragma unroll
-      for (int i = 0; i < W; ++i)
-#pragma unroll
-        for (int j = 0; j < W; ++j) {
-          acc.v[i][j] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(bhf[j], ahf[i], acc.v[i][j], 0, 0, 0);
-          acc.v[i][j] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(blf[j], ahf[i], acc.v[i][j], 0, 0, 0);
-          acc.v[i][j] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(bhf[j], alf[i], acc.v[i][j], 0, 0, 0);
-        }
-    }
-    __syncthreads();
-  }
-}
-
-// Visits every accumulator element with its (m, n) coordinates.
-template <int TILE, class F>
-__device__ __forceinline__ void acc_foreach(const Acc<TILE> &acc, int tm0, int tn0, F f) {
-  constexpr int W = TILE / 64;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int wm0 = (wave >> 1) * (TILE / 2), wn0 = (wave & 1) * (TILE / 2);
-#pragma unroll
-  for (int i = 0; i < W; ++i)
-#pragma unroll
-    for (int j = 0; j < W; ++j)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        int m = tm0 + wm0 + i * 32 + (lane & 31);
-        int n = tn0 + wn0 + j * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
-        f(m, n, acc.v[i][j][r]);
-      }
-}
+#include "k_gemm.h"
 
 __device__ __forceinline__ double wg_sum_f64(double v, double *red) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -213,22 +39,6 @@ __device__ __forceinline__ double wg_sum_f64(double v, double *red) {
   double r = red[0] + red[1] + red[2] + red[3];
   __syncthreads();
   return r;
-}
-
-// Full-size GEMM over kp x kp output tiles. sym: output is symmetric, only
-// tiles with I <= J are computed and the transposed tile is mirrored.
-template <int TILE, class Epi>
-__device__ void wg_gemm(TileSmem<TILE> &sm, const Opnd &A, const Opnd &B, int kp, int K, bool sym, Epi epi) {
-  const int nt = (kp + TILE - 1) / TILE;
-  for (int I = 0; I < nt; ++I)
-    for (int J = sym ? I : 0; J < nt; ++J) {
-      Acc<TILE> acc;
-      tile_mma<TILE>(sm, A, B, I * TILE, J * TILE, K, acc);
-      const bool mirror = sym && (I != J);
-      acc_foreach<TILE>(acc, I * TILE, J * TILE, [&](int m, int n, float v) {
-        if (m < kp && n < kp) epi(m, n, v, mirror);
-      });
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -257,20 +67,20 @@ __device__ __forceinline__ int next_item(int32_t *queue, int32_t *sh) {
   return *sh;
 }
 
-// cross-covariance A = X[:, slx]^T Y[:, sly] into buffer Ab (kp x kp)
+// cross-covariance A = X[:, slx]^T Y[:, sly] into the FP32 buffer Ab (kp x kp)
 template <int TILE>
-__device__ void crosscov(TileSmem<TILE> &sm, const HakLocalArgs &a, int s, int k, int kp, float *Ab) {
+__device__ void crosscov(GemmSmem<TILE> &sm, const HakLocalArgs &a, int s, int k, int kp, float *Ab) {
   const int o = a.sl_off[s];
-  Opnd oa = {a.X, a.slx_idx + o, a.ld, k, 0};
-  Opnd ob = {a.Y, a.sly_idx + o, a.ld, k, 0};
-  wg_gemm<TILE>(sm, oa, ob, kp, a.ld, false, [&](int m, int n, float v, bool) {
+  Opnd oa = opnd_gather(a.X, a.slx_idx + o, a.ld, k);
+  Opnd ob = opnd_gather(a.Y, a.sly_idx + o, a.ld, k);
+  wg_gemm<TILE, OP_F32_GATHER, OP_F32_GATHER>(sm, oa, ob, kp, a.ld, false, [&](int m, int n, float v) {
     Ab[m + (int64_t)n * kp] = v;
   });
 }
 
 // Learned-scaling features of A (12 per searchlight), written when a.features != NULL.
-__device__ void features_of(const HakLocalArgs &a, int s, int k, int kp, const float *Ab, float *Pv,
-                            float *feat_sh, double *red) {
+__device__ __noinline__ void features_of(const HakLocalArgs &a, int s, int k, int kp, const float *Ab, float *Pv,
+                                         float *feat_sh, double *red) {
   // Frobenius, trace, diagonal statistics
   double fro2 = 0.0;
   for (int64_t e = threadIdx.x; e < (int64_t)kp * kp; e += NT) fro2 += (double)Ab[e] * Ab[e];
@@ -358,9 +168,14 @@ __device__ void features_of(const HakLocalArgs &a, int s, int k, int kp, const f
   __syncthreads();
 }
 
+// predictions of the MLP for one searchlight (out of line: runs once per searchlight)
+__device__ __noinline__ void mlp_predict(const HakLocalArgs &a, const float *feat_sh, float *mlp_sh, float *pred) {
+  hak_mlp_forward_wg(a.mlp, a.mlp_norm, feat_sh, mlp_sh, pred);
+}
+
 template <int TILE>
-__global__ __launch_bounds__(NT) void k_polar(HakLocalArgs a) {
-  __shared__ TileSmem<TILE> sm;
+__global__ __launch_bounds__(NT, TILE == 128 ? 1 : 2) void k_polar(HakLocalArgs a) {
+  __shared__ GemmSmem<TILE> sm;
   __shared__ double red[4];
   __shared__ int32_t item_sh;
   __shared__ float feat_sh[HAK_NFEAT];
@@ -383,7 +198,6 @@ __global__ __launch_bounds__(NT) void k_polar(HakLocalArgs a) {
     }
     crosscov<TILE>(sm, a, s, k, kp, w.A);
     __syncthreads();
-    __threadfence_block();
     // scale and schedule bucket
     double fro2 = 0.0;
     for (int64_t e = threadIdx.x; e < (int64_t)kp * kp; e += NT) fro2 += (double)w.A[e] * w.A[e];
@@ -392,11 +206,12 @@ __global__ __launch_bounds__(NT) void k_polar(HakLocalArgs a) {
     int bucket = a.sl_bucket ? a.sl_bucket[s] : a.default_bucket;
     if (a.features || a.mlp) {
       features_of(a, s, k, kp, w.A, w.P, feat_sh, red);
-      if (a.features && threadIdx.x < HAK_NFEAT) a.features[(int64_t)s * HAK_NFEAT + threadIdx.x] = feat_sh[threadIdx.x];
+      if (a.features && threadIdx.x < HAK_NFEAT)
+        a.features[(int64_t)s * HAK_NFEAT + threadIdx.x] = feat_sh[threadIdx.x];
       if (a.mlp) {
         float pred[2];
-        hak_mlp_forward_wg(a.mlp, a.mlp_norm, feat_sh, mlp_sh, pred);
-        // pred[0] = log(sigma_max / ||A||_F), pred[1] = log10(sigma_min / sigma_max)
+        mlp_predict(a, feat_sh, mlp_sh, pred);
+        // pred[0] = log(sigma_max / ||A||_F), pred[1] = log10(sigma_min / sigma_max).
         // scale = predicted sigma_max inflated by the margin (never above ||A||_F);
         // ell = predicted sigma_min / scale, deflated by its margin
         float lf = fminf(pred[0] + a.mlp_margin_scale, 0.f);
@@ -414,32 +229,38 @@ __global__ __launch_bounds__(NT) void k_polar(HakLocalArgs a) {
       scale = a.sl_scale[s];
     }
     if (!(scale > 0.f) || !isfinite(scale)) {
-      if (threadIdx.x == 0) { a.flag[s] = 1; if (a.steps) a.steps[s] = 0; }
+      if (threadIdx.x == 0) {
+        a.flag[s] = 1;
+        if (a.steps) a.steps[s] = 0;
+      }
       continue;
     }
+    // X0 = A / scale, stored as bf16 hi/lo planes
     const float inv = 1.0f / scale;
-    for (int64_t e = threadIdx.x; e < (int64_t)kp * kp; e += NT) w.X[e] = w.A[e] * inv;
+    {
+      const SplitM x0 = as_split(w.X, kp);
+      for (int64_t e = threadIdx.x; e < (int64_t)kp * kp; e += NT) put_split(x0, e, w.A[e] * inv);
+    }
     __syncthreads();
-    float *X = w.X, *Xn = w.X2;
+    float *Xb = w.X, *Xnb = w.X2;
+    const SplitM G = as_split(w.G, kp), P = as_split(w.P, kp);
     int step = 0;
     bool done = false;
     for (step = 0; step < a.max_steps; ++step) {
+      const SplitM X = as_split(Xb, kp), Xn = as_split(Xnb, kp);
       // G = X^T X and r = ||G - I||_F over the k x k block
       double r2 = 0.0;
       {
-        Opnd oa = {X, nullptr, kp, kp, 0};
-        float *G = w.G;
-        wg_gemm<TILE>(sm, oa, oa, kp, kp, true, [&](int m, int n, float v, bool mirror) {
-          G[m + (int64_t)n * kp] = v;
-          if (mirror) G[n + (int64_t)m * kp] = v;
+        const Opnd ox = opnd_split(X, OP_SPLIT_K, kp, kp);
+        wg_gemm<TILE, OP_SPLIT_K, OP_SPLIT_K>(sm, ox, ox, kp, kp, true, [&](int m, int n, float v) {
+          put_split(G, m + n * kp, v);
           if (m < k && n < k) {
-            float d = v - (m == n ? 1.f : 0.f);
-            r2 += (mirror ? 2.0 : 1.0) * (double)d * d;
+            const float d = v - (m == n ? 1.f : 0.f);
+            r2 += (double)d * d;
           }
         });
       }
       r2 = wg_sum_f64(r2, red);
-      __threadfence_block();
       const double r = sqrt(r2);
       if (a.trace && threadIdx.x == 0) a.trace[(int64_t)s * a.max_steps + step] = (float)r;
       if (!isfinite(r)) break;
@@ -447,51 +268,56 @@ __global__ __launch_bounds__(NT) void k_polar(HakLocalArgs a) {
       float ca, cb, cc;
       if (finish) {
         ca = 1.5f; cb = -0.5f; cc = 0.f;
+      } else if (step < a.max_sched) {
+        const float *co = a.sched + ((int64_t)bucket * a.max_sched + step) * 3;
+        ca = co[0]; cb = co[1]; cc = co[2];
       } else {
-        if (step < a.max_sched) {
-          const float *co = a.sched + ((int64_t)bucket * a.max_sched + step) * 3;
-          ca = co[0]; cb = co[1]; cc = co[2];
-        } else {
-          ca = 1.875f; cb = -1.25f; cc = 0.375f;  // quintic Newton-Schulz
-        }
+        ca = 1.875f; cb = -1.25f; cc = 0.375f;  // quintic Newton-Schulz
       }
-      // P = ca I + cb G + cc G^2  (G^2 only when cc != 0)
-      float *P = w.P;
-      const float *G = w.G;
+      // P = ca I + cb G + cc G^2 (G^2 only when cc != 0), zero outside the k x k block
       if (cc != 0.f) {
-        Opnd og = {G, nullptr, kp, kp, 0};
-        wg_gemm<TILE>(sm, og, og, kp, kp, true, [&](int m, int n, float v, bool mirror) {
+        const Opnd og = opnd_split(G, OP_SPLIT_K, kp, kp);
+        wg_gemm<TILE, OP_SPLIT_K, OP_SPLIT_K>(sm, og, og, kp, kp, true, [&](int m, int n, float v) {
+          const int i = m + n * kp;
           float pv = 0.f;
-          if (m < k && n < k) pv = (m == n ? ca : 0.f) + cb * G[m + (int64_t)n * kp] + cc * v;
-          P[m + (int64_t)n * kp] = pv;
-          if (mirror) P[n + (int64_t)m * kp] = pv;
+          if (m < k && n < k) pv = (m == n ? ca : 0.f) + cb * get_split(G, i) + cc * v;
+          put_split(P, i, pv);
         });
       } else {
         for (int64_t e = threadIdx.x; e < (int64_t)kp * kp; e += NT) {
-          int m = (int)(e % kp), n = (int)(e / kp);
+          const int m = (int)(e % kp), n = (int)(e / kp);
           float pv = 0.f;
-          if (m < k && n < k) pv = (m == n ? ca : 0.f) + cb * G[e];
-          P[e] = pv;
+          if (m < k && n < k) pv = (m == n ? ca : 0.f) + cb * get_split(G, e);
+          put_split(P, e, pv);
         }
       }
       __syncthreads();
-      __threadfence_block();
-      // Xn = X P (the final product goes straight to the Q store)
-      float *dst = finish ? Q : Xn;
+      // Xn = X P; the finishing product goes to the FP32 Q store
       {
-        Opnd ox = {X, nullptr, kp, kp, 1};
-        Opnd op = {P, nullptr, kp, kp, 0};
-        wg_gemm<TILE>(sm, ox, op, kp, kp, false, [&](int m, int n, float v, bool) {
-          dst[m + (int64_t)n * kp] = v;
-        });
+        const Opnd ox = opnd_split(X, OP_SPLIT_M, kp, kp);
+        const Opnd op = opnd_split(P, OP_SPLIT_K, kp, kp);
+        if (finish) {
+          wg_gemm<TILE, OP_SPLIT_M, OP_SPLIT_K>(sm, ox, op, kp, kp, false, [&](int m, int n, float v) {
+            Q[m + n * kp] = v;
+          });
+        } else {
+          wg_gemm<TILE, OP_SPLIT_M, OP_SPLIT_K>(sm, ox, op, kp, kp, false, [&](int m, int n, float v) {
+            put_split(Xn, m + n * kp, v);
+          });
+        }
       }
       __syncthreads();
-      __threadfence_block();
-      if (finish) { done = true; ++step; break; }
-      float *t = X; X = Xn; Xn = t;
+      if (finish) {
+        done = true;
+        ++step;
+        break;
+      }
+      float *t = Xb;
+      Xb = Xnb;
+      Xnb = t;
     }
     if (!done) {
-      // budget exhausted or non-finite: FP64 Jacobi takes over (Q written later)
+      // budget exhausted or non-finite: the FP64 Jacobi pass takes over (writes Q)
       if (threadIdx.x == 0) a.flag[s] = 1;
     } else if (a.sigma_sum) {
       double tr = 0.0;
@@ -637,7 +463,7 @@ __device__ double jacobi_compose(T *Am, const T *Vm, int k, int kp, TO *Q, int q
 
 template <int TILE>
 __global__ __launch_bounds__(NT) void k_jacobi32(HakLocalArgs a) {
-  __shared__ TileSmem<TILE> sm;
+  __shared__ GemmSmem<TILE> sm;
   __shared__ double red[4];
   __shared__ int32_t item_sh, flag_sh, rank_sh;
   const int slot = blockIdx.x;
