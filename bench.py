@@ -87,10 +87,9 @@ def main():
           f"n={nt} (generated in {time.perf_counter() - t0:.1f}s)", file=sys.stderr, flush=True)
     ha.init()
     ha.set_options(method=args.method, tile=args.tile)
-    model = os.path.join(ROOT, "hyperalignment_amd", "model", "polar_mlp.npz")
-    if os.path.exists(model):
-        z = np.load(model)
-        ha.set_model(z["params"], z["norm"])
+    from hyperalignment_amd import learn
+    if os.path.exists(learn.MODEL_PATH):
+        learn.install(learn.MODEL_PATH)
     plan = ha.Plan(nt, s.nv, s.indices, s.offsets, s.dists, s.radius)
     plan.load(w.X, w.Y)
     print(f"[bench] plan ready: nnz {plan.nnz()}, pattern {plan.stats()['ms_pattern']:.1f} ms",

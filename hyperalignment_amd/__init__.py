@@ -111,7 +111,51 @@ _SIGS = {
     "ha_hip_plan_download": (ctypes.c_int, [ctypes.c_void_p, _P(_i32), _P(_i32), _P(_d)]),
     "ha_hip_plan_download_local": (ctypes.c_int, [ctypes.c_void_p, _P(_i32), _i32, _P(_f)]),
     "ha_hip_plan_stats": (ctypes.c_int, [ctypes.c_void_p, _P(HaHipStats)]),
+    "ha_hip_plan_spectra": (ctypes.c_int, [ctypes.c_void_p, _P(_i32), _i32, _P(_f), _P(_f)]),
+    "ha_hip_plan_steps": (ctypes.c_int, [ctypes.c_void_p, _P(_i32), _P(_i32)]),
+    "ha_hip_plan_predictions": (ctypes.c_int, [ctypes.c_void_p, _P(_f)]),
+    "ha_hip_mlp_forward": (ctypes.c_int, [_P(_f), _P(_f), _P(_f), ctypes.c_int64, _P(_f)]),
+    "ha_hip_mlp_grad": (ctypes.c_int, [_P(_f), _P(_f), _P(_f), _P(_f), ctypes.c_int64, _f, _f, _P(_f), _P(_f)]),
+    "ha_hip_mlp_train": (ctypes.c_int, [_P(_f), _P(_f), _P(_f), _P(_f), ctypes.c_int64, _i32, _i32, _f, _f, _f,
+                                        ctypes.c_uint64, _P(_f)]),
 }
+
+
+def _f32(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if shape is not None and a.size != int(np.prod(shape)):
+        raise ValueError(f"expected {shape}, got {a.shape}")
+    return a
+
+
+def mlp_forward(params, norm, X):
+    """Batch MLP inference on the GPU: (N, 12) features -> (N, 2)."""
+    p, n, x = _f32(params, (MLP_NPARAM,)), _f32(norm, (2 * MLP_IN,)), _f32(X)
+    y = np.zeros((x.shape[0], MLP_OUT), dtype=np.float32)
+    _check("ha_hip_mlp_forward", lib().ha_hip_mlp_forward(_ptr(p, _f), _ptr(n, _f), _ptr(x, _f), x.shape[0],
+                                                          _ptr(y, _f)))
+    return y
+
+
+def mlp_grad(params, norm, X, Y, w=(1.0, 1.0)):
+    """Full-batch loss gradient on the GPU; returns (grad, loss)."""
+    p, n, x, y = _f32(params, (MLP_NPARAM,)), _f32(norm, (2 * MLP_IN,)), _f32(X), _f32(Y)
+    g = np.zeros(MLP_NPARAM, dtype=np.float32)
+    loss = np.zeros(1, dtype=np.float32)
+    _check("ha_hip_mlp_grad", lib().ha_hip_mlp_grad(_ptr(p, _f), _ptr(n, _f), _ptr(x, _f), _ptr(y, _f), x.shape[0],
+                                                    float(w[0]), float(w[1]), _ptr(g, _f), _ptr(loss, _f)))
+    return g, float(loss[0])
+
+
+def mlp_train(params, norm, X, Y, epochs=20, batch=4096, lr=1e-3, w=(1.0, 1.0), seed=0):
+    """Adam training on the GPU; returns (trained params, per-epoch loss)."""
+    p = _f32(params, (MLP_NPARAM,)).copy()
+    n, x, y = _f32(norm, (2 * MLP_IN,)), _f32(X), _f32(Y)
+    hist = np.zeros(max(epochs, 1), dtype=np.float32)
+    _check("ha_hip_mlp_train", lib().ha_hip_mlp_train(_ptr(p, _f), _ptr(n, _f), _ptr(x, _f), _ptr(y, _f),
+                                                      x.shape[0], int(epochs), int(batch), float(lr), float(w[0]),
+                                                      float(w[1]), int(seed), _ptr(hist, _f)))
+    return p, hist[:epochs]
 
 
 def lib():
@@ -380,6 +424,28 @@ class Plan:
             res.append(out[o:o + k * k].reshape(k, k))
             o += k * k
         return res
+
+    def spectra(self, which):
+        """(features (n, 12), [sigma_max, sigma_min] (n, 2)) from the FP32 Jacobi path."""
+        which = np.ascontiguousarray(which, dtype=np.int32)
+        f = np.zeros((len(which), MLP_IN), dtype=np.float32)
+        s = np.zeros((len(which), 2), dtype=np.float32)
+        _check("ha_hip_plan_spectra", lib().ha_hip_plan_spectra(self.h, _ptr(which, ctypes.c_int32), len(which),
+                                                                _ptr(f, ctypes.c_float), _ptr(s, ctypes.c_float)))
+        return f, s
+
+    def steps(self):
+        """(steps, fallback flags) per searchlight of the last pass."""
+        st = np.zeros(self.count, dtype=np.int32)
+        fl = np.zeros(self.count, dtype=np.int32)
+        _check("ha_hip_plan_steps", lib().ha_hip_plan_steps(self.h, _ptr(st, ctypes.c_int32),
+                                                            _ptr(fl, ctypes.c_int32)))
+        return st, fl
+
+    def predictions(self):
+        p = np.zeros((self.count, 2), dtype=np.float32)
+        _check("ha_hip_plan_predictions", lib().ha_hip_plan_predictions(self.h, _ptr(p, ctypes.c_float)))
+        return p
 
     def stats(self):
         s = HaHipStats()

@@ -29,6 +29,7 @@ static struct {
 	int n_cu;
 	HaHipOptions opt;
 	float *d_sched;
+	int32_t *d_sched_len;
 	float *d_mlp;
 	float *d_mlp_norm;
 	bool has_model;
@@ -83,18 +84,23 @@ static int build_schedules(void) {
 	int rc = kHaSuccess;
 	size_t n = (size_t)HA_SCHED_BUCKETS * HA_SCHED_STEPS * 3;
 	float *h = (float *)malloc(n * sizeof(float));
+	int32_t len[HA_SCHED_BUCKETS];
 	if (!h) return kHaErrorAlloc;
 	for (int b = 0; b < HA_SCHED_BUCKETS; ++b) {
 		double ell = pow(10.0, -(double)b / HA_SCHED_PER_DECADE);
 		if (b == 0) ell = 0.96; /* already converged: quintic Newton-Schulz */
-		if (ha_polar_schedule(ell, HA_SCHED_STEPS, h + (size_t)b * HA_SCHED_STEPS * 3) < 0) {
+		int gl = 0;
+		if (ha_polar_schedule(ell, HA_SCHED_STEPS, h + (size_t)b * HA_SCHED_STEPS * 3, &gl) < 0) {
 			ERR("polar schedule for ell=%g failed\n", ell);
 			free(h);
 			return kHaErrorInternal;
 		}
+		len[b] = gl;
 	}
 	if (!g.d_sched) ACK(g.d_sched, n * sizeof(float));
+	if (!g.d_sched_len) ACK(g.d_sched_len, sizeof(len));
 	HCK(hipMemcpy(g.d_sched, h, n * sizeof(float), hipMemcpyHostToDevice));
+	HCK(hipMemcpy(g.d_sched_len, len, sizeof(len), hipMemcpyHostToDevice));
 fail:
 	free(h);
 	return rc;
@@ -126,6 +132,8 @@ void ha_hip_cleanup(void) {
 	if (!g.ready) return;
 	hipStreamSynchronize(g.stream);
 	if (g.d_sched) hipFree(g.d_sched);
+	if (g.d_sched_len) hipFree(g.d_sched_len);
+	g.d_sched_len = NULL;
 	if (g.d_mlp) hipFree(g.d_mlp);
 	if (g.d_mlp_norm) hipFree(g.d_mlp_norm);
 	g.d_sched = g.d_mlp = g.d_mlp_norm = NULL;
@@ -210,7 +218,8 @@ struct HaHipPlan {
 	double *work64; int64_t stride64; int32_t n_slots64;
 	int32_t *flag, *steps, *queues, *order;
 	double *sigma_sum, *var_sum, *scale;
-	float *features, *pred, *trace;
+	float *features, *pred, *trace, *sig;
+	int32_t *d_which;
 	HaHipStats stats;
 	hipEvent_t ev[8];
 	bool events;
@@ -223,7 +232,7 @@ static void plan_free(HaHipPlan *p) {
 	                p->occx_ptr, p->occx, p->wsum, p->w, p->indptr64, p->indptr, p->indices, p->data,
 	                p->qld, p->qoff, p->q, p->q64, p->X, p->Y, p->X64, p->Y64, p->stage, p->work, p->work64,
 	                p->flag, p->steps, p->queues, p->order, p->sigma_sum, p->var_sum, p->scale,
-	                p->features, p->pred, p->trace};
+	                p->features, p->pred, p->trace, p->sig, p->d_which};
 	for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i)
 		if (ptrs[i]) hipFree(ptrs[i]);
 	if (p->events)
@@ -556,6 +565,7 @@ static HakLocalArgs local_args(HaHipPlan *p) {
 	a.flag = p->flag; a.steps = p->steps;
 	a.work = p->work; a.slot_stride = p->slot_stride; a.n_slots = p->n_slots;
 	a.sched = g.d_sched;
+	a.sched_len = g.d_sched_len;
 	a.n_buckets = HA_SCHED_BUCKETS; a.max_sched = HA_SCHED_STEPS;
 	a.bucket_per_decade = HA_SCHED_PER_DECADE;
 	a.max_steps = g.opt.max_polar_steps;
@@ -567,6 +577,7 @@ static HakLocalArgs local_args(HaHipPlan *p) {
 	a.rank_check_nt = g.opt.fp64_fallback;
 	a.mlp_margin_scale = g.opt.learned_margin_scale;
 	a.mlp_margin_ell = g.opt.learned_margin_ell;
+	a.mlp_fallback_log10 = -6.0f; /* sigma_min / sigma_max < 1e-6: FP64 Jacobi */
 	return a;
 }
 
@@ -629,6 +640,8 @@ int ha_plan_run(HaHipPlan *p, bool isReflection, bool isScaling, bool zero_data,
 	if (method == kHaHipMethodPolarLearned) {
 		a.mlp = g.d_mlp;
 		a.mlp_norm = g.d_mlp_norm;
+		if (!p->pred) ACK(p->pred, (size_t)p->count * 2 * sizeof(float));
+		a.pred_out = p->pred;
 	}
 	HCK(hipEventRecord(p->ev[0], st));
 	KCK(hak_fill_i32(p->queues, 8, 0, st));
@@ -839,5 +852,73 @@ fail:
 	if (q64) hipFree(q64);
 	free(h);
 	free(qoff);
+	return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* learned-scaling support: spectra, per-searchlight steps and predictions   */
+/* ------------------------------------------------------------------------ */
+
+/* FP32 Jacobi on the searchlights `which`: HAK_NFEAT features and
+ * (sigma_max, sigma_min) of A = X_sl^T Y_sl for each, in the order given. */
+int ha_hip_plan_spectra(HaHipPlan *p, const int32_t *which, int32_t n, float *features, float *sig) {
+	int rc = kHaSuccess;
+	if (!p || !p->loaded || !which || n <= 0 || n > p->count) return kHaErrorArg;
+	for (int32_t i = 0; i < n; ++i)
+		if (which[i] < 0 || which[i] >= p->count) return kHaErrorArg;
+	hipStream_t st = g.stream;
+	float *hf = NULL, *hs = NULL;
+	if (!p->features) ACK(p->features, (size_t)p->count * HAK_NFEAT * sizeof(float));
+	if (!p->sig) ACK(p->sig, (size_t)p->count * 2 * sizeof(float));
+	if (p->d_which) { hipFree(p->d_which); p->d_which = NULL; }
+	ACK(p->d_which, (size_t)n * sizeof(int32_t));
+	HCK(hipMemcpyAsync(p->d_which, which, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, st));
+	HakLocalArgs a = local_args(p);
+	a.order = p->d_which;
+	a.n_order = n;
+	a.features = p->features;
+	a.sig_out = p->sig;
+	a.fallback_enabled = 0;
+	a.rank_check_nt = 0;
+	a.queue = p->queues + 4;
+	KCK(hak_fill_i32(p->queues, 8, 0, st));
+	KCK(hak_jacobi32(&a, st));
+	hf = (float *)malloc((size_t)p->count * HAK_NFEAT * sizeof(float));
+	hs = (float *)malloc((size_t)p->count * 2 * sizeof(float));
+	if (!hf || !hs) { rc = kHaErrorAlloc; goto fail; }
+	HCK(hipMemcpyAsync(hf, p->features, (size_t)p->count * HAK_NFEAT * sizeof(float), hipMemcpyDeviceToHost, st));
+	HCK(hipMemcpyAsync(hs, p->sig, (size_t)p->count * 2 * sizeof(float), hipMemcpyDeviceToHost, st));
+	HCK(hipStreamSynchronize(st));
+	for (int32_t i = 0; i < n; ++i) {
+		if (features) memcpy(features + (size_t)i * HAK_NFEAT, hf + (size_t)which[i] * HAK_NFEAT, HAK_NFEAT * sizeof(float));
+		if (sig) { sig[2 * i] = hs[2 * which[i]]; sig[2 * i + 1] = hs[2 * which[i] + 1]; }
+	}
+fail:
+	hipStreamSynchronize(st);
+	free(hf);
+	free(hs);
+	return rc;
+}
+
+/* Per-searchlight outcome of the last pass: steps (polar steps, Jacobi sweeps;
+ * negative = budget exhausted, >= 1000 = FP64 path) and fallback flags. */
+int ha_hip_plan_steps(const HaHipPlan *p, int32_t *steps, int32_t *flags) {
+	int rc = kHaSuccess;
+	if (!p) return kHaErrorArg;
+	if (steps) HCK(hipMemcpy(steps, p->steps, (size_t)p->count * sizeof(int32_t), hipMemcpyDeviceToHost));
+	if (flags) HCK(hipMemcpy(flags, p->flag, (size_t)p->count * sizeof(int32_t), hipMemcpyDeviceToHost));
+	return kHaSuccess;
+fail:
+	return rc;
+}
+
+/* MLP predictions (y0, y1 per searchlight) of the last learned pass. */
+int ha_hip_plan_predictions(const HaHipPlan *p, float *pred) {
+	int rc = kHaSuccess;
+	if (!p || !pred) return kHaErrorArg;
+	if (!p->pred) { ERR("no learned pass has run on this plan\n"); return kHaErrorArg; }
+	HCK(hipMemcpy(pred, p->pred, (size_t)p->count * 2 * sizeof(float), hipMemcpyDeviceToHost));
+	return kHaSuccess;
+fail:
 	return rc;
 }

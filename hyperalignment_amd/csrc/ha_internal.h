@@ -14,8 +14,10 @@ extern "C" {
 #endif
 
 /* Fills out[max_steps][3] with the polar coefficient schedule for an initial
- * lower bound ell0 on sigma_min / scale. Returns the step count or -1. */
-int ha_polar_schedule(double ell0, int max_steps, float *out);
+ * lower bound ell0 on sigma_min / scale; growth_len (may be NULL) receives the
+ * number of minimax steps before the Newton-Schulz tail. Returns the step
+ * count or -1. */
+int ha_polar_schedule(double ell0, int max_steps, float *out, int *growth_len);
 
 /* Schedule table geometry shared by host and kernels. */
 #define HA_SCHED_BUCKETS 37           /* ell_b = 10^(-b/4), b = 0..36 (down to 1e-9) */

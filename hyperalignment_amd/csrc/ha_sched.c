@@ -86,13 +86,15 @@ static double remez5(double l, double *a, double *b, double *c) {
 	return E;
 }
 
-int ha_polar_schedule(double ell0, int max_steps, float *out) {
+int ha_polar_schedule(double ell0, int max_steps, float *out, int *growth_len) {
 	double l = ell0 > 0 ? ell0 : 1e-12;
 	double la = 0, lb = 0, lc = 0;
 	int have = 0, n = 0;
+	if (growth_len) *growth_len = max_steps;
 	for (int t = 0; t < max_steps; ++t) {
 		double a, b, c, E;
 		if (1.0 - l < 0.05) {
+			if (growth_len && *growth_len == max_steps) *growth_len = t;
 			a = 15.0 / 8; b = -10.0 / 8; c = 3.0 / 8;
 			double x = l;
 			l = (15 * x - 10 * x * x * x + 3 * x * x * x * x * x) / 8;

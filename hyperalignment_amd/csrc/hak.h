@@ -98,6 +98,7 @@ typedef struct {
 	/* polar schedule: sched[n_buckets][max_sched][3]; bucket b assumes
 	 * sigma_min / scale >= 10^(-b / bucket_per_decade) */
 	const float *sched;
+	const int32_t *sched_len;            /* per bucket: minimax steps before the Newton-Schulz tail */
 	int32_t n_buckets, max_sched, max_steps, default_bucket;
 	float bucket_per_decade;
 	float tol;                           /* finish when ||X^T X - I||_F < tol */
@@ -106,9 +107,11 @@ typedef struct {
 	/* learned scaling (NULL mlp = off) */
 	const float *mlp; const float *mlp_norm;
 	float mlp_margin_scale, mlp_margin_ell;
+	float mlp_fallback_log10;            /* predicted log10(sigma_min/sigma_max) below this -> FP64 path */
 	float *pred_out;                     /* per searchlight 2 predictions (may be NULL) */
 	float *features;                     /* per searchlight HAK_NFEAT features (may be NULL) */
 	float *trace;                        /* per searchlight max_steps residuals ||G - I||_F (may be NULL) */
+	float *sig_out;                      /* Jacobi paths: per searchlight (sigma_max, sigma_min) (may be NULL) */
 	int32_t tile;
 	int32_t fallback_enabled;
 	int32_t rank_check_nt;               /* flag k >= nt as rank deficient */
@@ -121,6 +124,19 @@ int hak_jacobi32(const HakLocalArgs *a, void *stream);
  * correction when want_det_fix. work64: n_slots * 4 * stride64 doubles. */
 int hak_jacobi64(const HakLocalArgs *a, int want_det_fix, double *work64, int64_t stride64,
                  int32_t n_slots, int32_t count, void *stream);
+
+/* ---- spectral-scaling MLP (k_mlp.hip) ---- */
+/* Y[N][2] = MLP(X[N][12]) with the given parameters and input normalisation. */
+int hak_mlp_forward(const float *params, const float *norm, const float *X, int64_t N, float *Y, void *stream);
+/* One minibatch step over rows idx[0..nb) (identity when NULL): partial
+ * gradients per 64-row chunk (partial: ceil(nb/64) * NPARAM floats, loss_part:
+ * ceil(nb/64)), deterministic reduction, then Adam with learning rate lr
+ * (lr == 0: gradient only, into grad_out when not NULL). loss_out receives the
+ * batch loss sum_j w_j (out_j - y_j)^2 averaged over the batch. */
+int hak_mlp_grad_step(float *params, float *adam_m, float *adam_v, const float *norm, const float *X,
+                      const float *Y, const int32_t *idx, int64_t nb, float w0, float w1, float lr,
+                      int step, float *partial, float *loss_part, float *loss_out, float *grad_out,
+                      void *stream);
 
 /* Per-searchlight sum of population variances of X columns (FP64), for scaling. */
 int hak_var_sum(const double *X64, int32_t nt, const int32_t *slx_idx, const int32_t *sl_off,

@@ -224,6 +224,14 @@ __global__ __launch_bounds__(NT, TILE == 128 ? 1 : 2) void k_polar(HakLocalArgs 
           a.pred_out[2 * s] = pred[0];
           a.pred_out[2 * s + 1] = pred[1];
         }
+        if (pred[1] < a.mlp_fallback_log10 && a.fallback_enabled) {
+          // predicted too ill-conditioned for the FP32-accurate iteration
+          if (threadIdx.x == 0) {
+            a.flag[s] = 1;
+            if (a.steps) a.steps[s] = 0;
+          }
+          continue;
+        }
       }
     } else if (a.sl_scale) {
       scale = a.sl_scale[s];
@@ -244,7 +252,7 @@ __global__ __launch_bounds__(NT, TILE == 128 ? 1 : 2) void k_polar(HakLocalArgs 
     __syncthreads();
     float *Xb = w.X, *Xnb = w.X2;
     const SplitM G = as_split(w.G, kp), P = as_split(w.P, kp);
-    int step = 0;
+    int step = 0, restart = 0, restarts = 0;
     bool done = false;
     for (step = 0; step < a.max_steps; ++step) {
       const SplitM X = as_split(Xb, kp), Xn = as_split(Xnb, kp);
@@ -265,11 +273,23 @@ __global__ __launch_bounds__(NT, TILE == 128 ? 1 : 2) void k_polar(HakLocalArgs 
       if (a.trace && threadIdx.x == 0) a.trace[(int64_t)s * a.max_steps + step] = (float)r;
       if (!isfinite(r)) break;
       const bool finish = (r < a.tol);
+      int ts = step - restart;
+      if (!finish && ts >= a.sched_len[bucket] && r > 0.05 && restarts < 2) {
+        // The growth phase of the schedule is over but some singular values lag:
+        // with r = ||G - I||_F >= 1 - s^2 for every lagging value s,
+        // sqrt(1 - r) bounds them from below. Restart from that bucket.
+        const double s_lo = r < 1.0 ? sqrt(1.0 - r) : 1e-3;
+        const int b = (int)ceil(-log10(fmax(0.5 * s_lo, 1e-9)) * a.bucket_per_decade);
+        bucket = b < 1 ? 1 : (b >= a.n_buckets ? a.n_buckets - 1 : b);
+        restart = step;
+        ++restarts;
+        ts = 0;
+      }
       float ca, cb, cc;
       if (finish) {
         ca = 1.5f; cb = -0.5f; cc = 0.f;
-      } else if (step < a.max_sched) {
-        const float *co = a.sched + ((int64_t)bucket * a.max_sched + step) * 3;
+      } else if (ts < a.max_sched) {
+        const float *co = a.sched + ((int64_t)bucket * a.max_sched + ts) * 3;
         ca = co[0]; cb = co[1]; cc = co[2];
       } else {
         ca = 1.875f; cb = -1.25f; cc = 0.375f;  // quintic Newton-Schulz
@@ -357,12 +377,15 @@ __device__ __forceinline__ T grp16_sum(T v) {
   return v;
 }
 
+// Rotation threshold |a_p . a_q| > tol * |a_p| |a_q| with tol = 4 eps sqrt(k):
+// the rounding level of the computed dot products, so the last sweep finds
+// nothing left to rotate instead of chasing noise.
 template <typename T>
 struct JacobiTol;
 template <>
-struct JacobiTol<float> { static constexpr float tol = 1.2e-7f; static constexpr int sweeps = 30; };
+struct JacobiTol<float> { static constexpr double eps = 1.19e-7; static constexpr int sweeps = 30; };
 template <>
-struct JacobiTol<double> { static constexpr double tol = 2.3e-16; static constexpr int sweeps = 40; };
+struct JacobiTol<double> { static constexpr double eps = 2.22e-16; static constexpr int sweeps = 30; };
 
 // Orthogonalises the k columns of Am (col-major, ld kp) while accumulating the
 // rotations in Vm (starts as identity). Returns the number of sweeps used
@@ -371,6 +394,7 @@ template <typename T>
 __device__ int jacobi_sweeps(T *Am, T *Vm, int k, int kp, int32_t *flag_sh) {
   const int n = k + (k & 1);
   const int grp = threadIdx.x >> 4, gl = threadIdx.x & 15;
+  const double tol = 4.0 * JacobiTol<T>::eps * sqrt((double)k);
   int sweep;
   for (sweep = 0; sweep < JacobiTol<T>::sweeps; ++sweep) {
     if (threadIdx.x == 0) *flag_sh = 0;
@@ -389,7 +413,7 @@ __device__ int jacobi_sweeps(T *Am, T *Vm, int k, int kp, int32_t *flag_sh) {
               al += x * x; be += y * y; ga += x * y;
             }
             al = grp16_sum(al); be = grp16_sum(be); ga = grp16_sum(ga);
-            if (fabs((double)ga) > (double)JacobiTol<T>::tol * sqrt((double)al * (double)be) && ga != (T)0) {
+            if (fabs((double)ga) > tol * sqrt((double)al * (double)be) && ga != (T)0) {
               T zeta = (be - al) / (2 * ga);
               T tt = (zeta >= 0 ? (T)1 : (T)-1) / (fabs(zeta) + sqrt((T)1 + zeta * zeta));
               T c = (T)1 / sqrt((T)1 + tt * tt), sn = c * tt;
@@ -419,11 +443,52 @@ __device__ int jacobi_sweeps(T *Am, T *Vm, int k, int kp, int32_t *flag_sh) {
   return -sweep;
 }
 
+// Q(m, n) = sum_{j < k} U(m, j) V(n, j) - [flipj >= 0] 2 U(m, flipj) V(n, flipj)
+// for m, n < kp (zero outside k x k). U, V column-major with leading dim kp.
+// 32 x 32 output tiles with 32-column chunks of U and V staged in LDS.
+template <typename T, typename TO>
+__device__ void compose_uvt(const T *U, const T *V, int k, int kp, TO *Q, double *Q64, T (*us)[33], T (*vs)[33],
+                            int flipj) {
+  const int mi = threadIdx.x & 31, ni = threadIdx.x >> 5;
+  for (int tm = 0; tm < kp; tm += 32)
+    for (int tn = 0; tn < kp; tn += 32) {
+      T acc[4] = {0, 0, 0, 0};
+      for (int j0 = 0; j0 < k; j0 += 32) {
+        for (int e = threadIdx.x; e < 32 * 32; e += NT) {
+          const int c = e & 31, jj = e >> 5;
+          const bool okj = j0 + jj < k;
+          us[c][jj] = (okj && tm + c < k) ? U[(tm + c) + (int64_t)(j0 + jj) * kp] : (T)0;
+          vs[c][jj] = (okj && tn + c < k) ? V[(tn + c) + (int64_t)(j0 + jj) * kp] : (T)0;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int jj = 0; jj < 32; ++jj) {
+          const T u = us[mi][jj];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) acc[q] += u * vs[ni + 8 * q][jj];
+        }
+        __syncthreads();
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int m = tm + mi, n = tn + ni + 8 * q;
+        T v = 0;
+        if (m < k && n < k) {
+          v = acc[q];
+          if (flipj >= 0) v -= (T)2 * U[m + (int64_t)flipj * kp] * V[n + (int64_t)flipj * kp];
+        }
+        Q[m + (int64_t)n * kp] = (TO)v;
+        if (Q64) Q64[m + (int64_t)n * kp] = (double)v;
+      }
+    }
+  __syncthreads();
+}
+
 // Q = sum_j (a_j / |a_j|) v_j^T over columns with |a_j| > cut; returns sum |a_j|.
 // Columns with |a_j| <= cut are reported through *rank_def.
 template <typename T, typename TO>
 __device__ double jacobi_compose(T *Am, const T *Vm, int k, int kp, TO *Q, int qld, T *sig, int32_t *rank_sh,
-                                 double *red) {
+                                 double *red, T (*us)[33], T (*vs)[33]) {
   // singular values and normalised U columns
   if (threadIdx.x == 0) *rank_sh = 0;
   __syncthreads();
@@ -447,15 +512,7 @@ __device__ double jacobi_compose(T *Am, const T *Vm, int k, int kp, TO *Q, int q
     Am[e] = (sj > cut) ? Am[e] / sj : (T)0;
   }
   __syncthreads();
-  // Q(m, n) = sum_j U(m, j) V(n, j)
-  for (int64_t e = threadIdx.x; e < (int64_t)qld * qld; e += NT) {
-    int m = (int)(e % qld), n = (int)(e / qld);
-    T acc = 0;
-    if (m < k && n < k)
-      for (int j = 0; j < k; ++j) acc += Am[m + (int64_t)j * kp] * Vm[n + (int64_t)j * kp];
-    Q[e] = (TO)acc;
-  }
-  __syncthreads();
+  compose_uvt<T, TO>(Am, Vm, k, kp, Q, nullptr, us, vs, -1);
   double ssum = 0.0;
   for (int j = threadIdx.x; j < k; j += NT) ssum += (double)sig[j];
   return wg_sum_f64(ssum, red);
@@ -466,6 +523,8 @@ __global__ __launch_bounds__(NT) void k_jacobi32(HakLocalArgs a) {
   __shared__ GemmSmem<TILE> sm;
   __shared__ double red[4];
   __shared__ int32_t item_sh, flag_sh, rank_sh;
+  __shared__ float feat_sh[HAK_NFEAT];
+  __shared__ float us32[32][33], vs32[32][33];
   const int slot = blockIdx.x;
   WorkSlots w = slot_buffers(a, slot);
   for (;;) {
@@ -481,6 +540,11 @@ __global__ __launch_bounds__(NT) void k_jacobi32(HakLocalArgs a) {
     // cross-covariance in bf16x3 MFMA (same kernel as the polar path), then Jacobi in FP32
     crosscov<TILE>(sm, a, s, k, kp, w.A);
     __syncthreads();
+    if (a.features) {
+      features_of(a, s, k, kp, w.A, w.P, feat_sh, red);
+      if (threadIdx.x < HAK_NFEAT) a.features[(int64_t)s * HAK_NFEAT + threadIdx.x] = feat_sh[threadIdx.x];
+      __syncthreads();
+    }
     if (a.sigma_sum)  // keep A for trace(Q^T A)
       for (int64_t e = threadIdx.x; e < (int64_t)kp * kp; e += NT) w.X2[e] = w.A[e];
     for (int64_t e = threadIdx.x; e < (int64_t)kp * kp; e += NT) {
@@ -490,7 +554,16 @@ __global__ __launch_bounds__(NT) void k_jacobi32(HakLocalArgs a) {
     __syncthreads();
     int sweeps = jacobi_sweeps<float>(w.A, w.G, k, kp, &flag_sh);
     float *Q = a.q + a.qoff[s];
-    jacobi_compose<float, float>(w.A, w.G, k, kp, Q, kp, w.P, &rank_sh, red);
+    jacobi_compose<float, float>(w.A, w.G, k, kp, Q, kp, w.P, &rank_sh, red, us32, vs32);
+    if (a.sig_out && threadIdx.x == 0) {
+      float smax = 0.f, smin = 3.0e38f;
+      for (int j = 0; j < k; ++j) {
+        smax = fmaxf(smax, w.P[j]);
+        smin = fminf(smin, w.P[j]);
+      }
+      a.sig_out[2 * s] = smax;
+      a.sig_out[2 * s + 1] = smin;
+    }
     if (a.sigma_sum) {
       double tr = 0.0;
       for (int64_t e = threadIdx.x; e < (int64_t)kp * kp; e += NT) tr += (double)Q[e] * w.X2[e];
@@ -523,6 +596,8 @@ __global__ __launch_bounds__(NT) void k_jacobi64(HakLocalArgs a, int want_det_fi
   __shared__ int32_t item_sh, flag_sh, rank_sh, sign_sh, piv_sh;
   __shared__ double pmax_sh[4];
   __shared__ int32_t pidx_sh[4];
+  __shared__ double xs64[32][33], ys64[32][33];
+  __shared__ double wdot[1024];
   double *Am = work64 + (int64_t)blockIdx.x * 4 * stride64;
   double *Vm = Am + stride64;
   double *Lm = Am + 2 * stride64;
@@ -534,18 +609,35 @@ __global__ __launch_bounds__(NT) void k_jacobi64(HakLocalArgs a, int want_det_fi
     const int o = a.sl_off[s];
     const int k = a.sl_off[s + 1] - o;
     const int kp = a.qld[s];
-    // A = X64[:, slx]^T Y64[:, sly] in FP64
-    for (int64_t e = threadIdx.x; e < (int64_t)kp * kp; e += NT) {
-      int m = (int)(e % kp), n = (int)(e / kp);
-      double acc = 0.0;
-      if (m < k && n < k) {
-        const double *xc = a.X64 + (int64_t)a.slx_idx[o + m] * a.nt;
-        const double *yc = a.Y64 + (int64_t)a.sly_idx[o + n] * a.nt;
-        for (int t = 0; t < a.nt; ++t) acc += xc[t] * yc[t];
+    // A = X64[:, slx]^T Y64[:, sly] in FP64: 32 x 32 output tiles, 32-sample
+    // chunks of the gathered columns staged in LDS (coalesced column reads)
+    for (int tm = 0; tm < kp; tm += 32)
+      for (int tn = 0; tn < kp; tn += 32) {
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        const int mi = threadIdx.x & 31, ni = threadIdx.x >> 5;  // ni in 0..7
+        for (int t0 = 0; t0 < a.nt; t0 += 32) {
+          for (int e = threadIdx.x; e < 32 * 32; e += NT) {
+            const int c = e >> 5, t = t0 + (e & 31);
+            const int gm = tm + c, gn = tn + c;
+            xs64[c][e & 31] = (gm < k && t < a.nt) ? a.X64[(int64_t)a.slx_idx[o + gm] * a.nt + t] : 0.0;
+            ys64[c][e & 31] = (gn < k && t < a.nt) ? a.Y64[(int64_t)a.sly_idx[o + gn] * a.nt + t] : 0.0;
+          }
+          __syncthreads();
+#pragma unroll 4
+          for (int t = 0; t < 32; ++t) {
+            const double xv = xs64[mi][t];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc[q] += xv * ys64[ni + 8 * q][t];
+          }
+          __syncthreads();
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int m = tm + mi, n = tn + ni + 8 * q;
+          Am[m + (int64_t)n * kp] = acc[q];
+          Vm[m + (int64_t)n * kp] = (m == n && m < k) ? 1.0 : 0.0;
+        }
       }
-      Am[e] = acc;
-      Vm[e] = (m == n && m < k) ? 1.0 : 0.0;
-    }
     __syncthreads();
     if (want_det_fix) {
       // sign(det Q) = sign(det A) for nonsingular A: LU with partial pivoting on a copy
@@ -615,28 +707,48 @@ __global__ __launch_bounds__(NT) void k_jacobi64(HakLocalArgs a, int want_det_fi
       Am[e] = (sig[j] > cut) ? Am[e] / sig[j] : 0.0;
     }
     __syncthreads();
-    // complete U for deficient columns: Gram-Schmidt of unit vectors against
-    // all current columns (sequential over deficient columns, two passes)
+    // Complete U for columns with zero singular value (rank-deficient A): each
+    // deficient column starts as a fixed pseudo-random vector, is projected off
+    // every finished column twice (classical Gram-Schmidt, all dot products in
+    // parallel), and normalised. Finished = full-rank columns plus the deficient
+    // columns completed before it.
     for (int j = 0; j < k; ++j) {
       if (sig[j] > cut) continue;
       double *uj = Am + (int64_t)j * kp;
-      for (int cand = 0; cand < k; ++cand) {
-        for (int t = threadIdx.x; t < k; t += NT) uj[t] = (t == cand) ? 1.0 : 0.0;
+      for (int attempt = 0; attempt < 4; ++attempt) {
+        for (int t = threadIdx.x; t < k; t += NT) {
+          uint64_t z = ((uint64_t)j << 32) ^ ((uint64_t)t * 0x9E3779B97F4A7C15ULL) ^ ((uint64_t)attempt << 20);
+          z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+          z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+          z ^= z >> 31;
+          uj[t] = (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+        }
         __syncthreads();
-        for (int pass = 0; pass < 2; ++pass)
-          for (int i = 0; i < k; ++i) {
-            if (i == j) continue;
-            const double *ui = Am + (int64_t)i * kp;
-            double dot = 0.0;
-            for (int t = threadIdx.x; t < k; t += NT) dot += ui[t] * uj[t];
-            dot = wg_sum_f64(dot, red);
-            for (int t = threadIdx.x; t < k; t += NT) uj[t] -= dot * ui[t];
+        for (int pass = 0; pass < 2; ++pass) {
+          for (int i0 = 0; i0 < k; i0 += 1024) {
+            // dot products with the finished columns i0 .. i0+1023
+            for (int i = i0 + threadIdx.x; i < k && i < i0 + 1024; i += NT) {
+              double d = 0.0;
+              const bool finished = (i != j) && (sig[i] > cut || i < j);
+              if (finished) {
+                const double *ui = Am + (int64_t)i * kp;
+                for (int t = 0; t < k; ++t) d += ui[t] * uj[t];
+              }
+              wdot[i - i0] = d;
+            }
+            __syncthreads();
+            for (int t = threadIdx.x; t < k; t += NT) {
+              double v = uj[t];
+              for (int i = i0; i < k && i < i0 + 1024; ++i) v -= wdot[i - i0] * Am[(int64_t)i * kp + t];
+              uj[t] = v;
+            }
             __syncthreads();
           }
+        }
         double nn = 0.0;
         for (int t = threadIdx.x; t < k; t += NT) nn += uj[t] * uj[t];
         nn = wg_sum_f64(nn, red);
-        if (nn > 1e-6) {
+        if (nn > 1e-20) {
           const double is = 1.0 / sqrt(nn);
           for (int t = threadIdx.x; t < k; t += NT) uj[t] *= is;
           __syncthreads();
@@ -655,17 +767,7 @@ __global__ __launch_bounds__(NT) void k_jacobi64(HakLocalArgs a, int want_det_fi
     const bool flip = want_det_fix && sign_sh < 0;
     float *Q = a.q + a.qoff[s];
     double *Q64 = a.q64 ? a.q64 + a.qoff[s] : nullptr;
-    for (int64_t e = threadIdx.x; e < (int64_t)kp * kp; e += NT) {
-      int m = (int)(e % kp), n = (int)(e / kp);
-      double acc = 0.0;
-      if (m < k && n < k) {
-        for (int j = 0; j < k; ++j) acc += Am[m + (int64_t)j * kp] * Vm[n + (int64_t)j * kp];
-        if (flip) acc -= 2.0 * Am[m + (int64_t)jmin * kp] * Vm[n + (int64_t)jmin * kp];
-      }
-      Q[e] = (float)acc;
-      if (Q64) Q64[e] = acc;
-    }
-    __syncthreads();
+    compose_uvt<double, float>(Am, Vm, k, kp, Q, Q64, xs64, ys64, flip ? jmin : -1);
     if (a.sigma_sum) {
       double ss = 0.0;
       for (int j = threadIdx.x; j < k; j += NT) ss += sig[j];

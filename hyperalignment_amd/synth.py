@@ -198,6 +198,28 @@ class Workload:
     noise: float
 
 
+_SL_CACHE: dict = {}
+_MIX_CACHE: dict = {}
+
+
+def _cached_searchlights(order, radius):
+    key = (order, float(radius))
+    if key not in _SL_CACHE:
+        _SL_CACHE.clear()
+        _SL_CACHE[key] = searchlights(order, radius)
+    return _SL_CACHE[key]
+
+
+def _cached_mixing(order, radius, seed):
+    key = (order, seed)
+    if key not in _MIX_CACHE:
+        if len(_MIX_CACHE) > 4:
+            _MIX_CACHE.clear()
+        coords, _ = _cached_searchlights(order, radius)
+        _MIX_CACHE[key] = mixing_matrix(coords, seed)
+    return _MIX_CACHE[key]
+
+
 CONFIGS = {
     "c0": dict(order=4, radius=15.0, n=200, dtype="float64", noise=0.3),
     "c1": dict(order=5, radius=20.0, n=300, dtype="float32", noise=0.3),
@@ -217,10 +239,10 @@ def make_workload(name: str, seed: int = 0, n: int | None = None, noise: float |
         cfg["order"] = order
     if radius is not None:
         cfg["radius"] = radius
-    coords, sls = searchlights(cfg["order"], cfg["radius"])
+    coords, sls = _cached_searchlights(cfg["order"], cfg["radius"])
     nv = sls.nv
     X = zscore_columns(normal_matrix(stream_seed(seed, 1), cfg["n"], nv))
-    M = mixing_matrix(coords, stream_seed(seed, 2))
+    M = _cached_mixing(cfg["order"], cfg["radius"], stream_seed(seed, 2))
     E = normal_matrix(stream_seed(seed, 3), cfg["n"], nv)
     Y = np.asarray((M.T @ X.T).T) + cfg["noise"] * E
     if cfg["dtype"] == "float32":

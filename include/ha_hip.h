@@ -139,6 +139,16 @@ int ha_hip_plan_download_local(const HaHipPlan *plan, const int32_t *which, int3
                                float *out);
 int ha_hip_plan_stats(const HaHipPlan *plan, HaHipStats *stats);
 
+/* Features and (sigma_max, sigma_min) of the chosen searchlights from the FP32
+ * Jacobi SVD path (training labels of the learned scaling). features: n x 12,
+ * sig: n x 2, in the order of `which`. Requires loaded data. */
+int ha_hip_plan_spectra(HaHipPlan *plan, const int32_t *which, int32_t n, float *features, float *sig);
+/* Per-searchlight polar steps / Jacobi sweeps of the last pass (negative: budget
+ * exhausted; >= 1000: solved on the FP64 path) and FP64-fallback flags. */
+int ha_hip_plan_steps(const HaHipPlan *plan, int32_t *steps, int32_t *flags);
+/* MLP predictions (2 per searchlight) of the last learned pass. */
+int ha_hip_plan_predictions(const HaHipPlan *plan, float *pred);
+
 /* ---- learned polar scaling ---- */
 
 /* Number of features per searchlight and of MLP parameters (12 -> 64 -> 64 -> 2). */
@@ -154,6 +164,19 @@ int ha_hip_plan_stats(const HaHipPlan *plan, HaHipStats *stats);
  * (row index = input unit), plus the feature normalisation: mean[IN] then
  * std[IN]. NULL params unloads the model. */
 int ha_hip_set_model(const float *params, const float *feat_norm);
+
+/* Batch inference: Y[N][2] = MLP(X[N][12]) on the GPU. */
+int ha_hip_mlp_forward(const float *params, const float *feat_norm, const float *X, int64_t N, float *Y);
+/* Full-batch gradient of the loss mean_s sum_j w_j (out_sj - Y_sj)^2 at params
+ * (HA_HIP_MLP_NPARAM floats into grad) and the loss itself, on the GPU. */
+int ha_hip_mlp_grad(const float *params, const float *feat_norm, const float *X, const float *Y, int64_t N,
+                    float w0, float w1, float *grad, float *loss);
+/* Adam training on the GPU (beta 0.9 / 0.999, eps 1e-8): `epochs` passes of
+ * minibatches of `batch` rows in a splitmix64-seeded shuffle; params are
+ * updated in place; loss_hist[epochs] (may be NULL) gets the mean batch loss. */
+int ha_hip_mlp_train(float *params, const float *feat_norm, const float *X, const float *Y, int64_t N,
+                     int32_t epochs, int32_t batch, float lr, float w0, float w1, uint64_t seed,
+                     float *loss_hist);
 
 #ifdef __cplusplus
 }

@@ -223,3 +223,45 @@ def test_c1_polar_columns():
     st = plan.stats()
     assert st["n_unconverged"] == 0
     plan.close()
+
+
+# ---- rank deficiency: parity by invariants (the polar factor is not unique) ----
+
+def _check_rank_deficient(X, Y, T, tol):
+    k = X.shape[1]
+    assert np.abs(T.T @ T - np.eye(k)).max() < tol
+    H = T.T @ (X.T @ Y)
+    assert np.abs(H - H.T).max() < tol * max(1.0, np.abs(H).max())
+    assert np.linalg.eigvalsh(0.5 * (H + H.T)).min() > -tol * max(1.0, np.abs(H).max())
+    To = O.procrustes(X, Y)
+    r, ro = np.linalg.norm(X @ T - Y), np.linalg.norm(X @ To - Y)
+    assert abs(r - ro) <= tol * max(1.0, ro)
+
+
+def test_rank_deficient_local():
+    rng = np.random.default_rng(11)
+    X = rng.standard_normal((8, 20))
+    Y = rng.standard_normal((8, 20))
+    _check_rank_deficient(X, Y, ha.procrustes(X, Y), 1e-9)
+
+
+@pytest.mark.parametrize("method", ["polar", "jacobi32"])
+def test_rank_deficient_searchlights(method):
+    """k >= n_samples routes to the FP64 fallback; every local transform is an
+    optimal orthogonal Procrustes solution."""
+    rng = np.random.default_rng(12)
+    nv, nt = 60, 10
+    X = rng.standard_normal((nt, nv))
+    Y = rng.standard_normal((nt, nv))
+    lists = [np.sort(rng.choice(nv, 16, replace=False)) for _ in range(20)]
+    sl = synth.toy_searchlights(lists, nv=nv)
+    plan = ha.Plan(nt, nv, sl.indices, sl.offsets, None, 0.0)
+    plan.load(X, Y)
+    ha.set_options(method=method)
+    plan.execute()
+    st, fl = plan.steps()
+    assert fl.all()
+    for s, T in enumerate(plan.local(np.arange(len(lists)))):
+        l = lists[s]
+        _check_rank_deficient(X[:, l], Y[:, l], T.astype(np.float64), 2e-6)
+    plan.close()
