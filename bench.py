@@ -63,6 +63,63 @@ def cpu_baseline(w, seconds, threads):
     return len(done) / dt, np.array(done), Ts
 
 
+def comparator(w, args, every):
+    """The plain paths the learned one is measured against, on the same inputs:
+    the plain polar iteration (fixed scale ||A||_F and the 1e-7 schedule) over
+    the whole workload, and the FP32 one-sided Jacobi SVD path on every
+    `every`-th searchlight, its rate extrapolated per searchlight."""
+    from hyperalignment_amd import learn
+    s = w.sls
+    nt = w.X.shape[0]
+    res = {}
+    ha.set_options(method="polar")
+    plan = ha.Plan(nt, s.nv, s.indices, s.offsets, s.dists, s.radius)
+    plan.load(w.X, w.Y)
+    plan.execute()
+    t0 = time.perf_counter()
+    plan.execute()
+    dt = time.perf_counter() - t0
+    st = plan.stats()
+    res["plain_polar"] = {"value": s.count / dt, "hemisphere_s": dt,
+                          "mean_steps": st["steps_total"] / max(1, s.count),
+                          "fp64_fallback": st["n_fallback"]}
+    plan.close()
+    print(f"[bench] plain polar {1e3 * dt:.1f} ms", file=sys.stderr, flush=True)
+    which = np.arange(0, s.count, every)
+    idx, off, d = learn.sub_searchlights(s, which)
+    ha.set_options(method="jacobi32")
+    plan = ha.Plan(nt, s.nv, idx, off, d, s.radius)
+    plan.load(w.X, w.Y)
+    t0 = time.perf_counter()
+    plan.execute()
+    dt = time.perf_counter() - t0
+    st = plan.stats()
+    res["jacobi32"] = {"value": len(which) / dt, "hemisphere_s_est": s.count * dt / len(which),
+                       "sample": f"every {every}th searchlight ({len(which)}), one pass",
+                       "mean_sweeps": st["steps_total"] / max(1, len(which))}
+    plan.close()
+    print(f"[bench] jacobi32 sample {1e3 * dt:.1f} ms", file=sys.stderr, flush=True)
+    ha.set_options(method=args.method)
+    return res
+
+
+def pmc_traffic(config, method):
+    """HBM bytes per k_polar launch from the committed rocprofv3 counter summary
+    (FETCH_SIZE + WRITE_SIZE, calibrated against a copy kernel), when one exists
+    for this workload and method."""
+    path = os.path.join(ROOT, "profiles", f"k_polar_{config}_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    kv = {}
+    for line in open(path):
+        if "=" in line and not line.startswith("#"):
+            k, v = line.split("=", 1)
+            kv[k.strip()] = v.strip()
+    if kv.get("method") != method:
+        return None
+    return float(kv["traffic_bytes"])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,6 +130,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile", type=int, default=0)
+    ap.add_argument("--no-compare", action="store_true", help="skip the plain polar / Jacobi32 comparison")
+    ap.add_argument("--compare-every", type=int, default=40)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -112,11 +171,19 @@ def main():
     # dominant kernel: the local-transform kernel (cross-covariance + polar iteration)
     polar_avg = float(np.mean(polar_ms))
     achieved = st["gemm_flops"] / (polar_avg * 1e-3)
+    method = ha.resolved_method(args.method, int(s.sizes.max()))
     roofline = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_BF16 / 1e12, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_BF16, "traffic": None, "kernel": "k_polar",
+                "frac": achieved / PEAK_BF16, "traffic": pmc_traffic(args.config, method), "kernel": "k_polar",
                 "kernel_ms": polar_avg, "flops_per_launch": st["gemm_flops"]}
+    stats_main = st
+    compare = None
+    if not args.no_compare and method in ("polar", "polar_learned"):
+        compare = comparator(w, args, args.compare_every)
+        compare["speedup_vs_jacobi32"] = value / compare["jacobi32"]["value"]
+        compare["speedup_vs_plain_polar"] = value / compare["plain_polar"]["value"]
 
     # parity on a sample of searchlights against the FP64 oracle
+    st = stats_main
     cpu = None
     parity = None
     if not args.no_cpu:
@@ -141,15 +208,15 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16x3" if args.method in ("auto", "polar", "polar_learned") else args.method,
+        "dtype": "bf16x3" if method in ("polar", "polar_learned") else method,
         "data": "synthetic",
-        "config": {"workload": f"{args.config}: {WORKLOADS[args.config]}", "method": args.method,
+        "config": {"workload": f"{args.config}: {WORKLOADS[args.config]}", "method": method,
                    "searchlights": s.count, "hemisphere_s": dt / args.steps,
                    "pattern_ms": st["ms_pattern"], "nnz": st["nnz"], "fp64_fallback": st["n_fallback"],
                    "unconverged": st["n_unconverged"],
                    "mean_steps": st["steps_total"] / max(1, s.count),
                    "stage_ms": {k: st[k] for k in ("ms_weights", "ms_polar", "ms_fallback", "ms_scatter")},
-                   "parity_vs_fp64": parity},
+                   "parity_vs_fp64": parity, "compare": compare},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

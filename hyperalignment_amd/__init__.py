@@ -210,16 +210,33 @@ def set_options(**kw):
     return o
 
 
+_model_loaded = False
+
+
+def resolved_method(method, kmax):
+    """The local solver a pass with this method and largest searchlight runs:
+    "auto" picks FP64 Jacobi for k <= 32, else the learned polar iteration when
+    a model is loaded, else the plain polar iteration."""
+    if method != "auto":
+        return method
+    if kmax <= 32:
+        return "jacobi64"
+    return "polar_learned" if _model_loaded else "polar"
+
+
 def set_model(params, feat_norm):
     """Installs MLP parameters (MLP_NPARAM floats) and feature normalisation (2*MLP_IN)."""
+    global _model_loaded
     if params is None:
         _check("ha_hip_set_model", lib().ha_hip_set_model(None, None))
+        _model_loaded = False
         return
     p = np.ascontiguousarray(params, dtype=np.float32).ravel()
     n = np.ascontiguousarray(feat_norm, dtype=np.float32).ravel()
     if p.size != MLP_NPARAM or n.size != 2 * MLP_IN:
         raise ValueError("bad MLP parameter sizes")
     _check("ha_hip_set_model", lib().ha_hip_set_model(_ptr(p, ctypes.c_float), _ptr(n, ctypes.c_float)))
+    _model_loaded = True
 
 
 def _flat(sls, dists):

@@ -46,8 +46,8 @@ static struct {
 		.use_graph = 0,
 		.chunk = 0,
 		.verbose = 0,
-		.learned_margin_scale = 0.05f,
-		.learned_margin_ell = 0.5f,
+		.learned_margin_scale = 0.15f,
+		.learned_margin_ell = 1.0f,
 	},
 	.host_threads = 0,
 };
@@ -569,15 +569,13 @@ static HakLocalArgs local_args(HaHipPlan *p) {
 	a.n_buckets = HA_SCHED_BUCKETS; a.max_sched = HA_SCHED_STEPS;
 	a.bucket_per_decade = HA_SCHED_PER_DECADE;
 	a.max_steps = g.opt.max_polar_steps;
-	int b = (int)ceil(-log10(g.opt.plain_ell) * HA_SCHED_PER_DECADE);
-	a.default_bucket = b < 0 ? 0 : (b >= HA_SCHED_BUCKETS ? HA_SCHED_BUCKETS - 1 : b);
+	a.plain_ell = (float)g.opt.plain_ell;
 	a.tol = (float)g.opt.polar_tol;
 	a.tile = pick_tile(p->kpmax);
 	a.fallback_enabled = g.opt.fp64_fallback;
 	a.rank_check_nt = g.opt.fp64_fallback;
 	a.mlp_margin_scale = g.opt.learned_margin_scale;
 	a.mlp_margin_ell = g.opt.learned_margin_ell;
-	a.mlp_fallback_log10 = -6.0f; /* sigma_min / sigma_max < 1e-6: FP64 Jacobi */
 	return a;
 }
 
@@ -672,7 +670,30 @@ int ha_plan_run(HaHipPlan *p, bool isReflection, bool isScaling, bool zero_data,
 			KCK(hak_fill_i32((int32_t *)p->trace, (int64_t)p->count * a.max_steps, 0, st));
 			a.trace = p->trace;
 		}
+		uint64_t *clk = NULL;
+		if (getenv("HA_HIP_CLOCK")) {
+			/* per-phase busy time of the polar slots (100 MHz wall clock) */
+			ACK(clk, (size_t)p->n_slots * 4 * sizeof(uint64_t));
+			KCK(hak_fill_i32((int32_t *)clk, (int64_t)p->n_slots * 8, 0, st));
+			a.clk = clk;
+		}
 		KCK(hak_polar(&a, st));
+		if (clk) {
+			uint64_t *h = (uint64_t *)calloc((size_t)p->n_slots * 4, sizeof(uint64_t));
+			if (h) {
+				HCK(hipMemcpyAsync(h, clk, (size_t)p->n_slots * 4 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+				HCK(hipStreamSynchronize(st));
+				double sum[4] = {0, 0, 0, 0};
+				for (int i = 0; i < p->n_slots; ++i)
+					for (int j = 0; j < 4; ++j) sum[j] += (double)h[4 * i + j];
+				fprintf(stderr, "ha_hip clock: per slot ms gather+crosscov %.1f, scale+features %.1f, "
+				        "iteration %.1f; searchlights %.0f\n", sum[0] / p->n_slots * 1e-5,
+				        sum[1] / p->n_slots * 1e-5, sum[2] / p->n_slots * 1e-5, sum[3]);
+				free(h);
+			}
+			hipFree(clk);
+			a.clk = NULL;
+		}
 		if (a.trace) {
 			int ns = atoi(getenv("HA_HIP_TRACE"));
 			if (ns > p->count) ns = p->count;

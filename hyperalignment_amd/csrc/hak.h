@@ -96,21 +96,21 @@ typedef struct {
 	float *work; int64_t slot_stride; int32_t n_slots;
 	int32_t *queue;                      /* atomic work counter, zeroed before the launch */
 	/* polar schedule: sched[n_buckets][max_sched][3]; bucket b assumes
-	 * sigma_min / scale >= 10^(-b / bucket_per_decade) */
+	 * sigma_min(X0) >= 10^(-b / bucket_per_decade) and sigma_max(X0) <= 1 */
 	const float *sched;
 	const int32_t *sched_len;            /* per bucket: minimax steps before the Newton-Schulz tail */
-	int32_t n_buckets, max_sched, max_steps, default_bucket;
+	int32_t n_buckets, max_sched, max_steps;
 	float bucket_per_decade;
 	float tol;                           /* finish when ||X^T X - I||_F < tol */
-	const float *sl_scale;               /* per searchlight scale (NULL = ||A||_F) */
-	const int32_t *sl_bucket;            /* per searchlight bucket (NULL = default_bucket) */
+	float plain_ell;                     /* sigma_min / ||A||_F assumed without a model */
 	/* learned scaling (NULL mlp = off) */
 	const float *mlp; const float *mlp_norm;
-	float mlp_margin_scale, mlp_margin_ell;
-	float mlp_fallback_log10;            /* predicted log10(sigma_min/sigma_max) below this -> FP64 path */
+	float mlp_margin_scale;              /* added to the predicted ln(sigma_max / ||A||_F) */
+	float mlp_margin_ell;                /* decades below the predicted log10(sigma_min / sigma_max) */
 	float *pred_out;                     /* per searchlight 2 predictions (may be NULL) */
 	float *features;                     /* per searchlight HAK_NFEAT features (may be NULL) */
 	float *trace;                        /* per searchlight max_steps residuals ||G - I||_F (may be NULL) */
+	uint64_t *clk;                       /* polar: per slot 4 phase times in 100 MHz ticks (may be NULL) */
 	float *sig_out;                      /* Jacobi paths: per searchlight (sigma_max, sigma_min) (may be NULL) */
 	int32_t tile;
 	int32_t fallback_enabled;

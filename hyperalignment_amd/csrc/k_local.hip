@@ -41,6 +41,56 @@ __device__ __forceinline__ double wg_sum_f64(double v, double *red) {
   return r;
 }
 
+// Sum of squares of a kp x kp FP32 block (kp a multiple of 32, 16-byte aligned),
+// float4 loads four deep per thread so the loop is not latency-bound.
+__device__ __forceinline__ double frob2(const float *M, int kp, double *red) {
+  const float4 *m4 = reinterpret_cast<const float4 *>(M);
+  const int64_t n4 = (int64_t)kp * kp / 4;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  int64_t e = threadIdx.x;
+  for (; e + 3 * NT < n4; e += 4 * NT) {
+    float4 x[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) x[c] = m4[e + c * NT];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      acc[c] += (double)x[c].x * x[c].x + (double)x[c].y * x[c].y + (double)x[c].z * x[c].z +
+                (double)x[c].w * x[c].w;
+  }
+  for (; e < n4; e += NT) {
+    const float4 x = m4[e];
+    acc[0] += (double)x.x * x.x + (double)x.y * x.y + (double)x.z * x.z + (double)x.w * x.w;
+  }
+  return wg_sum_f64((acc[0] + acc[1]) + (acc[2] + acc[3]), red);
+}
+
+// x = split(M * f) for a kp x kp FP32 block, float4 loads four deep per thread
+__device__ __forceinline__ void split_scaled(const float *M, const SplitM &x, int kp, float f) {
+  const float4 *m4 = reinterpret_cast<const float4 *>(M);
+  uint2 *h4 = reinterpret_cast<uint2 *>(x.hi), *l4 = reinterpret_cast<uint2 *>(x.lo);
+  const int64_t n4 = (int64_t)kp * kp / 4;
+  auto put4 = [&](int64_t e, float4 v) {
+    const float z[4] = {v.x * f, v.y * f, v.z * f, v.w * f};
+    uint16_t h[4], l[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      h[c] = f2bf(z[c]);
+      l[c] = f2bf(z[c] - bf2f(h[c]));
+    }
+    h4[e] = make_uint2((uint32_t)h[0] | ((uint32_t)h[1] << 16), (uint32_t)h[2] | ((uint32_t)h[3] << 16));
+    l4[e] = make_uint2((uint32_t)l[0] | ((uint32_t)l[1] << 16), (uint32_t)l[2] | ((uint32_t)l[3] << 16));
+  };
+  int64_t e = threadIdx.x;
+  for (; e + 3 * NT < n4; e += 4 * NT) {
+    float4 v[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) v[c] = m4[e + c * NT];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) put4(e + c * NT, v[c]);
+  }
+  for (; e < n4; e += NT) put4(e, m4[e]);
+}
+
 // ---------------------------------------------------------------------------
 // polar iteration
 // ---------------------------------------------------------------------------
@@ -82,9 +132,7 @@ __device__ void crosscov(GemmSmem<TILE> &sm, const HakLocalArgs &a, int s, int k
 __device__ __noinline__ void features_of(const HakLocalArgs &a, int s, int k, int kp, const float *Ab, float *Pv,
                                          float *feat_sh, double *red) {
   // Frobenius, trace, diagonal statistics
-  double fro2 = 0.0;
-  for (int64_t e = threadIdx.x; e < (int64_t)kp * kp; e += NT) fro2 += (double)Ab[e] * Ab[e];
-  fro2 = wg_sum_f64(fro2, red);
+  double fro2 = frob2(Ab, kp, red);
   double tr = 0.0, dabs = 0.0, d2 = 0.0;
   float dmin = 3.0e38f, dmax = 0.f;
   for (int i = threadIdx.x; i < k; i += NT) {
@@ -113,34 +161,85 @@ __device__ __noinline__ void features_of(const HakLocalArgs &a, int s, int k, in
   dmin = fminf(fminf(mm_sh[0], mm_sh[1]), fminf(mm_sh[2], mm_sh[3]));
   dmax = fmaxf(fmaxf(mm_sh[4], mm_sh[5]), fmaxf(mm_sh[6], mm_sh[7]));
   __syncthreads();
-  // power iteration on A^T A with v0 = ones: estimates after 2, 4, 8 steps
-  float *v = Pv, *u = Pv + kp;
+  // power iteration on A^T A with v0 = ones: estimates after 2, 4, 8 steps.
+  // A (kp x kp, column-major, zero outside k x k, kp a multiple of 32) is read
+  // as float4 row quads with four columns per batch, so each lane keeps eight
+  // independent 16-byte loads in flight: the loop is bound by memory latency,
+  // not by arithmetic. u = A v splits the columns over the four waves (partial
+  // sums per wave); v = A^T u gives each wave whole columns.
+  float *v = Pv, *u = Pv + kp, *part = Pv + 2 * kp;  // part: 4 x kp partial sums
   for (int i = threadIdx.x; i < kp; i += NT) v[i] = (i < k) ? 1.0f : 0.f;
   __syncthreads();
   double est[3] = {0, 0, 0};
   int ei = 0;
+  const int nq = kp / 4;  // row quads per column
   for (int it = 1; it <= 8; ++it) {
-    // u = A v
-    for (int i = threadIdx.x; i < k; i += NT) {
-      float acc = 0.f;
-      for (int j = 0; j < k; ++j) acc += Ab[i + (int64_t)j * kp] * v[j];
-      u[i] = acc;
+    // u = A v: lane owns row quads q = lane + 64 c
+    for (int q0 = 0; q0 < nq; q0 += 128) {
+      float4 acc[2] = {make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f)};
+      const bool in0 = q0 + lane < nq, in1 = q0 + 64 + lane < nq;
+      for (int j0 = wave * 4; j0 < k; j0 += 16) {
+        float4 x0[4], x1[4];
+        float vj[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const int j = j0 + c;
+          vj[c] = (j < k) ? v[j] : 0.f;
+          const float4 *col = reinterpret_cast<const float4 *>(Ab + (int64_t)(j < k ? j : 0) * kp);
+          x0[c] = in0 ? col[q0 + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
+          x1[c] = in1 ? col[q0 + 64 + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          acc[0].x += x0[c].x * vj[c]; acc[0].y += x0[c].y * vj[c];
+          acc[0].z += x0[c].z * vj[c]; acc[0].w += x0[c].w * vj[c];
+          acc[1].x += x1[c].x * vj[c]; acc[1].y += x1[c].y * vj[c];
+          acc[1].z += x1[c].z * vj[c]; acc[1].w += x1[c].w * vj[c];
+        }
+      }
+      float4 *pw = reinterpret_cast<float4 *>(part + wave * kp);
+      if (in0) pw[q0 + lane] = acc[0];
+      if (in1) pw[q0 + 64 + lane] = acc[1];
     }
     __syncthreads();
-    // v = A^T u
+    double nu = 0.0;
+    for (int i = threadIdx.x; i < kp; i += NT) {
+      const float x = (part[i] + part[kp + i]) + (part[2 * kp + i] + part[3 * kp + i]);
+      u[i] = x;
+      nu += (double)x * x;
+    }
+    nu = wg_sum_f64(nu, red);  // its barriers publish u
+    // v = A^T u: wave w takes columns w*4 .. w*4+3, then +16
     double nv = 0.0;
-    for (int j = threadIdx.x; j < k; j += NT) {
-      float acc = 0.f;
-      const float *col = Ab + (int64_t)j * kp;
-      for (int i = 0; i < k; ++i) acc += col[i] * u[i];
-      v[j] = acc;
-      nv += (double)acc * acc;
+    for (int j0 = wave * 4; j0 < k; j0 += 16) {
+      float dot[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) dot[c] = 0.f;
+      for (int q0 = 0; q0 < nq; q0 += 64) {
+        const int q = q0 + lane;
+        const float4 uq = (q < nq) ? reinterpret_cast<const float4 *>(u)[q] : make_float4(0.f, 0.f, 0.f, 0.f);
+        float4 x[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const int j = j0 + c;
+          const float4 *col = reinterpret_cast<const float4 *>(Ab + (int64_t)(j < k ? j : 0) * kp);
+          x[c] = (q < nq && j < k) ? col[q] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) dot[c] += x[c].x * uq.x + x[c].y * uq.y + x[c].z * uq.z + x[c].w * uq.w;
+      }
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        float d = dot[c];
+        for (int o2 = 32; o2 > 0; o2 >>= 1) d += __shfl_xor(d, o2);
+        if (lane == 0 && j0 + c < k) {
+          v[j0 + c] = d;
+          nv += (double)d * d;
+        }
+      }
     }
     nv = wg_sum_f64(nv, red);
-    double nu = 0.0;
-    for (int i = threadIdx.x; i < k; i += NT) nu += (double)u[i] * u[i];
-    nu = wg_sum_f64(nu, red);
-    // Rayleigh estimate of sigma_max^2 = |A^T u|^2 / |u|^2 ... normalised: |A^T A v| / |A v|
+    // |A^T A v| / |A v| with |v| = 1 tends to sigma_max from below
     double sig = (nu > 0) ? sqrt(nv / nu) : 0.0;
     if (it == 2 || it == 4 || it == 8) est[ei++] = sig;
     double inv = (nv > 0) ? 1.0 / sqrt(nv) : 0.0;
@@ -196,14 +295,23 @@ __global__ __launch_bounds__(NT, TILE == 128 ? 1 : 2) void k_polar(HakLocalArgs 
       }
       continue;
     }
+    uint64_t t0 = a.clk ? wall_clock64() : 0;
     crosscov<TILE>(sm, a, s, k, kp, w.A);
     __syncthreads();
+    uint64_t t1 = a.clk ? wall_clock64() : 0;
     // scale and schedule bucket
-    double fro2 = 0.0;
-    for (int64_t e = threadIdx.x; e < (int64_t)kp * kp; e += NT) fro2 += (double)w.A[e] * w.A[e];
-    fro2 = wg_sum_f64(fro2, red);
-    float scale = (float)sqrt(fro2);
-    int bucket = a.sl_bucket ? a.sl_bucket[s] : a.default_bucket;
+    const double fro2 = frob2(w.A, kp, red);
+    const float fro = (float)sqrt(fro2);
+    if (!(fro > 0.f) || !isfinite(fro)) {
+      if (threadIdx.x == 0) {
+        a.flag[s] = 1;
+        if (a.steps) a.steps[s] = 0;
+      }
+      continue;
+    }
+    // plain: X0 = A / ||A||_F (sigma_max(X0) <= 1 always), sigma_min(X0) >= plain_ell assumed
+    float scale = fro, ell = a.plain_ell;
+    bool learned = false;
     if (a.features || a.mlp) {
       features_of(a, s, k, kp, w.A, w.P, feat_sh, red);
       if (a.features && threadIdx.x < HAK_NFEAT)
@@ -211,131 +319,132 @@ __global__ __launch_bounds__(NT, TILE == 128 ? 1 : 2) void k_polar(HakLocalArgs 
       if (a.mlp) {
         float pred[2];
         mlp_predict(a, feat_sh, mlp_sh, pred);
-        // pred[0] = log(sigma_max / ||A||_F), pred[1] = log10(sigma_min / sigma_max).
-        // scale = predicted sigma_max inflated by the margin (never above ||A||_F);
-        // ell = predicted sigma_min / scale, deflated by its margin
-        float lf = fminf(pred[0] + a.mlp_margin_scale, 0.f);
-        float smax = scale * expf(lf);
-        float ell = powf(10.f, pred[1] - a.mlp_margin_ell) * expf(pred[0] - lf);
-        scale = smax;
-        int b = (int)ceilf(-log10f(fmaxf(ell, 1e-30f)) * a.bucket_per_decade);
-        bucket = b < 0 ? 0 : (b >= a.n_buckets ? a.n_buckets - 1 : b);
+        // pred[0] = ln(sigma_max / ||A||_F), pred[1] = log10(sigma_min / sigma_max).
+        // scale = predicted sigma_max raised by its margin (never above ||A||_F);
+        // ell = predicted sigma_min / scale lowered by its margin
+        const float lf = fminf(pred[0] + a.mlp_margin_scale, 0.f);
+        scale = fro * expf(lf);
+        ell = exp10f(pred[1] - a.mlp_margin_ell) * expf(pred[0] - lf);
+        learned = true;
         if (a.pred_out && threadIdx.x == 0) {
           a.pred_out[2 * s] = pred[0];
           a.pred_out[2 * s + 1] = pred[1];
         }
-        if (pred[1] < a.mlp_fallback_log10 && a.fallback_enabled) {
-          // predicted too ill-conditioned for the FP32-accurate iteration
-          if (threadIdx.x == 0) {
-            a.flag[s] = 1;
-            if (a.steps) a.steps[s] = 0;
-          }
-          continue;
-        }
       }
-    } else if (a.sl_scale) {
-      scale = a.sl_scale[s];
     }
-    if (!(scale > 0.f) || !isfinite(scale)) {
-      if (threadIdx.x == 0) {
-        a.flag[s] = 1;
-        if (a.steps) a.steps[s] = 0;
-      }
-      continue;
-    }
-    // X0 = A / scale, stored as bf16 hi/lo planes
-    const float inv = 1.0f / scale;
-    {
-      const SplitM x0 = as_split(w.X, kp);
-      for (int64_t e = threadIdx.x; e < (int64_t)kp * kp; e += NT) put_split(x0, e, w.A[e] * inv);
-    }
-    __syncthreads();
-    float *Xb = w.X, *Xnb = w.X2;
     const SplitM G = as_split(w.G, kp), P = as_split(w.P, kp);
-    int step = 0, restart = 0, restarts = 0;
+    uint64_t t2 = a.clk ? wall_clock64() : 0;
+    int step = 0, total = 0;
     bool done = false;
-    for (step = 0; step < a.max_steps; ++step) {
-      const SplitM X = as_split(Xb, kp), Xn = as_split(Xnb, kp);
-      // G = X^T X and r = ||G - I||_F over the k x k block
-      double r2 = 0.0;
+    // A learned attempt whose scale fell below sigma_max diverges within a few
+    // steps (the schedules blow up singular values above about 1.01); it is
+    // redone once as the plain iteration from A before the FP64 fallback.
+    for (int attempt = 0; attempt < (learned ? 2 : 1) && !done; ++attempt) {
+      if (attempt == 1) {
+        scale = fro;
+        ell = a.plain_ell;
+      }
+      int bucket;
       {
-        const Opnd ox = opnd_split(X, OP_SPLIT_K, kp, kp);
-        wg_gemm<TILE, OP_SPLIT_K, OP_SPLIT_K>(sm, ox, ox, kp, kp, true, [&](int m, int n, float v) {
-          put_split(G, m + n * kp, v);
-          if (m < k && n < k) {
-            const float d = v - (m == n ? 1.f : 0.f);
-            r2 += (double)d * d;
-          }
-        });
+        // (the 1e-3 keeps an ell of exactly 10^(-b/4) in bucket b despite float rounding)
+        const int b = (int)ceilf(-log10f(fmaxf(ell, 1e-30f)) * a.bucket_per_decade - 1e-3f);
+        bucket = b < 0 ? 0 : (b >= a.n_buckets ? a.n_buckets - 1 : b);
       }
-      r2 = wg_sum_f64(r2, red);
-      const double r = sqrt(r2);
-      if (a.trace && threadIdx.x == 0) a.trace[(int64_t)s * a.max_steps + step] = (float)r;
-      if (!isfinite(r)) break;
-      const bool finish = (r < a.tol);
-      int ts = step - restart;
-      if (!finish && ts >= a.sched_len[bucket] && r > 0.05 && restarts < 2) {
-        // The growth phase of the schedule is over but some singular values lag:
-        // with r = ||G - I||_F >= 1 - s^2 for every lagging value s,
-        // sqrt(1 - r) bounds them from below. Restart from that bucket.
-        const double s_lo = r < 1.0 ? sqrt(1.0 - r) : 1e-3;
-        const int b = (int)ceil(-log10(fmax(0.5 * s_lo, 1e-9)) * a.bucket_per_decade);
-        bucket = b < 1 ? 1 : (b >= a.n_buckets ? a.n_buckets - 1 : b);
-        restart = step;
-        ++restarts;
-        ts = 0;
-      }
-      float ca, cb, cc;
-      if (finish) {
-        ca = 1.5f; cb = -0.5f; cc = 0.f;
-      } else if (ts < a.max_sched) {
-        const float *co = a.sched + ((int64_t)bucket * a.max_sched + ts) * 3;
-        ca = co[0]; cb = co[1]; cc = co[2];
-      } else {
-        ca = 1.875f; cb = -1.25f; cc = 0.375f;  // quintic Newton-Schulz
-      }
-      // P = ca I + cb G + cc G^2 (G^2 only when cc != 0), zero outside the k x k block
-      if (cc != 0.f) {
-        const Opnd og = opnd_split(G, OP_SPLIT_K, kp, kp);
-        wg_gemm<TILE, OP_SPLIT_K, OP_SPLIT_K>(sm, og, og, kp, kp, true, [&](int m, int n, float v) {
-          const int i = m + n * kp;
-          float pv = 0.f;
-          if (m < k && n < k) pv = (m == n ? ca : 0.f) + cb * get_split(G, i) + cc * v;
-          put_split(P, i, pv);
-        });
-      } else {
-        for (int64_t e = threadIdx.x; e < (int64_t)kp * kp; e += NT) {
-          const int m = (int)(e % kp), n = (int)(e / kp);
-          float pv = 0.f;
-          if (m < k && n < k) pv = (m == n ? ca : 0.f) + cb * get_split(G, e);
-          put_split(P, e, pv);
-        }
-      }
+      // X0 = A / scale, stored as bf16 hi/lo planes
+      const float inv = 1.0f / scale;
+      split_scaled(w.A, as_split(w.X, kp), kp, inv);
       __syncthreads();
-      // Xn = X P; the finishing product goes to the FP32 Q store
-      {
-        const Opnd ox = opnd_split(X, OP_SPLIT_M, kp, kp);
-        const Opnd op = opnd_split(P, OP_SPLIT_K, kp, kp);
+      float *Xb = w.X, *Xnb = w.X2;
+      int restart = 0, restarts = 0;
+      for (step = 0; step < a.max_steps; ++step) {
+        const SplitM X = as_split(Xb, kp), Xn = as_split(Xnb, kp);
+        // G = X^T X and r = ||G - I||_F over the k x k block
+        double r2 = 0.0;
+        {
+          const Opnd ox = opnd_split(X, OP_SPLIT_K, kp, kp);
+          wg_gemm<TILE, OP_SPLIT_K, OP_SPLIT_K>(sm, ox, ox, kp, kp, true, [&](int m, int n, float v) {
+            put_split(G, m + n * kp, v);
+            if (m < k && n < k) {
+              const float d = v - (m == n ? 1.f : 0.f);
+              r2 += (double)d * d;
+            }
+          });
+        }
+        r2 = wg_sum_f64(r2, red);
+        const double r = sqrt(r2);
+        if (a.trace && threadIdx.x == 0) a.trace[(int64_t)s * a.max_steps + step] = (float)r;
+        if (!isfinite(r)) {
+          ++step;
+          break;
+        }
+        const bool finish = (r < a.tol);
+        int ts = step - restart;
+        if (!finish && ts >= a.sched_len[bucket] && r > 0.7 && restarts < 2) {
+          // The growth phase of the schedule is over but some singular values lag
+          // far behind: with r = ||G - I||_F >= 1 - s^2 for every lagging value s,
+          // sqrt(1 - r) bounds them from below. Restart from that bucket. Smaller
+          // residuals (every s above 0.55) are left to the Newton-Schulz tail.
+          const double s_lo = r < 1.0 ? sqrt(1.0 - r) : 1e-3;
+          const int b = (int)ceil(-log10(fmax(0.5 * s_lo, 1e-9)) * a.bucket_per_decade);
+          bucket = b < 1 ? 1 : (b >= a.n_buckets ? a.n_buckets - 1 : b);
+          restart = step;
+          ++restarts;
+          ts = 0;
+        }
+        float ca, cb, cc;
         if (finish) {
-          wg_gemm<TILE, OP_SPLIT_M, OP_SPLIT_K>(sm, ox, op, kp, kp, false, [&](int m, int n, float v) {
-            Q[m + n * kp] = v;
+          ca = 1.5f; cb = -0.5f; cc = 0.f;
+        } else if (ts < a.max_sched) {
+          const float *co = a.sched + ((int64_t)bucket * a.max_sched + ts) * 3;
+          ca = co[0]; cb = co[1]; cc = co[2];
+        } else {
+          ca = 1.875f; cb = -1.25f; cc = 0.375f;  // quintic Newton-Schulz
+        }
+        // P = ca I + cb G + cc G^2 (G^2 only when cc != 0), zero outside the k x k block
+        if (cc != 0.f) {
+          const Opnd og = opnd_split(G, OP_SPLIT_K, kp, kp);
+          wg_gemm<TILE, OP_SPLIT_K, OP_SPLIT_K>(sm, og, og, kp, kp, true, [&](int m, int n, float v) {
+            const int i = m + n * kp;
+            float pv = 0.f;
+            if (m < k && n < k) pv = (m == n ? ca : 0.f) + cb * get_split(G, i) + cc * v;
+            put_split(P, i, pv);
           });
         } else {
-          wg_gemm<TILE, OP_SPLIT_M, OP_SPLIT_K>(sm, ox, op, kp, kp, false, [&](int m, int n, float v) {
-            put_split(Xn, m + n * kp, v);
-          });
+          for (int64_t e = threadIdx.x; e < (int64_t)kp * kp; e += NT) {
+            const int m = (int)(e % kp), n = (int)(e / kp);
+            float pv = 0.f;
+            if (m < k && n < k) pv = (m == n ? ca : 0.f) + cb * get_split(G, e);
+            put_split(P, e, pv);
+          }
         }
+        __syncthreads();
+        // Xn = X P; the finishing product goes to the FP32 Q store
+        {
+          const Opnd ox = opnd_split(X, OP_SPLIT_M, kp, kp);
+          const Opnd op = opnd_split(P, OP_SPLIT_K, kp, kp);
+          if (finish) {
+            wg_gemm<TILE, OP_SPLIT_M, OP_SPLIT_K>(sm, ox, op, kp, kp, false, [&](int m, int n, float v) {
+              Q[m + n * kp] = v;
+            });
+          } else {
+            wg_gemm<TILE, OP_SPLIT_M, OP_SPLIT_K>(sm, ox, op, kp, kp, false, [&](int m, int n, float v) {
+              put_split(Xn, m + n * kp, v);
+            });
+          }
+        }
+        __syncthreads();
+        if (finish) {
+          done = true;
+          ++step;
+          break;
+        }
+        float *t = Xb;
+        Xb = Xnb;
+        Xnb = t;
       }
-      __syncthreads();
-      if (finish) {
-        done = true;
-        ++step;
-        break;
-      }
-      float *t = Xb;
-      Xb = Xnb;
-      Xnb = t;
+      total += step;
     }
+    step = total;
     if (!done) {
       // budget exhausted or non-finite: the FP64 Jacobi pass takes over (writes Q)
       if (threadIdx.x == 0) a.flag[s] = 1;
@@ -346,6 +455,13 @@ __global__ __launch_bounds__(NT, TILE == 128 ? 1 : 2) void k_polar(HakLocalArgs 
       if (threadIdx.x == 0) a.sigma_sum[s] = tr;
     }
     if (threadIdx.x == 0 && a.steps) a.steps[s] = done ? step : -step;
+    if (a.clk && threadIdx.x == 0) {
+      const uint64_t t3 = wall_clock64();
+      a.clk[4 * slot + 0] += t1 - t0;  // gather + cross-covariance
+      a.clk[4 * slot + 1] += t2 - t1;  // scale, features, MLP
+      a.clk[4 * slot + 2] += t3 - t2;  // iteration
+      a.clk[4 * slot + 3] += 1;
+    }
     __syncthreads();
   }
 }

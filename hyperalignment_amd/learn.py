@@ -9,10 +9,13 @@ MLP that predicts
     y1 = log10(sigma_min / sigma_max)  (the schedule bucket, i.e. iteration count)
 
 Labels are the exact spectra of the FP32 one-sided Jacobi SVD path. Training
-uses the library's own GPU forward/backward/Adam kernels. Calibrated margins
-make the predicted scale an upper bound of sigma_max and the predicted ell a
-lower bound of sigma_min/sigma_max on held-out data; a wrong guess costs extra
-Newton-Schulz steps or, if the iteration fails, the FP64 Jacobi fallback.
+uses the library's own GPU forward/backward/Adam kernels. The kernel scales A by
+the predicted sigma_max raised by a margin and picks the schedule (the number
+of steps) from the predicted sigma_min / sigma_max lowered by a margin, both
+calibrated on held-out rows. A wrong prediction costs steps, never accuracy: an
+ell too small costs about half a step per decade, one too large a restart of the
+schedule, and a scale below sigma_max makes the attempt diverge, after which the
+kernel redoes the searchlight as the plain iteration.
 
 Workload grid (config 3): the full-hemisphere mesh (order-6 icosphere, 20 mm
 searchlights) with seeds 0-7, n_samples in {200, 300, 500} and noise sigma in
@@ -87,7 +90,7 @@ def dataset_rows(w: synth.Workload, sample: int, rng):
     return f, np.stack([y0, y1], axis=1).astype(np.float32), which
 
 
-ELL_FALLBACK_LOG10 = -6.0  # sigma_min / sigma_max below 1e-6 -> FP64 Jacobi
+ELL_FALLBACK_LOG10 = -6.0  # training rows: sigma_min / sigma_max >= 1e-6
 
 
 def usable(f, y):
@@ -131,14 +134,17 @@ def fit(X, Y, epochs=40, batch=4096, lr=2e-3, w=(10.0, 1.0), seed=0):
     return params, norm, hist
 
 
-def calibrate(params, norm, X, Y, q_scale=0.9995, q_ell=0.8):
-    """Margins so that exp(y0_pred + m0) >= sigma_max/||A||_F on almost every row
-    (an under-estimated sigma_max can make the iteration diverge) and
-    y1_pred - m1 <= log10(sigma_min/sigma_max) on most rows (the kernel rescues
-    the rest by restarting the schedule from the residual's lower bound)."""
+def calibrate(params, norm, X, Y, q_ell=0.95):
+    """Margins from held-out rows. Scale: the largest under-prediction of
+    ln(sigma_max/||A||_F) plus 0.05, since an attempt whose scale is below
+    sigma_max diverges and is redone. Ell: the q_ell quantile of the
+    over-prediction of log10(sigma_min/sigma_max) plus 0.1 decades; the
+    remaining rows restart their schedule, which costs a few steps. (A
+    simulation of the schedules on C2 spectra with the validation errors puts
+    the fewest mean steps near these margins.)"""
     P = ha.mlp_forward(params, norm, X).astype(np.float64)
-    m0 = float(np.quantile(Y[:, 0] - P[:, 0], q_scale)) + 0.01
-    m1 = float(np.quantile(P[:, 1] - Y[:, 1], q_ell)) + 0.05
+    m0 = float(np.max(Y[:, 0] - P[:, 0])) + 0.05
+    m1 = float(np.quantile(P[:, 1] - Y[:, 1], q_ell)) + 0.1
     rmse = np.sqrt(((P - Y) ** 2).mean(axis=0))
     return max(m0, 0.0), max(m1, 0.0), rmse
 

@@ -45,18 +45,20 @@ enum {
 
 typedef struct {
 	int32_t method;          /* kHaHipMethod* */
-	int32_t fp64_fallback;   /* 1: rank-deficient (k >= nt), unconverged or
-	                            sigma_min/sigma_max < 1e-6 searchlights use FP64 Jacobi */
-	int32_t max_polar_steps; /* iteration budget per searchlight (default 24) */
-	double  polar_tol;       /* stop when ||X^T X - I||_F < polar_tol (default 2e-2,
-	                            followed by one Newton-Schulz finishing step) */
-	double  plain_ell;       /* sigma_min/scale assumed by the plain schedule (default 1e-7) */
+	int32_t fp64_fallback;   /* 1: rank-deficient (k >= nt) searchlights and those whose
+	                            polar iteration does not converge (sigma_min/sigma_max
+	                            below about 1e-7) use FP64 Jacobi */
+	int32_t max_polar_steps; /* iteration budget per searchlight (default 28) */
+	double  polar_tol;       /* stop when ||X^T X - I||_F < polar_tol (default 1e-2),
+	                            followed by one Newton-Schulz finishing step */
+	double  plain_ell;       /* sigma_min/||A||_F assumed by the plain schedule (default 1e-7) */
 	int32_t tile;            /* GEMM tile edge 64 or 128; 0 = per-bucket autotuned */
 	int32_t use_graph;       /* 1: capture a plan's pass in a hipGraph and replay it */
 	int32_t chunk;           /* searchlights per polar chunk (0 = auto) */
 	int32_t verbose;         /* 1: timing lines on stderr */
-	float   learned_margin_scale; /* log-margin added to the predicted sigma_max (default 0.05) */
-	float   learned_margin_ell;   /* decades subtracted from the predicted sigma_min/sigma_max (default 0.5) */
+	float   learned_margin_scale; /* added to the predicted ln(sigma_max/||A||_F) (default 0.15) */
+	float   learned_margin_ell;   /* decades subtracted from the predicted log10(sigma_min/sigma_max)
+	                                 (default 1.0); a model file carries its calibrated margins */
 } HaHipOptions;
 
 void ha_hip_get_options(HaHipOptions *opt);
