@@ -114,6 +114,11 @@ _SIGS = {
     "ha_hip_plan_spectra": (ctypes.c_int, [ctypes.c_void_p, _P(_i32), _i32, _P(_f), _P(_f)]),
     "ha_hip_plan_steps": (ctypes.c_int, [ctypes.c_void_p, _P(_i32), _P(_i32)]),
     "ha_hip_plan_predictions": (ctypes.c_int, [ctypes.c_void_p, _P(_f)]),
+    "ha_searchlight_template_flat": (ctypes.c_int, [_P(_P(_d)), _i32, _i32, _i32, _P(_i32), _P(_i32), _P(_d), _i32,
+                                                     _d, ctypes.c_bool, ctypes.c_bool, ctypes.c_bool, _i32,
+                                                     _P(_d), ctypes.c_bool]),
+    "ha_template_procrustes": (ctypes.c_int, [_P(_P(TMat)), _i32, ctypes.c_bool, ctypes.c_bool, ctypes.c_bool,
+                                              _i32, _P(TMat)]),
     "ha_hip_mlp_forward": (ctypes.c_int, [_P(_f), _P(_f), _P(_f), ctypes.c_int64, _P(_f)]),
     "ha_hip_mlp_grad": (ctypes.c_int, [_P(_f), _P(_f), _P(_f), _P(_f), ctypes.c_int64, _f, _f, _P(_f), _P(_f)]),
     "ha_hip_mlp_train": (ctypes.c_int, [_P(_f), _P(_f), _P(_f), _P(_f), ctypes.c_int64, _i32, _i32, _f, _f, _f,
@@ -284,6 +289,45 @@ def searchlight_procrustes_flat(X, Y, idx, offsets, dists, radius, backend="hip"
         _ptr(T, ctypes.c_double), bool(reflection), bool(scaling), BACKENDS[backend], True)
     _check("ha_searchlight_procrustes_dense", rc)
     return T
+
+
+def searchlight_template_flat(dms, idx, offsets, dists, radius, reflection=True, scaling=False,
+                              zscore_common=True, level2_iter=2):
+    """Searchlight Procrustes template on the GPU (ha_searchlight_template_flat):
+    dms is a sequence of (nt, nv) arrays; returns the (nt, nv) float64 template,
+    tpl[:, sl] += local_template(sl) * w as src/hyperalignment/searchlight.py:215-236
+    with the Procrustes local template of csrc/ha_template.c:14-114."""
+    subs = [np.asfortranarray(d, dtype=np.float64) for d in dms]
+    nt, nv = subs[0].shape
+    if any(d.shape != (nt, nv) for d in subs):
+        raise ValueError("all subjects must have the same shape")
+    ptrs = (ctypes.POINTER(ctypes.c_double) * len(subs))(*[_ptr(d, ctypes.c_double) for d in subs])
+    idx = np.ascontiguousarray(idx, dtype=np.int32)
+    offsets = np.ascontiguousarray(offsets, dtype=np.int32)
+    d = None if dists is None else np.ascontiguousarray(dists, dtype=np.float64)
+    out = np.zeros((nt, nv), dtype=np.float64, order="F")
+    rc = lib().ha_searchlight_template_flat(ptrs, len(subs), nt, nv, _ptr(idx, ctypes.c_int32),
+                                            _ptr(offsets, ctypes.c_int32), _ptr(d, ctypes.c_double),
+                                            len(offsets) - 1, float(radius), bool(reflection), bool(scaling),
+                                            bool(zscore_common), int(level2_iter), _ptr(out, ctypes.c_double),
+                                            True)
+    _check("ha_searchlight_template_flat", rc)
+    return np.ascontiguousarray(out)
+
+
+def template_procrustes(dms, reflection=True, scaling=False, zscore_common=True, level2_iter=1):
+    """ha_template_procrustes on the GPU (csrc/ha_template.h:18-22): the
+    Procrustes template of whole (nt, nv) subject matrices."""
+    subs = [np.ascontiguousarray(d, dtype=np.float64) for d in dms]
+    nt, nv = subs[0].shape
+    mats = [TMat(_ptr(d, ctypes.c_double), nt, nv) for d in subs]
+    arr = (ctypes.POINTER(TMat) * len(mats))(*[ctypes.pointer(m) for m in mats])
+    out = np.zeros((nt, nv), dtype=np.float64)
+    tm = TMat(_ptr(out, ctypes.c_double), nt, nv)
+    rc = lib().ha_template_procrustes(arr, len(mats), bool(reflection), bool(scaling), bool(zscore_common),
+                                      int(level2_iter), ctypes.byref(tm))
+    _check("ha_template_procrustes", rc)
+    return out
 
 
 class _SlHolder:

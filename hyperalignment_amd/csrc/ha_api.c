@@ -10,6 +10,8 @@
  * ha_sparse_scatter_add             ha_sparse.h:35,         ha_sparse.c:128-142
  * ha_sparse_free                    ha_common.h:187-194
  * ha_procrustes / ha_procrustes_f32 ha_procrustes.h:20,27,  ha_procrustes.c:17-222
+ * ha_template_procrustes            ha_template.h:18-22,    ha_template.c:14-114
+ * ha_searchlight_template[_flat]    searchlight.py:215-236 with ha_template.c:14-114
  *
  * Only kHaBackendHIP is executed; other backend values are rejected with
  * kHaErrorArg (the reference's CPU backends are not part of this library).
@@ -22,6 +24,7 @@
 #include "ha_searchlight.h"
 #include "ha_sparse.h"
 #include "ha_procrustes.h"
+#include "ha_template.h"
 #include "hak.h"
 
 #define ERR(...) fprintf(stderr, "ha_hip: " __VA_ARGS__)
@@ -321,5 +324,83 @@ int ha_procrustes_f32(const TMatF *X, const TMatF *Y, TMatF *T, bool isReflectio
 		rc = kHaErrorArg;
 	}
 	free(xd); free(yd); free(td);
+	return rc;
+}
+
+/* ---- templates ---- */
+
+int ha_searchlight_template_flat(const double *const *dms, int32_t n_subj, int32_t nt, int32_t nv,
+                                 const int32_t *sl_indices, const int32_t *sl_offsets, const double *sl_dists,
+                                 int32_t count, double radius, bool isReflection, bool isScaling,
+                                 bool isZscoreCommon, int32_t level2_iter, double *tpl_out, bool col_major) {
+	if (!dms || !sl_indices || !sl_offsets || !tpl_out || n_subj < 2 || nt <= 0 || nv <= 0 || count <= 0 ||
+	    level2_iter < 0)
+		return kHaErrorArg;
+	if (!isReflection) {
+		ERR("the template path always allows reflections (isReflection = false is not supported)\n");
+		return kHaErrorArg;
+	}
+	HaPlanSetup su = {.build_pattern = false, .build_weights_index = true, .index_only = true,
+	                  .nv_x = nv, .nv_y = nv};
+	HaHipPlan *p = NULL;
+	int rc = ha_plan_create_ex(nt, &su, sl_indices, NULL, sl_offsets, sl_dists, count, radius, &p);
+	if (rc) return rc;
+	rc = ha_plan_template(p, dms, n_subj, col_major, isScaling, isZscoreCommon, level2_iter, tpl_out);
+	ha_hip_plan_destroy(p);
+	return rc;
+}
+
+static int template_tmats(const TMat **dms, int32_t n_subj, const TMat *tpl_out, const double ***flat) {
+	if (!dms || !tpl_out || !tpl_out->data || n_subj < 2) return kHaErrorArg;
+	const double **f = (const double **)malloc((size_t)n_subj * sizeof(double *));
+	if (!f) return kHaErrorAlloc;
+	for (int32_t s = 0; s < n_subj; ++s) {
+		if (!dms[s] || !dms[s]->data || dms[s]->rows != tpl_out->rows || dms[s]->cols != tpl_out->cols) {
+			ERR("template subjects must all be %d x %d\n", tpl_out->rows, tpl_out->cols);
+			free(f);
+			return kHaErrorArg;
+		}
+		f[s] = dms[s]->data;
+	}
+	*flat = f;
+	return kHaSuccess;
+}
+
+int ha_searchlight_template(const TMat **dms, int32_t n_subj, const TSearchlights *sls, bool isReflection,
+                            bool isScaling, bool isZscoreCommon, int32_t level2_iter, TMat *tpl_out,
+                            THaBackend backend) {
+	if (backend != kHaBackendHIP) {
+		ERR("backend %d is not provided by this library (use kHaBackendHIP = 4)\n", (int)backend);
+		return kHaErrorArg;
+	}
+	const double **flat = NULL;
+	int rc = template_tmats(dms, n_subj, tpl_out, &flat);
+	if (rc) return rc;
+	Flat f;
+	rc = flatten(sls, true, &f);
+	if (rc) { free(flat); return rc; }
+	rc = ha_searchlight_template_flat(flat, n_subj, tpl_out->rows, tpl_out->cols, f.idx, f.off, f.dist,
+	                                  sls->count, sls->radius, isReflection, isScaling, isZscoreCommon,
+	                                  level2_iter, tpl_out->data, false);
+	flat_free(&f);
+	free(flat);
+	return rc;
+}
+
+int ha_template_procrustes(const TMat **dms, int32_t n_subj, bool isReflection, bool isScaling,
+                           bool isZscoreCommon, int32_t level2_iter, TMat *tpl_out) {
+	const double **flat = NULL;
+	int rc = template_tmats(dms, n_subj, tpl_out, &flat);
+	if (rc) return rc;
+	const int32_t nv = tpl_out->cols;
+	int32_t *idx = (int32_t *)malloc((size_t)nv * sizeof(int32_t));
+	int32_t off[2] = {0, nv};
+	if (!idx) { free(flat); return kHaErrorAlloc; }
+	for (int32_t v = 0; v < nv; ++v) idx[v] = v;
+	/* one searchlight over every column: each vertex has weight 1 */
+	rc = ha_searchlight_template_flat(flat, n_subj, tpl_out->rows, nv, idx, off, NULL, 1, 0.0, isReflection,
+	                                  isScaling, isZscoreCommon, level2_iter, tpl_out->data, false);
+	free(idx);
+	free(flat);
 	return rc;
 }

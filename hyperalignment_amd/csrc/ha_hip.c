@@ -612,7 +612,7 @@ static void stats_update(HaHipPlan *p, int method) {
 			if (v < 0) { S->n_unconverged++; v = -v; }
 			S->steps_total += v;
 			S->steps_hist[v < 31 ? v : 31]++;
-			if (v > 0) flops += v * 4.0 * k * k * k + (v - 1) * 2.0 * k * k * k;
+			flops += v * 6.0 * k * k * k; /* X^T X, G^2 and X P per step */
 		} else {
 			if (v >= 1000) v -= 1000;
 			if (v < 0) { S->n_unconverged++; v = -v; }
@@ -802,6 +802,115 @@ int ha_hip_plan_stats(const HaHipPlan *p, HaHipStats *s) {
 	if (!p || !s) return kHaErrorArg;
 	*s = p->stats;
 	return kHaSuccess;
+}
+
+/* Searchlight Procrustes template over the plan's searchlights (see
+ * k_template.hip): subjects dms[s] are nt x nv FP64 (column- or row-major);
+ * tpl_out receives the weighted sum of the local templates in the same order.
+ * Reflection is always allowed (the reference's template kinds never disable it). */
+int ha_plan_template(HaHipPlan *p, const double *const *dms, int32_t ns, bool col_major, bool isScaling,
+                     bool isZscore, int32_t level2, double *tpl_out) {
+	int rc = kHaSuccess;
+	hipStream_t st = g.stream;
+	const int32_t nt = p->nt, nv = p->nv_x, ld = p->ld, count = p->count;
+	float *data = NULL, *loc = NULL, *work = NULL;
+	double *stage = NULL, *out64 = NULL, *host = NULL;
+	int32_t *order = NULL, *status = NULL, *queue = NULL, *h_order = NULL, *h_status = NULL;
+	int64_t *loc_off = NULL, *h_loc_off = NULL;
+	if (!p->weights_computed || ns < 2 || level2 < 0) { ERR("template needs >= 2 subjects\n"); return kHaErrorArg; }
+	for (int32_t s = 0; s < ns; ++s)
+		if (!dms[s]) { ERR("subject %d has no data\n", s); return kHaErrorArg; }
+	h_order = (int32_t *)malloc((size_t)count * sizeof(int32_t));
+	h_status = (int32_t *)malloc((size_t)count * sizeof(int32_t));
+	h_loc_off = (int64_t *)malloc((size_t)(count + 1) * sizeof(int64_t));
+	host = (double *)malloc((size_t)nt * nv * sizeof(double));
+	if (!h_order || !h_status || !h_loc_off || !host) { rc = kHaErrorAlloc; goto fail; }
+	h_loc_off[0] = 0;
+	for (int32_t s = 0; s < count; ++s) {
+		h_order[s] = s;
+		h_loc_off[s + 1] = h_loc_off[s] + (int64_t)nt * p->h_k[s];
+	}
+	qsort_r(h_order, (size_t)count, sizeof(int32_t), cmp_k_desc, p->h_k);
+	{
+		const int64_t *optr = p->occx_ptr ? p->occx_ptr : p->occ_ptr;
+		const int32_t *occ = p->occx ? p->occx : p->occ;
+		KCK(hak_weights(p->slx, p->dist, p->radius, p->total, optr, occ, nv, p->wsum, p->w, st));
+	}
+	ACK(data, (size_t)ns * nv * ld * sizeof(float));
+	ACK(stage, (size_t)nt * nv * sizeof(double));
+	for (int32_t s = 0; s < ns; ++s) {
+		HCK(hipMemcpyAsync(stage, dms[s], (size_t)nt * nv * sizeof(double), hipMemcpyHostToDevice, st));
+		KCK(hak_convert_data(stage, nt, nv, col_major ? 1 : 0, data + (size_t)s * nv * ld, ld, NULL, st));
+	}
+	ACK(order, (size_t)count * sizeof(int32_t));
+	ACK(status, (size_t)count * sizeof(int32_t));
+	ACK(queue, 8 * sizeof(int32_t));
+	ACK(loc_off, (size_t)(count + 1) * sizeof(int64_t));
+	ACK(loc, (size_t)h_loc_off[count] * sizeof(float));
+	HCK(hipMemcpyAsync(order, h_order, (size_t)count * sizeof(int32_t), hipMemcpyHostToDevice, st));
+	HCK(hipMemcpyAsync(loc_off, h_loc_off, (size_t)(count + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
+	{
+		const int64_t slot = hak_template_slot_floats(ns, ld, p->kpmax);
+		int32_t n_slots = 2 * g.n_cu;
+		if (n_slots > count) n_slots = count;
+		/* keep the work slots within 64 GiB of HBM */
+		while (n_slots > 1 && (double)n_slots * slot * sizeof(float) > 64.0 * (1u << 30)) n_slots /= 2;
+		ACK(work, (size_t)n_slots * slot * sizeof(float));
+		HakTplArgs t;
+		memset(&t, 0, sizeof(t));
+		t.data = data; t.ns = ns; t.nt = nt; t.ld = ld; t.nv = nv;
+		t.sl_idx = p->slx; t.sl_off = p->off; t.order = order; t.count = count; t.kpmax = p->kpmax;
+		t.loc = loc; t.loc_off = loc_off;
+		t.work = work; t.slot_stride = slot; t.n_slots = n_slots;
+		t.queue = queue; t.status = status;
+		t.zscore = isZscore ? 1 : 0; t.scaling = isScaling ? 1 : 0; t.level2 = level2;
+		HakLocalArgs pa = local_args(p);
+		KCK(hak_fill_i32(queue, 8, 0, st));
+		HCK(hipEventRecord(p->ev[0], st));
+		KCK(hak_template(&t, &pa, st));
+		HCK(hipEventRecord(p->ev[1], st));
+	}
+	ACK(out64, (size_t)nt * nv * sizeof(double));
+	KCK(hak_template_gather(loc, loc_off, p->off, p->sl_of, p->occ_ptr, p->occ, p->w, nt, nv, out64, st));
+	HCK(hipEventRecord(p->ev[2], st));
+	HCK(hipMemcpyAsync(h_status, status, (size_t)count * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+	HCK(hipMemcpyAsync(host, out64, (size_t)nt * nv * sizeof(double), hipMemcpyDeviceToHost, st));
+	HCK(hipStreamSynchronize(st));
+	{
+		float t0 = 0, t1 = 0;
+		hipEventElapsedTime(&t0, p->ev[0], p->ev[1]);
+		hipEventElapsedTime(&t1, p->ev[1], p->ev[2]);
+		memset(&p->stats, 0, sizeof(p->stats));
+		p->stats.ms_polar = t0;
+		p->stats.kernel_ms_polar_avg = t0;
+		p->stats.polar_launches = 1;
+		p->stats.ms_scatter = t1;
+		p->stats.ms_total = t0 + t1;
+		int64_t failed = 0, steps = 0;
+		for (int32_t s = 0; s < count; ++s) {
+			if (h_status[s] < 0) ++failed;
+			else steps += h_status[s];
+		}
+		p->stats.n_unconverged = failed;
+		p->stats.steps_total = steps;
+		if (failed) {
+			ERR("template: %lld searchlights had a local Procrustes that did not converge\n", (long long)failed);
+			rc = kHaErrorSvd;
+		}
+	}
+	/* device result is nt x nv column-major */
+	if (col_major) {
+		memcpy(tpl_out, host, (size_t)nt * nv * sizeof(double));
+	} else {
+		for (int32_t v = 0; v < nv; ++v)
+			for (int32_t t = 0; t < nt; ++t) tpl_out[(size_t)t * nv + v] = host[(size_t)v * nt + t];
+	}
+fail:
+	hipStreamSynchronize(st);
+	hipFree(data); hipFree(stage); hipFree(order); hipFree(status); hipFree(queue);
+	hipFree(loc_off); hipFree(loc); hipFree(work); hipFree(out64);
+	free(h_order); free(h_status); free(h_loc_off); free(host);
+	return rc;
 }
 
 int ha_plan_weights(HaHipPlan *p, double *w_flat) {

@@ -145,6 +145,31 @@ int hak_var_sum(const double *X64, int32_t nt, const int32_t *slx_idx, const int
 int hak_scale(const double *sigma_sum, const double *var_sum, int32_t nt, int32_t count,
               double *scale, void *stream);
 
+/* ---- searchlight Procrustes template (k_template.hip) ---- */
+typedef struct {
+	const float *data;       /* ns subjects, each ld x nv FP32 column-major (rows nt..ld-1 zero) */
+	int32_t ns, nt, ld, nv;
+	const int32_t *sl_idx, *sl_off, *order;
+	int32_t count, kpmax;
+	float *loc;              /* local templates, searchlight s: nt x k column-major at loc_off[s] */
+	const int64_t *loc_off;
+	float *work; int64_t slot_stride; int32_t n_slots;
+	int32_t *queue;          /* atomic work counter, zeroed before the launch */
+	int32_t *status;         /* per searchlight: polar steps over all solves, -1 if one failed */
+	int32_t zscore, scaling, level2;
+} HakTplArgs;
+/* Floats of one template work slot for ns subjects, leading dim ld, kp = kpmax. */
+int64_t hak_template_slot_floats(int32_t ns, int32_t ld, int32_t kpmax);
+/* Local templates of every searchlight (one workgroup each); pa carries the
+ * polar schedule (sched, sched_len, n_buckets, max_sched, max_steps,
+ * bucket_per_decade, tol, plain_ell). */
+int hak_template(const HakTplArgs *t, const HakLocalArgs *pa, void *stream);
+/* out[t + v * nt] = sum over the occurrences f of vertex v (searchlight order)
+ * of loc[s(f)](t, f - sl_off[s(f)]) * w[f], FP64. */
+int hak_template_gather(const float *loc, const int64_t *loc_off, const int32_t *sl_off, const int32_t *sl_of,
+                        const int64_t *occ_ptr, const int32_t *occ, const double *w, int32_t nt, int32_t nv,
+                        double *out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

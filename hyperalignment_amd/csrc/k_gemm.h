@@ -295,22 +295,24 @@ __device__ __forceinline__ void acc_to_lds(const Acc<TILE> &acc, float *ctile) {
       }
 }
 
-// Full product over kp x kp. Each finished tile is staged in LDS and handed to
-// epi(m, n, v) element by element with consecutive threads on consecutive rows
-// m (coalesced column-major writes). sym: only tiles I <= J are computed and
-// every off-diagonal tile is also delivered transposed, so epi still sees
-// every (m, n) of the full product exactly once.
+// Full product over M x N (rows m < M from A, columns n < N from B). Each
+// finished tile is staged in LDS and handed to epi(m, n, v) element by element
+// with consecutive threads on consecutive rows m (coalesced column-major
+// writes). sym (M == N, A == B or a symmetric result): only tiles I <= J are
+// computed and every off-diagonal tile is also delivered transposed, so epi
+// still sees every (m, n) of the full product exactly once.
 template <int TILE, int KA, int KB, class Epi>
-__device__ void wg_gemm(GemmSmem<TILE> &sm, const Opnd &A, const Opnd &B, int kp, int K, bool sym, Epi epi) {
-  const int nt = (kp + TILE - 1) / TILE;
+__device__ void wg_gemm_mn(GemmSmem<TILE> &sm, const Opnd &A, const Opnd &B, int M, int N, int K, bool sym,
+                           Epi epi) {
+  const int ntm = (M + TILE - 1) / TILE, ntn = (N + TILE - 1) / TILE;
   constexpr int LD = TILE + 1;
-  for (int I = 0; I < nt; ++I)
-    for (int J = sym ? I : 0; J < nt; ++J) {
+  for (int I = 0; I < ntm; ++I)
+    for (int J = sym ? I : 0; J < ntn; ++J) {
       Acc<TILE> acc;
       tile_gemm<TILE, KA, KB>(sm, A, B, I * TILE, J * TILE, K, acc);
       acc_to_lds<TILE>(acc, sm.ctile);
       __syncthreads();
-      const int mmax = min(TILE, kp - I * TILE), nmax = min(TILE, kp - J * TILE);
+      const int mmax = min(TILE, M - I * TILE), nmax = min(TILE, N - J * TILE);
       for (int e = threadIdx.x; e < TILE * TILE; e += NT) {
         const int ml = e % TILE, nl = e / TILE;
         if (ml < mmax && nl < nmax) epi(I * TILE + ml, J * TILE + nl, sm.ctile[nl * LD + ml]);
@@ -324,6 +326,13 @@ __device__ void wg_gemm(GemmSmem<TILE> &sm, const Opnd &A, const Opnd &B, int kp
       }
       __syncthreads();
     }
+}
+
+// Square kp x kp product (see wg_gemm_mn).
+template <int TILE, int KA, int KB, class Epi>
+__device__ __forceinline__ void wg_gemm(GemmSmem<TILE> &sm, const Opnd &A, const Opnd &B, int kp, int K, bool sym,
+                                        Epi epi) {
+  wg_gemm_mn<TILE, KA, KB>(sm, A, B, kp, kp, K, sym, epi);
 }
 
 #endif  // K_GEMM_H

@@ -1,0 +1,304 @@
+// k_template.hip -- searchlight Procrustes template on gfx950.
+//
+// One 256-thread workgroup per searchlight (persistent, atomic queue) builds
+// the searchlight's local template from the subjects' columns in it, following
+// ha_template.c:14-114 (spec local_template.py:165-221):
+//
+//   common = D_0; aligned_0 = D_0
+//   for s = 1 .. ns-1:    T = procrustes(D_s, common); aligned_s = z(D_s T)
+//                         common = z((common + aligned_s) / 2)
+//   level2 times:         sum = sum_s aligned_s
+//                         for s: T = procrustes(D_s, z((sum - aligned_s) / (ns-1)))
+//                                aligned_s = D_s T
+//   local = z(sum_s aligned_s)          (z = one-pass column z-score,
+//                                        ha_linalg.c:263-291; without z-scoring
+//                                        the plain averages of the C code)
+//
+// Every Procrustes runs on the bf16x3 MFMA engine: A = D_s^T R (K = time
+// points), the scheduled polar iteration (k_polar.h) and the product D_s Q.
+// Work slot layout (FP32 words, column-major, leading dim ld = roundup(nt, 32),
+// kp = roundup(k, 32), padding rows/columns exactly zero):
+//   dsp[ns]   subject columns as bf16 hi/lo planes (ld x kp each)
+//   al[ns]    aligned subjects, FP32 ld x kp
+//   com, sum  FP32 ld x kp; rsp: split planes of the reference (ld x kp)
+//   A, X, X2, G, P: the polar iteration's kp x kp buffers (kpmax x kpmax each)
+// The local templates (nt x k per searchlight) are combined per vertex by
+// k_template_gather with the searchlight weights (searchlight.py:215-236:
+// tpl[:, sl] += local * w), in searchlight order, in FP64.
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <math.h>
+#include "hak.h"
+#include "k_gemm.h"
+#include "k_polar.h"
+
+#define LAUNCH_RET() return (int)hipGetLastError()
+
+namespace {
+
+constexpr int TT = 64;  // GEMM tile edge of the template kernel (k is about 100 at 20 mm, order 5)
+
+struct TplSlot {
+  uint16_t *dsp;   // subject s: hi at dsp + 2 s ld kp, lo at + ld kp
+  float *al;       // subject s at al + s ld kp
+  float *com, *sum;
+  uint16_t *rsp;   // hi, lo = + ld kp
+  WorkSlots w;
+};
+
+__device__ __forceinline__ TplSlot tpl_slot(const HakTplArgs &t, int slot) {
+  const int64_t lk = (int64_t)t.ld * t.kpmax, kk = (int64_t)t.kpmax * t.kpmax;
+  float *b = t.work + (int64_t)slot * t.slot_stride;
+  TplSlot s;
+  s.dsp = reinterpret_cast<uint16_t *>(b);
+  s.al = b + t.ns * lk;
+  s.com = s.al + t.ns * lk;
+  s.sum = s.com + lk;
+  s.rsp = reinterpret_cast<uint16_t *>(s.sum + lk);
+  float *pw = s.sum + 2 * lk;
+  s.w.A = pw;
+  s.w.X = pw + kk;
+  s.w.X2 = pw + 2 * kk;
+  s.w.G = pw + 3 * kk;
+  s.w.P = pw + 4 * kk;
+  return s;
+}
+
+__device__ __forceinline__ SplitM planes(uint16_t *hi, int64_t n) {
+  SplitM m;
+  m.hi = hi;
+  m.lo = hi + n;
+  return m;
+}
+
+// One-pass column z-score of the nt x k block (ld x kp buffer) in FP64:
+// mean = sum/nt, var = sum2/nt - mean^2, columns with var < 1e-30 become 0
+// (ha_linalg.c:263-291). One wave per column.
+__device__ void zscore_cols(float *B, int nt, int ld, int k) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int j = wave; j < k; j += NT / 64) {
+    float *c = B + (int64_t)j * ld;
+    double s1 = 0.0, s2 = 0.0;
+    for (int t = lane; t < nt; t += 64) {
+      const double x = c[t];
+      s1 += x;
+      s2 += x * x;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      s1 += __shfl_xor(s1, o);
+      s2 += __shfl_xor(s2, o);
+    }
+    const double inv_n = 1.0 / nt;
+    const double mean = s1 * inv_n;
+    const double var = s2 * inv_n - mean * mean;
+    const double is = var < 1e-30 ? 0.0 : 1.0 / sqrt(var);
+    for (int t = lane; t < nt; t += 64) c[t] = (float)(((double)c[t] - mean) * is);
+  }
+  __syncthreads();
+}
+
+// R (FP32 ld x kp) -> bf16 hi/lo planes
+__device__ __forceinline__ void split_block(const float *R, const SplitM &r, int64_t n) {
+  for (int64_t e = threadIdx.x; e < n; e += NT) put_split(r, e, R[e]);
+  __syncthreads();
+}
+
+// out = D_s T with T = procrustes(D_s, R) (reflection allowed, optional
+// scaling sum(sigma) / sum_j ||D_s[:, j] - mean||^2). Returns the polar steps,
+// or -1 if the iteration did not converge (T = I is used then).
+__device__ int solve_one(GemmSmem<TT> &sm, const HakTplArgs &t, const HakLocalArgs &pa, const TplSlot &sl,
+                         const int32_t *idx, int sidx, const float *R, float *out, int k, int kp, double *red) {
+  const int64_t lk = (int64_t)t.ld * kp;
+  const SplitM d = planes(sl.dsp + (int64_t)2 * sidx * t.ld * t.kpmax, lk);
+  const SplitM r = planes(sl.rsp, lk);
+  split_block(R, r, lk);
+  // A = D_s^T R over the time points
+  {
+    const Opnd od = opnd_split(d, OP_SPLIT_K, t.ld, k);
+    const Opnd orr = opnd_split(r, OP_SPLIT_K, t.ld, k);
+    float *A = sl.w.A;
+    wg_gemm_mn<TT, OP_SPLIT_K, OP_SPLIT_K>(sm, od, orr, kp, kp, t.ld, false,
+                                           [&](int m, int n, float v) { A[m + n * kp] = v; });
+  }
+  __syncthreads();
+  const double fro2 = frob2(sl.w.A, kp, red);
+  const float fro = (float)sqrt(fro2);
+  const SplitM q = as_split(sl.w.G, kp);  // G is free while the final product runs
+  double tr = 0.0;
+  int steps = 0;
+  if (fro > 0.f && isfinite(fro)) {
+    const float *A = sl.w.A;
+    const PolarOut po = polar_solve<TT>(sm, pa, sl.w, k, kp, fro, fro, pa.plain_ell, false, red, nullptr,
+                                        [&](int m, int n, float v) {
+                                          put_split(q, m + n * kp, v);
+                                          tr += (double)v * A[m + n * kp];
+                                        });
+    steps = po.done ? po.steps : -1;
+  } else {
+    steps = -1;
+  }
+  if (steps < 0) {
+    // no polar factor (zero or non-finite A, or no convergence): identity
+    for (int64_t e = threadIdx.x; e < (int64_t)kp * kp; e += NT) {
+      const int m = (int)(e % kp), n = (int)(e / kp);
+      put_split(q, e, (m == n && m < k) ? 1.f : 0.f);
+    }
+  }
+  tr = wg_sum_f64(tr, red);
+  float f = 1.f;
+  if (t.scaling && steps >= 0) {
+    // sum_j ||D_s[:, j] - mean_j||^2 from the FP32 subject data, in FP64
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const float *D = t.data + (int64_t)sidx * t.nv * t.ld;
+    double vs = 0.0;
+    for (int j = wave; j < k; j += NT / 64) {
+      const float *c = D + (int64_t)idx[j] * t.ld;
+      double s1 = 0.0, s2 = 0.0;
+      for (int i = lane; i < t.nt; i += 64) {
+        s1 += c[i];
+        s2 += (double)c[i] * c[i];
+      }
+      for (int o = 32; o > 0; o >>= 1) {
+        s1 += __shfl_xor(s1, o);
+        s2 += __shfl_xor(s2, o);
+      }
+      if (lane == 0) vs += s2 - s1 * s1 / t.nt;
+    }
+    vs = wg_sum_f64(vs, red);
+    f = vs > 0.0 ? (float)(tr / vs) : 1.f;
+  }
+  __syncthreads();
+  // out = D_s Q: out^T(j, t) = sum_i Q(i, j) D_s(t, i)
+  {
+    const Opnd oq = opnd_split(q, OP_SPLIT_K, kp, k);
+    const Opnd odm = opnd_split(d, OP_SPLIT_M, t.ld, t.ld);
+    const int nt = t.nt, ld = t.ld;
+    wg_gemm_mn<TT, OP_SPLIT_K, OP_SPLIT_M>(sm, oq, odm, kp, ld, kp, false, [&](int j, int tt, float v) {
+      out[tt + (int64_t)j * ld] = (j < k && tt < nt) ? v * f : 0.f;
+    });
+  }
+  __syncthreads();
+  return steps;
+}
+
+__global__ __launch_bounds__(NT, 2) void k_template(HakTplArgs t, HakLocalArgs pa) {
+  __shared__ GemmSmem<TT> sm;
+  __shared__ double red[4];
+  __shared__ int32_t item_sh;
+  const TplSlot sl = tpl_slot(t, blockIdx.x);
+  for (;;) {
+    const int it = next_item(t.queue, &item_sh);
+    if (it >= t.count) break;
+    const int s = t.order ? t.order[it] : it;
+    const int32_t *idx = t.sl_idx + t.sl_off[s];
+    const int k = t.sl_off[s + 1] - t.sl_off[s];
+    const int kp = (k + 31) / 32 * 32;
+    const int ld = t.ld, nt = t.nt, ns = t.ns;
+    const int64_t lk = (int64_t)ld * kp, lkmax = (int64_t)ld * t.kpmax;
+    // subject columns: split planes for the products, FP32 copies for subject 0
+    for (int sub = 0; sub < ns; ++sub) {
+      const float *D = t.data + (int64_t)sub * t.nv * ld;
+      const SplitM d = planes(sl.dsp + 2 * sub * lkmax, lk);
+      for (int64_t e = threadIdx.x; e < lk; e += NT) {
+        const int tt = (int)(e % ld), i = (int)(e / ld);
+        const float v = (i < k && tt < nt) ? D[(int64_t)idx[i] * ld + tt] : 0.f;
+        put_split(d, e, v);
+        if (sub == 0) {
+          sl.al[e] = v;
+          sl.com[e] = v;
+        }
+      }
+    }
+    __syncthreads();
+    int total = 0;
+    bool failed = false;
+    auto acc_steps = [&](int st) {
+      if (st < 0) failed = true;
+      else total += st;
+    };
+    // sequential alignment to the running common space
+    for (int sub = 1; sub < ns; ++sub) {
+      float *a = sl.al + sub * lkmax;
+      acc_steps(solve_one(sm, t, pa, sl, idx, sub, sl.com, a, k, kp, red));
+      if (t.zscore) zscore_cols(a, nt, ld, k);
+      for (int64_t e = threadIdx.x; e < lk; e += NT) sl.com[e] = (sl.com[e] + a[e]) * 0.5f;
+      __syncthreads();
+      if (t.zscore) zscore_cols(sl.com, nt, ld, k);
+    }
+    // leave-one-out refinement
+    const float inv_n1 = 1.0f / (float)(ns - 1);
+    for (int lv = 0; lv < t.level2; ++lv) {
+      for (int64_t e = threadIdx.x; e < lk; e += NT) {
+        float acc = 0.f;
+        for (int sub = 0; sub < ns; ++sub) acc += sl.al[sub * lkmax + e];
+        sl.sum[e] = acc;
+      }
+      __syncthreads();
+      for (int sub = 0; sub < ns; ++sub) {
+        float *a = sl.al + sub * lkmax;
+        for (int64_t e = threadIdx.x; e < lk; e += NT) sl.com[e] = (sl.sum[e] - a[e]) * inv_n1;
+        __syncthreads();
+        if (t.zscore) zscore_cols(sl.com, nt, ld, k);
+        acc_steps(solve_one(sm, t, pa, sl, idx, sub, sl.com, a, k, kp, red));
+      }
+    }
+    // local template
+    for (int64_t e = threadIdx.x; e < lk; e += NT) {
+      float acc = 0.f;
+      for (int sub = 0; sub < ns; ++sub) acc += sl.al[sub * lkmax + e];
+      sl.com[e] = t.zscore ? acc : acc * (1.0f / (float)ns);
+    }
+    __syncthreads();
+    if (t.zscore) zscore_cols(sl.com, nt, ld, k);
+    float *loc = t.loc + t.loc_off[s];
+    for (int64_t e = threadIdx.x; e < (int64_t)nt * k; e += NT) {
+      const int tt = (int)(e % nt), i = (int)(e / nt);
+      loc[e] = sl.com[tt + (int64_t)i * ld];
+    }
+    if (threadIdx.x == 0) t.status[s] = failed ? -1 : total;
+    __syncthreads();
+  }
+}
+
+__global__ void k_template_gather(const float *__restrict__ loc, const int64_t *__restrict__ loc_off,
+                                  const int32_t *__restrict__ sl_off, const int32_t *__restrict__ sl_of,
+                                  const int64_t *__restrict__ occ_ptr, const int32_t *__restrict__ occ,
+                                  const double *__restrict__ w, int32_t nt, int32_t nv, double *__restrict__ out) {
+  for (int v = blockIdx.x; v < nv; v += gridDim.x) {
+    const int64_t o0 = occ_ptr[v], o1 = occ_ptr[v + 1];
+    for (int tt = threadIdx.x; tt < nt; tt += blockDim.x) {
+      double acc = 0.0;
+      for (int64_t o = o0; o < o1; ++o) {
+        const int32_t f = occ[o];
+        const int32_t s = sl_of[f];
+        const int32_t i = f - sl_off[s];
+        acc += (double)loc[loc_off[s] + tt + (int64_t)i * nt] * w[f];
+      }
+      out[tt + (int64_t)v * nt] = acc;
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" int64_t hak_template_slot_floats(int32_t ns, int32_t ld, int32_t kpmax) {
+  const int64_t lk = (int64_t)ld * kpmax;
+  return 2 * (int64_t)ns * lk + 3 * lk + 5 * (int64_t)kpmax * kpmax;
+}
+
+extern "C" int hak_template(const HakTplArgs *t, const HakLocalArgs *pa, void *stream) {
+  if (t->count <= 0) return 0;
+  hipLaunchKernelGGL(k_template, dim3(t->n_slots), dim3(NT), 0, (hipStream_t)stream, *t, *pa);
+  LAUNCH_RET();
+}
+
+extern "C" int hak_template_gather(const float *loc, const int64_t *loc_off, const int32_t *sl_off,
+                                   const int32_t *sl_of, const int64_t *occ_ptr, const int32_t *occ, const double *w,
+                                   int32_t nt, int32_t nv, double *out, void *stream) {
+  if (nv <= 0) return 0;
+  const int grid = nv < 65535 ? nv : 65535;
+  hipLaunchKernelGGL(k_template_gather, dim3(grid), dim3(256), 0, (hipStream_t)stream, loc, loc_off, sl_off, sl_of,
+                     occ_ptr, occ, w, nt, nv, out);
+  LAUNCH_RET();
+}

@@ -10,3 +10,12 @@ if ROOT not in sys.path:
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs through the HIP C-ABI)")
+
+
+@pytest.fixture(scope="session", autouse=True)
+def single_thread_blas():
+    """The oracle solves many small (about 100 x 100) SVDs, where a
+    multi-threaded BLAS only adds synchronisation: one thread for the session."""
+    from threadpoolctl import threadpool_limits
+    with threadpool_limits(limits=1, user_api="blas"):
+        yield

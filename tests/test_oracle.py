@@ -187,3 +187,14 @@ def test_driver_csc_equals_dense_c0():
     data = O.searchlight_procrustes_csc(w.X, w.Y, s.indices, s.offsets, indptr, indices, weights=wts)
     D = O.searchlight_procrustes_dense(w.X, w.Y, s.indices, s.offsets, s.dists, s.radius)
     assert np.array_equal(O.dense_from_csc(indptr, indices, data, s.nv), D)
+
+
+def test_searchlight_template_single_searchlight_is_template():
+    """One searchlight over every vertex (weight 1 each) gives ha_template.c's template."""
+    rng = np.random.default_rng(4)
+    dms = [O.zscore_columns(rng.standard_normal((40, 9))) for _ in range(4)]
+    idx = np.arange(9, dtype=np.int32)
+    off = np.array([0, 9], dtype=np.int32)
+    a = O.searchlight_template(dms, idx, off, None, 0.0, level2_iter=2)
+    b = O.template_procrustes(dms, True, False, True, 2)
+    assert np.abs(a - b).max() < 1e-12
