@@ -214,7 +214,8 @@ struct HaHipPlan {
 	int32_t *qld; int64_t *qoff; float *q; int64_t q_elems;
 	double *q64;
 	float *X, *Y; double *X64, *Y64; double *stage;
-	float *work; int64_t slot_stride; int32_t n_slots;
+	float *work; int64_t slot_stride; int32_t n_slots, n_slots_polar;
+	float *sl_par;
 	double *work64; int64_t stride64; int32_t n_slots64;
 	int32_t *flag, *steps, *queues, *order;
 	double *sigma_sum, *var_sum, *scale;
@@ -232,7 +233,7 @@ static void plan_free(HaHipPlan *p) {
 	                p->occx_ptr, p->occx, p->wsum, p->w, p->indptr64, p->indptr, p->indices, p->data,
 	                p->qld, p->qoff, p->q, p->q64, p->X, p->Y, p->X64, p->Y64, p->stage, p->work, p->work64,
 	                p->flag, p->steps, p->queues, p->order, p->sigma_sum, p->var_sum, p->scale,
-	                p->features, p->pred, p->trace, p->sig, p->d_which};
+	                p->features, p->pred, p->trace, p->sig, p->d_which, p->sl_par};
 	for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i)
 		if (ptrs[i]) hipFree(ptrs[i]);
 	if (p->events)
@@ -443,9 +444,18 @@ int ha_plan_create_ex(int32_t nt, const HaPlanSetup *su, const int32_t *slx_idx,
 	ACK(p->var_sum, (size_t)count * sizeof(double));
 	ACK(p->scale, (size_t)count * sizeof(double));
 	p->slot_stride = (int64_t)p->kpmax * p->kpmax;
+	/* Jacobi-32 and the cross-covariance stage: 2 workgroups per CU, 5 buffers;
+	 * the polar iteration: 3 per CU, 4 buffers (the work area serves both) */
 	p->n_slots = 2 * g.n_cu;
 	if (p->n_slots > count) p->n_slots = count;
-	ACK(p->work, (size_t)p->n_slots * 5 * p->slot_stride * sizeof(float));
+	p->n_slots_polar = 3 * g.n_cu;
+	if (p->n_slots_polar > count) p->n_slots_polar = count;
+	{
+		const int64_t nbuf = (int64_t)p->n_slots * 5 > (int64_t)p->n_slots_polar * 4
+		                         ? (int64_t)p->n_slots * 5 : (int64_t)p->n_slots_polar * 4;
+		ACK(p->work, (size_t)nbuf * p->slot_stride * sizeof(float));
+	}
+	ACK(p->sl_par, (size_t)count * 3 * sizeof(float));
 	p->stride64 = (int64_t)p->kpmax * p->kpmax;
 	p->n_slots64 = g.n_cu;
 	if (p->n_slots64 > count) p->n_slots64 = count;
@@ -564,6 +574,7 @@ static HakLocalArgs local_args(HaHipPlan *p) {
 	a.q = p->q; a.qoff = p->qoff; a.qld = p->qld;
 	a.flag = p->flag; a.steps = p->steps;
 	a.work = p->work; a.slot_stride = p->slot_stride; a.n_slots = p->n_slots;
+	a.sl_par = p->sl_par;
 	a.sched = g.d_sched;
 	a.sched_len = g.d_sched_len;
 	a.n_buckets = HA_SCHED_BUCKETS; a.max_sched = HA_SCHED_STEPS;
@@ -606,7 +617,6 @@ static void stats_update(HaHipPlan *p, int method) {
 		double k = p->h_k[s];
 		int32_t v = st[s];
 		if (fl[s]) S->n_fallback++;
-		flops += 2.0 * p->nt * k * k; /* cross-covariance */
 		if (method == kHaHipMethodPolar || method == kHaHipMethodPolarLearned) {
 			if (v >= 1000) v = 0;      /* solved by the FP64 path */
 			if (v < 0) { S->n_unconverged++; v = -v; }
@@ -655,6 +665,7 @@ int ha_plan_run(HaHipPlan *p, bool isReflection, bool isScaling, bool zero_data,
 	HCK(hipEventRecord(p->ev[1], st));
 	/* local transforms */
 	const bool det_fix = !isReflection;
+	const bool is_polar = !det_fix && (method == kHaHipMethodPolar || method == kHaHipMethodPolarLearned);
 	if (det_fix || method == kHaHipMethodJacobi64) {
 		a.flag = NULL;
 		a.queue = p->queues + 1;
@@ -671,24 +682,33 @@ int ha_plan_run(HaHipPlan *p, bool isReflection, bool isScaling, bool zero_data,
 			a.trace = p->trace;
 		}
 		uint64_t *clk = NULL;
+		const int32_t nclk = p->n_slots > p->n_slots_polar ? p->n_slots : p->n_slots_polar;
 		if (getenv("HA_HIP_CLOCK")) {
 			/* per-phase busy time of the polar slots (100 MHz wall clock) */
-			ACK(clk, (size_t)p->n_slots * 4 * sizeof(uint64_t));
-			KCK(hak_fill_i32((int32_t *)clk, (int64_t)p->n_slots * 8, 0, st));
+			ACK(clk, (size_t)nclk * 4 * sizeof(uint64_t));
+			KCK(hak_fill_i32((int32_t *)clk, (int64_t)nclk * 8, 0, st));
 			a.clk = clk;
 		}
+		a.learned_scale = (method == kHaHipMethodPolarLearned) ? 1 : 0;
+		a.queue = p->queues + 3;
+		KCK(hak_crosscov(&a, p->n_slots, st));
+		HCK(hipEventRecord(p->ev[6], st));
+		a.queue = p->queues + 0;
+		a.n_slots = a.tile == 64 ? p->n_slots_polar : (p->n_slots_polar + 2) / 3;
 		KCK(hak_polar(&a, st));
+		a.n_slots = p->n_slots;
 		if (clk) {
-			uint64_t *h = (uint64_t *)calloc((size_t)p->n_slots * 4, sizeof(uint64_t));
+			uint64_t *h = (uint64_t *)calloc((size_t)nclk * 4, sizeof(uint64_t));
 			if (h) {
-				HCK(hipMemcpyAsync(h, clk, (size_t)p->n_slots * 4 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+				HCK(hipMemcpyAsync(h, clk, (size_t)nclk * 4 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
 				HCK(hipStreamSynchronize(st));
 				double sum[4] = {0, 0, 0, 0};
-				for (int i = 0; i < p->n_slots; ++i)
+				for (int i = 0; i < nclk; ++i)
 					for (int j = 0; j < 4; ++j) sum[j] += (double)h[4 * i + j];
-				fprintf(stderr, "ha_hip clock: per slot ms gather+crosscov %.1f, scale+features %.1f, "
-				        "iteration %.1f; searchlights %.0f\n", sum[0] / p->n_slots * 1e-5,
-				        sum[1] / p->n_slots * 1e-5, sum[2] / p->n_slots * 1e-5, sum[3]);
+				fprintf(stderr, "ha_hip clock: per slot ms gather+crosscov %.1f, scale+features %.1f "
+				        "(%d slots), iteration %.1f (%d slots); searchlights %.0f\n",
+				        sum[0] / p->n_slots * 1e-5, sum[1] / p->n_slots * 1e-5, p->n_slots,
+				        sum[2] / p->n_slots_polar * 1e-5, p->n_slots_polar, sum[3]);
 				free(h);
 			}
 			hipFree(clk);
@@ -730,15 +750,19 @@ int ha_plan_run(HaHipPlan *p, bool isReflection, bool isScaling, bool zero_data,
 	{
 		float t[5];
 		for (int i = 0; i < 5; ++i) hipEventElapsedTime(&t[i], p->ev[i], p->ev[i + 1]);
-		float tot = 0;
+		float tot = 0, tc = 0, ti = t[1];
 		hipEventElapsedTime(&tot, p->ev[0], p->ev[5]);
+		if (is_polar) {
+			hipEventElapsedTime(&tc, p->ev[1], p->ev[6]);
+			hipEventElapsedTime(&ti, p->ev[6], p->ev[2]);
+		}
 		p->stats.ms_weights = t[0];
-		p->stats.ms_polar = t[1];
+		p->stats.ms_crosscov = tc;  /* polar path: gather + cross-covariance (+ features) stage */
+		p->stats.ms_polar = ti;     /* local solver (the iteration stage on the polar path) */
 		p->stats.ms_fallback = t[2];
-		p->stats.ms_crosscov = 0; /* fused into the local-transform kernel */
 		p->stats.ms_scatter = t[4] + t[3];
 		p->stats.ms_total = tot;
-		p->stats.kernel_ms_polar_avg = t[1];
+		p->stats.kernel_ms_polar_avg = ti;
 		p->stats.polar_launches = 1;
 	}
 	stats_update(p, method);

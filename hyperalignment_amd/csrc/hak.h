@@ -92,7 +92,7 @@ typedef struct {
 	double *sigma_sum;                   /* per searchlight trace(Q^T A) (may be NULL) */
 	int32_t *flag;                       /* per searchlight: 0 ok, 1 needs the FP64 path */
 	int32_t *steps;                      /* per searchlight polar steps / Jacobi sweeps (may be NULL) */
-	/* work: per slot 5 buffers of slot_stride floats */
+	/* work: per slot 5 (Jacobi-32) or 4 (polar) buffers of slot_stride floats */
 	float *work; int64_t slot_stride; int32_t n_slots;
 	int32_t *queue;                      /* atomic work counter, zeroed before the launch */
 	/* polar schedule: sched[n_buckets][max_sched][3]; bucket b assumes
@@ -103,6 +103,9 @@ typedef struct {
 	float bucket_per_decade;
 	float tol;                           /* finish when ||X^T X - I||_F < tol */
 	float plain_ell;                     /* sigma_min / ||A||_F assumed without a model */
+	/* polar stage 1 -> stage 2: per searchlight (scale, ell, ||A||_F); scale 0 = FP64 path */
+	float *sl_par;
+	int32_t learned_scale;               /* sl_par came from the model (retry plain on failure) */
 	/* learned scaling (NULL mlp = off) */
 	const float *mlp; const float *mlp_norm;
 	float mlp_margin_scale;              /* added to the predicted ln(sigma_max / ||A||_F) */
@@ -117,6 +120,10 @@ typedef struct {
 	int32_t rank_check_nt;               /* flag k >= nt as rank deficient */
 } HakLocalArgs;
 
+/* Polar path, stage 1: cross-covariance into the Q store, scale and schedule
+ * bound per searchlight (learned when a->mlp), n_slots workgroups. */
+int hak_crosscov(const HakLocalArgs *a, int n_slots, void *stream);
+/* Stage 2: the polar iteration, a->n_slots workgroups, 4 slot buffers each. */
 int hak_polar(const HakLocalArgs *a, void *stream);
 int hak_jacobi32(const HakLocalArgs *a, void *stream);
 /* FP64 cross-covariance + one-sided Jacobi + Q = U V^T for every searchlight

@@ -12,11 +12,19 @@
 //     MFMAs (edge tiles of kp that is not a multiple of TILE).
 //
 // Operand kinds (element (m, k) of the operand):
-//   OP_F32_GATHER  f32[col(m) * ld + k], col(m) = cols[m]: FP32 data, split
-//                  into bf16 hi/lo while staging (the cross-covariance)
+//   OP_F32_GATHER  f32[col(m) * ld + k], col(m) = cols[m] (cols == nullptr:
+//                  col(m) = m): FP32 data, k contiguous
 //   OP_SPLIT_K     hi/lo[m * ld + k]: pre-split planes, k contiguous
 //   OP_SPLIT_M     hi/lo[m + k * ld]: pre-split planes, m contiguous
+//   OP_F32_M       f32[m + k * ld]: FP32 data, m contiguous
 // Rows m >= mlimit read as zero; every k < K is in bounds.
+//
+// Two precisions share the staging buffers (PREC template argument):
+//   PREC_BF16X3  FP32 operands are split into bf16 hi/lo while staging and
+//                every product runs as the three bf16 MFMAs above;
+//   PREC_F32     operands stay FP32 (only the FP32 kinds) and every product
+//                runs on v_mfma_f32_32x32x2_f32: FP32 products at a sixth of
+//                the bf16x3 rate, for paths whose conditioning needs them.
 //
 // MFMA operand order is swapped (the B fragment feeds the instruction's A
 // slot), so lane % 32 indexes the output row m and output columns are
@@ -34,7 +42,8 @@ typedef float f32x4_t __attribute__((ext_vector_type(4)));
 #define NT 256  // threads per workgroup (4 waves)
 #define KC 32   // K chunk staged in LDS
 
-enum { OP_F32_GATHER = 0, OP_SPLIT_K = 1, OP_SPLIT_M = 2 };
+enum { OP_F32_GATHER = 0, OP_SPLIT_K = 1, OP_SPLIT_M = 2, OP_F32_M = 3 };
+enum { PREC_BF16X3 = 0, PREC_F32 = 1 };
 
 // ---- bf16 helpers (round to nearest even) ----
 __device__ __forceinline__ uint16_t f2bf(float x) {
@@ -75,6 +84,11 @@ __device__ __forceinline__ Opnd opnd_gather(const float *base, const int32_t *co
   o.kind = OP_F32_GATHER; o.f32 = base; o.cols = cols; o.hi = nullptr; o.lo = nullptr; o.ld = ld; o.mlimit = mlimit;
   return o;
 }
+__device__ __forceinline__ Opnd opnd_f32(const float *base, int kind, int ld, int mlimit) {
+  Opnd o;
+  o.kind = kind; o.f32 = base; o.cols = nullptr; o.hi = nullptr; o.lo = nullptr; o.ld = ld; o.mlimit = mlimit;
+  return o;
+}
 __device__ __forceinline__ Opnd opnd_split(const SplitM &s, int kind, int ld, int mlimit) {
   Opnd o;
   o.kind = kind; o.f32 = nullptr; o.cols = nullptr; o.hi = s.hi; o.lo = s.lo; o.ld = ld; o.mlimit = mlimit;
@@ -109,7 +123,17 @@ __device__ __forceinline__ void fetch(const Opnd &o, int m0, int kb, Stage<TILE>
       const int m = e / (KC / 4), kg = e % (KC / 4);
       const int gm = m0 + m;
       uint4 v = make_uint4(0, 0, 0, 0);
-      if (gm < o.mlimit) v = *(const uint4 *)(o.f32 + (int64_t)o.cols[gm] * o.ld + kb + kg * 4);
+      if (gm < o.mlimit) v = *(const uint4 *)(o.f32 + (int64_t)(o.cols ? o.cols[gm] : gm) * o.ld + kb + kg * 4);
+      st.r[it] = v;
+    }
+  } else if constexpr (KIND == OP_F32_M) {
+#pragma unroll
+    for (int it = 0; it < N; ++it) {
+      const int e = tid + it * NT;
+      const int kk = e / (TILE / 4), mq = e % (TILE / 4);
+      const int gm = m0 + mq * 4;
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (gm < o.mlimit) v = *(const uint4 *)(o.f32 + gm + (int64_t)(kb + kk) * o.ld);  // mlimit % 4 == 0
       st.r[it] = v;
     }
   } else if constexpr (KIND == OP_SPLIT_K) {
@@ -147,11 +171,36 @@ __device__ __forceinline__ void fetch(const Opnd &o, int m0, int kb, Stage<TILE>
 
 __device__ __forceinline__ uint32_t pack2(uint16_t a, uint16_t b) { return (uint32_t)a | ((uint32_t)b << 16); }
 
-template <int TILE, int KIND>
+// FP32 staging view of one operand buffer: k quad q (k = 4q .. 4q+3) of row m
+// lives in the hi array for q < KC/8 and in the lo array above.
+template <int TILE>
+__device__ __forceinline__ float4 *f32_quad(uint4 (*sh)[TILE], uint4 (*sl)[TILE], int q, int m) {
+  return reinterpret_cast<float4 *>(q < KC / 8 ? &sh[q][m] : &sl[q - KC / 8][m]);
+}
+
+template <int TILE, int KIND, int PREC = PREC_BF16X3>
 __device__ __forceinline__ void stash(const Stage<TILE> &st, uint4 (*sh)[TILE], uint4 (*sl)[TILE]) {
   constexpr int N = Stage<TILE>::N;
   const int tid = threadIdx.x;
-  if constexpr (KIND == OP_F32_GATHER) {
+  if constexpr (PREC == PREC_F32) {
+    static_assert(KIND == OP_F32_GATHER || KIND == OP_F32_M, "FP32 products take FP32 operands");
+#pragma unroll
+    for (int it = 0; it < N; ++it) {
+      const int e = tid + it * NT;
+      const uint4 u = st.r[it];
+      if constexpr (KIND == OP_F32_GATHER) {
+        const int m = e / (KC / 4), kg = e % (KC / 4);
+        *f32_quad<TILE>(sh, sl, kg, m) = make_float4(__uint_as_float(u.x), __uint_as_float(u.y),
+                                                     __uint_as_float(u.z), __uint_as_float(u.w));
+      } else {
+        const int kk = e / (TILE / 4), mq = e % (TILE / 4);
+        const uint32_t w[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+          reinterpret_cast<float *>(f32_quad<TILE>(sh, sl, kk >> 2, mq * 4 + p))[kk & 3] = __uint_as_float(w[p]);
+      }
+    }
+  } else if constexpr (KIND == OP_F32_GATHER) {
 #pragma unroll
     for (int it = 0; it < N; ++it) {
       const int e = tid + it * NT;
@@ -236,7 +285,42 @@ __device__ __forceinline__ void mma_chunk(const GemmSmem<TILE> &sm, int buf, Acc
   }
 }
 
-template <int TILE, int KA, int KB>
+// FP32 products: v_mfma_f32_32x32x2_f32 takes k = 2 s + lane/32 of step s, so
+// one float4 quad serves two steps (components lane/32 and 2 + lane/32).
+template <int TILE>
+__device__ __forceinline__ void mma_chunk_f32(GemmSmem<TILE> &sm, int buf, Acc<TILE> &acc, const bool *bm,
+                                              const bool *bn) {
+  constexpr int W = TILE / 64;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm0 = (wave >> 1) * (TILE / 2), wn0 = (wave & 1) * (TILE / 2);
+  const int h = lane >> 5;
+#pragma unroll
+  for (int q = 0; q < KC / 4; ++q) {
+    float a0[W], a1[W], b0[W], b1[W];
+#pragma unroll
+    for (int i = 0; i < W; ++i) {
+      const float4 av = *f32_quad<TILE>(sm.ah[buf], sm.al[buf], q, wm0 + i * 32 + (lane & 31));
+      a0[i] = h ? av.y : av.x;
+      a1[i] = h ? av.w : av.z;
+    }
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+      const float4 bv = *f32_quad<TILE>(sm.bh[buf], sm.bl[buf], q, wn0 + j * 32 + (lane & 31));
+      b0[j] = h ? bv.y : bv.x;
+      b1[j] = h ? bv.w : bv.z;
+    }
+#pragma unroll
+    for (int i = 0; i < W; ++i)
+#pragma unroll
+      for (int j = 0; j < W; ++j) {
+        if (!(bm[i] && bn[j])) continue;
+        acc.v[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(b0[j], a0[i], acc.v[i][j], 0, 0, 0);
+        acc.v[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(b1[j], a1[i], acc.v[i][j], 0, 0, 0);
+      }
+  }
+}
+
+template <int TILE, int KA, int KB, int PREC = PREC_BF16X3>
 __device__ __forceinline__ void tile_gemm(GemmSmem<TILE> &sm, const Opnd &A, const Opnd &B, int tm0, int tn0,
                                           int K, Acc<TILE> &acc) {
   constexpr int W = TILE / 64;
@@ -258,8 +342,8 @@ __device__ __forceinline__ void tile_gemm(GemmSmem<TILE> &sm, const Opnd &A, con
   Stage<TILE> sa, sb;
   fetch<TILE, KA>(A, tm0, 0, sa);
   fetch<TILE, KB>(B, tn0, 0, sb);
-  stash<TILE, KA>(sa, sm.ah[0], sm.al[0]);
-  stash<TILE, KB>(sb, sm.bh[0], sm.bl[0]);
+  stash<TILE, KA, PREC>(sa, sm.ah[0], sm.al[0]);
+  stash<TILE, KB, PREC>(sb, sm.bh[0], sm.bl[0]);
   __syncthreads();
   for (int c = 0; c < nc; ++c) {
     const int cur = c & 1;
@@ -268,10 +352,13 @@ __device__ __forceinline__ void tile_gemm(GemmSmem<TILE> &sm, const Opnd &A, con
       fetch<TILE, KA>(A, tm0, (c + 1) * KC, sa);
       fetch<TILE, KB>(B, tn0, (c + 1) * KC, sb);
     }
-    mma_chunk<TILE>(sm, cur, acc, bm, bn);
+    if constexpr (PREC == PREC_F32)
+      mma_chunk_f32<TILE>(sm, cur, acc, bm, bn);
+    else
+      mma_chunk<TILE>(sm, cur, acc, bm, bn);
     if (more) {
-      stash<TILE, KA>(sa, sm.ah[cur ^ 1], sm.al[cur ^ 1]);
-      stash<TILE, KB>(sb, sm.bh[cur ^ 1], sm.bl[cur ^ 1]);
+      stash<TILE, KA, PREC>(sa, sm.ah[cur ^ 1], sm.al[cur ^ 1]);
+      stash<TILE, KB, PREC>(sb, sm.bh[cur ^ 1], sm.bl[cur ^ 1]);
     }
     __syncthreads();
   }
@@ -301,7 +388,7 @@ __device__ __forceinline__ void acc_to_lds(const Acc<TILE> &acc, float *ctile) {
 // writes). sym (M == N, A == B or a symmetric result): only tiles I <= J are
 // computed and every off-diagonal tile is also delivered transposed, so epi
 // still sees every (m, n) of the full product exactly once.
-template <int TILE, int KA, int KB, class Epi>
+template <int TILE, int KA, int KB, int PREC = PREC_BF16X3, class Epi>
 __device__ void wg_gemm_mn(GemmSmem<TILE> &sm, const Opnd &A, const Opnd &B, int M, int N, int K, bool sym,
                            Epi epi) {
   const int ntm = (M + TILE - 1) / TILE, ntn = (N + TILE - 1) / TILE;
@@ -309,7 +396,7 @@ __device__ void wg_gemm_mn(GemmSmem<TILE> &sm, const Opnd &A, const Opnd &B, int
   for (int I = 0; I < ntm; ++I)
     for (int J = sym ? I : 0; J < ntn; ++J) {
       Acc<TILE> acc;
-      tile_gemm<TILE, KA, KB>(sm, A, B, I * TILE, J * TILE, K, acc);
+      tile_gemm<TILE, KA, KB, PREC>(sm, A, B, I * TILE, J * TILE, K, acc);
       acc_to_lds<TILE>(acc, sm.ctile);
       __syncthreads();
       const int mmax = min(TILE, M - I * TILE), nmax = min(TILE, N - J * TILE);
@@ -329,10 +416,10 @@ __device__ void wg_gemm_mn(GemmSmem<TILE> &sm, const Opnd &A, const Opnd &B, int
 }
 
 // Square kp x kp product (see wg_gemm_mn).
-template <int TILE, int KA, int KB, class Epi>
+template <int TILE, int KA, int KB, int PREC = PREC_BF16X3, class Epi>
 __device__ __forceinline__ void wg_gemm(GemmSmem<TILE> &sm, const Opnd &A, const Opnd &B, int kp, int K, bool sym,
                                         Epi epi) {
-  wg_gemm_mn<TILE, KA, KB>(sm, A, B, kp, kp, K, sym, epi);
+  wg_gemm_mn<TILE, KA, KB, PREC>(sm, A, B, kp, kp, K, sym, epi);
 }
 
 #endif  // K_GEMM_H

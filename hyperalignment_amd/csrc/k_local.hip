@@ -186,48 +186,51 @@ __device__ __noinline__ void mlp_predict(const HakLocalArgs &a, const float *fea
   hak_mlp_forward_wg(a.mlp, a.mlp_norm, feat_sh, mlp_sh, pred);
 }
 
+// Stage 1 of the polar path: per searchlight, the gathered cross-covariance
+// A = X[:, sl]^T Y[:, sl] (bf16x3 MFMA) written into the searchlight's place in
+// the Q store, ||A||_F, and the starting scale and schedule bound: plain
+// (scale = ||A||_F, ell = plain_ell) or learned (features + MLP prediction).
+// sl_par[3 s] = (scale, ell, ||A||_F); scale = 0 marks a searchlight left to
+// the FP64 path (k >= nt, or A zero / non-finite).
 template <int TILE>
-__global__ __launch_bounds__(NT, TILE == 128 ? 1 : 2) void k_polar(HakLocalArgs a) {
+__global__ __launch_bounds__(NT, 2) void k_crosscov(HakLocalArgs a) {
   __shared__ GemmSmem<TILE> sm;
   __shared__ double red[4];
   __shared__ int32_t item_sh;
   __shared__ float feat_sh[HAK_NFEAT];
   __shared__ float mlp_sh[2 * 64 + 2];
   const int slot = blockIdx.x;
-  WorkSlots w = slot_buffers(a, slot);
+  float *scratch = a.work + (int64_t)slot * a.slot_stride;  // features: 6 kp floats
   for (;;) {
     const int it = next_item(a.queue, &item_sh);
     if (it >= a.n_order) break;
     const int s = a.order ? a.order[it] : it;
     const int k = a.sl_off[s + 1] - a.sl_off[s];
     const int kp = a.qld[s];
-    float *Q = a.q + a.qoff[s];
+    float *A = a.q + a.qoff[s];
     if (a.rank_check_nt && k >= a.nt) {
       if (threadIdx.x == 0) {
         a.flag[s] = 1;
         if (a.steps) a.steps[s] = 0;
+        a.sl_par[3 * s] = 0.f;
       }
       continue;
     }
-    uint64_t t0 = a.clk ? wall_clock64() : 0;
-    crosscov<TILE>(sm, a, s, k, kp, w.A);
+    const uint64_t t0 = a.clk ? wall_clock64() : 0;
+    crosscov<TILE>(sm, a, s, k, kp, A);
     __syncthreads();
-    uint64_t t1 = a.clk ? wall_clock64() : 0;
-    // scale and schedule bucket
-    const double fro2 = frob2(w.A, kp, red);
-    const float fro = (float)sqrt(fro2);
+    const uint64_t t1 = a.clk ? wall_clock64() : 0;
+    const float fro = (float)sqrt(frob2(A, kp, red));
+    // plain: X0 = A / ||A||_F (sigma_max(X0) <= 1 always), sigma_min(X0) >= plain_ell assumed
+    float scale = fro, ell = a.plain_ell;
     if (!(fro > 0.f) || !isfinite(fro)) {
+      scale = 0.f;
       if (threadIdx.x == 0) {
         a.flag[s] = 1;
         if (a.steps) a.steps[s] = 0;
       }
-      continue;
-    }
-    // plain: X0 = A / ||A||_F (sigma_max(X0) <= 1 always), sigma_min(X0) >= plain_ell assumed
-    float scale = fro, ell = a.plain_ell;
-    bool learned = false;
-    if (a.features || a.mlp) {
-      features_of(a, s, k, kp, w.A, w.P, feat_sh, red);
+    } else if (a.features || a.mlp) {
+      features_of(a, s, k, kp, A, scratch, feat_sh, red);
       if (a.features && threadIdx.x < HAK_NFEAT)
         a.features[(int64_t)s * HAK_NFEAT + threadIdx.x] = feat_sh[threadIdx.x];
       if (a.mlp) {
@@ -239,38 +242,81 @@ __global__ __launch_bounds__(NT, TILE == 128 ? 1 : 2) void k_polar(HakLocalArgs 
         const float lf = fminf(pred[0] + a.mlp_margin_scale, 0.f);
         scale = fro * expf(lf);
         ell = exp10f(pred[1] - a.mlp_margin_ell) * expf(pred[0] - lf);
-        learned = true;
         if (a.pred_out && threadIdx.x == 0) {
           a.pred_out[2 * s] = pred[0];
           a.pred_out[2 * s + 1] = pred[1];
         }
       }
     }
-    uint64_t t2 = a.clk ? wall_clock64() : 0;
-    const PolarOut po = polar_solve<TILE>(sm, a, w, k, kp, fro, scale, ell, learned, red,
-                                          a.trace ? a.trace + (int64_t)s * a.max_steps : nullptr,
-                                          [&](int m, int n, float v) { Q[m + n * kp] = v; });
-    const bool done = po.done;
-    const int step = po.steps;
-    if (!done) {
-      // budget exhausted or non-finite: the FP64 Jacobi pass takes over (writes Q)
-      if (threadIdx.x == 0) a.flag[s] = 1;
-    } else if (a.sigma_sum) {
-      double tr = 0.0;
-      for (int64_t e = threadIdx.x; e < (int64_t)kp * kp; e += NT) tr += (double)Q[e] * w.A[e];
-      tr = wg_sum_f64(tr, red);
-      if (threadIdx.x == 0) a.sigma_sum[s] = tr;
+    if (threadIdx.x == 0) {
+      a.sl_par[3 * s] = scale;
+      a.sl_par[3 * s + 1] = ell;
+      a.sl_par[3 * s + 2] = fro;
+      if (a.clk) {
+        const uint64_t t2 = wall_clock64();
+        a.clk[4 * slot + 0] += t1 - t0;  // gather + cross-covariance
+        a.clk[4 * slot + 1] += t2 - t1;  // scale, features, MLP
+        a.clk[4 * slot + 3] += 1;
+      }
     }
-    if (threadIdx.x == 0 && a.steps) a.steps[s] = done ? step : -step;
-    if (a.clk && threadIdx.x == 0) {
-      const uint64_t t3 = wall_clock64();
-      a.clk[4 * slot + 0] += t1 - t0;  // gather + cross-covariance
-      a.clk[4 * slot + 1] += t2 - t1;  // scale, features, MLP
-      a.clk[4 * slot + 2] += t3 - t2;  // iteration
-      a.clk[4 * slot + 3] += 1;
+  }
+}
+
+// Stage 2: the polar iteration of every searchlight stage 1 kept. A is read
+// from the Q store and Q overwrites it in the finishing product (each element
+// is read before the same thread writes it). Without the gather and the
+// learned prologue the kernel fits three workgroups per CU, so three slots'
+// operand loads are in flight per CU.
+template <int TILE>
+__global__ __launch_bounds__(NT, TILE == 128 ? 1 : 3) void k_polar(HakLocalArgs a) {
+  __shared__ GemmSmem<TILE> sm;
+  __shared__ double red[4];
+  __shared__ int32_t item_sh;
+  const int slot = blockIdx.x;
+  WorkSlots w;
+  {
+    float *b = a.work + (int64_t)slot * 4 * a.slot_stride;
+    w.X = b;
+    w.X2 = b + a.slot_stride;
+    w.G = b + 2 * a.slot_stride;
+    w.P = b + 3 * a.slot_stride;
+  }
+  for (;;) {
+    const int it = next_item(a.queue, &item_sh);
+    if (it >= a.n_order) break;
+    const int s = a.order ? a.order[it] : it;
+    const float scale = a.sl_par[3 * s];
+    if (!(scale > 0.f)) continue;  // left to the FP64 path by stage 1
+    const float ell = a.sl_par[3 * s + 1], fro = a.sl_par[3 * s + 2];
+    const int k = a.sl_off[s + 1] - a.sl_off[s];
+    const int kp = a.qld[s];
+    float *Q = a.q + a.qoff[s];
+    w.A = Q;
+    const uint64_t t0 = a.clk ? wall_clock64() : 0;
+    double tr = 0.0;
+    const bool want_tr = a.sigma_sum != nullptr;
+    const PolarOut po = polar_solve<TILE, PREC_BF16X3>(sm, a, w, k, kp, fro, scale, ell, a.learned_scale != 0,
+                                                       red, a.trace ? a.trace + (int64_t)s * a.max_steps : nullptr,
+                                                       [&](int m, int n, float v) {
+                                                         const int64_t i = m + (int64_t)n * kp;
+                                                         if (want_tr) tr += (double)v * Q[i];
+                                                         Q[i] = v;
+                                                       });
+    if (want_tr) tr = wg_sum_f64(tr, red);
+    if (threadIdx.x == 0) {
+      if (!po.done) a.flag[s] = 1;  // budget exhausted or non-finite: the FP64 pass writes Q
+      else if (want_tr) a.sigma_sum[s] = tr;
+      if (a.steps) a.steps[s] = po.done ? po.steps : -po.steps;
+      if (a.clk) a.clk[4 * slot + 2] += wall_clock64() - t0;  // iteration
     }
     __syncthreads();
   }
+}
+
+extern "C" int hak_crosscov(const HakLocalArgs *a, int n_slots, void *stream) {
+  if (a->n_order <= 0) return 0;
+  hipLaunchKernelGGL(k_crosscov<64>, dim3(n_slots), dim3(NT), 0, (hipStream_t)stream, *a);
+  LAUNCH_RET();
 }
 
 extern "C" int hak_polar(const HakLocalArgs *a, void *stream) {

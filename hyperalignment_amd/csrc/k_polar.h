@@ -99,6 +99,44 @@ __device__ __forceinline__ int next_item(int32_t *queue, int32_t *sh) {
   return *sh;
 }
 
+// x = M * f for a kp x kp FP32 block (FP32 storage of the iterates)
+__device__ __forceinline__ void scale_block(const float *M, float *x, int kp, float f) {
+  const float4 *m4 = reinterpret_cast<const float4 *>(M);
+  float4 *x4 = reinterpret_cast<float4 *>(x);
+  const int64_t n4 = (int64_t)kp * kp / 4;
+  for (int64_t e = threadIdx.x; e < n4; e += NT) {
+    const float4 v = m4[e];
+    x4[e] = make_float4(v.x * f, v.y * f, v.z * f, v.w * f);
+  }
+}
+
+// Storage of a kp x kp iterate in one precision mode: bf16 hi/lo planes for
+// bf16x3 products, plain FP32 for FP32 products. KK / KM: the operand kinds
+// reading the matrix with k contiguous (column m of the matrix is row m of
+// the operand) or with m contiguous.
+template <int PREC>
+struct MatS;
+template <>
+struct MatS<PREC_BF16X3> {
+  static constexpr int KK = OP_SPLIT_K, KM = OP_SPLIT_M;
+  SplitM s;
+  __device__ MatS(float *buf, int kp) : s(as_split(buf, kp)) {}
+  __device__ __forceinline__ void put(int64_t i, float v) const { put_split(s, i, v); }
+  __device__ __forceinline__ float get(int64_t i) const { return get_split(s, i); }
+  __device__ __forceinline__ Opnd opnd(int kind, int ld, int mlimit) const { return opnd_split(s, kind, ld, mlimit); }
+  __device__ __forceinline__ void load_scaled(const float *A, int kp, float f) const { split_scaled(A, s, kp, f); }
+};
+template <>
+struct MatS<PREC_F32> {
+  static constexpr int KK = OP_F32_GATHER, KM = OP_F32_M;
+  float *f;
+  __device__ MatS(float *buf, int) : f(buf) {}
+  __device__ __forceinline__ void put(int64_t i, float v) const { f[i] = v; }
+  __device__ __forceinline__ float get(int64_t i) const { return f[i]; }
+  __device__ __forceinline__ Opnd opnd(int kind, int ld, int mlimit) const { return opnd_f32(f, kind, ld, mlimit); }
+  __device__ __forceinline__ void load_scaled(const float *A, int kp, float v) const { scale_block(A, f, kp, v); }
+};
+
 // Result of one polar factorisation: steps taken (all attempts) and whether it converged.
 struct PolarOut {
   int steps;
@@ -115,11 +153,12 @@ struct PolarOut {
 // learned: (scale, ell) came from a prediction; an attempt that does not
 // converge is redone once from (fro, plain_ell), which always holds
 // sigma_max(X0) <= 1. w.X, w.X2, w.G, w.P are overwritten.
-template <int TILE, class Fin>
+template <int TILE, int PREC, class Fin>
 __device__ __forceinline__ PolarOut polar_solve(GemmSmem<TILE> &sm, const HakLocalArgs &a, const WorkSlots &w,
                                                 int k, int kp, float fro, float scale, float ell, bool learned,
                                                 double *red, float *trace, Fin fin) {
-  const SplitM G = as_split(w.G, kp), P = as_split(w.P, kp);
+  using M = MatS<PREC>;
+  const M G(w.G, kp), P(w.P, kp);
   int step = 0, total = 0;
   bool done = false;
   // A learned attempt whose scale fell below sigma_max diverges within a few
@@ -138,18 +177,18 @@ __device__ __forceinline__ PolarOut polar_solve(GemmSmem<TILE> &sm, const HakLoc
     }
     // X0 = A / scale, stored as bf16 hi/lo planes
     const float inv = 1.0f / scale;
-    split_scaled(w.A, as_split(w.X, kp), kp, inv);
+    M(w.X, kp).load_scaled(w.A, kp, inv);
     __syncthreads();
     float *Xb = w.X, *Xnb = w.X2;
     int restart = 0, restarts = 0;
     for (step = 0; step < a.max_steps; ++step) {
-      const SplitM X = as_split(Xb, kp), Xn = as_split(Xnb, kp);
+      const M X(Xb, kp), Xn(Xnb, kp);
       // G = X^T X and r = ||G - I||_F over the k x k block
       double r2 = 0.0;
       {
-        const Opnd ox = opnd_split(X, OP_SPLIT_K, kp, kp);
-        wg_gemm<TILE, OP_SPLIT_K, OP_SPLIT_K>(sm, ox, ox, kp, kp, true, [&](int m, int n, float v) {
-          put_split(G, m + n * kp, v);
+        const Opnd ox = X.opnd(M::KK, kp, kp);
+        wg_gemm<TILE, M::KK, M::KK, PREC>(sm, ox, ox, kp, kp, true, [&](int m, int n, float v) {
+          G.put(m + n * kp, v);
           if (m < k && n < k) {
             const float d = v - (m == n ? 1.f : 0.f);
             r2 += (double)d * d;
@@ -190,33 +229,33 @@ __device__ __forceinline__ PolarOut polar_solve(GemmSmem<TILE> &sm, const HakLoc
       }
       // P = ca I + cb G + cc G^2 (G^2 only when cc != 0), zero outside the k x k block
       if (cc != 0.f) {
-        const Opnd og = opnd_split(G, OP_SPLIT_K, kp, kp);
-        wg_gemm<TILE, OP_SPLIT_K, OP_SPLIT_K>(sm, og, og, kp, kp, true, [&](int m, int n, float v) {
+        const Opnd og = G.opnd(M::KK, kp, kp);
+        wg_gemm<TILE, M::KK, M::KK, PREC>(sm, og, og, kp, kp, true, [&](int m, int n, float v) {
           const int i = m + n * kp;
           float pv = 0.f;
-          if (m < k && n < k) pv = (m == n ? ca : 0.f) + cb * get_split(G, i) + cc * v;
-          put_split(P, i, pv);
+          if (m < k && n < k) pv = (m == n ? ca : 0.f) + cb * G.get(i) + cc * v;
+          P.put(i, pv);
         });
       } else {
         for (int64_t e = threadIdx.x; e < (int64_t)kp * kp; e += NT) {
           const int m = (int)(e % kp), n = (int)(e / kp);
           float pv = 0.f;
-          if (m < k && n < k) pv = (m == n ? ca : 0.f) + cb * get_split(G, e);
-          put_split(P, e, pv);
+          if (m < k && n < k) pv = (m == n ? ca : 0.f) + cb * G.get(e);
+          P.put(e, pv);
         }
       }
       __syncthreads();
       // Xn = X P; the finishing product goes to fin(m, n, v)
       {
-        const Opnd ox = opnd_split(X, OP_SPLIT_M, kp, kp);
-        const Opnd op = opnd_split(P, OP_SPLIT_K, kp, kp);
+        const Opnd ox = X.opnd(M::KM, kp, kp);
+        const Opnd op = P.opnd(M::KK, kp, kp);
         if (finish) {
-          wg_gemm<TILE, OP_SPLIT_M, OP_SPLIT_K>(sm, ox, op, kp, kp, false, [&](int m, int n, float v) {
+          wg_gemm<TILE, M::KM, M::KK, PREC>(sm, ox, op, kp, kp, false, [&](int m, int n, float v) {
             fin(m, n, v);
           });
         } else {
-          wg_gemm<TILE, OP_SPLIT_M, OP_SPLIT_K>(sm, ox, op, kp, kp, false, [&](int m, int n, float v) {
-            put_split(Xn, m + n * kp, v);
+          wg_gemm<TILE, M::KM, M::KK, PREC>(sm, ox, op, kp, kp, false, [&](int m, int n, float v) {
+            Xn.put(m + n * kp, v);
           });
         }
       }

@@ -14,13 +14,16 @@
 //                                        ha_linalg.c:263-291; without z-scoring
 //                                        the plain averages of the C code)
 //
-// Every Procrustes runs on the bf16x3 MFMA engine: A = D_s^T R (K = time
-// points), the scheduled polar iteration (k_polar.h) and the product D_s Q.
+// Every Procrustes runs on the MFMA engine with FP32 products
+// (v_mfma_f32_32x32x2_f32): A = D_s^T R (K = time points), the scheduled
+// polar iteration (k_polar.h) and the product D_s Q. The chain is what needs
+// them: 47 dependent solves on cross-covariances with sigma_min / sigma_max
+// near 1e-4 (20 mm, n = 400) amplify bf16x3 rounding about a hundredfold.
 // Work slot layout (FP32 words, column-major, leading dim ld = roundup(nt, 32),
 // kp = roundup(k, 32), padding rows/columns exactly zero):
-//   dsp[ns]   subject columns as bf16 hi/lo planes (ld x kp each)
+//   dsp[ns]   subject columns, FP32 ld x kp each
 //   al[ns]    aligned subjects, FP32 ld x kp
-//   com, sum  FP32 ld x kp; rsp: split planes of the reference (ld x kp)
+//   com, sum  FP32 ld x kp; one more ld x kp block is spare
 //   A, X, X2, G, P: the polar iteration's kp x kp buffers (kpmax x kpmax each)
 // The local templates (nt x k per searchlight) are combined per vertex by
 // k_template_gather with the searchlight weights (searchlight.py:215-236:
@@ -40,10 +43,9 @@ namespace {
 constexpr int TT = 64;  // GEMM tile edge of the template kernel (k is about 100 at 20 mm, order 5)
 
 struct TplSlot {
-  uint16_t *dsp;   // subject s: hi at dsp + 2 s ld kp, lo at + ld kp
-  float *al;       // subject s at al + s ld kp
+  float *dsp;      // subject s at dsp + s ld kpmax
+  float *al;       // subject s at al + s ld kpmax
   float *com, *sum;
-  uint16_t *rsp;   // hi, lo = + ld kp
   WorkSlots w;
 };
 
@@ -51,11 +53,10 @@ __device__ __forceinline__ TplSlot tpl_slot(const HakTplArgs &t, int slot) {
   const int64_t lk = (int64_t)t.ld * t.kpmax, kk = (int64_t)t.kpmax * t.kpmax;
   float *b = t.work + (int64_t)slot * t.slot_stride;
   TplSlot s;
-  s.dsp = reinterpret_cast<uint16_t *>(b);
+  s.dsp = b;
   s.al = b + t.ns * lk;
   s.com = s.al + t.ns * lk;
   s.sum = s.com + lk;
-  s.rsp = reinterpret_cast<uint16_t *>(s.sum + lk);
   float *pw = s.sum + 2 * lk;
   s.w.A = pw;
   s.w.X = pw + kk;
@@ -63,13 +64,6 @@ __device__ __forceinline__ TplSlot tpl_slot(const HakTplArgs &t, int slot) {
   s.w.G = pw + 3 * kk;
   s.w.P = pw + 4 * kk;
   return s;
-}
-
-__device__ __forceinline__ SplitM planes(uint16_t *hi, int64_t n) {
-  SplitM m;
-  m.hi = hi;
-  m.lo = hi + n;
-  return m;
 }
 
 // One-pass column z-score of the nt x k block (ld x kp buffer) in FP64:
@@ -98,42 +92,33 @@ __device__ void zscore_cols(float *B, int nt, int ld, int k) {
   __syncthreads();
 }
 
-// R (FP32 ld x kp) -> bf16 hi/lo planes
-__device__ __forceinline__ void split_block(const float *R, const SplitM &r, int64_t n) {
-  for (int64_t e = threadIdx.x; e < n; e += NT) put_split(r, e, R[e]);
-  __syncthreads();
-}
-
 // out = D_s T with T = procrustes(D_s, R) (reflection allowed, optional
 // scaling sum(sigma) / sum_j ||D_s[:, j] - mean||^2). Returns the polar steps,
 // or -1 if the iteration did not converge (T = I is used then).
 __device__ int solve_one(GemmSmem<TT> &sm, const HakTplArgs &t, const HakLocalArgs &pa, const TplSlot &sl,
-                         const int32_t *idx, int sidx, const float *R, float *out, int k, int kp, double *red) {
-  const int64_t lk = (int64_t)t.ld * kp;
-  const SplitM d = planes(sl.dsp + (int64_t)2 * sidx * t.ld * t.kpmax, lk);
-  const SplitM r = planes(sl.rsp, lk);
-  split_block(R, r, lk);
+                         int sidx, const float *R, float *out, int k, int kp, double *red) {
+  const float *D = sl.dsp + (int64_t)sidx * t.ld * t.kpmax;
   // A = D_s^T R over the time points
   {
-    const Opnd od = opnd_split(d, OP_SPLIT_K, t.ld, k);
-    const Opnd orr = opnd_split(r, OP_SPLIT_K, t.ld, k);
+    const Opnd od = opnd_f32(D, OP_F32_GATHER, t.ld, k);
+    const Opnd orr = opnd_f32(R, OP_F32_GATHER, t.ld, k);
     float *A = sl.w.A;
-    wg_gemm_mn<TT, OP_SPLIT_K, OP_SPLIT_K>(sm, od, orr, kp, kp, t.ld, false,
-                                           [&](int m, int n, float v) { A[m + n * kp] = v; });
+    wg_gemm_mn<TT, OP_F32_GATHER, OP_F32_GATHER, PREC_F32>(sm, od, orr, kp, kp, t.ld, false,
+                                                          [&](int m, int n, float v) { A[m + n * kp] = v; });
   }
   __syncthreads();
   const double fro2 = frob2(sl.w.A, kp, red);
   const float fro = (float)sqrt(fro2);
-  const SplitM q = as_split(sl.w.G, kp);  // G is free while the final product runs
+  float *q = sl.w.G;  // G is free while the final product runs
   double tr = 0.0;
   int steps = 0;
   if (fro > 0.f && isfinite(fro)) {
     const float *A = sl.w.A;
-    const PolarOut po = polar_solve<TT>(sm, pa, sl.w, k, kp, fro, fro, pa.plain_ell, false, red, nullptr,
-                                        [&](int m, int n, float v) {
-                                          put_split(q, m + n * kp, v);
-                                          tr += (double)v * A[m + n * kp];
-                                        });
+    const PolarOut po = polar_solve<TT, PREC_F32>(sm, pa, sl.w, k, kp, fro, fro, pa.plain_ell, false, red,
+                                                  nullptr, [&](int m, int n, float v) {
+                                                    q[m + n * kp] = v;
+                                                    tr += (double)v * A[m + n * kp];
+                                                  });
     steps = po.done ? po.steps : -1;
   } else {
     steps = -1;
@@ -142,7 +127,7 @@ __device__ int solve_one(GemmSmem<TT> &sm, const HakTplArgs &t, const HakLocalAr
     // no polar factor (zero or non-finite A, or no convergence): identity
     for (int64_t e = threadIdx.x; e < (int64_t)kp * kp; e += NT) {
       const int m = (int)(e % kp), n = (int)(e / kp);
-      put_split(q, e, (m == n && m < k) ? 1.f : 0.f);
+      q[e] = (m == n && m < k) ? 1.f : 0.f;
     }
   }
   tr = wg_sum_f64(tr, red);
@@ -150,10 +135,9 @@ __device__ int solve_one(GemmSmem<TT> &sm, const HakTplArgs &t, const HakLocalAr
   if (t.scaling && steps >= 0) {
     // sum_j ||D_s[:, j] - mean_j||^2 from the FP32 subject data, in FP64
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const float *D = t.data + (int64_t)sidx * t.nv * t.ld;
     double vs = 0.0;
     for (int j = wave; j < k; j += NT / 64) {
-      const float *c = D + (int64_t)idx[j] * t.ld;
+      const float *c = D + (int64_t)j * t.ld;
       double s1 = 0.0, s2 = 0.0;
       for (int i = lane; i < t.nt; i += 64) {
         s1 += c[i];
@@ -171,10 +155,10 @@ __device__ int solve_one(GemmSmem<TT> &sm, const HakTplArgs &t, const HakLocalAr
   __syncthreads();
   // out = D_s Q: out^T(j, t) = sum_i Q(i, j) D_s(t, i)
   {
-    const Opnd oq = opnd_split(q, OP_SPLIT_K, kp, k);
-    const Opnd odm = opnd_split(d, OP_SPLIT_M, t.ld, t.ld);
+    const Opnd oq = opnd_f32(q, OP_F32_GATHER, kp, k);
+    const Opnd odm = opnd_f32(D, OP_F32_M, t.ld, t.ld);
     const int nt = t.nt, ld = t.ld;
-    wg_gemm_mn<TT, OP_SPLIT_K, OP_SPLIT_M>(sm, oq, odm, kp, ld, kp, false, [&](int j, int tt, float v) {
+    wg_gemm_mn<TT, OP_F32_GATHER, OP_F32_M, PREC_F32>(sm, oq, odm, kp, ld, kp, false, [&](int j, int tt, float v) {
       out[tt + (int64_t)j * ld] = (j < k && tt < nt) ? v * f : 0.f;
     });
   }
@@ -196,14 +180,14 @@ __global__ __launch_bounds__(NT, 2) void k_template(HakTplArgs t, HakLocalArgs p
     const int kp = (k + 31) / 32 * 32;
     const int ld = t.ld, nt = t.nt, ns = t.ns;
     const int64_t lk = (int64_t)ld * kp, lkmax = (int64_t)ld * t.kpmax;
-    // subject columns: split planes for the products, FP32 copies for subject 0
+    // the subjects' columns of this searchlight (subject 0 also starts the common space)
     for (int sub = 0; sub < ns; ++sub) {
       const float *D = t.data + (int64_t)sub * t.nv * ld;
-      const SplitM d = planes(sl.dsp + 2 * sub * lkmax, lk);
+      float *d = sl.dsp + sub * lkmax;
       for (int64_t e = threadIdx.x; e < lk; e += NT) {
         const int tt = (int)(e % ld), i = (int)(e / ld);
         const float v = (i < k && tt < nt) ? D[(int64_t)idx[i] * ld + tt] : 0.f;
-        put_split(d, e, v);
+        d[e] = v;
         if (sub == 0) {
           sl.al[e] = v;
           sl.com[e] = v;
@@ -220,7 +204,7 @@ __global__ __launch_bounds__(NT, 2) void k_template(HakTplArgs t, HakLocalArgs p
     // sequential alignment to the running common space
     for (int sub = 1; sub < ns; ++sub) {
       float *a = sl.al + sub * lkmax;
-      acc_steps(solve_one(sm, t, pa, sl, idx, sub, sl.com, a, k, kp, red));
+      acc_steps(solve_one(sm, t, pa, sl, sub, sl.com, a, k, kp, red));
       if (t.zscore) zscore_cols(a, nt, ld, k);
       for (int64_t e = threadIdx.x; e < lk; e += NT) sl.com[e] = (sl.com[e] + a[e]) * 0.5f;
       __syncthreads();
@@ -240,7 +224,7 @@ __global__ __launch_bounds__(NT, 2) void k_template(HakTplArgs t, HakLocalArgs p
         for (int64_t e = threadIdx.x; e < lk; e += NT) sl.com[e] = (sl.sum[e] - a[e]) * inv_n1;
         __syncthreads();
         if (t.zscore) zscore_cols(sl.com, nt, ld, k);
-        acc_steps(solve_one(sm, t, pa, sl, idx, sub, sl.com, a, k, kp, red));
+        acc_steps(solve_one(sm, t, pa, sl, sub, sl.com, a, k, kp, red));
       }
     }
     // local template

@@ -109,7 +109,8 @@ typedef struct {
 	int64_t steps_total;       /* sum of polar steps over searchlights */
 	int32_t steps_hist[32];    /* histogram of polar steps (last bin = overflow) */
 	int64_t nnz;
-	double  gemm_flops;        /* algorithmic MFMA flops of the last pass */
+	double  gemm_flops;        /* algorithmic flops of the polar iteration stage of the last pass
+	                              (6 k^3 per executed step and searchlight) */
 	double  kernel_ms_polar_avg; /* average duration of one polar launch */
 	int32_t polar_launches;
 } HaHipStats;
