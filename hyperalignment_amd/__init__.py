@@ -114,6 +114,7 @@ _SIGS = {
     "ha_hip_plan_spectra": (ctypes.c_int, [ctypes.c_void_p, _P(_i32), _i32, _P(_f), _P(_f)]),
     "ha_hip_plan_steps": (ctypes.c_int, [ctypes.c_void_p, _P(_i32), _P(_i32)]),
     "ha_hip_plan_predictions": (ctypes.c_int, [ctypes.c_void_p, _P(_f)]),
+    "ha_hip_plan_tiles": (ctypes.c_int, [ctypes.c_void_p, _P(_i32)]),
     "ha_searchlight_template_flat": (ctypes.c_int, [_P(_P(_d)), _i32, _i32, _i32, _P(_i32), _P(_i32), _P(_d), _i32,
                                                      _d, ctypes.c_bool, ctypes.c_bool, ctypes.c_bool, _i32,
                                                      _P(_d), ctypes.c_bool]),
@@ -507,6 +508,12 @@ class Plan:
         p = np.zeros((self.count, 2), dtype=np.float32)
         _check("ha_hip_plan_predictions", lib().ha_hip_plan_predictions(self.h, _ptr(p, ctypes.c_float)))
         return p
+
+    def tiles(self):
+        """Iteration tile per kp bucket (kp <= 64, 128, 256, 384, 512, larger)."""
+        t = np.zeros(6, dtype=np.int32)
+        _check("ha_hip_plan_tiles", lib().ha_hip_plan_tiles(self.h, _ptr(t, ctypes.c_int32)))
+        return t
 
     def stats(self):
         s = HaHipStats()

@@ -23,6 +23,8 @@ int ha_polar_schedule(double ell0, int max_steps, float *out, int *growth_len);
 #define HA_SCHED_BUCKETS 37           /* ell_b = 10^(-b/4), b = 0..36 (down to 1e-9) */
 #define HA_SCHED_PER_DECADE 4.0f
 #define HA_SCHED_STEPS 28
+/* kp buckets of the iteration tile choice: kp <= 64, 128, 256, 384, 512, larger */
+#define HA_TILE_BUCKETS 6
 
 /* Plan options beyond the public ones. */
 typedef struct {

@@ -120,6 +120,8 @@ typedef struct {
 	int32_t rank_check_nt;               /* flag k >= nt as rank deficient */
 } HakLocalArgs;
 
+/* Workgroups per CU of the polar iteration stage (launch bound and slot count). */
+#define HAK_POLAR_OCC 3
 /* Polar path, stage 1: cross-covariance into the Q store, scale and schedule
  * bound per searchlight (learned when a->mlp), n_slots workgroups. */
 int hak_crosscov(const HakLocalArgs *a, int n_slots, void *stream);

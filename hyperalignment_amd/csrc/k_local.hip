@@ -265,10 +265,10 @@ __global__ __launch_bounds__(NT, 2) void k_crosscov(HakLocalArgs a) {
 // Stage 2: the polar iteration of every searchlight stage 1 kept. A is read
 // from the Q store and Q overwrites it in the finishing product (each element
 // is read before the same thread writes it). Without the gather and the
-// learned prologue the kernel fits three workgroups per CU, so three slots'
-// operand loads are in flight per CU.
+// learned prologue the kernel fits HAK_POLAR_OCC workgroups per CU, so that
+// many slots' operand loads are in flight per CU.
 template <int TILE>
-__global__ __launch_bounds__(NT, TILE == 128 ? 1 : 3) void k_polar(HakLocalArgs a) {
+__global__ __launch_bounds__(NT, TILE == 128 ? 1 : HAK_POLAR_OCC) void k_polar(HakLocalArgs a) {
   __shared__ GemmSmem<TILE> sm;
   __shared__ double red[4];
   __shared__ int32_t item_sh;

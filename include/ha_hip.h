@@ -133,6 +133,11 @@ int ha_hip_plan_load(HaHipPlan *plan, const double *X, const double *Y, bool col
  * Synchronous. */
 int ha_hip_plan_execute(HaHipPlan *plan, bool isReflection, bool isScaling);
 
+/* Iteration tile (64 or 128) chosen for each kp bucket (kp <= 64, 128, 256,
+ * 384, 512, larger): the tile option when set, else timed on a sample of each
+ * large bucket on the first polar pass (the plan's data must be loaded). */
+int ha_hip_plan_tiles(HaHipPlan *plan, int32_t bucket_tile[6]);
+
 int ha_hip_plan_nnz(const HaHipPlan *plan, int64_t *nnz);
 /* Downloads the CSC pattern and values; any pointer may be NULL. */
 int ha_hip_plan_download(const HaHipPlan *plan, int32_t *indptr, int32_t *indices, double *data);

@@ -265,3 +265,49 @@ def test_rank_deficient_searchlights(method):
         l = lists[s]
         _check_rank_deficient(X[:, l], Y[:, l], T.astype(np.float64), 2e-6)
     plan.close()
+
+
+# ---- captured passes and the tile choice ----
+
+def test_graph_replay_matches_eager():
+    """A pass captured in a hipGraph and replayed gives the eager result bit for
+    bit on every replay, with eager passes and host reads between replays."""
+    w = synth.make_workload("c1")
+    s = w.sls
+    plan = ha.Plan(w.X.shape[0], s.nv, s.indices, s.offsets, s.dists, s.radius)
+    plan.load(w.X, w.Y)
+    ha.set_options(method="polar", use_graph=0)
+    plan.execute()
+    ref = plan.csc()[2].copy()
+    try:
+        for rep in range(3):
+            ha.set_options(use_graph=1)
+            plan.execute()
+            got = plan.csc()[2]
+            assert np.array_equal(got.view(np.uint64), ref.view(np.uint64)), rep
+            ha.set_options(use_graph=0)
+            plan.execute()
+            assert np.array_equal(plan.csc()[2].view(np.uint64), ref.view(np.uint64))
+    finally:
+        ha.set_options(use_graph=0)
+        plan.close()
+
+
+@pytest.mark.parametrize("tile", [64, 128])
+def test_forced_tile_matches_default(tile):
+    """Every iteration tile gives the same transforms within the bf16x3 bar."""
+    w = synth.make_workload("c1")
+    s = w.sls
+    plan = ha.Plan(w.X.shape[0], s.nv, s.indices, s.offsets, s.dists, s.radius)
+    plan.load(w.X, w.Y)
+    try:
+        ha.set_options(method="polar", tile=0)
+        plan.execute()
+        ref = plan.csc()[2].copy()
+        ha.set_options(tile=tile)
+        plan.execute()
+        assert set(plan.tiles().tolist()) == {tile}
+        assert np.abs(plan.csc()[2] - ref).max() < TOL_T
+    finally:
+        ha.set_options(tile=0)
+        plan.close()
