@@ -103,6 +103,18 @@ def comparator(w, args, every):
     return res
 
 
+def copy_calibration():
+    """One device-to-device copy of 2^28 float32 (1 GiB read, 1 GiB written),
+    the known-size kernel the FETCH_SIZE / WRITE_SIZE counters are scaled by."""
+    import torch
+    x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": 4 << 28}))
+
+
 def pmc_traffic(config, method):
     """HBM bytes per k_polar launch from the committed rocprofv3 counter summary
     (FETCH_SIZE + WRITE_SIZE, calibrated against a copy kernel), when one exists
@@ -132,7 +144,12 @@ def main():
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--no-compare", action="store_true", help="skip the plain polar / Jacobi32 comparison")
     ap.add_argument("--compare-every", type=int, default=40)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device-to-device copy (reference for the HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               f"(the searchlight pass shards by owner-computes column blocks, not run here)", file=sys.stderr)

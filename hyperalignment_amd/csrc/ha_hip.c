@@ -648,6 +648,12 @@ static void stats_update(HaHipPlan *p, int method) {
 
 /* ---- polar tiles per kp bucket: selection and autotuning ---- */
 
+/* iteration workgroups with 128 x 128 tiles (each needs 4 slot buffers, as with 64) */
+static int32_t slots128(const HaHipPlan *p) {
+	int32_t n = HAK_POLAR_OCC128 * g.n_cu;
+	return n < p->n_slots_polar ? n : p->n_slots_polar;
+}
+
 static int kp_bucket(int32_t kp) {
 	static const int32_t edges[HA_TILE_BUCKETS - 1] = {64, 128, 256, 384, 512};
 	int b = 0;
@@ -697,7 +703,7 @@ static int time_tile(HaHipPlan *p, const int32_t *sel, int32_t n, int tile, floa
 	a.queue = p->queues + 3;
 	KCK(hak_crosscov(&a, p->n_slots, st));
 	a.queue = p->queues + 0;
-	a.n_slots = tile == 64 ? p->n_slots_polar : (p->n_slots_polar + HAK_POLAR_OCC - 1) / HAK_POLAR_OCC;
+	a.n_slots = tile == 64 ? p->n_slots_polar : slots128(p);
 	HCK(hipEventRecord(p->ev[6], st));
 	KCK(hak_polar(&a, st));
 	HCK(hipEventRecord(p->ev[7], st));
@@ -807,7 +813,7 @@ static int enqueue_pass(HaHipPlan *p, HakLocalArgs a, int method, bool isReflect
 			b.order = p->tile_order[t];
 			b.n_order = p->tile_count[t];
 			b.queue = p->queues + 4 + t;
-			b.n_slots = t ? (p->n_slots_polar + HAK_POLAR_OCC - 1) / HAK_POLAR_OCC : p->n_slots_polar;
+			b.n_slots = t ? slots128(p) : p->n_slots_polar;
 			KCK(hak_polar(&b, st));
 		}
 	}

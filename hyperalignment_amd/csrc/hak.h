@@ -122,6 +122,7 @@ typedef struct {
 
 /* Workgroups per CU of the polar iteration stage (launch bound and slot count). */
 #define HAK_POLAR_OCC 3
+#define HAK_POLAR_OCC128 2  /* with 128 x 128 tiles (the autotuner picks them where they win) */
 /* Polar path, stage 1: cross-covariance into the Q store, scale and schedule
  * bound per searchlight (learned when a->mlp), n_slots workgroups. */
 int hak_crosscov(const HakLocalArgs *a, int n_slots, void *stream);

@@ -268,7 +268,7 @@ __global__ __launch_bounds__(NT, 2) void k_crosscov(HakLocalArgs a) {
 // learned prologue the kernel fits HAK_POLAR_OCC workgroups per CU, so that
 // many slots' operand loads are in flight per CU.
 template <int TILE>
-__global__ __launch_bounds__(NT, TILE == 128 ? 1 : HAK_POLAR_OCC) void k_polar(HakLocalArgs a) {
+__global__ __launch_bounds__(NT, TILE == 128 ? HAK_POLAR_OCC128 : HAK_POLAR_OCC) void k_polar(HakLocalArgs a) {
   __shared__ GemmSmem<TILE> sm;
   __shared__ double red[4];
   __shared__ int32_t item_sh;
