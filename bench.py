@@ -52,14 +52,17 @@ def cpu_baseline(w, seconds, threads):
         sl = lists[i]
         return O.procrustes(w.X[:, sl], w.Y[:, sl])
 
+    from threadpoolctl import threadpool_limits
     done, Ts = [], []
-    t0 = time.perf_counter()
-    with ThreadPoolExecutor(threads) as ex:
-        while time.perf_counter() - t0 < seconds and len(done) < s.count:
-            batch = perm[len(done):len(done) + threads]
-            Ts.extend(ex.map(one, batch))
-            done.extend(batch)
-    dt = time.perf_counter() - t0
+    # one LAPACK thread per worker: the pool supplies the parallelism
+    with threadpool_limits(limits=1, user_api="blas"):
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(threads) as ex:
+            while time.perf_counter() - t0 < seconds and len(done) < s.count:
+                batch = perm[len(done):len(done) + threads]
+                Ts.extend(ex.map(one, batch))
+                done.extend(batch)
+        dt = time.perf_counter() - t0
     return len(done) / dt, np.array(done), Ts
 
 
@@ -193,6 +196,15 @@ def main():
                 "frac": achieved / PEAK_BF16, "traffic": pmc_traffic(args.config, method), "kernel": "k_polar",
                 "kernel_ms": polar_avg, "flops_per_launch": st["gemm_flops"]}
     stats_main = st
+    # apply and score the transform on the training data (Yhat = X T, vertex-wise r)
+    t0 = time.perf_counter()
+    _, r = plan.evaluate(w.X, w.Y, want_yhat=False)
+    ev_st = plan.stats()
+    nnz = st["nnz"]
+    ev_bytes = 2 * nt * s.nv * 8 + nnz * 12  # X read once, T (data + index), Yhat written
+    evaluate = {"ms_apply": ev_st["ms_apply"], "ms_corr": ev_st["ms_corr"],
+                "apply_algorithmic_GBs": ev_bytes / (ev_st["ms_apply"] * 1e-3) / 1e9,
+                "r_mean": float(np.nanmean(r)), "host_s": time.perf_counter() - t0}
     compare = None
     if not args.no_compare and method in ("polar", "polar_learned"):
         compare = comparator(w, args, args.compare_every)
@@ -232,8 +244,9 @@ def main():
                    "pattern_ms": st["ms_pattern"], "nnz": st["nnz"], "fp64_fallback": st["n_fallback"],
                    "unconverged": st["n_unconverged"],
                    "mean_steps": st["steps_total"] / max(1, s.count),
-                   "stage_ms": {k: st[k] for k in ("ms_weights", "ms_polar", "ms_fallback", "ms_scatter")},
-                   "parity_vs_fp64": parity, "compare": compare},
+                   "stage_ms": {k: st[k] for k in ("ms_weights", "ms_crosscov", "ms_polar", "ms_fallback",
+                                                   "ms_scatter")},
+                   "parity_vs_fp64": parity, "compare": compare, "evaluate": evaluate},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -69,7 +69,8 @@ class HaHipStats(ctypes.Structure):
                 ("n_unconverged", ctypes.c_int64), ("steps_total", ctypes.c_int64),
                 ("steps_hist", ctypes.c_int32 * 32), ("nnz", ctypes.c_int64),
                 ("gemm_flops", ctypes.c_double), ("kernel_ms_polar_avg", ctypes.c_double),
-                ("polar_launches", ctypes.c_int32)]
+                ("polar_launches", ctypes.c_int32), ("ms_apply", ctypes.c_double),
+                ("ms_corr", ctypes.c_double)]
 
     def as_dict(self):
         d = {name: getattr(self, name) for name, _ in self._fields_}
@@ -115,6 +116,7 @@ _SIGS = {
     "ha_hip_plan_steps": (ctypes.c_int, [ctypes.c_void_p, _P(_i32), _P(_i32)]),
     "ha_hip_plan_predictions": (ctypes.c_int, [ctypes.c_void_p, _P(_f)]),
     "ha_hip_plan_tiles": (ctypes.c_int, [ctypes.c_void_p, _P(_i32)]),
+    "ha_hip_plan_evaluate": (ctypes.c_int, [ctypes.c_void_p, _P(_d), _P(_d), _i32, ctypes.c_bool, _P(_d), _P(_d)]),
     "ha_searchlight_template_flat": (ctypes.c_int, [_P(_P(_d)), _i32, _i32, _i32, _P(_i32), _P(_i32), _P(_d), _i32,
                                                      _d, ctypes.c_bool, ctypes.c_bool, ctypes.c_bool, _i32,
                                                      _P(_d), ctypes.c_bool]),
@@ -508,6 +510,20 @@ class Plan:
         p = np.zeros((self.count, 2), dtype=np.float32)
         _check("ha_hip_plan_predictions", lib().ha_hip_plan_predictions(self.h, _ptr(p, ctypes.c_float)))
         return p
+
+    def evaluate(self, X, Y=None, want_yhat=True):
+        """Yhat = X @ T with the plan's CSC transform and, when Y is given, the
+        vertex-wise correlation of Yhat with Y (benchmark_neuroboros.py:204-211,229-232).
+        Returns (Yhat or None, r or None)."""
+        X = np.asfortranarray(X, dtype=np.float64)
+        nt = X.shape[0]
+        Yf = None if Y is None else np.asfortranarray(Y, dtype=np.float64)
+        yh = np.zeros((nt, self.nv), dtype=np.float64, order="F") if want_yhat else None
+        r = None if Y is None else np.zeros(self.nv, dtype=np.float64)
+        _check("ha_hip_plan_evaluate", lib().ha_hip_plan_evaluate(
+            self.h, _ptr(X, ctypes.c_double), _ptr(Yf, ctypes.c_double), nt, True,
+            _ptr(yh, ctypes.c_double), _ptr(r, ctypes.c_double)))
+        return (None if yh is None else np.ascontiguousarray(yh)), r
 
     def tiles(self):
         """Iteration tile per kp bucket (kp <= 64, 128, 256, 384, 512, larger)."""

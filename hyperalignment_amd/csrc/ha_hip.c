@@ -1139,6 +1139,73 @@ fail:
 	return rc;
 }
 
+/* Yhat = X T (T: the plan's CSC transform) and, when Y is given, the vertex-wise
+ * correlation of Yhat with Y. X, Y: nt x nv_x / nt x ncols FP64. */
+int ha_hip_plan_evaluate(HaHipPlan *p, const double *X, const double *Y, int32_t nt, bool col_major,
+                         double *Yhat_out, double *r_out) {
+	int rc = kHaSuccess;
+	hipStream_t st = g.stream;
+	double *dx = NULL, *dy = NULL, *dyh = NULL, *dr = NULL, *stage = NULL, *host = NULL;
+	if (!p || !X || nt <= 0 || (!Yhat_out && !r_out) || (r_out && !Y)) return kHaErrorArg;
+	if (!p->data || !p->indptr64) { ERR("plan has no CSC transform\n"); return kHaErrorArg; }
+	const int32_t nvx = p->nv_x, nc = p->ncols;
+	ACK(stage, (size_t)nt * (nvx > nc ? nvx : nc) * sizeof(double));
+	ACK(dx, (size_t)nt * nvx * sizeof(double));
+	ACK(dyh, (size_t)nt * nc * sizeof(double));
+	/* column-major FP64 copies (k_convert writes the FP64 copy with leading dim nt) */
+	{
+		float *f32 = NULL;
+		const int32_t ld = (nt + 31) / 32 * 32;
+		const int32_t nmax = nvx > nc ? nvx : nc;
+		ACK(f32, (size_t)ld * nmax * sizeof(float));
+		HCK(hipMemcpyAsync(stage, X, (size_t)nt * nvx * sizeof(double), hipMemcpyHostToDevice, st));
+		rc = hak_convert_data(stage, nt, nvx, col_major ? 1 : 0, f32, ld, dx, st);
+		if (!rc && r_out) {
+			ACK(dy, (size_t)nt * nc * sizeof(double));
+			HCK(hipMemcpyAsync(stage, Y, (size_t)nt * nc * sizeof(double), hipMemcpyHostToDevice, st));
+			rc = hak_convert_data(stage, nt, nc, col_major ? 1 : 0, f32, ld, dy, st);
+		}
+		HCK(hipStreamSynchronize(st));
+		hipFree(f32);
+		if (rc) { rc = kHaErrorInternal; goto fail; }
+	}
+	HCK(hipEventRecord(p->ev[0], st));
+	KCK(hak_spmm_csc(dx, nt, p->indptr64, p->indices, p->data, nc, dyh, st));
+	HCK(hipEventRecord(p->ev[1], st));
+	if (r_out) {
+		ACK(dr, (size_t)nc * sizeof(double));
+		KCK(hak_corr_cols(dyh, dy, nt, nc, dr, st));
+		HCK(hipMemcpyAsync(r_out, dr, (size_t)nc * sizeof(double), hipMemcpyDeviceToHost, st));
+	}
+	HCK(hipEventRecord(p->ev[2], st));
+	if (Yhat_out) {
+		if (col_major) {
+			HCK(hipMemcpyAsync(Yhat_out, dyh, (size_t)nt * nc * sizeof(double), hipMemcpyDeviceToHost, st));
+			HCK(hipStreamSynchronize(st));
+		} else {
+			host = (double *)malloc((size_t)nt * nc * sizeof(double));
+			if (!host) { rc = kHaErrorAlloc; goto fail; }
+			HCK(hipMemcpyAsync(host, dyh, (size_t)nt * nc * sizeof(double), hipMemcpyDeviceToHost, st));
+			HCK(hipStreamSynchronize(st));
+			for (int32_t c = 0; c < nc; ++c)
+				for (int32_t t = 0; t < nt; ++t) Yhat_out[(size_t)t * nc + c] = host[(size_t)c * nt + t];
+		}
+	}
+	HCK(hipStreamSynchronize(st));
+	{
+		float t0 = 0, t1 = 0;
+		hipEventElapsedTime(&t0, p->ev[0], p->ev[1]);
+		hipEventElapsedTime(&t1, p->ev[1], p->ev[2]);
+		p->stats.ms_apply = t0;
+		p->stats.ms_corr = r_out ? t1 : 0.0;
+	}
+fail:
+	hipStreamSynchronize(st);
+	hipFree(stage); hipFree(dx); hipFree(dy); hipFree(dyh); hipFree(dr);
+	free(host);
+	return rc;
+}
+
 int ha_plan_weights(HaHipPlan *p, double *w_flat) {
 	int rc = kHaSuccess;
 	hipStream_t st = g.stream;

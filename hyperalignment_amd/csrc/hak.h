@@ -155,6 +155,14 @@ int hak_var_sum(const double *X64, int32_t nt, const int32_t *slx_idx, const int
 int hak_scale(const double *sigma_sum, const double *var_sum, int32_t nt, int32_t count,
               double *scale, void *stream);
 
+/* ---- applying and scoring a transform (k_apply.hip) ---- */
+/* Y = X T: X nt x nv FP64 column-major (leading dim nt), T CSC with ncols
+ * columns (int64 indptr), Y nt x ncols column-major. */
+int hak_spmm_csc(const double *X, int32_t nt, const int64_t *indptr, const int32_t *indices,
+                 const double *data, int32_t ncols, double *Y, void *stream);
+/* r[v] = population-z-score correlation of column v of A and B (nt x ncols, column-major). */
+int hak_corr_cols(const double *A, const double *B, int32_t nt, int32_t ncols, double *r, void *stream);
+
 /* ---- searchlight Procrustes template (k_template.hip) ---- */
 typedef struct {
 	const float *data;       /* ns subjects, each ld x nv FP32 column-major (rows nt..ld-1 zero) */

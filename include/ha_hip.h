@@ -113,6 +113,8 @@ typedef struct {
 	                              (6 k^3 per executed step and searchlight) */
 	double  kernel_ms_polar_avg; /* average duration of one polar launch */
 	int32_t polar_launches;
+	double  ms_apply;          /* last ha_hip_plan_evaluate: Yhat = X T */
+	double  ms_corr;           /* and the vertex-wise correlation */
 } HaHipStats;
 
 /* Uploads the flat searchlight arrays (layout of hyperalignment_c.py:161-169)
@@ -137,6 +139,15 @@ int ha_hip_plan_execute(HaHipPlan *plan, bool isReflection, bool isScaling);
  * 384, 512, larger): the tile option when set, else timed on a sample of each
  * large bucket on the first polar pass (the plan's data must be loaded). */
 int ha_hip_plan_tiles(HaHipPlan *plan, int32_t bucket_tile[6]);
+
+/* Applies the plan's CSC transform to new data and scores it, the evaluation of
+ * benchmark_neuroboros.py:204-211,229-232: Yhat = X T (X: nt x nv samples, the
+ * plan's nv source vertices; Yhat: nt x nv target vertices) and, when Y and
+ * r_out are given, r[v] = mean_t z(Yhat[:, v]) z(Y[:, v]) with population
+ * z-scores (NaN for a constant column). X, Y, Yhat in the same order
+ * (col_major or row-major); Yhat_out may be NULL when only r is wanted. FP64. */
+int ha_hip_plan_evaluate(HaHipPlan *plan, const double *X, const double *Y, int32_t nt, bool col_major,
+                         double *Yhat_out, double *r_out);
 
 int ha_hip_plan_nnz(const HaHipPlan *plan, int64_t *nnz);
 /* Downloads the CSC pattern and values; any pointer may be NULL. */

@@ -312,3 +312,33 @@ def searchlight_template(dms, idx, offsets, dists, radius, reflection=True, scal
         loc = template_procrustes([d[:, sl[s]] for d in dms], reflection, scaling, zscore_common, level2_iter)
         tpl[:, sl[s]] += loc * w[offsets[s]:offsets[s + 1]][None, :]
     return tpl
+
+
+# ---------------------------------------------------------------------------
+# applying and scoring a transform
+# ---------------------------------------------------------------------------
+
+def apply_transform(X, indptr, indices, data, ncols):
+    """Yhat = X @ W for the CSC transform W (rows: source vertices), as
+    benchmark_neuroboros.py:229-232 (X_test @ Ws). Column c accumulates
+    X[:, indices[q]] * data[q] over its nonzeros in index order, each multiply
+    and add rounded separately (the order of a sequential CSC product)."""
+    X = np.asarray(X, dtype=np.float64)
+    Y = np.zeros((X.shape[0], ncols))
+    for c in range(ncols):
+        acc = np.zeros(X.shape[0])
+        for q in range(indptr[c], indptr[c + 1]):
+            acc = acc + X[:, indices[q]] * data[q]
+        Y[:, c] = acc
+    return Y
+
+
+def vertex_correlation(A, B):
+    """np.mean(zscore(A, axis=0) * zscore(B, axis=0), axis=0) with population
+    z-scores (scipy.stats.zscore, ddof=0), as benchmark_neuroboros.py:204-211."""
+    A = np.asarray(A, dtype=np.float64)
+    B = np.asarray(B, dtype=np.float64)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        za = (A - A.mean(axis=0)) / A.std(axis=0)
+        zb = (B - B.mean(axis=0)) / B.std(axis=0)
+        return (za * zb).mean(axis=0)

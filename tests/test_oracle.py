@@ -198,3 +198,22 @@ def test_searchlight_template_single_searchlight_is_template():
     a = O.searchlight_template(dms, idx, off, None, 0.0, level2_iter=2)
     b = O.template_procrustes(dms, True, False, True, 2)
     assert np.abs(a - b).max() < 1e-12
+
+
+def test_apply_transform_matches_scipy():
+    import scipy.sparse as sp
+    rng = np.random.default_rng(8)
+    W = sp.random(30, 25, density=0.2, random_state=3, format="csc")
+    X = rng.standard_normal((17, 30))
+    got = O.apply_transform(X, W.indptr, W.indices, W.data, 25)
+    assert np.abs(got - X @ W).max() < 1e-13
+
+
+def test_vertex_correlation_is_pearson():
+    rng = np.random.default_rng(9)
+    A = rng.standard_normal((50, 6))
+    B = A + rng.standard_normal((50, 6))
+    r = O.vertex_correlation(A, B)
+    ref = [np.corrcoef(A[:, j], B[:, j])[0, 1] for j in range(6)]
+    assert np.abs(r - ref).max() < 1e-13
+    assert np.isnan(O.vertex_correlation(np.ones((5, 1)), B[:5, :1]))[0]
