@@ -414,7 +414,7 @@ extern "C" int hak_pattern_fill(const int32_t *sl_idx, const int32_t *sl_off, co
 // ---------------------------------------------------------------------------
 
 #define SC_T 256
-#define SC_LDS 4096   // column entries held in LDS (rows int32 + acc double = 48 KB)
+#define SC_LDS 3072   // column entries held in LDS (rows int32 + acc double = 36 KB)
 
 __device__ __forceinline__ int32_t find_row(const int32_t *rows, int32_t L, int32_t r) {
   int32_t lo = 0, hi = L;
@@ -425,6 +425,21 @@ __device__ __forceinline__ int32_t find_row(const int32_t *rows, int32_t L, int3
   return (lo < L && rows[lo] == r) ? lo : -1;
 }
 
+// Per column c (one workgroup), the contributions of every searchlight holding
+// c, in increasing searchlight order with a barrier between searchlights, so
+// each element of data is summed in one fixed order. The occurrences' metadata
+// (searchlight, position, size, Q offset, scale) is staged in LDS a batch at a
+// time by all threads at once, and each searchlight's Q values, weights and
+// vertex ids are loaded one searchlight ahead into registers, so the barrier
+// wait overlaps the next loads instead of a chain of dependent ones.
+#define SC_OCC 256  // occurrences staged per batch
+#define SC_V 2      // values per thread per searchlight held in registers (k <= SC_V * SC_T pipelined)
+
+struct ScVals {
+  double v[SC_V];
+  int32_t r[SC_V];
+};
+
 __global__ __launch_bounds__(SC_T) void k_scatter(
     const int32_t *__restrict__ slx_idx, const int32_t *__restrict__ off, const int32_t *__restrict__ sl_of,
     const int64_t *__restrict__ ptr, const int32_t *__restrict__ occ, int32_t ncols,
@@ -433,10 +448,14 @@ __global__ __launch_bounds__(SC_T) void k_scatter(
     const double *__restrict__ scale, const double *__restrict__ w, int32_t s_begin, int32_t s_end) {
   __shared__ int32_t rows_l[SC_LDS];
   __shared__ double acc_l[SC_LDS];
+  __shared__ int32_t m_o[SC_OCC], m_ks[SC_OCC];
+  __shared__ int64_t m_q[SC_OCC];
+  __shared__ double m_sc[SC_OCC];
+  const int tid = threadIdx.x;
   for (int32_t c = blockIdx.x; c < ncols; c += gridDim.x) {
     int64_t pb = ptr[c], pe = ptr[c + 1];
-    // skip occurrences outside [s_begin, s_end): occ lists are sorted by flat
-    // position, i.e. by searchlight
+    // occurrences outside [s_begin, s_end) are skipped: occ lists are sorted
+    // by flat position, i.e. by searchlight
     while (pb < pe && sl_of[occ[pb]] < s_begin) ++pb;
     int64_t pe2 = pb;
     while (pe2 < pe && sl_of[occ[pe2]] < s_end) ++pe2;
@@ -447,33 +466,74 @@ __global__ __launch_bounds__(SC_T) void k_scatter(
     const int32_t *rows = in_lds ? rows_l : indices + cb;
     double *acc = in_lds ? acc_l : data + cb;
     if (in_lds) {
-      for (int32_t i = threadIdx.x; i < L; i += SC_T) {
+      for (int32_t i = tid; i < L; i += SC_T) {
         rows_l[i] = indices[cb + i];
         acc_l[i] = data[cb + i];
       }
     }
-    __syncthreads();
-    for (int64_t p = pb; p < pe2; ++p) {
-      const int32_t k = occ[p];
-      const int32_t s = sl_of[k];
-      const int32_t o = off[s];
-      const int32_t j = k - o;
-      const int32_t ks = off[s + 1] - o;
-      const float *qc = q + qoff[s] + (int64_t)j * qld[s];
-      const double sc = scale ? scale[s] : 1.0;
-      for (int32_t i = threadIdx.x; i < ks; i += SC_T) {
-        int32_t pos = find_row(rows, L, slx_idx[o + i]);
-        if (pos >= 0) {
-          double t = (double)qc[i];
-          if (scale) t = t * sc;
-          double val = w ? t * w[o + i] : t;
-          acc[pos] += val;
-        }
+    for (int64_t b0 = pb; b0 < pe2; b0 += SC_OCC) {
+      const int32_t nb = (int32_t)((pe2 - b0) < SC_OCC ? (pe2 - b0) : SC_OCC);
+      __syncthreads();  // previous batch done with the metadata
+      for (int32_t e = tid; e < nb; e += SC_T) {
+        const int32_t k = occ[b0 + e];
+        const int32_t s = sl_of[k];
+        const int32_t o = off[s];
+        m_o[e] = o;
+        m_ks[e] = off[s + 1] - o;
+        m_q[e] = qoff[s] + (int64_t)(k - o) * qld[s];  // column j = k - o of Q_s
+        m_sc[e] = scale ? scale[s] : 1.0;
       }
       __syncthreads();
+      auto load = [&](int32_t e, ScVals &x) {
+        const int32_t o = m_o[e], ks = m_ks[e];
+        const float *qc = q + m_q[e];
+        const double sc = m_sc[e];
+#pragma unroll
+        for (int u = 0; u < SC_V; ++u) {
+          const int32_t i = tid + u * SC_T;
+          x.r[u] = -1;
+          x.v[u] = 0.0;
+          if (i < ks) {
+            double t = (double)qc[i];
+            if (scale) t = t * sc;
+            x.v[u] = w ? t * w[o + i] : t;
+            x.r[u] = slx_idx[o + i];
+          }
+        }
+      };
+      ScVals cur;
+      load(0, cur);
+      for (int32_t e = 0; e < nb; ++e) {
+        ScVals nxt;
+        if (e + 1 < nb) load(e + 1, nxt);
+#pragma unroll
+        for (int u = 0; u < SC_V; ++u) {
+          if (cur.r[u] >= 0) {
+            const int32_t pos = find_row(rows, L, cur.r[u]);
+            if (pos >= 0) acc[pos] += cur.v[u];
+          }
+        }
+        // searchlights larger than SC_V * SC_T: the rest, unpipelined
+        const int32_t ks = m_ks[e];
+        if (ks > SC_V * SC_T) {
+          const int32_t o = m_o[e];
+          const float *qc = q + m_q[e];
+          const double sc = m_sc[e];
+          for (int32_t i = tid + SC_V * SC_T; i < ks; i += SC_T) {
+            const int32_t pos = find_row(rows, L, slx_idx[o + i]);
+            if (pos >= 0) {
+              double t = (double)qc[i];
+              if (scale) t = t * sc;
+              acc[pos] += w ? t * w[o + i] : t;
+            }
+          }
+        }
+        __syncthreads();
+        cur = nxt;
+      }
     }
     if (in_lds) {
-      for (int32_t i = threadIdx.x; i < L; i += SC_T) data[cb + i] = acc_l[i];
+      for (int32_t i = tid; i < L; i += SC_T) data[cb + i] = acc_l[i];
     }
     __syncthreads();
   }
