@@ -379,6 +379,45 @@ def searchlight_weights(sls, dists=None, radius=0.0):
     return res
 
 
+# ---- the Python package's own names (src/hyperalignment/__init__.py:1-7) ----
+
+def compute_searchlight_weights(sls, dists=None, radius=None, return_sparse=False):
+    """searchlight.py:12-79 on the GPU: per-searchlight weights summing to one per
+    vertex (distance-based when dists is given). return_sparse: a CSR matrix with
+    one row per searchlight and one column per vertex."""
+    import scipy.sparse as sp
+    if dists is not None and radius is None:
+        raise ValueError("Radius must be provided if distances are given.")
+    w = searchlight_weights(sls, dists, 0.0 if radius is None else radius)
+    if not return_sparse:
+        return w
+    nv = int(max(int(np.max(sl)) for sl in sls)) + 1
+    rows = np.concatenate([np.full(len(sl), i, dtype=np.int64) for i, sl in enumerate(sls)])
+    cols = np.concatenate([np.asarray(sl, dtype=np.int64) for sl in sls])
+    return sp.csr_matrix((np.concatenate(w), (rows, cols)), shape=(len(sls), nv))
+
+
+def initialize_sparse_matrix(sls, nv=None, dtype=np.float64, cache_fn=None):
+    """sparse.py:6-43 with the pattern built on the GPU (ha_sparse_init): a CSC
+    matrix of zeros over the union of the searchlights' vertex pairs. cache_fn
+    is read with scipy.sparse.load_npz when it exists and written with
+    scipy.sparse.save_npz otherwise, so cache files are interchangeable with
+    the reference's."""
+    import scipy.sparse as sp
+    if cache_fn is not None and os.path.exists(cache_fn):
+        return sp.load_npz(cache_fn)
+    if nv is None:
+        nv = int(max(int(np.max(sl)) for sl in sls)) + 1
+    mat = sparse_init(sls, nv)
+    mat.data = np.zeros(mat.nnz, dtype=dtype)
+    if cache_fn is not None:
+        d = os.path.dirname(cache_fn)
+        if d:
+            os.makedirs(d, exist_ok=True)
+        sp.save_npz(cache_fn, mat)
+    return mat
+
+
 def searchlight_procrustes_csc(X, Y, sls, mat, sls_Y=None, weights=None, reflection=True, scaling=False):
     """ha_searchlight_procrustes (CSC API) with backend HIP; mat is accumulated in place.
 

@@ -311,3 +311,34 @@ def test_forced_tile_matches_default(tile):
     finally:
         ha.set_options(tile=0)
         plan.close()
+
+
+# ---- the Python package's API names and the scipy .npz cache ----
+
+def test_initialize_sparse_matrix_npz_cache(tmp_path):
+    """The pattern saved through the cache is a scipy .npz; loading it back, or
+    building it again, gives the reference's union pattern (sparse.py:29-42)."""
+    import scipy.sparse as sp
+    w = synth.make_workload("c0")
+    s = w.sls
+    fn = str(tmp_path / "cache" / "pattern.npz")
+    m = ha.initialize_sparse_matrix(s.lists(), s.nv, cache_fn=fn)
+    p, i = O.sparse_init(s.indices, s.offsets, s.nv)
+    assert np.array_equal(m.indptr.astype(np.int64), p) and np.array_equal(m.indices, i)
+    assert np.all(m.data == 0) and m.data.dtype == np.float64
+    loaded = sp.load_npz(fn)
+    assert loaded.format == "csc" and (loaded != m).nnz == 0
+    again = ha.initialize_sparse_matrix(s.lists(), s.nv, cache_fn=fn)
+    assert np.array_equal(again.indptr, m.indptr) and np.array_equal(again.indices, m.indices)
+
+
+def test_compute_searchlight_weights_sparse():
+    w = synth.make_workload("c0")
+    s = w.sls
+    dl = s.dist_lists()
+    lists = ha.compute_searchlight_weights(s.lists(), dl, s.radius)
+    mat = ha.compute_searchlight_weights(s.lists(), dl, s.radius, return_sparse=True)
+    ref = O.searchlight_weights_flat(s.indices, s.offsets, s.dists, s.radius, s.nv)
+    assert np.array_equal(np.concatenate(lists), ref)
+    assert mat.shape == (s.count, s.nv)
+    assert np.abs(np.asarray(mat.sum(axis=0)).ravel() - 1).max() < 1e-12
