@@ -103,6 +103,11 @@ _SIGS = {
     "ha_sparse_free": (None, [_P(TSparseCSC)]),
     "ha_sparse_scatter_add": (None, [_P(TSparseCSC), _P(_i32), _P(_i32), _i32, _i32, _P(_d), _P(_d)]),
     "ha_procrustes": (ctypes.c_int, [_P(TMat), _P(TMat), _P(TMat), ctypes.c_bool, ctypes.c_bool]),
+    "ha_ridge": (ctypes.c_int, [_P(TMat), _P(TMat), _d, _P(TMat)]),
+    "ha_ridge_vec": (ctypes.c_int, [_P(TMat), _P(_d), _d, _P(_d)]),
+    "ha_searchlight_ridge": (ctypes.c_int, [_P(TMat), _P(TMat), _P(TSearchlights), _P(TSearchlights),
+                                            _P(TSparseCSC), _P(_P(_d)), _d]),
+    "ha_hip_plan_ridge": (ctypes.c_int, [ctypes.c_void_p, _d]),
     "ha_hip_plan_create": (ctypes.c_int, [_i32, _i32, _P(_i32), _P(_i32), _P(_d), _i32, _d,
                                           _P(ctypes.c_void_p)]),
     "ha_hip_plan_destroy": (None, [ctypes.c_void_p]),
@@ -418,11 +423,9 @@ def initialize_sparse_matrix(sls, nv=None, dtype=np.float64, cache_fn=None):
     return mat
 
 
-def searchlight_procrustes_csc(X, Y, sls, mat, sls_Y=None, weights=None, reflection=True, scaling=False):
-    """ha_searchlight_procrustes (CSC API) with backend HIP; mat is accumulated in place.
-
-    mat: scipy.sparse.csc_matrix with sorted indices (e.g. from sparse_init).
-    """
+def _csc_call(fn, X, Y, sls, mat, sls_Y, weights, *extra):
+    """Runs a CSC searchlight entry point (TMat X, Y; TSearchlights; TSparseCSC;
+    per-searchlight weights) and copies the accumulated data back into mat."""
     X = np.ascontiguousarray(X, dtype=np.float64)
     Y = np.ascontiguousarray(Y, dtype=np.float64)
     hx = _SlHolder(sls)
@@ -438,12 +441,70 @@ def searchlight_procrustes_csc(X, Y, sls, mat, sls_Y=None, weights=None, reflect
     if weights is not None:
         warr = [np.ascontiguousarray(w, dtype=np.float64) for w in weights]
         wp = (ctypes.POINTER(ctypes.c_double) * len(warr))(*[_ptr(w, ctypes.c_double) for w in warr])
-    rc = lib().ha_searchlight_procrustes(ctypes.byref(tx), ctypes.byref(ty), ctypes.byref(hx.st),
-                                         ctypes.byref(hy.st) if hy else None, ctypes.byref(csc), wp,
-                                         bool(reflection), bool(scaling), BACKENDS["hip"])
-    _check("ha_searchlight_procrustes", rc)
+    rc = getattr(lib(), fn)(ctypes.byref(tx), ctypes.byref(ty), ctypes.byref(hx.st),
+                            ctypes.byref(hy.st) if hy else None, ctypes.byref(csc), wp, *extra)
+    _check(fn, rc)
     mat.data[:] = data
     return mat
+
+
+def searchlight_procrustes_csc(X, Y, sls, mat, sls_Y=None, weights=None, reflection=True, scaling=False):
+    """ha_searchlight_procrustes (CSC API) with backend HIP; mat is accumulated in place.
+
+    mat: scipy.sparse.csc_matrix with sorted indices (e.g. from sparse_init).
+    """
+    return _csc_call("ha_searchlight_procrustes", X, Y, sls, mat, sls_Y, weights, bool(reflection), bool(scaling),
+                     BACKENDS["hip"])
+
+
+def searchlight_ridge_csc(X, Y, sls, mat, sls_Y=None, weights=None, alpha=1e3):
+    """ha_searchlight_ridge (csrc/ha_searchlight.c:1378-1429) on the GPU: FP64
+    ridge betas per searchlight, accumulated into mat in place (row weights,
+    as the C scatter)."""
+    return _csc_call("ha_searchlight_ridge", X, Y, sls, mat, sls_Y, weights, float(alpha))
+
+
+def searchlight_ridge(X, Y, sls, sls_Y=None, mat0=None, alpha=1e3, weights=None):
+    """searchlight_ridge of src/hyperalignment/searchlight.py:176-212 on the GPU.
+
+    mat0: scipy CSC matrix whose pattern holds every searchlight pair (e.g.
+    from initialize_sparse_matrix); its copy is accumulated and returned. With
+    mat0 None the pattern is built for the searchlights and a dense array is
+    returned, as the reference returns a dense array then. weights multiply the
+    rows of each local betas matrix (the C library's scatter,
+    csrc/ha_sparse.c:128-142); the Python reference multiplies columns
+    (searchlight.py:125).
+    """
+    import scipy.sparse as sp
+    if mat0 is None:
+        if X.shape[1] != Y.shape[1] or sls_Y is not None:
+            raise ValueError("mat0 is required with sls_Y or when X and Y have different numbers of vertices "
+                             "(its pattern must hold every X x Y pair)")
+        mat = initialize_sparse_matrix(sls, nv=X.shape[1])
+        searchlight_ridge_csc(X, Y, sls, mat, None, weights, alpha)
+        return mat.toarray()
+    mat = sp.csc_matrix(mat0, dtype=np.float64, copy=True)
+    mat.sort_indices()
+    return searchlight_ridge_csc(X, Y, sls, mat, sls_Y, weights, alpha)
+
+
+def ridge(X, Y, alpha):
+    """ridge of src/hyperalignment/ridge.py:7-12 / csrc/ha_ridge.c:13-54 on the
+    GPU (FP64 normal equations, the same betas for alpha > 0). Y may be 1-D
+    (ha_ridge_vec, csrc/ha_ridge.c:56-95)."""
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    Y = np.ascontiguousarray(Y, dtype=np.float64)
+    tx = TMat(_ptr(X, ctypes.c_double), X.shape[0], X.shape[1])
+    if Y.ndim == 1:
+        b = np.zeros(X.shape[1], dtype=np.float64)
+        _check("ha_ridge_vec", lib().ha_ridge_vec(ctypes.byref(tx), _ptr(Y, ctypes.c_double), float(alpha),
+                                                  _ptr(b, ctypes.c_double)))
+        return b
+    b = np.zeros((X.shape[1], Y.shape[1]), dtype=np.float64)
+    ty = TMat(_ptr(Y, ctypes.c_double), Y.shape[0], Y.shape[1])
+    tb = TMat(_ptr(b, ctypes.c_double), b.shape[0], b.shape[1])
+    _check("ha_ridge", lib().ha_ridge(ctypes.byref(tx), ctypes.byref(ty), float(alpha), ctypes.byref(tb)))
+    return b
 
 
 def sparse_scatter_add(mat, rows, cols, local_T, weights=None):
@@ -500,6 +561,10 @@ class Plan:
 
     def execute(self, reflection=True, scaling=False):
         _check("ha_hip_plan_execute", lib().ha_hip_plan_execute(self.h, bool(reflection), bool(scaling)))
+
+    def ridge(self, alpha):
+        """Searchlight ridge pass into the plan's CSC data (ha_hip_plan_ridge)."""
+        _check("ha_hip_plan_ridge", lib().ha_hip_plan_ridge(self.h, float(alpha)))
 
     def nnz(self):
         n = ctypes.c_int64()

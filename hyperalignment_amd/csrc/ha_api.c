@@ -25,6 +25,7 @@
 #include "ha_sparse.h"
 #include "ha_procrustes.h"
 #include "ha_template.h"
+#include "ha_ridge.h"
 #include "hak.h"
 
 #define ERR(...) fprintf(stderr, "ha_hip: " __VA_ARGS__)
@@ -285,6 +286,100 @@ int ha_searchlight_procrustes(const TMat *X, const TMat *Y, const TSearchlights 
 		return kHaErrorArg;
 	}
 	return ha_searchlight_procrustes_hip_csc(X, Y, sls_X, sls_Y, mat, weights, isReflection, isScaling);
+}
+
+/* ---- searchlight ridge ---- */
+
+int ha_searchlight_ridge(const TMat *X, const TMat *Y, const TSearchlights *sls_X, const TSearchlights *sls_Y,
+                         TSparseCSC *mat, double **weights, double alpha) {
+	if (!X || !Y || !sls_X || !mat || !X->data || !Y->data) return kHaErrorArg;
+	if (X->rows != Y->rows || X->rows <= 0) return kHaErrorArg;
+	const TSearchlights *sy = sls_Y ? sls_Y : sls_X;
+	if (sy->count != sls_X->count) return kHaErrorArg;
+	for (int32_t s = 0; s < sls_X->count; ++s)
+		if (sy->sizes[s] != sls_X->sizes[s]) {
+			ERR("searchlight %d has %d X and %d Y vertices; this backend needs equal sizes\n", s,
+			    sls_X->sizes[s], sy->sizes[s]);
+			return kHaErrorArg;
+		}
+	Flat fx, fy;
+	int rc = flatten(sls_X, false, &fx);
+	if (rc) return rc;
+	memset(&fy, 0, sizeof(fy));
+	if (sls_Y && sls_Y != sls_X) {
+		rc = flatten(sls_Y, false, &fy);
+		if (rc) { flat_free(&fx); return rc; }
+	}
+	double *wflat = NULL;
+	HaHipPlan *p = NULL;
+	if (weights) {
+		wflat = (double *)malloc((size_t)fx.total * sizeof(double));
+		if (!wflat) { rc = kHaErrorAlloc; goto done; }
+		for (int32_t s = 0; s < sls_X->count; ++s)
+			memcpy(wflat + fx.off[s], weights[s], (size_t)sls_X->sizes[s] * sizeof(double));
+	}
+	HaPlanSetup su = {.build_pattern = false, .build_weights_index = false, .index_only = false,
+	                  .nv_x = X->cols, .nv_y = Y->cols};
+	rc = ha_plan_create_ex(X->rows, &su, fx.idx, fy.idx ? fy.idx : NULL, fx.off, NULL, sls_X->count, 0.0, &p);
+	if (rc) goto done;
+	rc = ha_plan_set_pattern(p, mat->cols, mat->indptr, mat->indices, mat->nnz, mat->data);
+	if (rc) goto done;
+	rc = ha_plan_set_weights(p, wflat, false);
+	if (rc) goto done;
+	rc = ha_plan_load_xy(p, X->data, Y->data, false);
+	if (rc) goto done;
+	rc = ha_plan_ridge(p, alpha, false, NULL);
+	if (rc) goto done;
+	rc = ha_plan_download_data(p, mat->data);
+done:
+	ha_hip_plan_destroy(p);
+	free(wflat);
+	flat_free(&fx);
+	flat_free(&fy);
+	return rc;
+}
+
+/* ---- ridge (one problem) ---- */
+
+/* X (M x N) and Y (M x P) are padded with zero columns to K = max(N, P) and
+ * solved as one K-vertex searchlight: for alpha > 0 the padded system is block
+ * diagonal, diag(X^T X + alpha I, alpha I), so betas[:N, :P] are unchanged. */
+int ha_ridge(const TMat *X, const TMat *Y, double alpha, TMat *betas) {
+	if (!X || !Y || !betas || !X->data || !Y->data || !betas->data) return kHaErrorArg;
+	const int32_t M = X->rows, N = X->cols, P = Y->cols;
+	if (M <= 0 || N <= 0 || P <= 0 || Y->rows != M || betas->rows != N || betas->cols != P) return kHaErrorArg;
+	const int32_t K = N > P ? N : P;
+	double *xp = (double *)calloc((size_t)M * K, sizeof(double));
+	double *yp = (double *)calloc((size_t)M * K, sizeof(double));
+	double *bk = (double *)malloc((size_t)K * K * sizeof(double));
+	int32_t *idx = (int32_t *)malloc((size_t)K * sizeof(int32_t));
+	int32_t off[2] = {0, K};
+	HaHipPlan *p = NULL;
+	int rc = kHaErrorAlloc;
+	if (!xp || !yp || !bk || !idx) goto done;
+	for (int32_t t = 0; t < M; ++t) {
+		memcpy(xp + (int64_t)t * K, X->data + (int64_t)t * N, (size_t)N * sizeof(double));
+		memcpy(yp + (int64_t)t * K, Y->data + (int64_t)t * P, (size_t)P * sizeof(double));
+	}
+	for (int32_t i = 0; i < K; ++i) idx[i] = i;
+	HaPlanSetup su = {.build_pattern = false, .build_weights_index = false, .index_only = false,
+	                  .nv_x = K, .nv_y = K};
+	rc = ha_plan_create_ex(M, &su, idx, NULL, off, NULL, 1, 0.0, &p);
+	if (!rc) rc = ha_plan_load_xy(p, xp, yp, false);
+	if (!rc) rc = ha_plan_ridge(p, alpha, false, bk);
+	if (!rc)
+		for (int32_t r = 0; r < N; ++r)
+			memcpy(betas->data + (int64_t)r * P, bk + (int64_t)r * K, (size_t)P * sizeof(double));
+done:
+	ha_hip_plan_destroy(p);
+	free(xp); free(yp); free(bk); free(idx);
+	return rc;
+}
+
+int ha_ridge_vec(const TMat *X, const double *y, double alpha, double *betas_out) {
+	if (!X || !y || !betas_out) return kHaErrorArg;
+	TMat Ym = {(double *)y, X->rows, 1}, Bm = {betas_out, X->cols, 1};
+	return ha_ridge(X, &Ym, alpha, &Bm);
 }
 
 /* ---- local Procrustes (one problem) on the FP64 GPU path ---- */

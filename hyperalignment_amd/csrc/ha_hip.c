@@ -1278,6 +1278,119 @@ fail:
 	return rc;
 }
 
+/* ---- searchlight ridge ---- */
+
+/* Searchlight ridge (ha_searchlight.c:1378-1429, ha_ridge.c:13-54): the FP64
+ * betas (X_sl^T X_sl + alpha I)^-1 X_sl^T Y_sl of every searchlight, then the
+ * weighted deterministic scatter into the CSC data (accumulating unless
+ * zero_data). With betas64 set, the betas are returned instead (row-major k x k
+ * per searchlight, in searchlight order) and nothing is scattered. */
+int ha_plan_ridge(HaHipPlan *p, double alpha, bool zero_data, double *betas64) {
+	int rc = kHaSuccess;
+	hipStream_t st = g.stream;
+	double *q64 = NULL, *h = NULL;
+	int64_t *qoff = NULL;
+	int32_t *fl = NULL;
+	if (!p || !p->loaded) { ERR("plan has no data loaded\n"); return kHaErrorArg; }
+	if (!betas64 && (!p->data || !p->indptr64)) { ERR("plan has no CSC pattern\n"); return kHaErrorArg; }
+	if (!(alpha >= 0.0) || isinf(alpha)) { ERR("ridge alpha must be finite and >= 0 (got %g)\n", alpha); return kHaErrorArg; }
+	HakLocalArgs a = local_args(p);
+	a.queue = p->queues + 1;
+	if (betas64) {
+		ACK(q64, (size_t)p->q_elems * sizeof(double));
+		a.q64 = q64;
+	}
+	HCK(hipEventRecord(p->ev[0], st));
+	KCK(hak_fill_i32(p->queues, 8, 0, st));
+	KCK(hak_fill_i32(p->flag, p->count, 0, st));
+	KCK(hak_fill_i32(p->steps, p->count, 0, st));
+	if (!betas64) {
+		if (zero_data) KCK(hak_fill_f64(p->data, p->nnz, 0.0, st));
+		if (p->weights_computed && !p->weights_none) {
+			const int64_t *optr = p->occx_ptr ? p->occx_ptr : p->occ_ptr;
+			const int32_t *occ = p->occx ? p->occx : p->occ;
+			KCK(hak_weights(p->slx, p->dist, p->radius, p->total, optr, occ, p->nv_x, p->wsum, p->w, st));
+		}
+	}
+	HCK(hipEventRecord(p->ev[1], st));
+	/* two of the four FP64 buffers per Jacobi slot suffice: twice the workgroups */
+	int32_t nwg = 2 * p->n_slots64;
+	if (nwg > p->count) nwg = p->count;
+	if ((int64_t)nwg * 2 > (int64_t)p->n_slots64 * 4 || p->stride64 < (int64_t)p->kpmax * p->kpmax) {
+		ERR("ridge launch does not fit the FP64 work area\n");
+		rc = kHaErrorInternal;
+		goto fail;
+	}
+	KCK(hak_ridge64(&a, alpha, p->work64, p->stride64, nwg, p->count, st));
+	HCK(hipEventRecord(p->ev[2], st));
+	if (!betas64)
+		KCK(hak_scatter(p->slx, p->off, p->sl_of, p->occ_ptr, p->occ, p->ncols, p->indptr64, p->indices, p->data,
+		                p->q, p->qoff, p->qld, NULL, p->weights_none ? NULL : p->w, 0, p->count, st));
+	HCK(hipEventRecord(p->ev[5], st));
+	HCK(hipStreamSynchronize(st));
+	{
+		float t0 = 0, t1 = 0, t2 = 0;
+		hipEventElapsedTime(&t0, p->ev[0], p->ev[1]);
+		hipEventElapsedTime(&t1, p->ev[1], p->ev[2]);
+		hipEventElapsedTime(&t2, p->ev[2], p->ev[5]);
+		memset(&p->stats, 0, sizeof(p->stats));
+		p->stats.ms_weights = t0;
+		p->stats.ms_polar = t1;   /* the local solver: here the ridge solve */
+		p->stats.kernel_ms_polar_avg = t1;
+		p->stats.polar_launches = 1;
+		p->stats.ms_scatter = t2;
+		p->stats.ms_total = t0 + t1 + t2;
+		p->stats.nnz = p->nnz;
+		/* 4 n k^2 + k^3 / 3 + 2 k^3 flops: the two products over n time points, the
+		 * Cholesky, and the two triangular solves with k right-hand sides */
+		double flops = 0;
+		for (int32_t s = 0; s < p->count; ++s) {
+			const double k = p->h_k[s];
+			flops += (2.0 * p->nt * k * k * 2.0) + k * k * k / 3.0 + 2.0 * k * k * k;
+		}
+		p->stats.gemm_flops = flops;
+	}
+	fl = (int32_t *)malloc((size_t)p->count * sizeof(int32_t));
+	if (!fl) { rc = kHaErrorAlloc; goto fail; }
+	HCK(hipMemcpy(fl, p->flag, (size_t)p->count * sizeof(int32_t), hipMemcpyDeviceToHost));
+	{
+		int32_t bad = 0, first = -1;
+		for (int32_t s = 0; s < p->count; ++s)
+			if (fl[s]) { if (first < 0) first = s; ++bad; }
+		p->stats.n_fallback = bad;
+		if (bad) {
+			ERR("ridge: X_sl^T X_sl + alpha I is not positive definite for %d searchlight(s) (first %d, alpha %g); "
+			    "alpha must be > 0 where a searchlight has fewer time points than vertices or a constant column\n",
+			    bad, first, alpha);
+			rc = kHaErrorSvd;
+			goto fail;
+		}
+	}
+	if (betas64) {
+		h = (double *)malloc((size_t)p->q_elems * sizeof(double));
+		qoff = (int64_t *)malloc((size_t)p->count * sizeof(int64_t));
+		if (!h || !qoff) { rc = kHaErrorAlloc; goto fail; }
+		HCK(hipMemcpy(h, q64, (size_t)p->q_elems * sizeof(double), hipMemcpyDeviceToHost));
+		HCK(hipMemcpy(qoff, p->qoff, (size_t)p->count * sizeof(int64_t), hipMemcpyDeviceToHost));
+		int64_t o = 0;
+		for (int32_t s = 0; s < p->count; ++s) {
+			const int32_t k = p->h_k[s], kp = (k + 31) / 32 * 32;
+			for (int32_t r = 0; r < k; ++r)
+				for (int32_t c = 0; c < k; ++c) betas64[o + (int64_t)r * k + c] = h[qoff[s] + r + (int64_t)c * kp];
+			o += (int64_t)k * k;
+		}
+	}
+fail:
+	if (q64) hipFree(q64);
+	free(h);
+	free(qoff);
+	free(fl);
+	if (rc) hipStreamSynchronize(st);
+	return rc;
+}
+
+int ha_hip_plan_ridge(HaHipPlan *plan, double alpha) { return ha_plan_ridge(plan, alpha, true, NULL); }
+
 /* ------------------------------------------------------------------------ */
 /* learned-scaling support: spectra, per-searchlight steps and predictions   */
 /* ------------------------------------------------------------------------ */

@@ -49,6 +49,7 @@ int ha_plan_run(HaHipPlan *p, bool isReflection, bool isScaling, bool zero_data,
 int ha_plan_download_data(const HaHipPlan *p, double *data);
 /* FP64 local transforms (Jacobi-64 path) of every searchlight, row-major k x k each. */
 int ha_plan_local64(HaHipPlan *p, bool isReflection, bool isScaling, double *out);
+int ha_plan_ridge(HaHipPlan *p, double alpha, bool zero_data, double *betas64);
 /* Weights computed on the GPU for the plan's X lists (flat). */
 int ha_plan_weights(HaHipPlan *p, double *w_flat);
 int ha_plan_pattern(HaHipPlan *p, int32_t **indptr, int32_t **indices, int64_t *nnz);

@@ -155,6 +155,14 @@ int hak_var_sum(const double *X64, int32_t nt, const int32_t *slx_idx, const int
 int hak_scale(const double *sigma_sum, const double *var_sum, int32_t nt, int32_t count,
               double *scale, void *stream);
 
+/* ---- searchlight ridge (k_ridge.hip) ---- */
+/* Per searchlight (order, count): betas = (X^T X + alpha I)^-1 X^T Y in FP64
+ * into the Q store (FP32, and q64 when set); flag[s] = 1 if X^T X + alpha I is
+ * not positive definite. n_wg workgroups; work64: n_wg * 2 * stride64 doubles
+ * (stride64 >= kpmax^2). */
+int hak_ridge64(const HakLocalArgs *a, double alpha, double *work64, int64_t stride64, int n_wg, int32_t count,
+                void *stream);
+
 /* ---- applying and scoring a transform (k_apply.hip) ---- */
 /* Y = X T: X nt x nv FP64 column-major (leading dim nt), T CSC with ncols
  * columns (int64 indptr), Y nt x ncols column-major. */

@@ -135,6 +135,13 @@ int ha_hip_plan_load(HaHipPlan *plan, const double *X, const double *Y, bool col
  * Synchronous. */
 int ha_hip_plan_execute(HaHipPlan *plan, bool isReflection, bool isScaling);
 
+/* The searchlight ridge pass on data already in HBM (ha_searchlight.c:1378-1429):
+ * weights, FP64 ridge betas (X_sl^T X_sl + alpha I)^-1 X_sl^T Y_sl per
+ * searchlight, deterministic scatter into the plan's CSC data (zeroed first).
+ * kHaErrorSvd if some X_sl^T X_sl + alpha I is not positive definite.
+ * Synchronous. */
+int ha_hip_plan_ridge(HaHipPlan *plan, double alpha);
+
 /* Iteration tile (64 or 128) chosen for each kp bucket (kp <= 64, 128, 256,
  * 384, 512, larger): the tile option when set, else timed on a sample of each
  * large bucket on the first polar pass (the plan's data must be loaded). */

@@ -1,6 +1,6 @@
 /*
- * ha_searchlight.h -- searchlight weights and the two searchlight Procrustes
- * entry points, served by the HIP backend.
+ * ha_searchlight.h -- searchlight weights, the two searchlight Procrustes
+ * entry points and searchlight ridge, served by the HIP backend.
  *
  * Signatures are those of the reference's csrc/ha_searchlight.h:14-80. The
  * only backend this library executes is kHaBackendHIP; any other value
@@ -58,6 +58,20 @@ int ha_searchlight_procrustes_dense(
     bool isReflection, bool isScaling,
     THaBackend backend,
     bool col_major);
+
+/* Replaces ha_searchlight.h:83-91 / ha_searchlight.c:1378-1429 (searchlight
+ * ridge). Per searchlight the ridge betas of ha_ridge (ha_ridge.c:13-54), in
+ * FP64 on the GPU, scattered like the Procrustes transforms: mat->data +=
+ * w_s[i] * betas_s[i, j], in searchlight order. alpha must be >= 0; a
+ * searchlight whose X_sl^T X_sl + alpha I is not positive definite (alpha = 0
+ * with fewer time points than vertices, or a zero column) returns kHaErrorSvd.
+ * sls_Y: NULL or the same sizes as sls_X. The reference's function has no
+ * backend argument; this one runs on the HIP backend. */
+int ha_searchlight_ridge(const TMat *X, const TMat *Y,
+                         const TSearchlights *sls_X,
+                         const TSearchlights *sls_Y,
+                         TSparseCSC *mat,
+                         double **weights, double alpha);
 
 #ifdef __cplusplus
 }

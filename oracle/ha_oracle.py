@@ -24,6 +24,11 @@ How each function is pinned (tests/test_oracle.py):
                             CSC/dense agreement
   template_procrustes       identical subjects give the input back
                             (csrc/ha_test.c:503-535)
+  ridge                     recovers exact betas at alpha 1e-6 (csrc/ha_test.c:336-362);
+                            normal equations (X^T X + alpha I) B = X^T Y, also with
+                            fewer samples than features; alpha -> inf limit X^T Y / alpha;
+                            alpha = 0 equals least squares
+  searchlight_ridge_csc     brute-force dense accumulation on a tiny mesh
 """
 
 from __future__ import annotations
@@ -209,6 +214,13 @@ def searchlight_procrustes_csc(X, Y, idx, offsets, indptr, indices, data0=None, 
         colmask[list(which_cols)] = True
         todo = [s for s in range(count) if colmask[sly[s]].any()]
     Ts = local_transforms(X, Y, idx, offsets, reflection, scaling, which=todo, idx_y=idx_y, n_jobs=n_jobs)
+    _scatter_csc(data, indptr, indices, slx, sly, offsets, weights, todo, Ts, colmask)
+    return data
+
+
+def _scatter_csc(data, indptr, indices, slx, sly, offsets, weights, todo, Ts, colmask=None):
+    """ha_sparse_scatter_add (csrc/ha_sparse.c:128-142) of each T_s in order:
+    data[pos(slx[i], sly[j])] += T[i, j] * w[i]; pairs outside the pattern skipped."""
     for s, T in zip(todo, Ts):
         rows, cols = slx[s], sly[s]
         w = weights[offsets[s]:offsets[s + 1]] if weights is not None else None
@@ -221,7 +233,6 @@ def searchlight_procrustes_csc(X, Y, idx, offsets, indptr, indices, data0=None, 
             ok[ok] = seg[pos[ok]] == rows[ok]
             val = T[:, j] * w if w is not None else T[:, j] * 1.0
             data[indptr[c] + pos[ok]] += val[ok]
-    return data
 
 
 def searchlight_procrustes_dense(X, Y, idx, offsets, dists, radius, T_out=None, reflection=True,
@@ -245,6 +256,46 @@ def dense_from_csc(indptr, indices, data, n):
     for c in range(n):
         out[indices[indptr[c]:indptr[c + 1]], c] = data[indptr[c]:indptr[c + 1]]
     return out
+
+
+# ---------------------------------------------------------------------------
+# ridge
+# ---------------------------------------------------------------------------
+
+def ridge(X, Y, alpha):
+    """Ridge betas through the thin SVD of X, as src/hyperalignment/ridge.py:7-12
+    and csrc/ha_ridge.c:13-54: X = U diag(s) Vt, d = s / (alpha + s^2),
+    betas = Vt^T (d * (U^T Y)). Y may be 1-D (csrc/ha_ridge.c:56-95)."""
+    X = np.asarray(X, dtype=np.float64)
+    Y = np.asarray(Y, dtype=np.float64)
+    U, s, Vt = safe_svd(X)
+    d = s / (alpha + s * s)
+    UtY = U.T @ Y
+    return Vt.T @ (d[:, None] * UtY if UtY.ndim == 2 else d * UtY)
+
+
+def searchlight_ridge_csc(X, Y, idx, offsets, indptr, indices, alpha, data0=None, weights=None, idx_y=None,
+                          which_cols=None):
+    """csrc/ha_searchlight.c:1378-1429: ridge betas of every searchlight,
+    scattered in searchlight order with ha_sparse_scatter_add (row weights).
+
+    which_cols: optional set of columns to compute (only searchlights touching
+    them are solved); other columns keep data0.
+    """
+    indptr = np.asarray(indptr, dtype=np.int64)
+    indices = np.asarray(indices, dtype=np.int64)
+    data = np.zeros(len(indices)) if data0 is None else np.array(data0, dtype=np.float64)
+    slx = _lists(idx, offsets)
+    sly = slx if idx_y is None else _lists(idx_y, offsets)
+    colmask = None
+    todo = list(range(len(slx)))
+    if which_cols is not None:
+        colmask = np.zeros(int(len(indptr) - 1), dtype=bool)
+        colmask[list(which_cols)] = True
+        todo = [s for s in todo if colmask[sly[s]].any()]
+    Ts = [ridge(X[:, slx[s]], Y[:, sly[s]], alpha) for s in todo]
+    _scatter_csc(data, indptr, indices, slx, sly, offsets, weights, todo, Ts, colmask)
+    return data
 
 
 # ---------------------------------------------------------------------------

@@ -64,3 +64,12 @@ def template_case():
 SPARSE_TOY = [[0, 1, 2], [1, 2, 3, 4]]
 # csrc/ha_test.c:439-470 (three searchlights on six vertices)
 WEIGHTS_TOY = [[0, 1, 2], [1, 2, 3, 4], [3, 4, 5]]
+
+
+def ridge_case():
+    """test_ridge_small_alpha: seed 101, X 50 x 10 then true betas 10 x 5, Y = X B;
+    ridge with alpha 1e-6 recovers B to 1e-6 (csrc/ha_test.c:336-362)."""
+    r = RefRng(101)
+    X = r.mat(50, 10)
+    B = r.mat(10, 5)
+    return X, X @ B, B

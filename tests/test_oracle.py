@@ -217,3 +217,54 @@ def test_vertex_correlation_is_pearson():
     ref = [np.corrcoef(A[:, j], B[:, j])[0, 1] for j in range(6)]
     assert np.abs(r - ref).max() < 1e-13
     assert np.isnan(O.vertex_correlation(np.ones((5, 1)), B[:5, :1]))[0]
+
+
+# ---- ridge ----
+
+def test_ridge_recovers_true_betas():
+    """csrc/ha_test.c:336-362: Y = X B, alpha 1e-6 -> max |betas - B| < 1e-6."""
+    X, Y, B = refcases.ridge_case()
+    assert np.abs(O.ridge(X, Y, 1e-6) - B).max() < 1e-6
+
+
+@pytest.mark.parametrize("M,N,P,alpha", [(60, 20, 7, 3.0), (15, 40, 40, 10.0), (30, 12, 20, 1e3)])
+def test_ridge_normal_equations(M, N, P, alpha):
+    """(X^T X + alpha I) betas = X^T Y, including fewer samples than features."""
+    rng = np.random.default_rng(M + N + P)
+    X = rng.standard_normal((M, N))
+    Y = rng.standard_normal((M, P))
+    b = O.ridge(X, Y, alpha)
+    lhs = (X.T @ X + alpha * np.eye(N)) @ b
+    assert np.abs(lhs - X.T @ Y).max() < 1e-10 * max(1.0, np.abs(X.T @ Y).max())
+
+
+def test_ridge_limits():
+    rng = np.random.default_rng(12)
+    X = rng.standard_normal((40, 9))
+    Y = rng.standard_normal((40, 4))
+    # alpha = 0 and full column rank: least squares
+    assert np.abs(O.ridge(X, Y, 0.0) - np.linalg.lstsq(X, Y, rcond=None)[0]).max() < 1e-12
+    # alpha -> inf: alpha betas -> X^T Y
+    a = 1e12
+    assert np.abs(a * O.ridge(X, Y, a) - X.T @ Y).max() < 1e-6 * np.abs(X.T @ Y).max()
+    # one response vector (csrc/ha_ridge.c:56-95) is a column of the matrix form
+    assert np.abs(O.ridge(X, Y[:, 2], 5.0) - O.ridge(X, Y, 5.0)[:, 2]).max() < 1e-14
+
+
+def test_searchlight_ridge_brute_force():
+    """mat[np.ix_(sl, sl)] += betas_s * w_s[:, None] over the searchlights in order
+    (csrc/ha_searchlight.c:1378-1429), through the CSC pattern."""
+    rng = np.random.default_rng(13)
+    lists = [[0, 1, 2, 5], [1, 2, 3], [3, 4, 5, 6, 0], [6, 2]]
+    nv = 7
+    sl = synth.toy_searchlights(lists, nv=nv)
+    X = rng.standard_normal((12, nv))
+    Y = rng.standard_normal((12, nv))
+    wts = rng.uniform(0.2, 1.0, size=int(sl.offsets[-1]))
+    indptr, indices = O.sparse_init(sl.indices, sl.offsets, nv)
+    data = O.searchlight_ridge_csc(X, Y, sl.indices, sl.offsets, indptr, indices, 2.5, weights=wts)
+    D = np.zeros((nv, nv))
+    for s, l in enumerate(lists):
+        b = np.linalg.solve(X[:, l].T @ X[:, l] + 2.5 * np.eye(len(l)), X[:, l].T @ Y[:, l])
+        D[np.ix_(l, l)] += b * wts[sl.offsets[s]:sl.offsets[s + 1], None]
+    assert np.abs(O.dense_from_csc(indptr, indices, data, nv) - D).max() < 1e-12
