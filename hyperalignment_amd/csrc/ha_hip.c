@@ -217,6 +217,7 @@ struct HaHipPlan {
 	float *work; int64_t slot_stride; int32_t n_slots, n_slots_polar;
 	float *sl_par;
 	double *work64; int64_t stride64; int32_t n_slots64;
+	double *work_ridge; int32_t n_wg_ridge;  /* ridge: 2 x stride64 doubles per workgroup */
 	int32_t *flag, *steps, *queues, *order;
 	double *sigma_sum, *var_sum, *scale;
 	float *features, *pred, *trace, *sig;
@@ -242,7 +243,7 @@ static void plan_free(HaHipPlan *p) {
 	                p->qld, p->qoff, p->q, p->q64, p->X, p->Y, p->X64, p->Y64, p->stage, p->work, p->work64,
 	                p->flag, p->steps, p->queues, p->order, p->sigma_sum, p->var_sum, p->scale,
 	                p->features, p->pred, p->trace, p->sig, p->d_which, p->sl_par, p->tile_order[0],
-	                p->tile_order[1]};
+	                p->tile_order[1], p->work_ridge};
 	for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i)
 		if (ptrs[i]) hipFree(ptrs[i]);
 	if (p->gexec) hipGraphExecDestroy(p->gexec);
@@ -1313,15 +1314,20 @@ int ha_plan_ridge(HaHipPlan *p, double alpha, bool zero_data, double *betas64) {
 		}
 	}
 	HCK(hipEventRecord(p->ev[1], st));
-	/* two of the four FP64 buffers per Jacobi slot suffice: twice the workgroups */
-	int32_t nwg = 2 * p->n_slots64;
-	if (nwg > p->count) nwg = p->count;
-	if ((int64_t)nwg * 2 > (int64_t)p->n_slots64 * 4 || p->stride64 < (int64_t)p->kpmax * p->kpmax) {
+	/* HAK_RIDGE_OCC workgroups per CU, two kp x kp FP64 buffers each (allocated on
+	 * the first ridge pass of the plan) */
+	if (!p->work_ridge) {
+		int32_t n = HAK_RIDGE_OCC * g.n_cu;
+		if (n > p->count) n = p->count;
+		ACK(p->work_ridge, (size_t)n * 2 * p->stride64 * sizeof(double));
+		p->n_wg_ridge = n;
+	}
+	if (p->stride64 < (int64_t)p->kpmax * p->kpmax) {
 		ERR("ridge launch does not fit the FP64 work area\n");
 		rc = kHaErrorInternal;
 		goto fail;
 	}
-	KCK(hak_ridge64(&a, alpha, p->work64, p->stride64, nwg, p->count, st));
+	KCK(hak_ridge64(&a, alpha, p->work_ridge, p->stride64, p->n_wg_ridge, p->count, st));
 	HCK(hipEventRecord(p->ev[2], st));
 	if (!betas64)
 		KCK(hak_scatter(p->slx, p->off, p->sl_of, p->occ_ptr, p->occ, p->ncols, p->indptr64, p->indices, p->data,
@@ -1341,12 +1347,13 @@ int ha_plan_ridge(HaHipPlan *p, double alpha, bool zero_data, double *betas64) {
 		p->stats.ms_scatter = t2;
 		p->stats.ms_total = t0 + t1 + t2;
 		p->stats.nnz = p->nnz;
-		/* 4 n k^2 + k^3 / 3 + 2 k^3 flops: the two products over n time points, the
-		 * Cholesky, and the two triangular solves with k right-hand sides */
+		/* algorithmic flops 3 n k^2 + 7 k^3 / 3: X^T X (symmetric, n k^2), X^T Y
+		 * (2 n k^2), the Cholesky (k^3 / 3) and two triangular solves with k
+		 * right-hand sides (2 k^3) */
 		double flops = 0;
 		for (int32_t s = 0; s < p->count; ++s) {
 			const double k = p->h_k[s];
-			flops += (2.0 * p->nt * k * k * 2.0) + k * k * k / 3.0 + 2.0 * k * k * k;
+			flops += 3.0 * p->nt * k * k + 7.0 * k * k * k / 3.0;
 		}
 		p->stats.gemm_flops = flops;
 	}

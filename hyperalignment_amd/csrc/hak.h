@@ -156,6 +156,7 @@ int hak_scale(const double *sigma_sum, const double *var_sum, int32_t nt, int32_
               double *scale, void *stream);
 
 /* ---- searchlight ridge (k_ridge.hip) ---- */
+#define HAK_RIDGE_OCC 4  /* ridge workgroups per CU (33.8 KB LDS, at most 128 VGPRs each) */
 /* Per searchlight (order, count): betas = (X^T X + alpha I)^-1 X^T Y in FP64
  * into the Q store (FP32, and q64 when set); flag[s] = 1 if X^T X + alpha I is
  * not positive definite. n_wg workgroups; work64: n_wg * 2 * stride64 doubles

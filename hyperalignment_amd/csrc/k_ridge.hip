@@ -13,7 +13,7 @@
 // and its solution is zero there. The betas (k x k) go to the searchlight's
 // place in the Q store (FP32; FP64 too when a.q64 is set) for the
 // deterministic CSC scatter. Each workgroup uses two kp x kp FP64 buffers of
-// work64 (G, then L, and C, then the betas).
+// the work area (G, then L, and C, then the betas).
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -36,12 +36,13 @@ __device__ __forceinline__ int next_sl(int32_t *queue, int32_t *sh) {
 
 // out[m + n kp] = sum_t P[cols_p[m]] Q[cols_q[n]] over t < nt (m, n < k; zero
 // elsewhere) + diag on m == n < k and 1 on m == n >= k.
+// lower: only the tiles on and below the diagonal (the Cholesky reads no others).
 __device__ void crossprod64(const double *__restrict__ P, const int32_t *cols_p, const double *__restrict__ Q,
                             const int32_t *cols_q, int nt, int k, int kp, double diag, double pad_diag, double *out,
-                            double (*xs)[B + 1], double (*ys)[B + 1]) {
+                            bool lower, double (*xs)[B + 1], double (*ys)[B + 1]) {
   const int mi = threadIdx.x & 31, ni = threadIdx.x >> 5;  // ni in 0..7
   for (int tm = 0; tm < kp; tm += B)
-    for (int tn = 0; tn < kp; tn += B) {
+    for (int tn = 0; tn < (lower ? tm + B : kp); tn += B) {
       double acc[4] = {0.0, 0.0, 0.0, 0.0};
       for (int t0 = 0; t0 < nt; t0 += B) {
         for (int e = threadIdx.x; e < B * B; e += RT) {
@@ -213,7 +214,7 @@ __device__ void trsm_lower_t64(const double *L, double *C, int kp, double (*d)[B
   __syncthreads();
 }
 
-__global__ __launch_bounds__(RT) void k_ridge64(HakLocalArgs a, double alpha, double *work64, int64_t stride64,
+__global__ __launch_bounds__(RT, HAK_RIDGE_OCC) void k_ridge64(HakLocalArgs a, double alpha, double *work64, int64_t stride64,
                                                 int32_t count) {
   __shared__ double t0[B][B + 1], t1[B][B + 1], t2[B][B + 1], t3[B][B + 1];
   __shared__ int32_t item_sh, ok_sh;
@@ -226,8 +227,8 @@ __global__ __launch_bounds__(RT) void k_ridge64(HakLocalArgs a, double alpha, do
     const int o = a.sl_off[s];
     const int k = a.sl_off[s + 1] - o;
     const int kp = a.qld[s];
-    crossprod64(a.X64, a.slx_idx + o, a.X64, a.slx_idx + o, a.nt, k, kp, alpha, 1.0, Gm, t0, t1);
-    crossprod64(a.X64, a.slx_idx + o, a.Y64, a.sly_idx + o, a.nt, k, kp, 0.0, 0.0, Cm, t0, t1);
+    crossprod64(a.X64, a.slx_idx + o, a.X64, a.slx_idx + o, a.nt, k, kp, alpha, 1.0, Gm, true, t0, t1);
+    crossprod64(a.X64, a.slx_idx + o, a.Y64, a.sly_idx + o, a.nt, k, kp, 0.0, 0.0, Cm, false, t0, t1);
     const bool ok = cholesky64(Gm, kp, t0, t3, t1, t2, &ok_sh);
     trsm_lower64(Gm, Cm, kp, t0, t3, t1, t2);
     trsm_lower_t64(Gm, Cm, kp, t0, t3, t1, t2);
