@@ -7,7 +7,11 @@
 //   * bf16x3 products: C += Bhi*Ahi + Blo*Ahi + Bhi*Alo (the Blo*Alo term,
 //     2^-16 relative, is dropped), so FP32-level accuracy at bf16 MFMA rate;
 //   * K chunks of 32 staged in double-buffered LDS: the next chunk's global
-//     loads are in flight while the current chunk's MFMAs run;
+//     loads are in flight while the current chunk's MFMAs run; k-contiguous
+//     operands are stored as 16-byte slots of 8 k-values per row m, and an
+//     m-contiguous A operand (OP_SPLIT_M) as k rows of m-contiguous bf16 whose
+//     MFMA fragments come from transposing LDS reads (ds_read_b64_tr_b16), so
+//     every staging write is a 128-bit store free of bank conflicts;
 //   * 32x32 blocks that lie entirely outside the valid rows/columns skip their
 //     MFMAs (edge tiles of kp that is not a multiple of TILE).
 //
@@ -36,6 +40,8 @@
 #include <stdint.h>
 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4_t __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) bf16x4_t lds_bf16x4_t;
 typedef float f32x16_t __attribute__((ext_vector_type(16)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 
@@ -95,12 +101,33 @@ __device__ __forceinline__ Opnd opnd_split(const SplitM &s, int kind, int ld, in
   return o;
 }
 
+// LDS row strides. A k-contiguous row holds TILE slots (8 k-values of one m
+// each, 16 bytes) padded to TILE + 2: 128-bit LDS accesses are served 8 lanes
+// at a time, and the 8 lanes of a staging write (4 k-slots of 2 rows m) then
+// fall in 8 distinct 16-byte bank groups. An m-contiguous row holds TILE 16-bit
+// values padded by 16: a row stride of 32 bytes modulo 128, so the 4 rows of
+// one transposing read (16 lanes, 4 rows x 32 bytes) fall in distinct banks.
+template <int TILE>
+struct Lds {
+  static constexpr int SR = TILE + 2;
+  static constexpr int TR = TILE + 16;
+};
+template <int TILE>
+using SRow = uint4[Lds<TILE>::SR];
+
 template <int TILE>
 struct GemmSmem {
   union {
     struct {
-      uint4 ah[2][KC / 8][TILE], al[2][KC / 8][TILE];
-      uint4 bh[2][KC / 8][TILE], bl[2][KC / 8][TILE];
+      union {
+        struct {
+          uint4 ah[2][KC / 8][Lds<TILE>::SR], al[2][KC / 8][Lds<TILE>::SR];
+        };
+        struct {  // A operand of kind OP_SPLIT_M: k rows of m-contiguous bf16
+          uint16_t th[2][KC][Lds<TILE>::TR], tl[2][KC][Lds<TILE>::TR];
+        };
+      };
+      uint4 bh[2][KC / 8][Lds<TILE>::SR], bl[2][KC / 8][Lds<TILE>::SR];
     };
     float ctile[TILE * (TILE + 1)];  // finished tile, column-major, padded leading dim
   };
@@ -174,12 +201,12 @@ __device__ __forceinline__ uint32_t pack2(uint16_t a, uint16_t b) { return (uint
 // FP32 staging view of one operand buffer: k quad q (k = 4q .. 4q+3) of row m
 // lives in the hi array for q < KC/8 and in the lo array above.
 template <int TILE>
-__device__ __forceinline__ float4 *f32_quad(uint4 (*sh)[TILE], uint4 (*sl)[TILE], int q, int m) {
+__device__ __forceinline__ float4 *f32_quad(SRow<TILE> *sh, SRow<TILE> *sl, int q, int m) {
   return reinterpret_cast<float4 *>(q < KC / 8 ? &sh[q][m] : &sl[q - KC / 8][m]);
 }
 
 template <int TILE, int KIND, int PREC = PREC_BF16X3>
-__device__ __forceinline__ void stash(const Stage<TILE> &st, uint4 (*sh)[TILE], uint4 (*sl)[TILE]) {
+__device__ __forceinline__ void stash(const Stage<TILE> &st, SRow<TILE> *sh, SRow<TILE> *sl) {
   constexpr int N = Stage<TILE>::N;
   const int tid = threadIdx.x;
   if constexpr (PREC == PREC_F32) {
@@ -223,21 +250,24 @@ __device__ __forceinline__ void stash(const Stage<TILE> &st, uint4 (*sh)[TILE], 
       sl[kg][m] = st.r[N / 2 + it];
     }
   } else {
+    static_assert(KIND != OP_SPLIT_M && KIND != OP_F32_M,
+                  "OP_SPLIT_M operands are staged by stash_m; OP_F32_M needs PREC_F32");
+  }
+}
+
+// Stages an OP_SPLIT_M operand chunk (8 consecutive m of one k per 16 bytes)
+// as k rows of m-contiguous bf16: 8 consecutive lanes write one 128-byte run.
+template <int TILE>
+__device__ __forceinline__ void stash_m(const Stage<TILE> &st, uint16_t (*th)[Lds<TILE>::TR],
+                                        uint16_t (*tl)[Lds<TILE>::TR]) {
+  constexpr int N = Stage<TILE>::N;
+  const int tid = threadIdx.x;
 #pragma unroll
-    for (int it = 0; it < N / 2; ++it) {
-      const int e = tid + it * NT;
-      const int kk = e / (TILE / 8), mg = e % (TILE / 8);
-      const uint4 h = st.r[it], l = st.r[N / 2 + it];
-      const uint32_t hw[4] = {h.x, h.y, h.z, h.w}, lw[4] = {l.x, l.y, l.z, l.w};
-      const int g = kk >> 3, q = kk & 7;
-#pragma unroll
-      for (int p = 0; p < 4; ++p) {
-        ((uint16_t *)&sh[g][mg * 8 + 2 * p])[q] = (uint16_t)(hw[p] & 0xFFFF);
-        ((uint16_t *)&sh[g][mg * 8 + 2 * p + 1])[q] = (uint16_t)(hw[p] >> 16);
-        ((uint16_t *)&sl[g][mg * 8 + 2 * p])[q] = (uint16_t)(lw[p] & 0xFFFF);
-        ((uint16_t *)&sl[g][mg * 8 + 2 * p + 1])[q] = (uint16_t)(lw[p] >> 16);
-      }
-    }
+  for (int it = 0; it < N / 2; ++it) {
+    const int e = tid + it * NT;
+    const int kk = e / (TILE / 8), mg = e % (TILE / 8);
+    *reinterpret_cast<uint4 *>(&th[kk][mg * 8]) = st.r[it];
+    *reinterpret_cast<uint4 *>(&tl[kk][mg * 8]) = st.r[N / 2 + it];
   }
 }
 
@@ -249,7 +279,23 @@ struct Acc {
   f32x16_t v[W][W];
 };
 
+// MFMA fragment (8 consecutive k from k0, row mrow0 + lane % 16) from k rows of
+// m-contiguous bf16 by two ds_read_b64_tr_b16. Within each 16-lane group that
+// instruction hands lane i the 4 values M[4j + i/4][i % 4] (j = 0..3), where
+// M[r] is the 64-bit piece addressed by lane r: lane r therefore addresses row
+// k0 + r/4 at column mrow0 + 4 (r % 4), and lane i receives rows k0 .. k0+3 of
+// column mrow0 + i.
 template <int TILE>
+__device__ __forceinline__ uint4 frag_m(const uint16_t (*t)[Lds<TILE>::TR], int k0, int mrow0, int lane) {
+  const int r = lane & 15;
+  uint16_t *p0 = const_cast<uint16_t *>(&t[k0 + (r >> 2)][mrow0 + 4 * (r & 3)]);
+  uint16_t *p1 = const_cast<uint16_t *>(&t[k0 + 4 + (r >> 2)][mrow0 + 4 * (r & 3)]);
+  const uint2 a = __builtin_bit_cast(uint2, __builtin_amdgcn_ds_read_tr16_b64_v4bf16((lds_bf16x4_t *)p0));
+  const uint2 b = __builtin_bit_cast(uint2, __builtin_amdgcn_ds_read_tr16_b64_v4bf16((lds_bf16x4_t *)p1));
+  return make_uint4(a.x, a.y, b.x, b.y);
+}
+
+template <int TILE, int KA = OP_SPLIT_K>
 __device__ __forceinline__ void mma_chunk(const GemmSmem<TILE> &sm, int buf, Acc<TILE> &acc, const bool *bm,
                                           const bool *bn) {
   constexpr int W = TILE / 64;
@@ -261,8 +307,15 @@ __device__ __forceinline__ void mma_chunk(const GemmSmem<TILE> &sm, int buf, Acc
     bf16x8_t ahf[W], alf[W], bhf[W], blf[W];
 #pragma unroll
     for (int i = 0; i < W; ++i) {
-      uint4 h = sm.ah[buf][g][wm0 + i * 32 + (lane & 31)];
-      uint4 l = sm.al[buf][g][wm0 + i * 32 + (lane & 31)];
+      uint4 h, l;
+      if constexpr (KA == OP_SPLIT_M) {
+        const int k0 = ks * 16 + 8 * (lane >> 5), mrow0 = wm0 + i * 32 + 16 * ((lane >> 4) & 1);
+        h = frag_m<TILE>(sm.th[buf], k0, mrow0, lane);
+        l = frag_m<TILE>(sm.tl[buf], k0, mrow0, lane);
+      } else {
+        h = sm.ah[buf][g][wm0 + i * 32 + (lane & 31)];
+        l = sm.al[buf][g][wm0 + i * 32 + (lane & 31)];
+      }
       ahf[i] = __builtin_bit_cast(bf16x8_t, h);
       alf[i] = __builtin_bit_cast(bf16x8_t, l);
     }
@@ -338,11 +391,15 @@ __device__ __forceinline__ void tile_gemm(GemmSmem<TILE> &sm, const Opnd &A, con
     for (int j = 0; j < W; ++j)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc.v[i][j][r] = 0.f;
+  static_assert(KB != OP_SPLIT_M, "only the A operand may be read with m contiguous");
   const int nc = K / KC;
   Stage<TILE> sa, sb;
   fetch<TILE, KA>(A, tm0, 0, sa);
   fetch<TILE, KB>(B, tn0, 0, sb);
-  stash<TILE, KA, PREC>(sa, sm.ah[0], sm.al[0]);
+  if constexpr (KA == OP_SPLIT_M)
+    stash_m<TILE>(sa, sm.th[0], sm.tl[0]);
+  else
+    stash<TILE, KA, PREC>(sa, sm.ah[0], sm.al[0]);
   stash<TILE, KB, PREC>(sb, sm.bh[0], sm.bl[0]);
   __syncthreads();
   for (int c = 0; c < nc; ++c) {
@@ -355,9 +412,12 @@ __device__ __forceinline__ void tile_gemm(GemmSmem<TILE> &sm, const Opnd &A, con
     if constexpr (PREC == PREC_F32)
       mma_chunk_f32<TILE>(sm, cur, acc, bm, bn);
     else
-      mma_chunk<TILE>(sm, cur, acc, bm, bn);
+      mma_chunk<TILE, KA>(sm, cur, acc, bm, bn);
     if (more) {
-      stash<TILE, KA, PREC>(sa, sm.ah[cur ^ 1], sm.al[cur ^ 1]);
+      if constexpr (KA == OP_SPLIT_M)
+        stash_m<TILE>(sa, sm.th[cur ^ 1], sm.tl[cur ^ 1]);
+      else
+        stash<TILE, KA, PREC>(sa, sm.ah[cur ^ 1], sm.al[cur ^ 1]);
       stash<TILE, KB, PREC>(sb, sm.bh[cur ^ 1], sm.bl[cur ^ 1]);
     }
     __syncthreads();
