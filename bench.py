@@ -118,10 +118,10 @@ def copy_calibration():
     print(json.dumps({"copy_bytes": 4 << 28}))
 
 
-def pmc_traffic(config, method):
+def pmc_traffic(config, method, tiles_used):
     """HBM bytes per k_polar launch from the committed rocprofv3 counter summary
     (FETCH_SIZE + WRITE_SIZE, calibrated against a copy kernel), when one exists
-    for this workload and method."""
+    for this workload, method and iteration tile."""
     path = os.path.join(ROOT, "profiles", f"k_polar_{config}_traffic.txt")
     if not os.path.exists(path):
         return None
@@ -130,9 +130,18 @@ def pmc_traffic(config, method):
         if "=" in line and not line.startswith("#"):
             k, v = line.split("=", 1)
             kv[k.strip()] = v.strip()
-    if kv.get("method") != method:
+    if kv.get("method") != method or kv.get("tile") != tiles_used:
         return None
     return float(kv["traffic_bytes"])
+
+
+def kp_bucket(kp):
+    """Index of the iteration-tile bucket of a padded searchlight size (kp <= 64,
+    128, 256, 384, 512, larger), as the library's autotuner groups them."""
+    for b, edge in enumerate((64, 128, 256, 384, 512)):
+        if kp <= edge:
+            return b
+    return 5
 
 
 def main():
@@ -192,8 +201,17 @@ def main():
     polar_avg = float(np.mean(polar_ms))
     achieved = st["gemm_flops"] / (polar_avg * 1e-3)
     method = ha.resolved_method(args.method, int(s.sizes.max()))
+    tiles = None
+    tiles_used = None
+    if method in ("polar", "polar_learned"):
+        edges = ["kp<=64", "kp<=128", "kp<=256", "kp<=384", "kp<=512", "kp>512"]
+        bt = [int(t) for t in plan.tiles()]
+        tiles = dict(zip(edges, bt))
+        kps = set(((s.sizes + 31) // 32 * 32).tolist())
+        tiles_used = "+".join(str(t) for t in sorted({bt[kp_bucket(kp)] for kp in kps}))
     roofline = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_BF16 / 1e12, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_BF16, "traffic": pmc_traffic(args.config, method), "kernel": "k_polar",
+                "frac": achieved / PEAK_BF16, "traffic": pmc_traffic(args.config, method, tiles_used),
+                "kernel": "k_polar" + (f"<{tiles_used}>" if tiles_used else ""),
                 "kernel_ms": polar_avg, "flops_per_launch": st["gemm_flops"]}
     stats_main = st
     # apply and score the transform on the training data (Yhat = X T, vertex-wise r)
@@ -246,7 +264,8 @@ def main():
                    "mean_steps": st["steps_total"] / max(1, s.count),
                    "stage_ms": {k: st[k] for k in ("ms_weights", "ms_crosscov", "ms_polar", "ms_fallback",
                                                    "ms_scatter")},
-                   "parity_vs_fp64": parity, "compare": compare, "evaluate": evaluate},
+                   "iteration_tile": tiles, "parity_vs_fp64": parity, "compare": compare,
+                   "evaluate": evaluate},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
