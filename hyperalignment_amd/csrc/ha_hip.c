@@ -48,6 +48,8 @@ static struct {
 		.verbose = 0,
 		.learned_margin_scale = 0.15f,
 		.learned_margin_ell = 1.0f,
+		.team = 1,
+		.team_spin = 0,
 	},
 	.host_threads = 0,
 };
@@ -218,6 +220,7 @@ struct HaHipPlan {
 	float *sl_par;
 	double *work64; int64_t stride64; int32_t n_slots64;
 	double *work_ridge; int32_t n_wg_ridge;  /* ridge: 2 x stride64 doubles per workgroup */
+	void *grp; int32_t n_teams;               /* polar teams (128 x 128 tiles): state per team */
 	int32_t *flag, *steps, *queues, *order;
 	double *sigma_sum, *var_sum, *scale;
 	float *features, *pred, *trace, *sig;
@@ -232,7 +235,8 @@ struct HaHipPlan {
 	int32_t *tile_order[2]; int32_t tile_count[2];
 	/* captured pass (use_graph) and the arguments it was captured with */
 	hipGraphExec_t gexec;
-	struct { HakLocalArgs a; int method; bool refl, scal, zero; } gkey;
+	struct { HakLocalArgs a; int method; bool refl, scal, zero; int32_t team; uint32_t team_spin; } gkey;
+	int32_t tuned_team;
 	bool loaded;
 };
 
@@ -243,7 +247,7 @@ static void plan_free(HaHipPlan *p) {
 	                p->qld, p->qoff, p->q, p->q64, p->X, p->Y, p->X64, p->Y64, p->stage, p->work, p->work64,
 	                p->flag, p->steps, p->queues, p->order, p->sigma_sum, p->var_sum, p->scale,
 	                p->features, p->pred, p->trace, p->sig, p->d_which, p->sl_par, p->tile_order[0],
-	                p->tile_order[1], p->work_ridge};
+	                p->tile_order[1], p->work_ridge, p->grp};
 	for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i)
 		if (ptrs[i]) hipFree(ptrs[i]);
 	if (p->gexec) hipGraphExecDestroy(p->gexec);
@@ -655,6 +659,53 @@ static int32_t slots128(const HaHipPlan *p) {
 	return n < p->n_slots_polar ? n : p->n_slots_polar;
 }
 
+/* Teams of HAK_GRP workgroups for the 128-tile iteration: as many as fit
+ * HAK_POLAR_OCC128 workgroups per CU and the work area (4 buffers per team), a
+ * multiple of 8 (one team per XCD per group of 8); 0 when fewer than 8. */
+static int32_t team_count(const HaHipPlan *p) {
+	const int64_t nbuf = (int64_t)p->n_slots * 5 > (int64_t)p->n_slots_polar * 4
+	                         ? (int64_t)p->n_slots * 5 : (int64_t)p->n_slots_polar * 4;
+	int64_t n = (int64_t)HAK_POLAR_OCC128 * g.n_cu / HAK_GRP;
+	if (n > nbuf / 4) n = nbuf / 4;
+	n -= n % 8;
+	return n >= 8 ? (int32_t)n : 0;
+}
+
+/* The 128-tile iteration over b's searchlights (b.order, b.n_order): the team
+ * kernel first when teams are on, then the one-workgroup kernel over what the
+ * teams left (nothing, unless a team barrier gave up). Needs steps[] zeroed. */
+/* Allocates the team state (outside any graph capture). */
+static int ensure_teams(HaHipPlan *p) {
+	int rc = kHaSuccess;
+	if (!g.opt.team || p->grp || p->n_teams < 0) return kHaSuccess;
+	p->n_teams = team_count(p);
+	if (p->n_teams == 0) { p->n_teams = -1; return kHaSuccess; }
+	ACK(p->grp, (size_t)p->n_teams * HAK_GRP_STATE_BYTES);
+fail:
+	return rc;
+}
+
+static int launch_polar128(HaHipPlan *p, HakLocalArgs b, hipStream_t st) {
+	int rc = kHaSuccess;
+	b.tile = 128;
+	b.skip_done = 0;
+	if (g.opt.team && b.n_order > 0) {
+		if (p->grp) {
+			KCK(hak_fill_i32((int32_t *)p->grp, (int64_t)p->n_teams * (HAK_GRP_STATE_BYTES / 4), 0, st));
+			HakLocalArgs t = b;
+			t.grp = p->grp;
+			t.grp_spin = g.opt.team_spin;
+			t.queue = p->queues + 6;
+			KCK(hak_polar_grp(&t, p->n_teams, st));
+			b.skip_done = 1;
+		}
+	}
+	b.n_slots = slots128(p);
+	KCK(hak_polar(&b, st));
+fail:
+	return rc;
+}
+
 static int kp_bucket(int32_t kp) {
 	static const int32_t edges[HA_TILE_BUCKETS - 1] = {64, 128, 256, 384, 512};
 	int b = 0;
@@ -701,12 +752,18 @@ static int time_tile(HaHipPlan *p, const int32_t *sel, int32_t n, int tile, floa
 	a.tile = tile;
 	KCK(hak_fill_i32(p->queues, 8, 0, st));
 	KCK(hak_fill_i32(p->flag, p->count, 0, st));
+	KCK(hak_fill_i32(p->steps, p->count, 0, st));
 	a.queue = p->queues + 3;
 	KCK(hak_crosscov(&a, p->n_slots, st));
-	a.queue = p->queues + 0;
-	a.n_slots = tile == 64 ? p->n_slots_polar : slots128(p);
+	a.queue = p->queues + 5;
 	HCK(hipEventRecord(p->ev[6], st));
-	KCK(hak_polar(&a, st));
+	if (tile == 64) {
+		a.n_slots = p->n_slots_polar;
+		KCK(hak_polar(&a, st));
+	} else {
+		rc = launch_polar128(p, a, st);
+		if (rc) goto fail;
+	}
 	HCK(hipEventRecord(p->ev[7], st));
 	HCK(hipStreamSynchronize(st));
 	HCK(hipEventElapsedTime(ms, p->ev[6], p->ev[7]));
@@ -722,6 +779,8 @@ fail:
 static int tune_tiles(HaHipPlan *p) {
 	int rc = kHaSuccess;
 	int32_t *ord = NULL, *sel = NULL, *d_sel = NULL;
+	rc = ensure_teams(p);
+	if (rc) return rc;
 	for (int b = 0; b < HA_TILE_BUCKETS; ++b) p->bucket_tile[b] = g.opt.tile ? g.opt.tile : 64;
 	if (!g.opt.tile) {
 		ord = (int32_t *)malloc((size_t)p->count * sizeof(int32_t));
@@ -753,6 +812,7 @@ static int tune_tiles(HaHipPlan *p) {
 	if (!rc) {
 		p->tuned = true;
 		p->tuned_opt = g.opt.tile;
+		p->tuned_team = g.opt.team;
 	}
 fail:
 	free(ord);
@@ -764,7 +824,7 @@ fail:
 int ha_hip_plan_tiles(HaHipPlan *p, int32_t *bucket_tile) {
 	if (!p || !bucket_tile) return kHaErrorArg;
 	if (!p->loaded) { ERR("plan has no data loaded\n"); return kHaErrorArg; }
-	if (!p->tuned || p->tuned_opt != g.opt.tile) {
+	if (!p->tuned || p->tuned_opt != g.opt.tile || p->tuned_team != g.opt.team) {
 		int rc = tune_tiles(p);
 		if (rc) return rc;
 	}
@@ -814,8 +874,13 @@ static int enqueue_pass(HaHipPlan *p, HakLocalArgs a, int method, bool isReflect
 			b.order = p->tile_order[t];
 			b.n_order = p->tile_count[t];
 			b.queue = p->queues + 4 + t;
-			b.n_slots = t ? slots128(p) : p->n_slots_polar;
-			KCK(hak_polar(&b, st));
+			if (t) {
+				rc = launch_polar128(p, b, st);
+				if (rc) goto fail;
+			} else {
+				b.n_slots = p->n_slots_polar;
+				KCK(hak_polar(&b, st));
+			}
 		}
 	}
 	if (timed) HCK(hipEventRecord(p->ev[2], st));
@@ -882,7 +947,11 @@ int ha_plan_run(HaHipPlan *p, bool isReflection, bool isScaling, bool zero_data,
 	if (method < 0) return kHaErrorArg;
 	hipStream_t st = g.stream;
 	const bool is_polar = isReflection && (method == kHaHipMethodPolar || method == kHaHipMethodPolarLearned);
-	if (is_polar && (!p->tuned || p->tuned_opt != g.opt.tile)) {
+	if (is_polar) {
+		rc = ensure_teams(p);
+		if (rc) return rc;
+	}
+	if (is_polar && (!p->tuned || p->tuned_opt != g.opt.tile || p->tuned_team != g.opt.team)) {
 		rc = tune_tiles(p);
 		if (rc) return rc;
 	}
@@ -933,6 +1002,7 @@ int ha_plan_run(HaHipPlan *p, bool isReflection, bool isScaling, bool zero_data,
 		/* replay the captured pass when nothing it depends on changed */
 		const bool same = p->gexec && p->gkey.method == method && p->gkey.refl == isReflection &&
 		                  p->gkey.scal == isScaling && p->gkey.zero == zero_data &&
+		                  p->gkey.team == g.opt.team && p->gkey.team_spin == g.opt.team_spin &&
 		                  memcmp(&p->gkey.a, &a, sizeof(a)) == 0;
 		if (!same) {
 			if (p->gexec) { hipGraphExecDestroy(p->gexec); p->gexec = NULL; }
@@ -949,6 +1019,7 @@ int ha_plan_run(HaHipPlan *p, bool isReflection, bool isScaling, bool zero_data,
 			memset(&p->gkey, 0, sizeof(p->gkey));
 			p->gkey.a = a; p->gkey.method = method; p->gkey.refl = isReflection;
 			p->gkey.scal = isScaling; p->gkey.zero = zero_data;
+			p->gkey.team = g.opt.team; p->gkey.team_spin = g.opt.team_spin;
 		}
 		HCK(hipEventRecord(p->ev[0], st));
 		HCK(hipGraphLaunch(p->gexec, st));

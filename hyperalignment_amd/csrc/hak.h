@@ -118,6 +118,10 @@ typedef struct {
 	int32_t tile;
 	int32_t fallback_enabled;
 	int32_t rank_check_nt;               /* flag k >= nt as rank deficient */
+	/* teams of HAK_GRP workgroups per searchlight (k_polar_grp) */
+	void *grp;                           /* per team HAK_GRP_STATE_BYTES of state, zeroed before the launch */
+	uint32_t grp_spin;                   /* polls of a team barrier before it gives up (0: default) */
+	int32_t skip_done;                   /* polar: skip searchlights whose steps[] is already nonzero */
 } HakLocalArgs;
 
 /* Workgroups per CU of the polar iteration stage (launch bound and slot count). */
@@ -128,6 +132,15 @@ typedef struct {
 int hak_crosscov(const HakLocalArgs *a, int n_slots, void *stream);
 /* Stage 2: the polar iteration, a->n_slots workgroups, 4 slot buffers each. */
 int hak_polar(const HakLocalArgs *a, void *stream);
+/* Stage 2 with teams of HAK_GRP workgroups of one XCD per searchlight (128 x 128
+ * tiles): n_teams teams (a multiple of 8), 4 slot buffers per team, a->grp holds
+ * n_teams states. A searchlight it finishes gets steps[s] != 0; one it leaves
+ * (a team barrier gave up) keeps steps[s] == 0 and A in the Q store, for a
+ * following hak_polar with skip_done; one cut short while Q was being written is
+ * flagged for the FP64 path. */
+#define HAK_GRP 4  /* workgroups per team */
+#define HAK_GRP_STATE_BYTES 128
+int hak_polar_grp(const HakLocalArgs *a, int n_teams, void *stream);
 int hak_jacobi32(const HakLocalArgs *a, void *stream);
 /* FP64 cross-covariance + one-sided Jacobi + Q = U V^T for every searchlight
  * s < count with flag[s] != 0 (all of them when a->flag is NULL); det-sign

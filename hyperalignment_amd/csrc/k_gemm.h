@@ -75,6 +75,25 @@ __device__ __forceinline__ void put_split(const SplitM &s, int64_t i, float v) {
   s.lo[i] = f2bf(v - bf2f(h));
 }
 __device__ __forceinline__ float get_split(const SplitM &s, int64_t i) { return bf2f(s.hi[i]) + bf2f(s.lo[i]); }
+// four consecutive elements i..i+3 (i a multiple of 4): one 8-byte access per plane
+__device__ __forceinline__ void put_split4(const SplitM &s, int64_t i, float4 v) {
+  const float z[4] = {v.x, v.y, v.z, v.w};
+  uint16_t h[4], l[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    h[c] = f2bf(z[c]);
+    l[c] = f2bf(z[c] - bf2f(h[c]));
+  }
+  *reinterpret_cast<uint2 *>(s.hi + i) =
+      make_uint2((uint32_t)h[0] | ((uint32_t)h[1] << 16), (uint32_t)h[2] | ((uint32_t)h[3] << 16));
+  *reinterpret_cast<uint2 *>(s.lo + i) =
+      make_uint2((uint32_t)l[0] | ((uint32_t)l[1] << 16), (uint32_t)l[2] | ((uint32_t)l[3] << 16));
+}
+__device__ __forceinline__ float4 get_split4(const SplitM &s, int64_t i) {
+  const uint2 h = *reinterpret_cast<const uint2 *>(s.hi + i), l = *reinterpret_cast<const uint2 *>(s.lo + i);
+  return make_float4(bf2f((uint16_t)h.x) + bf2f((uint16_t)l.x), bf2f((uint16_t)(h.x >> 16)) + bf2f((uint16_t)(l.x >> 16)),
+                     bf2f((uint16_t)h.y) + bf2f((uint16_t)l.y), bf2f((uint16_t)(h.y >> 16)) + bf2f((uint16_t)(l.y >> 16)));
+}
 
 struct Opnd {
   int32_t kind;
@@ -442,33 +461,54 @@ __device__ __forceinline__ void acc_to_lds(const Acc<TILE> &acc, float *ctile) {
       }
 }
 
-// Full product over M x N (rows m < M from A, columns n < N from B). Each
-// finished tile is staged in LDS and handed to epi(m, n, v) element by element
-// with consecutive threads on consecutive rows m (coalesced column-major
-// writes). sym (M == N, A == B or a symmetric result): only tiles I <= J are
-// computed and every off-diagonal tile is also delivered transposed, so epi
-// still sees every (m, n) of the full product exactly once.
+// Full product over M x N (rows m < M from A, columns n < N from B; M and N are
+// multiples of 32). Each finished tile is staged in LDS and handed to
+// epi(m, n, v) four rows at a time: v holds C(m..m+3, n), m a multiple of 4.
+// sym (M == N, A == B or a symmetric result): only tiles I <= J are computed
+// and every off-diagonal tile is also delivered transposed, so epi still sees
+// every (m, n) of the full product exactly once.
+//
+// Tile subsets (several workgroups sharing one product): the tiles are taken in
+// blocks of gm x gn, and this workgroup computes tile (BI gm + ga, BJ gn + gb) of
+// every block (BI, BJ), blocks in the same order on every workgroup, so the gm gn
+// workgroups of a block read gm row panels and gn column panels together.
 template <int TILE, int KA, int KB, int PREC = PREC_BF16X3, class Epi>
 __device__ void wg_gemm_mn(GemmSmem<TILE> &sm, const Opnd &A, const Opnd &B, int M, int N, int K, bool sym,
-                           Epi epi) {
+                           Epi epi, int gm = 1, int gn = 1, int ga = 0, int gb = 0) {
   const int ntm = (M + TILE - 1) / TILE, ntn = (N + TILE - 1) / TILE;
+  const int nbm = (ntm + gm - 1) / gm, nbn = (ntn + gn - 1) / gn;
   constexpr int LD = TILE + 1;
-  for (int I = 0; I < ntm; ++I)
-    for (int J = sym ? I : 0; J < ntn; ++J) {
+  for (int BI = 0; BI < nbm; ++BI)
+    for (int BJ = sym ? BI * gm / gn : 0; BJ < nbn; ++BJ) {
+      const int I = BI * gm + ga, J = BJ * gn + gb;
+      if (I >= ntm || J >= ntn || (sym && I > J)) continue;
       Acc<TILE> acc;
       tile_gemm<TILE, KA, KB, PREC>(sm, A, B, I * TILE, J * TILE, K, acc);
       acc_to_lds<TILE>(acc, sm.ctile);
       __syncthreads();
       const int mmax = min(TILE, M - I * TILE), nmax = min(TILE, N - J * TILE);
-      for (int e = threadIdx.x; e < TILE * TILE; e += NT) {
-        const int ml = e % TILE, nl = e / TILE;
-        if (ml < mmax && nl < nmax) epi(I * TILE + ml, J * TILE + nl, sm.ctile[nl * LD + ml]);
+      // four consecutive rows per call; 8 consecutive lanes take 8 row groups of
+      // one column and the next lanes the next columns, so with LD = 1 mod 32 the
+      // 32 lanes of an LDS read hit 32 distinct banks and each column's stores
+      // are 64-byte runs per plane
+      constexpr int TQ = TILE / 4;
+      for (int e = threadIdx.x; e < TQ * TILE; e += NT) {
+        const int rg = (e & 7) + 8 * (e / (8 * TILE)), nl = (e >> 3) % TILE;
+        const int ml = 4 * rg;
+        if (ml < mmax && nl < nmax) {
+          const float *c = &sm.ctile[nl * LD + ml];
+          epi(I * TILE + ml, J * TILE + nl, make_float4(c[0], c[1], c[2], c[3]));
+        }
       }
       if (sym && I != J) {
         // transposed delivery: output (J*TILE + a, I*TILE + b) = tile element (b, a)
-        for (int e = threadIdx.x; e < TILE * TILE; e += NT) {
-          const int al = e % TILE, bl = e / TILE;
-          if (al < nmax && bl < mmax) epi(J * TILE + al, I * TILE + bl, sm.ctile[al * LD + bl]);
+        for (int e = threadIdx.x; e < TQ * TILE; e += NT) {
+          const int ag = (e & 7) + 8 * (e / (8 * TILE)), bl = (e >> 3) % TILE;
+          const int al = 4 * ag;
+          if (al < nmax && bl < mmax) {
+            const float *c = &sm.ctile[al * LD + bl];
+            epi(J * TILE + al, I * TILE + bl, make_float4(c[0], c[LD], c[2 * LD], c[3 * LD]));
+          }
         }
       }
       __syncthreads();
@@ -478,8 +518,8 @@ __device__ void wg_gemm_mn(GemmSmem<TILE> &sm, const Opnd &A, const Opnd &B, int
 // Square kp x kp product (see wg_gemm_mn).
 template <int TILE, int KA, int KB, int PREC = PREC_BF16X3, class Epi>
 __device__ __forceinline__ void wg_gemm(GemmSmem<TILE> &sm, const Opnd &A, const Opnd &B, int kp, int K, bool sym,
-                                        Epi epi) {
-  wg_gemm_mn<TILE, KA, KB, PREC>(sm, A, B, kp, kp, K, sym, epi);
+                                        Epi epi, int gm = 1, int gn = 1, int ga = 0, int gb = 0) {
+  wg_gemm_mn<TILE, KA, KB, PREC>(sm, A, B, kp, kp, K, sym, epi, gm, gn, ga, gb);
 }
 
 #endif  // K_GEMM_H

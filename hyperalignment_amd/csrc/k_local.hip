@@ -37,8 +37,8 @@ __device__ void crosscov(GemmSmem<TILE> &sm, const HakLocalArgs &a, int s, int k
   const int o = a.sl_off[s];
   Opnd oa = opnd_gather(a.X, a.slx_idx + o, a.ld, k);
   Opnd ob = opnd_gather(a.Y, a.sly_idx + o, a.ld, k);
-  wg_gemm<TILE, OP_F32_GATHER, OP_F32_GATHER>(sm, oa, ob, kp, a.ld, false, [&](int m, int n, float v) {
-    Ab[m + (int64_t)n * kp] = v;
+  wg_gemm<TILE, OP_F32_GATHER, OP_F32_GATHER>(sm, oa, ob, kp, a.ld, false, [&](int m, int n, float4 v) {
+    *reinterpret_cast<float4 *>(Ab + m + (int64_t)n * kp) = v;
   });
 }
 
@@ -287,6 +287,7 @@ __global__ __launch_bounds__(NT, TILE == 128 ? HAK_POLAR_OCC128 : HAK_POLAR_OCC)
     const int s = a.order ? a.order[it] : it;
     const float scale = a.sl_par[3 * s];
     if (!(scale > 0.f)) continue;  // left to the FP64 path by stage 1
+    if (a.skip_done && a.steps[s] != 0) continue;  // finished (or given up) by the team kernel
     const float ell = a.sl_par[3 * s + 1], fro = a.sl_par[3 * s + 2];
     const int k = a.sl_off[s + 1] - a.sl_off[s];
     const int kp = a.qld[s];
@@ -295,12 +296,17 @@ __global__ __launch_bounds__(NT, TILE == 128 ? HAK_POLAR_OCC128 : HAK_POLAR_OCC)
     const uint64_t t0 = a.clk ? wall_clock64() : 0;
     double tr = 0.0;
     const bool want_tr = a.sigma_sum != nullptr;
+    SoloTeam team;
     const PolarOut po = polar_solve<TILE, PREC_BF16X3>(sm, a, w, k, kp, fro, scale, ell, a.learned_scale != 0,
                                                        red, a.trace ? a.trace + (int64_t)s * a.max_steps : nullptr,
-                                                       [&](int m, int n, float v) {
-                                                         const int64_t i = m + (int64_t)n * kp;
-                                                         if (want_tr) tr += (double)v * Q[i];
-                                                         Q[i] = v;
+                                                       team, [&](int m, int n, float4 v) {
+                                                         float4 *q4 = reinterpret_cast<float4 *>(Q + m + (int64_t)n * kp);
+                                                         if (want_tr) {
+                                                           const float4 o = *q4;
+                                                           tr += (double)v.x * o.x + (double)v.y * o.y +
+                                                                 (double)v.z * o.z + (double)v.w * o.w;
+                                                         }
+                                                         *q4 = v;
                                                        });
     if (want_tr) tr = wg_sum_f64(tr, red);
     if (threadIdx.x == 0) {
@@ -311,6 +317,101 @@ __global__ __launch_bounds__(NT, TILE == 128 ? HAK_POLAR_OCC128 : HAK_POLAR_OCC)
     }
     __syncthreads();
   }
+}
+
+// Stage 2 with teams: HAK_GRP workgroups of one XCD (workgroups are dispatched
+// round-robin over the 8 XCDs, so workgroup b runs on XCD b % 8) iterate one
+// searchlight together, sharing panel reads through that XCD's L2. The finishing
+// product goes to the team's G buffer, and Q replaces A in the Q store only after
+// a barrier, so a searchlight the team gives up on still holds A for the
+// one-workgroup kernel that follows.
+__global__ __launch_bounds__(NT, HAK_POLAR_OCC128) void k_polar_grp(HakLocalArgs a) {
+  constexpr int TILE = 128;
+  __shared__ GemmSmem<TILE> sm;
+  __shared__ double red[4];
+  __shared__ int ok_sh;
+  const int xcd = blockIdx.x & 7, j = blockIdx.x >> 3;
+  const int team_id = (j / HAK_GRP) * 8 + xcd, member = j % HAK_GRP;
+  GrpState *st = reinterpret_cast<GrpState *>(a.grp) + team_id;
+  GroupTeam team{st, member, 0u, a.grp_spin ? a.grp_spin : HAK_GRP_SPIN, &ok_sh};
+  WorkSlots w;
+  {
+    float *b = a.work + (int64_t)team_id * 4 * a.slot_stride;
+    w.X = b;
+    w.X2 = b + a.slot_stride;
+    w.G = b + 2 * a.slot_stride;
+    w.P = b + 3 * a.slot_stride;
+  }
+  const bool lead = member == 0 && threadIdx.x == 0;
+  // the barrier relies on the members sharing one L2: each publishes its XCD
+  if (threadIdx.x == 0) {
+    uint32_t xcc;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(xcc));
+    __hip_atomic_fetch_or(&st->xcc_mask, 1u << (xcc & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  if (!team.sync()) return;
+  if (__popc(__hip_atomic_load(&st->xcc_mask, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) != 1)
+    return;  // members on different XCDs: every searchlight is left to the one-workgroup kernel
+  for (;;) {
+    if (lead) st->item = atomicAdd(a.queue, 1);
+    if (!team.sync()) break;
+    const int it = __hip_atomic_load(&st->item, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (it >= a.n_order) break;
+    const int s = a.order ? a.order[it] : it;
+    const float scale = a.sl_par[3 * s];
+    if (!(scale > 0.f)) continue;  // left to the FP64 path by stage 1
+    const float ell = a.sl_par[3 * s + 1], fro = a.sl_par[3 * s + 2];
+    const int k = a.sl_off[s + 1] - a.sl_off[s];
+    const int kp = a.qld[s];
+    float *Q = a.q + a.qoff[s];
+    w.A = Q;
+    float *Qn = w.G;  // free once the finishing step has formed P
+    const PolarOut po = polar_solve<TILE, PREC_BF16X3>(
+        sm, a, w, k, kp, fro, scale, ell, a.learned_scale != 0, red,
+        a.trace ? a.trace + (int64_t)s * a.max_steps : nullptr, team,
+        [&](int m, int n, float4 v) { *reinterpret_cast<float4 *>(Qn + m + (int64_t)n * kp) = v; });
+    if (po.aborted) break;
+    if (!po.done) {
+      if (lead) {
+        a.flag[s] = 1;  // budget exhausted or non-finite: the FP64 pass writes Q
+        a.steps[s] = -po.steps;
+      }
+      continue;
+    }
+    // Q replaces A: marked first, so a copy cut short sends the searchlight to the FP64 pass
+    if (lead) {
+      a.flag[s] = 1;
+      a.steps[s] = -po.steps;
+    }
+    if (!team.sync()) break;
+    const bool want_tr = a.sigma_sum != nullptr;
+    double tr = 0.0;
+    {
+      float4 *q4 = reinterpret_cast<float4 *>(Q);
+      const float4 *n4 = reinterpret_cast<const float4 *>(Qn);
+      const int64_t nq = (int64_t)kp * kp / 4;
+      for (int64_t e = team.e0(); e < nq; e += team.es()) {
+        const float4 o = q4[e], v = n4[e];
+        if (want_tr) tr += (double)v.x * o.x + (double)v.y * o.y + (double)v.z * o.z + (double)v.w * o.w;
+        q4[e] = v;
+      }
+    }
+    bool ok;
+    tr = team.sum(tr, red, &ok);
+    if (!ok) break;
+    if (lead) {
+      a.flag[s] = 0;
+      if (want_tr) a.sigma_sum[s] = tr;
+      a.steps[s] = po.steps;
+    }
+  }
+}
+
+extern "C" int hak_polar_grp(const HakLocalArgs *a, int n_teams, void *stream) {
+  if (a->n_order <= 0) return 0;
+  if (n_teams <= 0 || n_teams % 8 != 0 || !a->grp || !a->steps || !a->flag) return (int)hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_polar_grp, dim3(n_teams * HAK_GRP), dim3(NT), 0, (hipStream_t)stream, *a);
+  LAUNCH_RET();
 }
 
 extern "C" int hak_crosscov(const HakLocalArgs *a, int n_slots, void *stream) {

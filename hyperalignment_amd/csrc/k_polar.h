@@ -46,8 +46,10 @@ __device__ __forceinline__ double frob2(const float *M, int kp, double *red) {
   return wg_sum_f64((acc[0] + acc[1]) + (acc[2] + acc[3]), red);
 }
 
-// x = split(M * f) for a kp x kp FP32 block, float4 loads four deep per thread
-__device__ __forceinline__ void split_scaled(const float *M, const SplitM &x, int kp, float f) {
+// x = split(M * f) for a kp x kp FP32 block, float4 loads four deep per thread;
+// a workgroup of a team handles float4 e0, e0 + es, ... (es a multiple of NT)
+__device__ __forceinline__ void split_scaled(const float *M, const SplitM &x, int kp, float f,
+                                             int64_t e0 = -1, int64_t es = NT) {
   const float4 *m4 = reinterpret_cast<const float4 *>(M);
   uint2 *h4 = reinterpret_cast<uint2 *>(x.hi), *l4 = reinterpret_cast<uint2 *>(x.lo);
   const int64_t n4 = (int64_t)kp * kp / 4;
@@ -62,15 +64,17 @@ __device__ __forceinline__ void split_scaled(const float *M, const SplitM &x, in
     h4[e] = make_uint2((uint32_t)h[0] | ((uint32_t)h[1] << 16), (uint32_t)h[2] | ((uint32_t)h[3] << 16));
     l4[e] = make_uint2((uint32_t)l[0] | ((uint32_t)l[1] << 16), (uint32_t)l[2] | ((uint32_t)l[3] << 16));
   };
-  int64_t e = threadIdx.x;
-  for (; e + 3 * NT < n4; e += 4 * NT) {
-    float4 v[4];
+  int64_t e = e0 < 0 ? threadIdx.x : e0;
+  if (es == NT) {
+    for (; e + 3 * NT < n4; e += 4 * NT) {
+      float4 v[4];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) v[c] = m4[e + c * NT];
+      for (int c = 0; c < 4; ++c) v[c] = m4[e + c * NT];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) put4(e + c * NT, v[c]);
+      for (int c = 0; c < 4; ++c) put4(e + c * NT, v[c]);
+    }
   }
-  for (; e < n4; e += NT) put4(e, m4[e]);
+  for (; e < n4; e += es) put4(e, m4[e]);
 }
 
 // ---------------------------------------------------------------------------
@@ -100,11 +104,12 @@ __device__ __forceinline__ int next_item(int32_t *queue, int32_t *sh) {
 }
 
 // x = M * f for a kp x kp FP32 block (FP32 storage of the iterates)
-__device__ __forceinline__ void scale_block(const float *M, float *x, int kp, float f) {
+__device__ __forceinline__ void scale_block(const float *M, float *x, int kp, float f, int64_t e0 = -1,
+                                            int64_t es = NT) {
   const float4 *m4 = reinterpret_cast<const float4 *>(M);
   float4 *x4 = reinterpret_cast<float4 *>(x);
   const int64_t n4 = (int64_t)kp * kp / 4;
-  for (int64_t e = threadIdx.x; e < n4; e += NT) {
+  for (int64_t e = e0 < 0 ? threadIdx.x : e0; e < n4; e += es) {
     const float4 v = m4[e];
     x4[e] = make_float4(v.x * f, v.y * f, v.z * f, v.w * f);
   }
@@ -123,8 +128,13 @@ struct MatS<PREC_BF16X3> {
   __device__ MatS(float *buf, int kp) : s(as_split(buf, kp)) {}
   __device__ __forceinline__ void put(int64_t i, float v) const { put_split(s, i, v); }
   __device__ __forceinline__ float get(int64_t i) const { return get_split(s, i); }
+  __device__ __forceinline__ void put4(int64_t i, float4 v) const { put_split4(s, i, v); }
+  __device__ __forceinline__ float4 get4(int64_t i) const { return get_split4(s, i); }
   __device__ __forceinline__ Opnd opnd(int kind, int ld, int mlimit) const { return opnd_split(s, kind, ld, mlimit); }
-  __device__ __forceinline__ void load_scaled(const float *A, int kp, float f) const { split_scaled(A, s, kp, f); }
+  __device__ __forceinline__ void load_scaled(const float *A, int kp, float f, int64_t e0 = -1,
+                                              int64_t es = NT) const {
+    split_scaled(A, s, kp, f, e0, es);
+  }
 };
 template <>
 struct MatS<PREC_F32> {
@@ -133,14 +143,113 @@ struct MatS<PREC_F32> {
   __device__ MatS(float *buf, int) : f(buf) {}
   __device__ __forceinline__ void put(int64_t i, float v) const { f[i] = v; }
   __device__ __forceinline__ float get(int64_t i) const { return f[i]; }
+  __device__ __forceinline__ void put4(int64_t i, float4 v) const { *reinterpret_cast<float4 *>(f + i) = v; }
+  __device__ __forceinline__ float4 get4(int64_t i) const { return *reinterpret_cast<const float4 *>(f + i); }
   __device__ __forceinline__ Opnd opnd(int kind, int ld, int mlimit) const { return opnd_f32(f, kind, ld, mlimit); }
-  __device__ __forceinline__ void load_scaled(const float *A, int kp, float v) const { scale_block(A, f, kp, v); }
+  __device__ __forceinline__ void load_scaled(const float *A, int kp, float v, int64_t e0 = -1,
+                                              int64_t es = NT) const {
+    scale_block(A, f, kp, v, e0, es);
+  }
 };
 
-// Result of one polar factorisation: steps taken (all attempts) and whether it converged.
+// Result of one polar factorisation: steps taken (all attempts), whether it
+// converged, and whether a team barrier gave up (the team's state is then lost).
 struct PolarOut {
   int steps;
   bool done;
+  bool aborted;
+};
+
+// The workgroups that run one factorisation. SoloTeam: one workgroup does it all.
+// GroupTeam: 4 workgroups of one XCD share it. Each computes one tile of every
+// 2 x 2 block of output tiles (both panels of a block are read by two of them
+// at about the same time, so the second read is an L2 hit) and a quarter of the
+// elementwise passes; they meet at a barrier on a counter in global memory after
+// every product. Reductions go through per-member partial sums added in member
+// order, so the result does not depend on timing.
+struct SoloTeam {
+  static constexpr int gm = 1, gn = 1;
+  __device__ int ga() const { return 0; }
+  __device__ int gb() const { return 0; }
+  __device__ bool leader() const { return true; }
+  __device__ int64_t e0() const { return threadIdx.x; }
+  __device__ int64_t es() const { return NT; }
+  __device__ bool sync() {
+    __syncthreads();
+    return true;
+  }
+  __device__ double sum(double v, double *red, bool *ok) {
+    *ok = true;
+    return wg_sum_f64(v, red);
+  }
+};
+
+#define HAK_GRP_SPIN (1u << 18)  // polls of a team barrier before it gives up (a fraction of a second)
+
+// Per-team state in global memory, zeroed before the launch.
+struct GrpState {
+  uint32_t count;     // barrier arrivals (monotonic)
+  int32_t item;       // the team's current searchlight position
+  uint32_t xcc_mask;  // bit x set by a member running on XCD x
+  uint32_t pad;
+  double part[HAK_GRP];
+  double pad2[16 - 2 - HAK_GRP];
+};
+static_assert(sizeof(GrpState) == HAK_GRP_STATE_BYTES, "team state size");
+
+struct GroupTeam {
+  static constexpr int gm = 2, gn = 2;
+  GrpState *st;
+  int member;
+  uint32_t gen;     // barriers passed by this workgroup
+  uint32_t limit;
+  int *ok_sh;       // LDS flag broadcast by thread 0
+  __device__ int ga() const { return member >> 1; }
+  __device__ int gb() const { return member & 1; }
+  __device__ bool leader() const { return member == 0; }
+  __device__ int64_t e0() const { return (int64_t)member * NT + threadIdx.x; }
+  __device__ int64_t es() const { return (int64_t)HAK_GRP * NT; }
+  // Barrier of the team, valid because all members run on one XCD and so share
+  // its L2 (checked by the kernel through xcc_mask): each thread waits until its
+  // global stores have reached L2 (the vector L1 is write-through), the arrival
+  // counter is an L2 atomic polled past L1, and after the wait the L1 is
+  // invalidated so that later loads see the other members' writes. Device-scope
+  // fences are avoided on purpose: on this GPU they write back and invalidate
+  // the whole L2 of the XCD, the cache the team shares panels through. Returns
+  // false (in every thread) when the wait gives up.
+  __device__ bool sync() {
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    ++gen;
+    if (threadIdx.x == 0) {
+      __hip_atomic_fetch_add(&st->count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const uint32_t target = gen * HAK_GRP;
+      uint32_t n = 0;
+      int ok = 1;
+      while (__hip_atomic_load(&st->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        __builtin_amdgcn_s_sleep(1);
+        if (++n > limit) {
+          ok = 0;
+          break;
+        }
+      }
+      *ok_sh = ok;
+    }
+    __syncthreads();
+    asm volatile("buffer_inv sc0" ::: "memory");
+    const bool ok = *ok_sh != 0;
+    __syncthreads();
+    return ok;
+  }
+  __device__ double sum(double v, double *red, bool *ok) {
+    const double p = wg_sum_f64(v, red);
+    if (threadIdx.x == 0) st->part[member] = p;
+    *ok = sync();
+    double t = 0.0;
+#pragma unroll
+    for (int m = 0; m < HAK_GRP; ++m) t += __hip_atomic_load(&st->part[m], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return t;
+  }
 };
 
 // Polar factor Q of the k x k block of w.A (FP32, kp x kp) by the scheduled
@@ -149,18 +258,22 @@ struct PolarOut {
 // schedule table picks (ca, cb, cc) per step for the bucket of ell; when the
 // growth phase ends with a large residual the schedule restarts from the bound
 // the residual gives; once ||X^T X - I||_F < tol a final quintic Newton-Schulz
-// step writes Q through fin(m, n, v) for every (m, n) of the kp x kp block.
+// step writes Q through fin(m, n, v), v = Q(m..m+3, n), for every (m, n) of
+// the kp x kp block (m a multiple of 4).
 // learned: (scale, ell) came from a prediction; an attempt that does not
 // converge is redone once from (fro, plain_ell), which always holds
 // sigma_max(X0) <= 1. w.X, w.X2, w.G, w.P are overwritten.
-template <int TILE, int PREC, class Fin>
+template <int TILE, int PREC, class Team, class Fin>
 __device__ __forceinline__ PolarOut polar_solve(GemmSmem<TILE> &sm, const HakLocalArgs &a, const WorkSlots &w,
                                                 int k, int kp, float fro, float scale, float ell, bool learned,
-                                                double *red, float *trace, Fin fin) {
+                                                double *red, float *trace, Team &team, Fin fin) {
   using M = MatS<PREC>;
   const M G(w.G, kp), P(w.P, kp);
   int step = 0, total = 0;
   bool done = false;
+  constexpr int gm = Team::gm, gn = Team::gn;
+  const int ga = team.ga(), gb = team.gb();
+  if (!team.leader()) trace = nullptr;
   // A learned attempt whose scale fell below sigma_max diverges within a few
   // steps (the schedules blow up singular values above about 1.01); it is
   // redone once as the plain iteration from A.
@@ -177,8 +290,8 @@ __device__ __forceinline__ PolarOut polar_solve(GemmSmem<TILE> &sm, const HakLoc
     }
     // X0 = A / scale, stored as bf16 hi/lo planes
     const float inv = 1.0f / scale;
-    M(w.X, kp).load_scaled(w.A, kp, inv);
-    __syncthreads();
+    M(w.X, kp).load_scaled(w.A, kp, inv, team.e0(), team.es());
+    if (!team.sync()) return PolarOut{total + step, false, true};
     float *Xb = w.X, *Xnb = w.X2;
     int restart = 0, restarts = 0;
     for (step = 0; step < a.max_steps; ++step) {
@@ -187,15 +300,27 @@ __device__ __forceinline__ PolarOut polar_solve(GemmSmem<TILE> &sm, const HakLoc
       double r2 = 0.0;
       {
         const Opnd ox = X.opnd(M::KK, kp, kp);
-        wg_gemm<TILE, M::KK, M::KK, PREC>(sm, ox, ox, kp, kp, true, [&](int m, int n, float v) {
-          G.put(m + n * kp, v);
-          if (m < k && n < k) {
-            const float d = v - (m == n ? 1.f : 0.f);
-            r2 += (double)d * d;
-          }
-        });
+        wg_gemm<TILE, M::KK, M::KK, PREC>(
+            sm, ox, ox, kp, kp, true,
+            [&](int m, int n, float4 v) {
+              G.put4(m + n * kp, v);
+              if (n < k) {
+                const float z[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                  if (m + c < k) {
+                    const float d = z[c] - (m + c == n ? 1.f : 0.f);
+                    r2 += (double)d * d;
+                  }
+              }
+            },
+            gm, gn, ga, gb);
       }
-      r2 = wg_sum_f64(r2, red);
+      {
+        bool ok;
+        r2 = team.sum(r2, red, &ok);
+        if (!ok) return PolarOut{total + step, false, true};
+      }
       const double r = sqrt(r2);
       if (trace && threadIdx.x == 0) trace[step] = (float)r;
       if (!isfinite(r)) {
@@ -230,36 +355,43 @@ __device__ __forceinline__ PolarOut polar_solve(GemmSmem<TILE> &sm, const HakLoc
       // P = ca I + cb G + cc G^2 (G^2 only when cc != 0), zero outside the k x k block
       if (cc != 0.f) {
         const Opnd og = G.opnd(M::KK, kp, kp);
-        wg_gemm<TILE, M::KK, M::KK, PREC>(sm, og, og, kp, kp, true, [&](int m, int n, float v) {
-          const int i = m + n * kp;
-          float pv = 0.f;
-          if (m < k && n < k) pv = (m == n ? ca : 0.f) + cb * G.get(i) + cc * v;
-          P.put(i, pv);
-        });
+        wg_gemm<TILE, M::KK, M::KK, PREC>(
+            sm, og, og, kp, kp, true,
+            [&](int m, int n, float4 v) {
+              const int i = m + n * kp;
+              float4 pv = make_float4(0.f, 0.f, 0.f, 0.f);
+              if (n < k && m < k) {
+                const float4 g = G.get4(i);
+                pv.x = (m == n ? ca : 0.f) + cb * g.x + cc * v.x;
+                pv.y = (m + 1 < k) ? (m + 1 == n ? ca : 0.f) + cb * g.y + cc * v.y : 0.f;
+                pv.z = (m + 2 < k) ? (m + 2 == n ? ca : 0.f) + cb * g.z + cc * v.z : 0.f;
+                pv.w = (m + 3 < k) ? (m + 3 == n ? ca : 0.f) + cb * g.w + cc * v.w : 0.f;
+              }
+              P.put4(i, pv);
+            },
+            gm, gn, ga, gb);
       } else {
-        for (int64_t e = threadIdx.x; e < (int64_t)kp * kp; e += NT) {
+        for (int64_t e = team.e0(); e < (int64_t)kp * kp; e += team.es()) {
           const int m = (int)(e % kp), n = (int)(e / kp);
           float pv = 0.f;
           if (m < k && n < k) pv = (m == n ? ca : 0.f) + cb * G.get(e);
           P.put(e, pv);
         }
       }
-      __syncthreads();
+      if (!team.sync()) return PolarOut{total + step, false, true};
       // Xn = X P; the finishing product goes to fin(m, n, v)
       {
         const Opnd ox = X.opnd(M::KM, kp, kp);
         const Opnd op = P.opnd(M::KK, kp, kp);
         if (finish) {
-          wg_gemm<TILE, M::KM, M::KK, PREC>(sm, ox, op, kp, kp, false, [&](int m, int n, float v) {
-            fin(m, n, v);
-          });
+          wg_gemm<TILE, M::KM, M::KK, PREC>(
+              sm, ox, op, kp, kp, false, [&](int m, int n, float4 v) { fin(m, n, v); }, gm, gn, ga, gb);
         } else {
-          wg_gemm<TILE, M::KM, M::KK, PREC>(sm, ox, op, kp, kp, false, [&](int m, int n, float v) {
-            Xn.put(m + n * kp, v);
-          });
+          wg_gemm<TILE, M::KM, M::KK, PREC>(
+              sm, ox, op, kp, kp, false, [&](int m, int n, float4 v) { Xn.put4(m + n * kp, v); }, gm, gn, ga, gb);
         }
       }
-      __syncthreads();
+      if (!team.sync()) return PolarOut{total + step, false, true};
       if (finish) {
         done = true;
         ++step;
@@ -271,7 +403,7 @@ __device__ __forceinline__ PolarOut polar_solve(GemmSmem<TILE> &sm, const HakLoc
     }
     total += step;
   }
-  return PolarOut{total, done};
+  return PolarOut{total, done, false};
 }
 
 #endif  // K_POLAR_H

@@ -104,7 +104,9 @@ __device__ int solve_one(GemmSmem<TT> &sm, const HakTplArgs &t, const HakLocalAr
     const Opnd orr = opnd_f32(R, OP_F32_GATHER, t.ld, k);
     float *A = sl.w.A;
     wg_gemm_mn<TT, OP_F32_GATHER, OP_F32_GATHER, PREC_F32>(sm, od, orr, kp, kp, t.ld, false,
-                                                          [&](int m, int n, float v) { A[m + n * kp] = v; });
+                                                          [&](int m, int n, float4 v) {
+                                                            *reinterpret_cast<float4 *>(A + m + n * kp) = v;
+                                                          });
   }
   __syncthreads();
   const double fro2 = frob2(sl.w.A, kp, red);
@@ -114,10 +116,13 @@ __device__ int solve_one(GemmSmem<TT> &sm, const HakTplArgs &t, const HakLocalAr
   int steps = 0;
   if (fro > 0.f && isfinite(fro)) {
     const float *A = sl.w.A;
+    SoloTeam team;
     const PolarOut po = polar_solve<TT, PREC_F32>(sm, pa, sl.w, k, kp, fro, fro, pa.plain_ell, false, red,
-                                                  nullptr, [&](int m, int n, float v) {
-                                                    q[m + n * kp] = v;
-                                                    tr += (double)v * A[m + n * kp];
+                                                  nullptr, team, [&](int m, int n, float4 v) {
+                                                    const float4 o = *reinterpret_cast<const float4 *>(A + m + n * kp);
+                                                    *reinterpret_cast<float4 *>(q + m + n * kp) = v;
+                                                    tr += (double)v.x * o.x + (double)v.y * o.y + (double)v.z * o.z +
+                                                          (double)v.w * o.w;
                                                   });
     steps = po.done ? po.steps : -1;
   } else {
@@ -158,8 +163,10 @@ __device__ int solve_one(GemmSmem<TT> &sm, const HakTplArgs &t, const HakLocalAr
     const Opnd oq = opnd_f32(q, OP_F32_GATHER, kp, k);
     const Opnd odm = opnd_f32(D, OP_F32_M, t.ld, t.ld);
     const int nt = t.nt, ld = t.ld;
-    wg_gemm_mn<TT, OP_F32_GATHER, OP_F32_M, PREC_F32>(sm, oq, odm, kp, ld, kp, false, [&](int j, int tt, float v) {
-      out[tt + (int64_t)j * ld] = (j < k && tt < nt) ? v * f : 0.f;
+    wg_gemm_mn<TT, OP_F32_GATHER, OP_F32_M, PREC_F32>(sm, oq, odm, kp, ld, kp, false, [&](int j, int tt, float4 v) {
+      const float z[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int c = 0; c < 4; ++c) out[tt + (int64_t)(j + c) * ld] = (j + c < k && tt < nt) ? z[c] * f : 0.f;
     });
   }
   __syncthreads();

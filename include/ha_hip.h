@@ -59,6 +59,11 @@ typedef struct {
 	float   learned_margin_scale; /* added to the predicted ln(sigma_max/||A||_F) (default 0.15) */
 	float   learned_margin_ell;   /* decades subtracted from the predicted log10(sigma_min/sigma_max)
 	                                 (default 1.0); a model file carries its calibrated margins */
+	int32_t team;            /* 1 (default): with 128 x 128 tiles, 4 workgroups of one XCD
+	                            iterate each searchlight together and share panel reads
+	                            through L2; 0: one workgroup per searchlight */
+	uint32_t team_spin;      /* polls of a team barrier before the team gives its searchlight
+	                            back to the one-workgroup kernel (0: default, about 0.1 s) */
 } HaHipOptions;
 
 void ha_hip_get_options(HaHipOptions *opt);

@@ -1,0 +1,86 @@
+"""Teams of four workgroups per searchlight (128 x 128 iteration tiles): parity
+with the one-workgroup kernel and the FP64 oracle, determinism, and the
+give-back path when a team barrier gives up (forced from the host with a poll
+limit of 1, so every team wait times out at once and the one-workgroup kernel
+finishes the pass).
+
+Tolerance: max |T - T_fp64| <= 1e-4 on sampled CSC columns, the bar of the
+other polar tests; team and one-workgroup results agree to 1e-4 (their
+residual sums differ in order, which can move a stopping decision by a step).
+"""
+
+import numpy as np
+import pytest
+
+import hyperalignment_amd as ha
+from hyperalignment_amd import synth
+from oracle import ha_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL_T = 1e-4
+
+
+@pytest.fixture(scope="module", autouse=True)
+def hip():
+    ha.init()
+    yield
+    ha.set_options(method="auto", tile=0, team=1, team_spin=0)
+
+
+def run(w, team, team_spin=0, method="polar"):
+    ha.set_options(method=method, tile=128, team=team, team_spin=team_spin)
+    s = w.sls
+    plan = ha.Plan(w.X.shape[0], s.nv, s.indices, s.offsets, s.dists, s.radius)
+    plan.load(w.X, w.Y)
+    plan.execute()
+    indptr, indices, data = plan.csc()
+    st = plan.stats()
+    steps, flags = plan.steps()
+    plan.close()
+    return indptr, indices, data, st, steps
+
+
+@pytest.fixture(scope="module", params=[("c1", 28.0, None), ("c0", 75.0, 500)], ids=["kp224", "kp352"])
+def case(request):
+    name, radius, n = request.param
+    w = synth.make_workload(name, radius=radius, n=n)
+    print(f"[team] {name} radius {radius}: k {w.sls.sizes.min()}-{w.sls.sizes.max()}", flush=True)
+    return w
+
+
+def oracle_cols(w, indptr, indices, cols):
+    s = w.sls
+    wts = O.searchlight_weights_flat(s.indices, s.offsets, s.dists, s.radius, s.nv)
+    return O.searchlight_procrustes_csc(np.asarray(w.X, np.float64), np.asarray(w.Y, np.float64), s.indices,
+                                        s.offsets, indptr, indices, weights=wts, which_cols=cols, n_jobs=8)
+
+
+def test_team_matches_solo_and_oracle(case):
+    w = case
+    ip, ix, d_team, st, steps = run(w, team=1)
+    _, _, d_solo, st0, _ = run(w, team=0)
+    print(f"[team] pass team {st['ms_total']:.1f} ms, solo {st0['ms_total']:.1f} ms", flush=True)
+    assert st["n_unconverged"] == 0 and np.all(steps != 0)
+    assert np.abs(d_team - d_solo).max() < TOL_T
+    cols = [0, w.sls.nv // 3, w.sls.nv - 1]
+    ref = oracle_cols(w, ip, ix, cols)
+    for c in cols:
+        sl = slice(ip[c], ip[c + 1])
+        assert np.abs(d_team[sl] - ref[sl]).max() < TOL_T
+
+
+def test_team_deterministic(case):
+    w = case
+    a = run(w, team=1)[2]
+    b = run(w, team=1)[2]
+    assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
+
+
+def test_team_gives_back_when_barrier_gives_up(case):
+    w = case
+    ip, ix, d_fail, st, steps = run(w, team=1, team_spin=1)
+    _, _, d_solo, _, _ = run(w, team=0)
+    print(f"[team] forced give-up: pass {st['ms_total']:.1f} ms, fp64 fallback {st['n_fallback']}", flush=True)
+    assert st["n_unconverged"] == 0 and np.all(steps != 0)
+    assert np.abs(d_fail - d_solo).max() < TOL_T
