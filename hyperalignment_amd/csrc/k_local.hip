@@ -416,7 +416,10 @@ extern "C" int hak_polar_grp(const HakLocalArgs *a, int n_teams, void *stream) {
 
 extern "C" int hak_crosscov(const HakLocalArgs *a, int n_slots, void *stream) {
   if (a->n_order <= 0) return 0;
-  hipLaunchKernelGGL(k_crosscov<64>, dim3(n_slots), dim3(NT), 0, (hipStream_t)stream, *a);
+  if (a->tile == 128)
+    hipLaunchKernelGGL(k_crosscov<128>, dim3(n_slots), dim3(NT), 0, (hipStream_t)stream, *a);
+  else
+    hipLaunchKernelGGL(k_crosscov<64>, dim3(n_slots), dim3(NT), 0, (hipStream_t)stream, *a);
   LAUNCH_RET();
 }
 
