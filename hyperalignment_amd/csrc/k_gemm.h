@@ -468,20 +468,19 @@ __device__ __forceinline__ void acc_to_lds(const Acc<TILE> &acc, float *ctile) {
 // and every off-diagonal tile is also delivered transposed, so epi still sees
 // every (m, n) of the full product exactly once.
 //
-// Tile subsets (several workgroups sharing one product): the tiles are taken in
-// blocks of gm x gn, and this workgroup computes tile (BI gm + ga, BJ gn + gb) of
-// every block (BI, BJ), blocks in the same order on every workgroup, so the gm gn
-// workgroups of a block read gm row panels and gn column panels together.
+// Tile subsets (nw workgroups sharing one product, this one number w): tile
+// (I, J) goes to workgroup (I + J) mod nw, each in row-major order. Every
+// workgroup then has about one tile per row and per column, so the cheap edge
+// tiles (kp not a multiple of TILE) spread evenly, and at any time the
+// workgroups work on tiles of the same row and read its row panel together.
 template <int TILE, int KA, int KB, int PREC = PREC_BF16X3, class Epi>
 __device__ void wg_gemm_mn(GemmSmem<TILE> &sm, const Opnd &A, const Opnd &B, int M, int N, int K, bool sym,
-                           Epi epi, int gm = 1, int gn = 1, int ga = 0, int gb = 0) {
+                           Epi epi, int nw = 1, int w = 0) {
   const int ntm = (M + TILE - 1) / TILE, ntn = (N + TILE - 1) / TILE;
-  const int nbm = (ntm + gm - 1) / gm, nbn = (ntn + gn - 1) / gn;
   constexpr int LD = TILE + 1;
-  for (int BI = 0; BI < nbm; ++BI)
-    for (int BJ = sym ? BI * gm / gn : 0; BJ < nbn; ++BJ) {
-      const int I = BI * gm + ga, J = BJ * gn + gb;
-      if (I >= ntm || J >= ntn || (sym && I > J)) continue;
+  for (int I = 0; I < ntm; ++I)
+    for (int J = sym ? I : 0; J < ntn; ++J) {
+      if ((I + J) % nw != w) continue;
       Acc<TILE> acc;
       tile_gemm<TILE, KA, KB, PREC>(sm, A, B, I * TILE, J * TILE, K, acc);
       acc_to_lds<TILE>(acc, sm.ctile);
@@ -518,8 +517,8 @@ __device__ void wg_gemm_mn(GemmSmem<TILE> &sm, const Opnd &A, const Opnd &B, int
 // Square kp x kp product (see wg_gemm_mn).
 template <int TILE, int KA, int KB, int PREC = PREC_BF16X3, class Epi>
 __device__ __forceinline__ void wg_gemm(GemmSmem<TILE> &sm, const Opnd &A, const Opnd &B, int kp, int K, bool sym,
-                                        Epi epi, int gm = 1, int gn = 1, int ga = 0, int gb = 0) {
-  wg_gemm_mn<TILE, KA, KB, PREC>(sm, A, B, kp, kp, K, sym, epi, gm, gn, ga, gb);
+                                        Epi epi, int nw = 1, int w = 0) {
+  wg_gemm_mn<TILE, KA, KB, PREC>(sm, A, B, kp, kp, K, sym, epi, nw, w);
 }
 
 #endif  // K_GEMM_H

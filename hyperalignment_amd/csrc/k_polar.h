@@ -161,16 +161,15 @@ struct PolarOut {
 };
 
 // The workgroups that run one factorisation. SoloTeam: one workgroup does it all.
-// GroupTeam: 4 workgroups of one XCD share it. Each computes one tile of every
-// 2 x 2 block of output tiles (both panels of a block are read by two of them
-// at about the same time, so the second read is an L2 hit) and a quarter of the
-// elementwise passes; they meet at a barrier on a counter in global memory after
-// every product. Reductions go through per-member partial sums added in member
+// GroupTeam: 4 workgroups of one XCD share it. Tile (I, J) of every product goes
+// to member (I + J) mod 4 (the members work on tiles of one row at a time, so
+// its row panel is read from HBM once and from L2 by the others), and each does
+// a quarter of the elementwise passes; they meet at a barrier on a counter in
+// global memory after every product. Reductions go through per-member partial sums added in member
 // order, so the result does not depend on timing.
 struct SoloTeam {
-  static constexpr int gm = 1, gn = 1;
-  __device__ int ga() const { return 0; }
-  __device__ int gb() const { return 0; }
+  static constexpr int nw = 1;
+  __device__ int member_id() const { return 0; }
   __device__ bool leader() const { return true; }
   __device__ int64_t e0() const { return threadIdx.x; }
   __device__ int64_t es() const { return NT; }
@@ -198,14 +197,13 @@ struct GrpState {
 static_assert(sizeof(GrpState) == HAK_GRP_STATE_BYTES, "team state size");
 
 struct GroupTeam {
-  static constexpr int gm = 2, gn = 2;
+  static constexpr int nw = HAK_GRP;
   GrpState *st;
   int member;
   uint32_t gen;     // barriers passed by this workgroup
   uint32_t limit;
   int *ok_sh;       // LDS flag broadcast by thread 0
-  __device__ int ga() const { return member >> 1; }
-  __device__ int gb() const { return member & 1; }
+  __device__ int member_id() const { return member; }
   __device__ bool leader() const { return member == 0; }
   __device__ int64_t e0() const { return (int64_t)member * NT + threadIdx.x; }
   __device__ int64_t es() const { return (int64_t)HAK_GRP * NT; }
@@ -271,8 +269,8 @@ __device__ __forceinline__ PolarOut polar_solve(GemmSmem<TILE> &sm, const HakLoc
   const M G(w.G, kp), P(w.P, kp);
   int step = 0, total = 0;
   bool done = false;
-  constexpr int gm = Team::gm, gn = Team::gn;
-  const int ga = team.ga(), gb = team.gb();
+  constexpr int nw = Team::nw;
+  const int tw = team.member_id();
   if (!team.leader()) trace = nullptr;
   // A learned attempt whose scale fell below sigma_max diverges within a few
   // steps (the schedules blow up singular values above about 1.01); it is
@@ -314,7 +312,7 @@ __device__ __forceinline__ PolarOut polar_solve(GemmSmem<TILE> &sm, const HakLoc
                   }
               }
             },
-            gm, gn, ga, gb);
+            nw, tw);
       }
       {
         bool ok;
@@ -369,7 +367,7 @@ __device__ __forceinline__ PolarOut polar_solve(GemmSmem<TILE> &sm, const HakLoc
               }
               P.put4(i, pv);
             },
-            gm, gn, ga, gb);
+            nw, tw);
       } else {
         for (int64_t e = team.e0(); e < (int64_t)kp * kp; e += team.es()) {
           const int m = (int)(e % kp), n = (int)(e / kp);
@@ -385,10 +383,10 @@ __device__ __forceinline__ PolarOut polar_solve(GemmSmem<TILE> &sm, const HakLoc
         const Opnd op = P.opnd(M::KK, kp, kp);
         if (finish) {
           wg_gemm<TILE, M::KM, M::KK, PREC>(
-              sm, ox, op, kp, kp, false, [&](int m, int n, float4 v) { fin(m, n, v); }, gm, gn, ga, gb);
+              sm, ox, op, kp, kp, false, [&](int m, int n, float4 v) { fin(m, n, v); }, nw, tw);
         } else {
           wg_gemm<TILE, M::KM, M::KK, PREC>(
-              sm, ox, op, kp, kp, false, [&](int m, int n, float4 v) { Xn.put4(m + n * kp, v); }, gm, gn, ga, gb);
+              sm, ox, op, kp, kp, false, [&](int m, int n, float4 v) { Xn.put4(m + n * kp, v); }, nw, tw);
         }
       }
       if (!team.sync()) return PolarOut{total + step, false, true};
