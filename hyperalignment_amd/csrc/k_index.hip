@@ -414,7 +414,24 @@ extern "C" int hak_pattern_fill(const int32_t *sl_idx, const int32_t *sl_off, co
 // ---------------------------------------------------------------------------
 
 #define SC_T 256
-#define SC_LDS 3072   // column entries held in LDS (rows int32 + acc double = 36 KB)
+#define SC_LDS 3072   // column entries held in LDS (FP64 accumulators, 24 KB)
+#define SC_H 4096     // LDS hash of the column's rows: ((row + 1) << 12) | position, 0 = empty
+#define SC_HROW (1 << 19)  // rows below this fit the packing (positions < 4096 = SC_H)
+
+__device__ __forceinline__ uint32_t sc_hash(int32_t r) { return ((uint32_t)r * 2654435761u) >> 20; }  // 12 bits
+
+// position of row r in the column, from the LDS hash (r is in the column)
+__device__ __forceinline__ int32_t hash_row(const uint32_t *ht, int32_t r) {
+  const uint32_t key = (uint32_t)(r + 1);
+  uint32_t h = sc_hash(r);
+  for (int probe = 0; probe < SC_H; ++probe) {
+    const uint32_t e = ht[h];
+    if (e == 0u) return -1;
+    if ((e >> 12) == key) return (int32_t)(e & 0xFFFu);
+    h = (h + 1) & (SC_H - 1);
+  }
+  return -1;
+}
 
 __device__ __forceinline__ int32_t find_row(const int32_t *rows, int32_t L, int32_t r) {
   int32_t lo = 0, hi = L;
@@ -427,7 +444,10 @@ __device__ __forceinline__ int32_t find_row(const int32_t *rows, int32_t L, int3
 
 // Per column c (one workgroup), the contributions of every searchlight holding
 // c, in increasing searchlight order with a barrier between searchlights, so
-// each element of data is summed in one fixed order. The occurrences' metadata
+// each element of data is summed in one fixed order. A column of up to SC_LDS
+// rows keeps its FP64 accumulators in LDS and finds a row's position through an
+// LDS hash (about one probe; a row outside the pattern ends at an empty slot
+// and is skipped); longer columns work in global memory with a binary search. The occurrences' metadata
 // (searchlight, position, size, Q offset, scale) is staged in LDS a batch at a
 // time by all threads at once, and each searchlight's Q values, weights and
 // vertex ids are loaded one searchlight ahead into registers, so the barrier
@@ -446,7 +466,7 @@ __global__ __launch_bounds__(SC_T) void k_scatter(
     const int64_t *__restrict__ indptr, const int32_t *__restrict__ indices, double *__restrict__ data,
     const float *__restrict__ q, const int64_t *__restrict__ qoff, const int32_t *__restrict__ qld,
     const double *__restrict__ scale, const double *__restrict__ w, int32_t s_begin, int32_t s_end) {
-  __shared__ int32_t rows_l[SC_LDS];
+  __shared__ uint32_t ht[SC_H];
   __shared__ double acc_l[SC_LDS];
   __shared__ int32_t m_o[SC_OCC], m_ks[SC_OCC];
   __shared__ int64_t m_q[SC_OCC];
@@ -462,15 +482,22 @@ __global__ __launch_bounds__(SC_T) void k_scatter(
     if (pb == pe2) continue;
     const int64_t cb = indptr[c];
     const int32_t L = (int32_t)(indptr[c + 1] - cb);
-    const bool in_lds = (L <= SC_LDS);
-    const int32_t *rows = in_lds ? rows_l : indices + cb;
+    // rows sorted: the last is the largest
+    const bool in_lds = L > 0 && L <= SC_LDS && indices[cb + L - 1] < SC_HROW;
+    const int32_t *rows = indices + cb;
     double *acc = in_lds ? acc_l : data + cb;
     if (in_lds) {
+      for (int32_t i = tid; i < SC_H; i += SC_T) ht[i] = 0u;
+      __syncthreads();
       for (int32_t i = tid; i < L; i += SC_T) {
-        rows_l[i] = indices[cb + i];
+        const int32_t r = indices[cb + i];
+        const uint32_t v = ((uint32_t)(r + 1) << 12) | (uint32_t)i;
+        uint32_t h = sc_hash(r);
+        while (atomicCAS(&ht[h], 0u, v) != 0u) h = (h + 1) & (SC_H - 1);
         acc_l[i] = data[cb + i];
       }
     }
+    auto pos_of = [&](int32_t r) { return in_lds ? hash_row(ht, r) : find_row(rows, L, r); };
     for (int64_t b0 = pb; b0 < pe2; b0 += SC_OCC) {
       const int32_t nb = (int32_t)((pe2 - b0) < SC_OCC ? (pe2 - b0) : SC_OCC);
       __syncthreads();  // previous batch done with the metadata
@@ -509,7 +536,7 @@ __global__ __launch_bounds__(SC_T) void k_scatter(
 #pragma unroll
         for (int u = 0; u < SC_V; ++u) {
           if (cur.r[u] >= 0) {
-            const int32_t pos = find_row(rows, L, cur.r[u]);
+            const int32_t pos = pos_of(cur.r[u]);
             if (pos >= 0) acc[pos] += cur.v[u];
           }
         }
@@ -520,7 +547,7 @@ __global__ __launch_bounds__(SC_T) void k_scatter(
           const float *qc = q + m_q[e];
           const double sc = m_sc[e];
           for (int32_t i = tid + SC_V * SC_T; i < ks; i += SC_T) {
-            const int32_t pos = find_row(rows, L, slx_idx[o + i]);
+            const int32_t pos = pos_of(slx_idx[o + i]);
             if (pos >= 0) {
               double t = (double)qc[i];
               if (scale) t = t * sc;
