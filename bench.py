@@ -118,10 +118,10 @@ def copy_calibration():
     print(json.dumps({"copy_bytes": 4 << 28}))
 
 
-def pmc_traffic(config, method, tiles_used):
-    """HBM bytes per k_polar launch from the committed rocprofv3 counter summary
+def pmc_traffic(config, method, tiles_used, team):
+    """HBM bytes per iteration pass from the committed rocprofv3 counter summary
     (FETCH_SIZE + WRITE_SIZE, calibrated against a copy kernel), when one exists
-    for this workload, method and iteration tile."""
+    for this workload, method, iteration tile and team setting."""
     path = os.path.join(ROOT, "profiles", f"k_polar_{config}_traffic.txt")
     if not os.path.exists(path):
         return None
@@ -130,7 +130,7 @@ def pmc_traffic(config, method, tiles_used):
         if "=" in line and not line.startswith("#"):
             k, v = line.split("=", 1)
             kv[k.strip()] = v.strip()
-    if kv.get("method") != method or kv.get("tile") != tiles_used:
+    if kv.get("method") != method or kv.get("tile") != tiles_used or kv.get("team") != str(team):
         return None
     return float(kv["traffic_bytes"])
 
@@ -209,9 +209,13 @@ def main():
         tiles = dict(zip(edges, bt))
         kps = set(((s.sizes + 31) // 32 * 32).tolist())
         tiles_used = "+".join(str(t) for t in sorted({bt[kp_bucket(kp)] for kp in kps}))
+    team = int(ha.get_options().team)
+    kernel = "k_polar" + (f"<{tiles_used}>" if tiles_used else "")
+    if team and tiles_used and "128" in tiles_used:
+        kernel = "k_polar_grp (teams of 4, 128 x 128 tiles) + " + kernel
     roofline = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_BF16 / 1e12, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_BF16, "traffic": pmc_traffic(args.config, method, tiles_used),
-                "kernel": "k_polar" + (f"<{tiles_used}>" if tiles_used else ""),
+                "frac": achieved / PEAK_BF16, "traffic": pmc_traffic(args.config, method, tiles_used, team),
+                "kernel": kernel,
                 "kernel_ms": polar_avg, "flops_per_launch": st["gemm_flops"]}
     stats_main = st
     # apply and score the transform on the training data (Yhat = X T, vertex-wise r)
