@@ -109,6 +109,9 @@ _SIGS = {
     "ha_searchlight_ridge": (ctypes.c_int, [_P(TMat), _P(TMat), _P(TSearchlights), _P(TSearchlights),
                                             _P(TSparseCSC), _P(_P(_d)), _d]),
     "ha_hip_plan_ridge": (ctypes.c_int, [ctypes.c_void_p, _d]),
+    "ha_mesh_searchlights": (ctypes.c_int, [_P(_d), _i32, _P(_i32), _i32, _d, _P(_i32), _i32, _P(_P(_i32)),
+                                            _P(_P(_i32)), _P(_P(_d))]),
+    "ha_mesh_free": (None, [ctypes.c_void_p]),
     "ha_hip_plan_create": (ctypes.c_int, [_i32, _i32, _P(_i32), _P(_i32), _P(_d), _i32, _d,
                                           _P(ctypes.c_void_p)]),
     "ha_hip_plan_destroy": (None, [ctypes.c_void_p]),
@@ -487,6 +490,97 @@ def searchlight_ridge(X, Y, sls, sls_Y=None, mat0=None, alpha=1e3, weights=None)
     mat = sp.csc_matrix(mat0, dtype=np.float64, copy=True)
     mat.sort_indices()
     return searchlight_ridge_csc(X, Y, sls, mat, sls_Y, weights, alpha)
+
+
+def mesh_searchlights_flat(coords, faces, radius, centers=None):
+    """ha_mesh_searchlights: searchlights of `radius` along the mesh edges
+    (Dijkstra) around each centre (every vertex by default), as the flat arrays
+    the searchlight entry points take: (indices int32, offsets int32, dists float64),
+    each searchlight sorted by vertex id."""
+    V = np.ascontiguousarray(coords, dtype=np.float64)
+    F = np.ascontiguousarray(faces, dtype=np.int32)
+    if V.ndim != 2 or V.shape[1] != 3 or F.ndim != 2 or F.shape[1] != 3:
+        raise ValueError("coords must be (nv, 3) and faces (nf, 3)")
+    C = None if centers is None else np.ascontiguousarray(centers, dtype=np.int32)
+    pi, po = ctypes.POINTER(ctypes.c_int32)(), ctypes.POINTER(ctypes.c_int32)()
+    pd = ctypes.POINTER(ctypes.c_double)()
+    nc = V.shape[0] if C is None else len(C)
+    L = lib()
+    _check("ha_mesh_searchlights", L.ha_mesh_searchlights(
+        _ptr(V, ctypes.c_double), V.shape[0], _ptr(F, ctypes.c_int32), F.shape[0], float(radius),
+        _ptr(C, ctypes.c_int32), 0 if C is None else len(C), ctypes.byref(pi), ctypes.byref(po), ctypes.byref(pd)))
+    try:
+        off = np.ctypeslib.as_array(po, shape=(nc + 1,)).copy()
+        total = int(off[-1])
+        idx = np.ctypeslib.as_array(pi, shape=(max(total, 1),))[:total].copy()
+        dists = np.ctypeslib.as_array(pd, shape=(max(total, 1),))[:total].copy()
+    finally:
+        for ptr in (pi, po, pd):
+            L.ha_mesh_free(ctypes.cast(ptr, ctypes.c_void_p))
+    return idx, off, dists
+
+
+def mesh_searchlights(coords, faces, radius, centers=None, return_dists=False):
+    """Searchlights on a triangle mesh in the form neuroboros' nb.sls returns
+    them (benchmark_neuroboros.py:153): a list of vertex-id arrays, and with
+    return_dists the matching distance arrays. Distances run along the mesh
+    edges (Dijkstra); a searchlight holds the vertices closer than radius."""
+    idx, off, d = mesh_searchlights_flat(coords, faces, radius, centers)
+    sls = [idx[off[i]:off[i + 1]] for i in range(len(off) - 1)]
+    if not return_dists:
+        return sls
+    return sls, [d[off[i]:off[i + 1]] for i in range(len(off) - 1)]
+
+
+def read_surface(path):
+    """(coords (nv, 3) float64, faces (nf, 3) int32) of a FreeSurfer triangle
+    surface file (lh.white, ...) or a GIFTI surface (.gii: the POINTSET and
+    TRIANGLE data arrays; ASCII, Base64Binary or GZipBase64Binary encodings)."""
+    with open(path, "rb") as fh:
+        head = fh.read(3)
+    if head == b"\xff\xff\xfe":
+        return _read_freesurfer(path)
+    return _read_gifti(path)
+
+
+def _read_freesurfer(path):
+    with open(path, "rb") as fh:
+        fh.read(3)
+        fh.readline()  # creator line
+        fh.readline()  # blank line
+        nv, nf = np.frombuffer(fh.read(8), dtype=">i4")
+        coords = np.frombuffer(fh.read(12 * int(nv)), dtype=">f4").reshape(int(nv), 3)
+        faces = np.frombuffer(fh.read(12 * int(nf)), dtype=">i4").reshape(int(nf), 3)
+    return coords.astype(np.float64), faces.astype(np.int32)
+
+
+def _read_gifti(path):
+    import base64
+    import xml.etree.ElementTree as ET
+    import zlib
+    dtypes = {"NIFTI_TYPE_FLOAT32": np.float32, "NIFTI_TYPE_FLOAT64": np.float64, "NIFTI_TYPE_INT32": np.int32,
+              "NIFTI_TYPE_UINT8": np.uint8}
+    out = {}
+    for da in ET.parse(path).getroot().iter("DataArray"):
+        intent = da.get("Intent")
+        if intent not in ("NIFTI_INTENT_POINTSET", "NIFTI_INTENT_TRIANGLE"):
+            continue
+        dims = [int(da.get(f"Dim{i}")) for i in range(int(da.get("Dimensionality")))]
+        dt = np.dtype(dtypes[da.get("DataType")])
+        if da.get("Endian", "LittleEndian") == "BigEndian":
+            dt = dt.newbyteorder(">")
+        text = da.find("Data").text or ""
+        enc = da.get("Encoding", "ASCII")
+        if enc == "ASCII":
+            a = np.array(text.split(), dtype=dt)
+        else:
+            raw = base64.b64decode("".join(text.split()))
+            if enc == "GZipBase64Binary":
+                raw = zlib.decompress(raw, 15 + 32)  # zlib stream (the standard's) or gzip framing
+            a = np.frombuffer(raw, dtype=dt)
+        order = "F" if da.get("ArrayIndexingOrder") == "ColumnMajorOrder" else "C"
+        out[intent] = a.reshape(dims, order=order)
+    return (out["NIFTI_INTENT_POINTSET"].astype(np.float64), out["NIFTI_INTENT_TRIANGLE"].astype(np.int32))
 
 
 def ridge(X, Y, alpha):

@@ -76,6 +76,12 @@ def icosphere(order: int, radius: float = SPHERE_RADIUS_MM) -> np.ndarray:
     Vertex order: the 12 icosahedron vertices, then each level's edge
     midpoints in lexicographic order of their (lo, hi) endpoint ids.
     """
+    return icosphere_mesh(order, radius)[0]
+
+
+def icosphere_mesh(order: int, radius: float = SPHERE_RADIUS_MM):
+    """(vertices (10*4^order + 2, 3), faces (20*4^order, 3) int32) of the subdivided
+    icosahedron of icosphere()."""
     p = (1.0 + 5.0 ** 0.5) / 2.0
     v = np.array([[-1, p, 0], [1, p, 0], [-1, -p, 0], [1, -p, 0],
                   [0, -1, p], [0, 1, p], [0, -1, -p], [0, 1, -p],
@@ -102,7 +108,7 @@ def icosphere(order: int, radius: float = SPHERE_RADIUS_MM) -> np.ndarray:
         f = np.concatenate([np.stack([a, ab, ca], 1), np.stack([b, bc, ab], 1),
                             np.stack([c, ca, bc], 1), np.stack([ab, bc, ca], 1)], axis=0)
         v = np.concatenate([v, mid], axis=0)
-    return v * radius
+    return v * radius, f.astype(np.int32)
 
 
 def _neighbourhoods(coords: np.ndarray, radius_mm: float):

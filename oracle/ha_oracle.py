@@ -299,6 +299,47 @@ def searchlight_ridge_csc(X, Y, idx, offsets, indptr, indices, alpha, data0=None
 
 
 # ---------------------------------------------------------------------------
+# searchlights on a mesh
+# ---------------------------------------------------------------------------
+
+def mesh_searchlights(coords, faces, radius, centers=None):
+    """Searchlights along the mesh edges, for the checker: per centre, a full
+    Dijkstra (heapq) over the edge graph (edge length = Euclidean distance of
+    its end points), the vertices with distance < radius, sorted by vertex id.
+    The reference takes these lists from neuroboros (benchmark_neuroboros.py:153),
+    absent here; parity with neuroboros itself is unpinned."""
+    import heapq
+    coords = np.asarray(coords, dtype=np.float64)
+    nv = coords.shape[0]
+    nbr = [set() for _ in range(nv)]
+    for a, b, c in np.asarray(faces):
+        for u, v in ((a, b), (b, c), (c, a)):
+            if u != v:
+                nbr[u].add(int(v))
+                nbr[v].add(int(u))
+    centers = range(nv) if centers is None else centers
+    sls, dists = [], []
+    for c0 in centers:
+        dist = {int(c0): 0.0}
+        done = set()
+        h = [(0.0, int(c0))]
+        while h:
+            d, v = heapq.heappop(h)
+            if v in done or d > dist[v]:
+                continue
+            done.add(v)
+            for u in nbr[v]:
+                nd = d + float(np.linalg.norm(coords[v] - coords[u]))
+                if nd < radius and nd < dist.get(u, np.inf):
+                    dist[u] = nd
+                    heapq.heappush(h, (nd, u))
+        keep = sorted(v for v in done)
+        sls.append(np.array(keep, dtype=np.int64))
+        dists.append(np.array([dist[v] for v in keep]))
+    return sls, dists
+
+
+# ---------------------------------------------------------------------------
 # templates
 # ---------------------------------------------------------------------------
 
