@@ -352,10 +352,14 @@ __global__ __launch_bounds__(NT, HAK_POLAR_OCC128) void k_polar_grp(HakLocalArgs
   if (!team.sync()) return;
   if (__popc(__hip_atomic_load(&st->xcc_mask, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) != 1)
     return;  // members on different XCDs: every searchlight is left to the one-workgroup kernel
-  for (;;) {
-    if (lead) st->item = atomicAdd(a.queue, 1);
+  // Turn t takes its searchlight from item[t & 1]: a member that skips a
+  // searchlight without a barrier (scale <= 0) cannot overwrite the slot another
+  // member has yet to read, since that slot is written again only after the
+  // next turn's barrier, which every member reaches after its read.
+  for (int turn = 0;; ++turn) {
+    if (lead) st->item[turn & 1] = atomicAdd(a.queue, 1);
     if (!team.sync()) break;
-    const int it = __hip_atomic_load(&st->item, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int it = __hip_atomic_load(&st->item[turn & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (it >= a.n_order) break;
     const int s = a.order ? a.order[it] : it;
     const float scale = a.sl_par[3 * s];

@@ -188,9 +188,8 @@ struct SoloTeam {
 // Per-team state in global memory, zeroed before the launch.
 struct GrpState {
   uint32_t count;     // barrier arrivals (monotonic)
-  int32_t item;       // the team's current searchlight position
+  int32_t item[2];    // the team's searchlight positions, by loop turn parity
   uint32_t xcc_mask;  // bit x set by a member running on XCD x
-  uint32_t pad;
   double part[HAK_GRP];
   double pad2[16 - 2 - HAK_GRP];
 };
@@ -208,13 +207,15 @@ struct GroupTeam {
   __device__ int64_t e0() const { return (int64_t)member * NT + threadIdx.x; }
   __device__ int64_t es() const { return (int64_t)HAK_GRP * NT; }
   // Barrier of the team, valid because all members run on one XCD and so share
-  // its L2 (checked by the kernel through xcc_mask): each thread waits until its
-  // global stores have reached L2 (the vector L1 is write-through), the arrival
-  // counter is an L2 atomic polled past L1, and after the wait the L1 is
-  // invalidated so that later loads see the other members' writes. Device-scope
-  // fences are avoided on purpose: on this GPU they write back and invalidate
-  // the whole L2 of the XCD, the cache the team shares panels through. Returns
-  // false (in every thread) when the wait gives up.
+  // its L2 (checked by the kernel through xcc_mask). Release: each thread waits
+  // until its global stores have reached L2 (the vector L1 is write-through);
+  // the L2 write-back of an agent-scope release fence is left out on purpose,
+  // since the members read through the same L2. The arrival counter is an L2
+  // atomic polled past L1. Acquire: an agent-scope acquire fence (buffer_inv
+  // sc1), which drops the CU's L1 lines so that later loads see the other
+  // members' writes; the workgroup-scope form (sc0) does not do that here, and
+  // stale L1 lines then give wrong results once nothing else evicts them.
+  // Returns false (in every thread) when the wait gives up.
   __device__ bool sync() {
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
@@ -234,7 +235,7 @@ struct GroupTeam {
       *ok_sh = ok;
     }
     __syncthreads();
-    asm volatile("buffer_inv sc0" ::: "memory");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     const bool ok = *ok_sh != 0;
     __syncthreads();
     return ok;

@@ -41,7 +41,10 @@ def run(w, team, team_spin=0, method="polar"):
     return indptr, indices, data, st, steps
 
 
-@pytest.fixture(scope="module", params=[("c1", 28.0, None), ("c0", 75.0, 500)], ids=["kp224", "kp352"])
+# kp128: one 128 tile per product, so member 0 forms every product and the
+# others only read what it wrote (the barrier's acquire side carries all of it)
+@pytest.fixture(scope="module", params=[("c1", None, None), ("c1", 28.0, None), ("c0", 75.0, 500)],
+                ids=["kp128", "kp224", "kp352"])
 def case(request):
     name, radius, n = request.param
     w = synth.make_workload(name, radius=radius, n=n)
