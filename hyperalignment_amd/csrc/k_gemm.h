@@ -392,9 +392,13 @@ __device__ __forceinline__ void mma_chunk_f32(GemmSmem<TILE> &sm, int buf, Acc<T
   }
 }
 
+// One output tile over K. sa, sb: the stage registers; with have_first the
+// caller has already fetched chunk 0 of this tile into them (during the
+// previous tile's epilogue), so its load latency is hidden.
 template <int TILE, int KA, int KB, int PREC = PREC_BF16X3>
 __device__ __forceinline__ void tile_gemm(GemmSmem<TILE> &sm, const Opnd &A, const Opnd &B, int tm0, int tn0,
-                                          int K, Acc<TILE> &acc) {
+                                          int K, Acc<TILE> &acc, Stage<TILE> &sa, Stage<TILE> &sb,
+                                          bool have_first) {
   constexpr int W = TILE / 64;
   const int wave = threadIdx.x >> 6;
   const int wm0 = (wave >> 1) * (TILE / 2), wn0 = (wave & 1) * (TILE / 2);
@@ -412,9 +416,10 @@ __device__ __forceinline__ void tile_gemm(GemmSmem<TILE> &sm, const Opnd &A, con
       for (int r = 0; r < 16; ++r) acc.v[i][j][r] = 0.f;
   static_assert(KB != OP_SPLIT_M, "only the A operand may be read with m contiguous");
   const int nc = K / KC;
-  Stage<TILE> sa, sb;
-  fetch<TILE, KA>(A, tm0, 0, sa);
-  fetch<TILE, KB>(B, tn0, 0, sb);
+  if (!have_first) {
+    fetch<TILE, KA>(A, tm0, 0, sa);
+    fetch<TILE, KB>(B, tn0, 0, sb);
+  }
   if constexpr (KA == OP_SPLIT_M)
     stash_m<TILE>(sa, sm.th[0], sm.tl[0]);
   else
@@ -478,40 +483,63 @@ __device__ void wg_gemm_mn(GemmSmem<TILE> &sm, const Opnd &A, const Opnd &B, int
                            Epi epi, int nw = 1, int w = 0) {
   const int ntm = (M + TILE - 1) / TILE, ntn = (N + TILE - 1) / TILE;
   constexpr int LD = TILE + 1;
-  for (int I = 0; I < ntm; ++I)
-    for (int J = sym ? I : 0; J < ntn; ++J) {
-      if ((I + J) % nw != w) continue;
-      Acc<TILE> acc;
-      tile_gemm<TILE, KA, KB, PREC>(sm, A, B, I * TILE, J * TILE, K, acc);
-      acc_to_lds<TILE>(acc, sm.ctile);
-      __syncthreads();
-      const int mmax = min(TILE, M - I * TILE), nmax = min(TILE, N - J * TILE);
-      // four consecutive rows per call; 8 consecutive lanes take 8 row groups of
-      // one column and the next lanes the next columns, so with LD = 1 mod 32 the
-      // 32 lanes of an LDS read hit 32 distinct banks and each column's stores
-      // are 64-byte runs per plane
-      constexpr int TQ = TILE / 4;
-      for (int e = threadIdx.x; e < TQ * TILE; e += NT) {
-        const int rg = (e & 7) + 8 * (e / (8 * TILE)), nl = (e >> 3) % TILE;
-        const int ml = 4 * rg;
-        if (ml < mmax && nl < nmax) {
-          const float *c = &sm.ctile[nl * LD + ml];
-          epi(I * TILE + ml, J * TILE + nl, make_float4(c[0], c[1], c[2], c[3]));
-        }
+  // this workgroup's tiles in row-major order (only I <= J when sym)
+  auto next_tile = [&](int &I, int &J) {
+    for (;;) {
+      if (++J >= ntn) {
+        if (++I >= ntm) return false;
+        J = sym ? I : 0;
       }
-      if (sym && I != J) {
-        // transposed delivery: output (J*TILE + a, I*TILE + b) = tile element (b, a)
-        for (int e = threadIdx.x; e < TQ * TILE; e += NT) {
-          const int ag = (e & 7) + 8 * (e / (8 * TILE)), bl = (e >> 3) % TILE;
-          const int al = 4 * ag;
-          if (al < nmax && bl < mmax) {
-            const float *c = &sm.ctile[al * LD + bl];
-            epi(J * TILE + al, I * TILE + bl, make_float4(c[0], c[LD], c[2 * LD], c[3 * LD]));
-          }
-        }
-      }
-      __syncthreads();
+      if ((I + J) % nw == w) return true;
     }
+  };
+  Stage<TILE> sa, sb;
+  bool have_first = false;
+  int I = 0, J = -1;
+  for (bool more = next_tile(I, J); more;) {
+    {
+      Acc<TILE> acc;
+      tile_gemm<TILE, KA, KB, PREC>(sm, A, B, I * TILE, J * TILE, K, acc, sa, sb, have_first);
+      acc_to_lds<TILE>(acc, sm.ctile);
+    }
+    // the next tile's first K chunk is loaded while this tile is delivered
+    int In = I, Jn = J;
+    more = next_tile(In, Jn);
+    if (more) {
+      fetch<TILE, KA>(A, In * TILE, 0, sa);
+      fetch<TILE, KB>(B, Jn * TILE, 0, sb);
+    }
+    have_first = more;
+    __syncthreads();
+    const int mmax = min(TILE, M - I * TILE), nmax = min(TILE, N - J * TILE);
+    // four consecutive rows per call; 8 consecutive lanes take 8 row groups of
+    // one column and the next lanes the next columns, so with LD = 1 mod 32 the
+    // 32 lanes of an LDS read hit 32 distinct banks and each column's stores
+    // are 64-byte runs per plane
+    constexpr int TQ = TILE / 4;
+    for (int e = threadIdx.x; e < TQ * TILE; e += NT) {
+      const int rg = (e & 7) + 8 * (e / (8 * TILE)), nl = (e >> 3) % TILE;
+      const int ml = 4 * rg;
+      if (ml < mmax && nl < nmax) {
+        const float *c = &sm.ctile[nl * LD + ml];
+        epi(I * TILE + ml, J * TILE + nl, make_float4(c[0], c[1], c[2], c[3]));
+      }
+    }
+    if (sym && I != J) {
+      // transposed delivery: output (J*TILE + a, I*TILE + b) = tile element (b, a)
+      for (int e = threadIdx.x; e < TQ * TILE; e += NT) {
+        const int ag = (e & 7) + 8 * (e / (8 * TILE)), bl = (e >> 3) % TILE;
+        const int al = 4 * ag;
+        if (al < nmax && bl < mmax) {
+          const float *c = &sm.ctile[al * LD + bl];
+          epi(J * TILE + al, I * TILE + bl, make_float4(c[0], c[LD], c[2 * LD], c[3 * LD]));
+        }
+      }
+    }
+    __syncthreads();
+    I = In;
+    J = Jn;
+  }
 }
 
 // Square kp x kp product (see wg_gemm_mn).
