@@ -74,7 +74,44 @@ __device__ __forceinline__ void put_split(const SplitM &s, int64_t i, float v) {
   s.hi[i] = h;
   s.lo[i] = f2bf(v - bf2f(h));
 }
-__device__ __forceinline__ float get_split(const SplitM &s, int64_t i) { return bf2f(s.hi[i]) + bf2f(s.lo[i]); }
+
+// Device-scope loads (cache policy sc1) of memory that other workgroups of the
+// same launch write between barriers (the members of a team): they skip the
+// CU's vector L1 and read the L2, so a barrier needs no L1 invalidate. They are
+// buffer loads: base must be the same in every lane (the buffer descriptor
+// lives in scalar registers) and byte offsets stay below 2^31. COH = false
+// gives the plain load.
+constexpr int CPOL_SC1 = 16;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t dev_rsrc(const void *base) {
+  // word 3 = 0x00020000: the raw (unformatted) buffer word of gfx9-family descriptors
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, 0x7fffffff, 0x00020000);
+}
+template <bool COH>
+__device__ __forceinline__ uint4 ld_b128(const void *base, int64_t ofs) {
+  if constexpr (COH)
+    return __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(dev_rsrc(base), (int)ofs, 0, CPOL_SC1));
+  else
+    return *reinterpret_cast<const uint4 *>(static_cast<const char *>(base) + ofs);
+}
+template <bool COH>
+__device__ __forceinline__ uint2 ld_b64(const void *base, int64_t ofs) {
+  if constexpr (COH)
+    return __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(dev_rsrc(base), (int)ofs, 0, CPOL_SC1));
+  else
+    return *reinterpret_cast<const uint2 *>(static_cast<const char *>(base) + ofs);
+}
+template <bool COH>
+__device__ __forceinline__ uint16_t ld_b16(const void *base, int64_t ofs) {
+  if constexpr (COH)
+    return __builtin_amdgcn_raw_buffer_load_b16(dev_rsrc(base), (int)ofs, 0, CPOL_SC1);
+  else
+    return *reinterpret_cast<const uint16_t *>(static_cast<const char *>(base) + ofs);
+}
+
+template <bool COH = false>
+__device__ __forceinline__ float get_split(const SplitM &s, int64_t i) {
+  return bf2f(ld_b16<COH>(s.hi, 2 * i)) + bf2f(ld_b16<COH>(s.lo, 2 * i));
+}
 // four consecutive elements i..i+3 (i a multiple of 4): one 8-byte access per plane
 __device__ __forceinline__ void put_split4(const SplitM &s, int64_t i, float4 v) {
   const float z[4] = {v.x, v.y, v.z, v.w};
@@ -89,8 +126,9 @@ __device__ __forceinline__ void put_split4(const SplitM &s, int64_t i, float4 v)
   *reinterpret_cast<uint2 *>(s.lo + i) =
       make_uint2((uint32_t)l[0] | ((uint32_t)l[1] << 16), (uint32_t)l[2] | ((uint32_t)l[3] << 16));
 }
+template <bool COH = false>
 __device__ __forceinline__ float4 get_split4(const SplitM &s, int64_t i) {
-  const uint2 h = *reinterpret_cast<const uint2 *>(s.hi + i), l = *reinterpret_cast<const uint2 *>(s.lo + i);
+  const uint2 h = ld_b64<COH>(s.hi, 2 * i), l = ld_b64<COH>(s.lo, 2 * i);
   return make_float4(bf2f((uint16_t)h.x) + bf2f((uint16_t)l.x), bf2f((uint16_t)(h.x >> 16)) + bf2f((uint16_t)(l.x >> 16)),
                      bf2f((uint16_t)h.y) + bf2f((uint16_t)l.y), bf2f((uint16_t)(h.y >> 16)) + bf2f((uint16_t)(l.y >> 16)));
 }
@@ -158,8 +196,11 @@ struct Stage {
   uint4 r[N];
 };
 
-template <int TILE, int KIND>
+// COH: device-scope loads (split operands only), for operands that other
+// workgroups of the launch wrote.
+template <int TILE, int KIND, bool COH = false>
 __device__ __forceinline__ void fetch(const Opnd &o, int m0, int kb, Stage<TILE> &st) {
+  static_assert(!COH || KIND == OP_SPLIT_K || KIND == OP_SPLIT_M, "device-scope loads for split operands only");
   constexpr int N = Stage<TILE>::N;
   const int tid = threadIdx.x;
   if constexpr (KIND == OP_F32_GATHER) {
@@ -191,8 +232,8 @@ __device__ __forceinline__ void fetch(const Opnd &o, int m0, int kb, Stage<TILE>
       uint4 h = make_uint4(0, 0, 0, 0), l = h;
       if (gm < o.mlimit) {
         const int64_t ofs = (int64_t)gm * o.ld + kb + kg * 8;
-        h = *(const uint4 *)(o.hi + ofs);
-        l = *(const uint4 *)(o.lo + ofs);
+        h = ld_b128<COH>(o.hi, 2 * ofs);
+        l = ld_b128<COH>(o.lo, 2 * ofs);
       }
       st.r[it] = h;
       st.r[N / 2 + it] = l;
@@ -206,8 +247,8 @@ __device__ __forceinline__ void fetch(const Opnd &o, int m0, int kb, Stage<TILE>
       uint4 h = make_uint4(0, 0, 0, 0), l = h;
       if (gm < o.mlimit) {  // mlimit is a multiple of 8 here
         const int64_t ofs = gm + (int64_t)(kb + kk) * o.ld;
-        h = *(const uint4 *)(o.hi + ofs);
-        l = *(const uint4 *)(o.lo + ofs);
+        h = ld_b128<COH>(o.hi, 2 * ofs);
+        l = ld_b128<COH>(o.lo, 2 * ofs);
       }
       st.r[it] = h;
       st.r[N / 2 + it] = l;
@@ -395,7 +436,7 @@ __device__ __forceinline__ void mma_chunk_f32(GemmSmem<TILE> &sm, int buf, Acc<T
 // One output tile over K. sa, sb: the stage registers; with have_first the
 // caller has already fetched chunk 0 of this tile into them (during the
 // previous tile's epilogue), so its load latency is hidden.
-template <int TILE, int KA, int KB, int PREC = PREC_BF16X3>
+template <int TILE, int KA, int KB, int PREC = PREC_BF16X3, bool COH = false>
 __device__ __forceinline__ void tile_gemm(GemmSmem<TILE> &sm, const Opnd &A, const Opnd &B, int tm0, int tn0,
                                           int K, Acc<TILE> &acc, Stage<TILE> &sa, Stage<TILE> &sb,
                                           bool have_first) {
@@ -417,8 +458,8 @@ __device__ __forceinline__ void tile_gemm(GemmSmem<TILE> &sm, const Opnd &A, con
   static_assert(KB != OP_SPLIT_M, "only the A operand may be read with m contiguous");
   const int nc = K / KC;
   if (!have_first) {
-    fetch<TILE, KA>(A, tm0, 0, sa);
-    fetch<TILE, KB>(B, tn0, 0, sb);
+    fetch<TILE, KA, COH>(A, tm0, 0, sa);
+    fetch<TILE, KB, COH>(B, tn0, 0, sb);
   }
   if constexpr (KA == OP_SPLIT_M)
     stash_m<TILE>(sa, sm.th[0], sm.tl[0]);
@@ -430,8 +471,8 @@ __device__ __forceinline__ void tile_gemm(GemmSmem<TILE> &sm, const Opnd &A, con
     const int cur = c & 1;
     const bool more = c + 1 < nc;
     if (more) {
-      fetch<TILE, KA>(A, tm0, (c + 1) * KC, sa);
-      fetch<TILE, KB>(B, tn0, (c + 1) * KC, sb);
+      fetch<TILE, KA, COH>(A, tm0, (c + 1) * KC, sa);
+      fetch<TILE, KB, COH>(B, tn0, (c + 1) * KC, sb);
     }
     if constexpr (PREC == PREC_F32)
       mma_chunk_f32<TILE>(sm, cur, acc, bm, bn);
@@ -478,7 +519,7 @@ __device__ __forceinline__ void acc_to_lds(const Acc<TILE> &acc, float *ctile) {
 // workgroup then has about one tile per row and per column, so the cheap edge
 // tiles (kp not a multiple of TILE) spread evenly, and at any time the
 // workgroups work on tiles of the same row and read its row panel together.
-template <int TILE, int KA, int KB, int PREC = PREC_BF16X3, class Epi>
+template <int TILE, int KA, int KB, int PREC = PREC_BF16X3, bool COH = false, class Epi>
 __device__ void wg_gemm_mn(GemmSmem<TILE> &sm, const Opnd &A, const Opnd &B, int M, int N, int K, bool sym,
                            Epi epi, int nw = 1, int w = 0) {
   const int ntm = (M + TILE - 1) / TILE, ntn = (N + TILE - 1) / TILE;
@@ -499,15 +540,15 @@ __device__ void wg_gemm_mn(GemmSmem<TILE> &sm, const Opnd &A, const Opnd &B, int
   for (bool more = next_tile(I, J); more;) {
     {
       Acc<TILE> acc;
-      tile_gemm<TILE, KA, KB, PREC>(sm, A, B, I * TILE, J * TILE, K, acc, sa, sb, have_first);
+      tile_gemm<TILE, KA, KB, PREC, COH>(sm, A, B, I * TILE, J * TILE, K, acc, sa, sb, have_first);
       acc_to_lds<TILE>(acc, sm.ctile);
     }
     // the next tile's first K chunk is loaded while this tile is delivered
     int In = I, Jn = J;
     more = next_tile(In, Jn);
     if (more) {
-      fetch<TILE, KA>(A, In * TILE, 0, sa);
-      fetch<TILE, KB>(B, Jn * TILE, 0, sb);
+      fetch<TILE, KA, COH>(A, In * TILE, 0, sa);
+      fetch<TILE, KB, COH>(B, Jn * TILE, 0, sb);
     }
     have_first = more;
     __syncthreads();
@@ -543,10 +584,10 @@ __device__ void wg_gemm_mn(GemmSmem<TILE> &sm, const Opnd &A, const Opnd &B, int
 }
 
 // Square kp x kp product (see wg_gemm_mn).
-template <int TILE, int KA, int KB, int PREC = PREC_BF16X3, class Epi>
+template <int TILE, int KA, int KB, int PREC = PREC_BF16X3, bool COH = false, class Epi>
 __device__ __forceinline__ void wg_gemm(GemmSmem<TILE> &sm, const Opnd &A, const Opnd &B, int kp, int K, bool sym,
                                         Epi epi, int nw = 1, int w = 0) {
-  wg_gemm_mn<TILE, KA, KB, PREC>(sm, A, B, kp, kp, K, sym, epi, nw, w);
+  wg_gemm_mn<TILE, KA, KB, PREC, COH>(sm, A, B, kp, kp, K, sym, epi, nw, w);
 }
 
 #endif  // K_GEMM_H

@@ -391,11 +391,12 @@ __global__ __launch_bounds__(NT, HAK_POLAR_OCC128) void k_polar_grp(HakLocalArgs
     const bool want_tr = a.sigma_sum != nullptr;
     double tr = 0.0;
     {
+      // Qn holds what other members wrote: device-scope loads; A (the old Q) is
+      // read and replaced by the same thread
       float4 *q4 = reinterpret_cast<float4 *>(Q);
-      const float4 *n4 = reinterpret_cast<const float4 *>(Qn);
       const int64_t nq = (int64_t)kp * kp / 4;
       for (int64_t e = team.e0(); e < nq; e += team.es()) {
-        const float4 o = q4[e], v = n4[e];
+        const float4 o = q4[e], v = __builtin_bit_cast(float4, ld_b128<true>(Qn, 16 * e));
         if (want_tr) tr += (double)v.x * o.x + (double)v.y * o.y + (double)v.z * o.z + (double)v.w * o.w;
         q4[e] = v;
       }

@@ -127,9 +127,11 @@ struct MatS<PREC_BF16X3> {
   SplitM s;
   __device__ MatS(float *buf, int kp) : s(as_split(buf, kp)) {}
   __device__ __forceinline__ void put(int64_t i, float v) const { put_split(s, i, v); }
-  __device__ __forceinline__ float get(int64_t i) const { return get_split(s, i); }
+  template <bool COH = false>
+  __device__ __forceinline__ float get(int64_t i) const { return get_split<COH>(s, i); }
   __device__ __forceinline__ void put4(int64_t i, float4 v) const { put_split4(s, i, v); }
-  __device__ __forceinline__ float4 get4(int64_t i) const { return get_split4(s, i); }
+  template <bool COH = false>
+  __device__ __forceinline__ float4 get4(int64_t i) const { return get_split4<COH>(s, i); }
   __device__ __forceinline__ Opnd opnd(int kind, int ld, int mlimit) const { return opnd_split(s, kind, ld, mlimit); }
   __device__ __forceinline__ void load_scaled(const float *A, int kp, float f, int64_t e0 = -1,
                                               int64_t es = NT) const {
@@ -142,9 +144,17 @@ struct MatS<PREC_F32> {
   float *f;
   __device__ MatS(float *buf, int) : f(buf) {}
   __device__ __forceinline__ void put(int64_t i, float v) const { f[i] = v; }
-  __device__ __forceinline__ float get(int64_t i) const { return f[i]; }
+  template <bool COH = false>
+  __device__ __forceinline__ float get(int64_t i) const {
+    static_assert(!COH, "FP32 storage is not shared between workgroups");
+    return f[i];
+  }
   __device__ __forceinline__ void put4(int64_t i, float4 v) const { *reinterpret_cast<float4 *>(f + i) = v; }
-  __device__ __forceinline__ float4 get4(int64_t i) const { return *reinterpret_cast<const float4 *>(f + i); }
+  template <bool COH = false>
+  __device__ __forceinline__ float4 get4(int64_t i) const {
+    static_assert(!COH, "FP32 storage is not shared between workgroups");
+    return *reinterpret_cast<const float4 *>(f + i);
+  }
   __device__ __forceinline__ Opnd opnd(int kind, int ld, int mlimit) const { return opnd_f32(f, kind, ld, mlimit); }
   __device__ __forceinline__ void load_scaled(const float *A, int kp, float v, int64_t e0 = -1,
                                               int64_t es = NT) const {
@@ -169,6 +179,7 @@ struct PolarOut {
 // order, so the result does not depend on timing.
 struct SoloTeam {
   static constexpr int nw = 1;
+  static constexpr bool coh = false;  // plain loads: nothing is shared with other workgroups
   __device__ int member_id() const { return 0; }
   __device__ bool leader() const { return true; }
   __device__ int64_t e0() const { return threadIdx.x; }
@@ -197,6 +208,9 @@ static_assert(sizeof(GrpState) == HAK_GRP_STATE_BYTES, "team state size");
 
 struct GroupTeam {
   static constexpr int nw = HAK_GRP;
+  // every load of what another member wrote (operands of the products, G in
+  // the elementwise passes) is a device-scope load: see sync()
+  static constexpr bool coh = true;
   GrpState *st;
   int member;
   uint32_t gen;     // barriers passed by this workgroup
@@ -211,10 +225,13 @@ struct GroupTeam {
   // until its global stores have reached L2 (the vector L1 is write-through);
   // the L2 write-back of an agent-scope release fence is left out on purpose,
   // since the members read through the same L2. The arrival counter is an L2
-  // atomic polled past L1. Acquire: an agent-scope acquire fence (buffer_inv
-  // sc1), which drops the CU's L1 lines so that later loads see the other
-  // members' writes; the workgroup-scope form (sc0) does not do that here, and
-  // stale L1 lines then give wrong results once nothing else evicts them.
+  // atomic polled past L1. Acquire: none in the cache. The members read each
+  // other's data only with device-scope loads (coh, sc1), which skip the L1 and
+  // so cannot see a stale line; an L1 invalidate here (an agent-scope acquire
+  // fence) costs about 5% of the iteration. The workgroup-scope invalidate
+  // (buffer_inv sc0) is not enough: it left stale L1 lines behind. Any new load
+  // of memory another member writes must be a device-scope load too. The asm
+  // statements keep the compiler from moving memory accesses across the wait.
   // Returns false (in every thread) when the wait gives up.
   __device__ bool sync() {
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
@@ -235,7 +252,7 @@ struct GroupTeam {
       *ok_sh = ok;
     }
     __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    asm volatile("" ::: "memory");
     const bool ok = *ok_sh != 0;
     __syncthreads();
     return ok;
@@ -299,7 +316,7 @@ __device__ __forceinline__ PolarOut polar_solve(GemmSmem<TILE> &sm, const HakLoc
       double r2 = 0.0;
       {
         const Opnd ox = X.opnd(M::KK, kp, kp);
-        wg_gemm<TILE, M::KK, M::KK, PREC>(
+        wg_gemm<TILE, M::KK, M::KK, PREC, Team::coh>(
             sm, ox, ox, kp, kp, true,
             [&](int m, int n, float4 v) {
               G.put4(m + n * kp, v);
@@ -354,13 +371,13 @@ __device__ __forceinline__ PolarOut polar_solve(GemmSmem<TILE> &sm, const HakLoc
       // P = ca I + cb G + cc G^2 (G^2 only when cc != 0), zero outside the k x k block
       if (cc != 0.f) {
         const Opnd og = G.opnd(M::KK, kp, kp);
-        wg_gemm<TILE, M::KK, M::KK, PREC>(
+        wg_gemm<TILE, M::KK, M::KK, PREC, Team::coh>(
             sm, og, og, kp, kp, true,
             [&](int m, int n, float4 v) {
               const int i = m + n * kp;
               float4 pv = make_float4(0.f, 0.f, 0.f, 0.f);
               if (n < k && m < k) {
-                const float4 g = G.get4(i);
+                const float4 g = G.template get4<Team::coh>(i);
                 pv.x = (m == n ? ca : 0.f) + cb * g.x + cc * v.x;
                 pv.y = (m + 1 < k) ? (m + 1 == n ? ca : 0.f) + cb * g.y + cc * v.y : 0.f;
                 pv.z = (m + 2 < k) ? (m + 2 == n ? ca : 0.f) + cb * g.z + cc * v.z : 0.f;
@@ -373,7 +390,7 @@ __device__ __forceinline__ PolarOut polar_solve(GemmSmem<TILE> &sm, const HakLoc
         for (int64_t e = team.e0(); e < (int64_t)kp * kp; e += team.es()) {
           const int m = (int)(e % kp), n = (int)(e / kp);
           float pv = 0.f;
-          if (m < k && n < k) pv = (m == n ? ca : 0.f) + cb * G.get(e);
+          if (m < k && n < k) pv = (m == n ? ca : 0.f) + cb * G.template get<Team::coh>(e);
           P.put(e, pv);
         }
       }
@@ -383,10 +400,10 @@ __device__ __forceinline__ PolarOut polar_solve(GemmSmem<TILE> &sm, const HakLoc
         const Opnd ox = X.opnd(M::KM, kp, kp);
         const Opnd op = P.opnd(M::KK, kp, kp);
         if (finish) {
-          wg_gemm<TILE, M::KM, M::KK, PREC>(
+          wg_gemm<TILE, M::KM, M::KK, PREC, Team::coh>(
               sm, ox, op, kp, kp, false, [&](int m, int n, float4 v) { fin(m, n, v); }, nw, tw);
         } else {
-          wg_gemm<TILE, M::KM, M::KK, PREC>(
+          wg_gemm<TILE, M::KM, M::KK, PREC, Team::coh>(
               sm, ox, op, kp, kp, false, [&](int m, int n, float4 v) { Xn.put4(m + n * kp, v); }, nw, tw);
         }
       }
