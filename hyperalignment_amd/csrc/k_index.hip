@@ -473,12 +473,21 @@ __global__ __launch_bounds__(SC_T) void k_scatter(
   __shared__ double m_sc[SC_OCC];
   const int tid = threadIdx.x;
   for (int32_t c = blockIdx.x; c < ncols; c += gridDim.x) {
-    int64_t pb = ptr[c], pe = ptr[c + 1];
     // occurrences outside [s_begin, s_end) are skipped: occ lists are sorted
-    // by flat position, i.e. by searchlight
-    while (pb < pe && sl_of[occ[pb]] < s_begin) ++pb;
-    int64_t pe2 = pb;
-    while (pe2 < pe && sl_of[occ[pe2]] < s_end) ++pe2;
+    // by flat position, i.e. by searchlight, so two binary searches find the
+    // range (a linear walk costs two dependent loads per occurrence)
+    auto first_at = [&](int64_t lo, int64_t hi, int32_t sv) {
+      while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (sl_of[occ[mid]] < sv)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      return lo;
+    };
+    const int64_t pb = first_at(ptr[c], ptr[c + 1], s_begin);
+    const int64_t pe2 = first_at(pb, ptr[c + 1], s_end);
     if (pb == pe2) continue;
     const int64_t cb = indptr[c];
     const int32_t L = (int32_t)(indptr[c + 1] - cb);
