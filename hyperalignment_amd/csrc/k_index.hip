@@ -454,9 +454,14 @@ __device__ __forceinline__ int32_t find_row(const int32_t *rows, int32_t L, int3
 // wait overlaps the next loads instead of a chain of dependent ones.
 #define SC_OCC 256  // occurrences staged per batch
 #define SC_V 2      // values per thread per searchlight held in registers (k <= SC_V * SC_T pipelined)
+#define SC_P 8      // searchlights whose values are loaded together
 
-struct ScVals {
-  double v[SC_V];
+// One thread's raw loads for one searchlight: Q values, weights, rows (-1: none).
+// Only loads, no arithmetic: the arithmetic waits for its operands, and done
+// here it would make each searchlight wait for its own loads.
+struct ScRaw {
+  float q[SC_V];
+  double w[SC_V];
   int32_t r[SC_V];
 };
 
@@ -520,52 +525,56 @@ __global__ __launch_bounds__(SC_T) void k_scatter(
         m_sc[e] = scale ? scale[s] : 1.0;
       }
       __syncthreads();
-      auto load = [&](int32_t e, ScVals &x) {
+      // indices clamped to the searchlight (ks >= 1: it holds column c's vertex),
+      // so every load is valid and none sits behind a branch
+      auto load = [&](int32_t e, ScRaw &x) {
         const int32_t o = m_o[e], ks = m_ks[e];
         const float *qc = q + m_q[e];
-        const double sc = m_sc[e];
 #pragma unroll
         for (int u = 0; u < SC_V; ++u) {
           const int32_t i = tid + u * SC_T;
-          x.r[u] = -1;
-          x.v[u] = 0.0;
-          if (i < ks) {
-            double t = (double)qc[i];
-            if (scale) t = t * sc;
-            x.v[u] = w ? t * w[o + i] : t;
-            x.r[u] = slx_idx[o + i];
-          }
+          const int32_t ic = i < ks ? i : ks - 1;
+          x.q[u] = qc[ic];
+          x.w[u] = w ? w[o + ic] : 1.0;
+          const int32_t r = slx_idx[o + ic];
+          x.r[u] = i < ks ? r : -1;
         }
       };
-      ScVals cur;
-      load(0, cur);
-      for (int32_t e = 0; e < nb; ++e) {
-        ScVals nxt;
-        if (e + 1 < nb) load(e + 1, nxt);
+      // contribution Q[i, j] * scale * w_i, in this order (as the oracle forms it)
+      auto contrib = [&](float qv, double sc, double wv) {
+        double t = (double)qv;
+        if (scale) t = t * sc;
+        return w ? t * wv : t;
+      };
+      for (int32_t e0 = 0; e0 < nb; e0 += SC_P) {
+        // SC_P searchlights' loads in flight together, then their sums in order
+        ScRaw x[SC_P];
 #pragma unroll
-        for (int u = 0; u < SC_V; ++u) {
-          if (cur.r[u] >= 0) {
-            const int32_t pos = pos_of(cur.r[u]);
-            if (pos >= 0) acc[pos] += cur.v[u];
-          }
-        }
-        // searchlights larger than SC_V * SC_T: the rest, unpipelined
-        const int32_t ks = m_ks[e];
-        if (ks > SC_V * SC_T) {
-          const int32_t o = m_o[e];
-          const float *qc = q + m_q[e];
+        for (int p = 0; p < SC_P; ++p) load(e0 + p < nb ? e0 + p : nb - 1, x[p]);
+#pragma unroll
+        for (int p = 0; p < SC_P; ++p) {
+          const int32_t e = e0 + p;
+          if (e >= nb) break;  // uniform
           const double sc = m_sc[e];
-          for (int32_t i = tid + SC_V * SC_T; i < ks; i += SC_T) {
-            const int32_t pos = pos_of(slx_idx[o + i]);
-            if (pos >= 0) {
-              double t = (double)qc[i];
-              if (scale) t = t * sc;
-              acc[pos] += w ? t * w[o + i] : t;
+#pragma unroll
+          for (int u = 0; u < SC_V; ++u) {
+            if (x[p].r[u] >= 0) {
+              const int32_t pos = pos_of(x[p].r[u]);
+              if (pos >= 0) acc[pos] += contrib(x[p].q[u], sc, x[p].w[u]);
             }
           }
+          // searchlights larger than SC_V * SC_T: the rest, unpipelined
+          const int32_t ks = m_ks[e];
+          if (ks > SC_V * SC_T) {
+            const int32_t o = m_o[e];
+            const float *qc = q + m_q[e];
+            for (int32_t i = tid + SC_V * SC_T; i < ks; i += SC_T) {
+              const int32_t pos = pos_of(slx_idx[o + i]);
+              if (pos >= 0) acc[pos] += contrib(qc[i], sc, w ? w[o + i] : 1.0);
+            }
+          }
+          __syncthreads();
         }
-        __syncthreads();
-        cur = nxt;
       }
     }
     if (in_lds) {
