@@ -210,6 +210,7 @@ struct HaHipPlan {
 	bool weights_none;
 	/* CSC */
 	int32_t ncols;
+	int32_t max_col;   /* longest CSC column (entries) */
 	int64_t nnz;
 	int64_t *indptr64; int32_t *indptr; int32_t *indices; double *data;
 	/* local transforms */
@@ -423,6 +424,19 @@ int ha_plan_create_ex(int32_t nt, const HaPlanSetup *su, const int32_t *slx_idx,
 		}
 		p->nnz = nnz;
 		p->ncols = p->nv_x;
+		{
+			int32_t *hc = (int32_t *)malloc((size_t)p->nv_x * sizeof(int32_t));
+			if (!hc) { hipFree(slmin); hipFree(slmax); hipFree(colcnt); rc = kHaErrorAlloc; goto fail; }
+			if (hipMemcpy(hc, colcnt, (size_t)p->nv_x * sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess) {
+				free(hc); hipFree(slmin); hipFree(slmax); hipFree(colcnt);
+				rc = kHaErrorInternal;
+				goto fail;
+			}
+			p->max_col = 0;
+			for (int32_t c = 0; c < p->nv_x; ++c)
+				if (hc[c] > p->max_col) p->max_col = hc[c];
+			free(hc);
+		}
 		ACK(p->indptr, (size_t)(p->nv_x + 1) * sizeof(int32_t));
 		ACK(p->indices, (size_t)nnz * sizeof(int32_t));
 		ACK(p->data, (size_t)nnz * sizeof(double));
@@ -510,7 +524,11 @@ int ha_plan_set_pattern(HaHipPlan *p, int32_t ncols, const int32_t *indptr, cons
 	p->nnz = nnz;
 	int64_t *h64 = (int64_t *)malloc((size_t)(ncols + 1) * sizeof(int64_t));
 	if (!h64) return kHaErrorAlloc;
-	for (int32_t c = 0; c <= ncols; ++c) h64[c] = indptr[c];
+	p->max_col = 0;
+	for (int32_t c = 0; c <= ncols; ++c) {
+		h64[c] = indptr[c];
+		if (c < ncols && indptr[c + 1] - indptr[c] > p->max_col) p->max_col = indptr[c + 1] - indptr[c];
+	}
 	ACK(p->indptr, (size_t)(ncols + 1) * sizeof(int32_t));
 	ACK(p->indptr64, (size_t)(ncols + 1) * sizeof(int64_t));
 	ACK(p->indices, (size_t)nnz * sizeof(int32_t));
@@ -903,7 +921,7 @@ static int enqueue_pass(HaHipPlan *p, HakLocalArgs a, int method, bool isReflect
 	if (timed) HCK(hipEventRecord(p->ev[4], st));
 	KCK(hak_scatter(p->slx, p->off, p->sl_of, p->occ_ptr, p->occ, p->ncols, p->indptr64, p->indices, p->data,
 	                p->q, p->qoff, p->qld, isScaling ? p->scale : NULL, p->weights_none ? NULL : p->w, 0,
-	                p->count, st));
+	                p->count, p->max_col, st));
 	if (timed) HCK(hipEventRecord(p->ev[5], st));
 fail:
 	return rc;
@@ -1408,7 +1426,7 @@ int ha_plan_ridge(HaHipPlan *p, double alpha, bool zero_data, double *betas64) {
 	HCK(hipEventRecord(p->ev[2], st));
 	if (!betas64)
 		KCK(hak_scatter(p->slx, p->off, p->sl_of, p->occ_ptr, p->occ, p->ncols, p->indptr64, p->indices, p->data,
-		                p->q, p->qoff, p->qld, NULL, p->weights_none ? NULL : p->w, 0, p->count, st));
+		                p->q, p->qoff, p->qld, NULL, p->weights_none ? NULL : p->w, 0, p->count, p->max_col, st));
 	HCK(hipEventRecord(p->ev[5], st));
 	HCK(hipStreamSynchronize(st));
 	{

@@ -57,13 +57,14 @@ int hak_pattern_fill(const int32_t *sl_idx, const int32_t *sl_off, const int32_t
  * local position j: data[pos(slx_s[i], c)] += (double)Q_s[i,j] * scale_s * w_s[i].
  * Q_s is k_s x k_s at q + qoff[s], column-major with leading dim qld[s].
  * scale may be NULL (1.0), w may be NULL (1.0). Pairs outside the pattern are
- * skipped. occ lists come from the Y-side searchlight lists. */
+ * skipped. occ lists come from the Y-side searchlight lists. max_col: the
+ * longest column (indptr[c + 1] - indptr[c]); it sizes the kernel's LDS. */
 int hak_scatter(const int32_t *slx_idx, const int32_t *sl_off, const int32_t *sl_of,
                 const int64_t *occ_ptr, const int32_t *occ, int32_t ncols,
                 const int64_t *indptr, const int32_t *indices, double *data,
                 const float *q, const int64_t *qoff, const int32_t *qld,
                 const double *scale, const double *w,
-                int32_t s_begin, int32_t s_end, void *stream);
+                int32_t s_begin, int32_t s_end, int32_t max_col, void *stream);
 
 /* Single local transform scatter (ha_sparse_scatter_add on the GPU). */
 int hak_scatter_one(const int32_t *indptr, const int32_t *indices, double *data,

@@ -414,21 +414,24 @@ extern "C" int hak_pattern_fill(const int32_t *sl_idx, const int32_t *sl_off, co
 // ---------------------------------------------------------------------------
 
 #define SC_T 256
-#define SC_LDS 3072   // column entries held in LDS (FP64 accumulators, 24 KB)
-#define SC_H 4096     // LDS hash of the column's rows: ((row + 1) << 12) | position, 0 = empty
-#define SC_HROW (1 << 19)  // rows below this fit the packing (positions < 4096 = SC_H)
+#define SC_LDS 3072   // most column entries held in LDS (FP64 accumulators)
+#define SC_HROW (1 << 19)  // rows below this fit the hash packing ((row + 1) << 12 | position)
 
-__device__ __forceinline__ uint32_t sc_hash(int32_t r) { return ((uint32_t)r * 2654435761u) >> 20; }  // 12 bits
+// LDS hash of a column's rows: 2^hbits words of ((row + 1) << 12) | position,
+// 0 = empty (positions < 4096 > SC_LDS), linear probing
+__device__ __forceinline__ uint32_t sc_hash(int32_t r, int hbits) {
+  return ((uint32_t)r * 2654435761u) >> (32 - hbits);
+}
 
-// position of row r in the column, from the LDS hash (r is in the column)
-__device__ __forceinline__ int32_t hash_row(const uint32_t *ht, int32_t r) {
-  const uint32_t key = (uint32_t)(r + 1);
-  uint32_t h = sc_hash(r);
-  for (int probe = 0; probe < SC_H; ++probe) {
+// position of row r in the column, from the LDS hash (-1: r is not in the column)
+__device__ __forceinline__ int32_t hash_row(const uint32_t *ht, int hbits, int32_t r) {
+  const uint32_t key = (uint32_t)(r + 1), mask = (1u << hbits) - 1u;
+  uint32_t h = sc_hash(r, hbits);
+  for (uint32_t probe = 0; probe <= mask; ++probe) {
     const uint32_t e = ht[h];
     if (e == 0u) return -1;
     if ((e >> 12) == key) return (int32_t)(e & 0xFFFu);
-    h = (h + 1) & (SC_H - 1);
+    h = (h + 1) & mask;
   }
   return -1;
 }
@@ -444,15 +447,16 @@ __device__ __forceinline__ int32_t find_row(const int32_t *rows, int32_t L, int3
 
 // Per column c (one workgroup), the contributions of every searchlight holding
 // c, in increasing searchlight order with a barrier between searchlights, so
-// each element of data is summed in one fixed order. A column of up to SC_LDS
+// each element of data is summed in one fixed order. A column of up to cap
 // rows keeps its FP64 accumulators in LDS and finds a row's position through an
-// LDS hash (about one probe; a row outside the pattern ends at an empty slot
-// and is skipped); longer columns work in global memory with a binary search. The occurrences' metadata
-// (searchlight, position, size, Q offset, scale) is staged in LDS a batch at a
-// time by all threads at once, and each searchlight's Q values, weights and
-// vertex ids are loaded one searchlight ahead into registers, so the barrier
-// wait overlaps the next loads instead of a chain of dependent ones.
-#define SC_OCC 256  // occurrences staged per batch
+// LDS hash of 2^hbits words (about one probe; a row outside the pattern ends at
+// an empty slot and is skipped); longer columns work in global memory with a
+// binary search. cap is the plan's longest column (at most SC_LDS), so the
+// dynamic LDS, and with it the workgroups per CU, fits the plan. The
+// occurrences' metadata (searchlight, position, size, Q offset, scale) is
+// staged in LDS a batch at a time by all threads at once, and the Q values,
+// weights and vertex ids of SC_P searchlights are loaded together.
+#define SC_OCC 64   // occurrences staged per batch
 #define SC_V 2      // values per thread per searchlight held in registers (k <= SC_V * SC_T pipelined)
 #define SC_P 8      // searchlights whose values are loaded together
 
@@ -470,12 +474,17 @@ __global__ __launch_bounds__(SC_T) void k_scatter(
     const int64_t *__restrict__ ptr, const int32_t *__restrict__ occ, int32_t ncols,
     const int64_t *__restrict__ indptr, const int32_t *__restrict__ indices, double *__restrict__ data,
     const float *__restrict__ q, const int64_t *__restrict__ qoff, const int32_t *__restrict__ qld,
-    const double *__restrict__ scale, const double *__restrict__ w, int32_t s_begin, int32_t s_end) {
-  __shared__ uint32_t ht[SC_H];
-  __shared__ double acc_l[SC_LDS];
-  __shared__ int32_t m_o[SC_OCC], m_ks[SC_OCC];
-  __shared__ int64_t m_q[SC_OCC];
-  __shared__ double m_sc[SC_OCC];
+    const double *__restrict__ scale, const double *__restrict__ w, int32_t s_begin, int32_t s_end,
+    int32_t cap, int32_t hbits) {
+  // dynamic LDS: accumulators (cap), metadata (SC_OCC each), hash (2^hbits words)
+  extern __shared__ __align__(16) unsigned char sc_lds[];
+  double *acc_l = reinterpret_cast<double *>(sc_lds);
+  int64_t *m_q = reinterpret_cast<int64_t *>(acc_l + cap);
+  double *m_sc = reinterpret_cast<double *>(m_q + SC_OCC);
+  int32_t *m_o = reinterpret_cast<int32_t *>(m_sc + SC_OCC);
+  int32_t *m_ks = m_o + SC_OCC;
+  uint32_t *ht = reinterpret_cast<uint32_t *>(m_ks + SC_OCC);
+  const int32_t hsize = 1 << hbits;
   const int tid = threadIdx.x;
   for (int32_t c = blockIdx.x; c < ncols; c += gridDim.x) {
     // occurrences outside [s_begin, s_end) are skipped: occ lists are sorted
@@ -497,21 +506,21 @@ __global__ __launch_bounds__(SC_T) void k_scatter(
     const int64_t cb = indptr[c];
     const int32_t L = (int32_t)(indptr[c + 1] - cb);
     // rows sorted: the last is the largest
-    const bool in_lds = L > 0 && L <= SC_LDS && indices[cb + L - 1] < SC_HROW;
+    const bool in_lds = L > 0 && L <= cap && indices[cb + L - 1] < SC_HROW;
     const int32_t *rows = indices + cb;
     double *acc = in_lds ? acc_l : data + cb;
     if (in_lds) {
-      for (int32_t i = tid; i < SC_H; i += SC_T) ht[i] = 0u;
+      for (int32_t i = tid; i < hsize; i += SC_T) ht[i] = 0u;
       __syncthreads();
       for (int32_t i = tid; i < L; i += SC_T) {
         const int32_t r = indices[cb + i];
         const uint32_t v = ((uint32_t)(r + 1) << 12) | (uint32_t)i;
-        uint32_t h = sc_hash(r);
-        while (atomicCAS(&ht[h], 0u, v) != 0u) h = (h + 1) & (SC_H - 1);
+        uint32_t h = sc_hash(r, hbits);
+        while (atomicCAS(&ht[h], 0u, v) != 0u) h = (h + 1) & (uint32_t)(hsize - 1);
         acc_l[i] = data[cb + i];
       }
     }
-    auto pos_of = [&](int32_t r) { return in_lds ? hash_row(ht, r) : find_row(rows, L, r); };
+    auto pos_of = [&](int32_t r) { return in_lds ? hash_row(ht, hbits, r) : find_row(rows, L, r); };
     for (int64_t b0 = pb; b0 < pe2; b0 += SC_OCC) {
       const int32_t nb = (int32_t)((pe2 - b0) < SC_OCC ? (pe2 - b0) : SC_OCC);
       __syncthreads();  // previous batch done with the metadata
@@ -589,11 +598,17 @@ extern "C" int hak_scatter(const int32_t *slx_idx, const int32_t *sl_off, const 
                            const int64_t *indptr, const int32_t *indices, double *data,
                            const float *q, const int64_t *qoff, const int32_t *qld,
                            const double *scale, const double *w,
-                           int32_t s_begin, int32_t s_end, void *stream) {
+                           int32_t s_begin, int32_t s_end, int32_t max_col, void *stream) {
   if (ncols <= 0) return 0;
-  hipLaunchKernelGGL(k_scatter, dim3(ncols < 65536 ? ncols : 65536), dim3(SC_T), 0, (hipStream_t)stream,
+  // LDS sized to the longest column (at most SC_LDS entries), hash at least twice that
+  const int32_t cap = max_col < 1 ? 1 : (max_col > SC_LDS ? SC_LDS : max_col);
+  int32_t hbits = 10;
+  while ((1 << hbits) < 2 * cap) ++hbits;
+  const size_t lds = (size_t)cap * sizeof(double) + (size_t)SC_OCC * (2 * sizeof(int64_t) + 2 * sizeof(int32_t)) +
+                     ((size_t)1 << hbits) * sizeof(uint32_t);
+  hipLaunchKernelGGL(k_scatter, dim3(ncols < 65536 ? ncols : 65536), dim3(SC_T), lds, (hipStream_t)stream,
                      slx_idx, sl_off, sl_of, occ_ptr, occ, ncols, indptr, indices, data, q, qoff, qld,
-                     scale, w, s_begin, s_end);
+                     scale, w, s_begin, s_end, cap, hbits);
   LAUNCH_RET();
 }
 
