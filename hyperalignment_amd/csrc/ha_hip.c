@@ -217,6 +217,7 @@ struct HaHipPlan {
 	int32_t *qld; int64_t *qoff; float *q; int64_t q_elems;
 	double *q64;
 	float *X, *Y; double *X64, *Y64; double *stage;
+	uint16_t *Xs, *Ys;   /* bf16 hi plane then lo plane of X and Y (ld x nv each) */
 	float *work; int64_t slot_stride; int32_t n_slots, n_slots_polar;
 	float *sl_par;
 	double *work64; int64_t stride64; int32_t n_slots64;
@@ -245,7 +246,7 @@ static void plan_free(HaHipPlan *p) {
 	if (!p) return;
 	void *ptrs[] = {p->slx, p->sly == p->slx ? NULL : p->sly, p->off, p->sl_of, p->dist, p->occ_ptr, p->occ,
 	                p->occx_ptr, p->occx, p->wsum, p->w, p->indptr64, p->indptr, p->indices, p->data,
-	                p->qld, p->qoff, p->q, p->q64, p->X, p->Y, p->X64, p->Y64, p->stage, p->work, p->work64,
+	                p->qld, p->qoff, p->q, p->q64, p->X, p->Y, p->X64, p->Y64, p->Xs, p->Ys, p->stage, p->work, p->work64,
 	                p->flag, p->steps, p->queues, p->order, p->sigma_sum, p->var_sum, p->scale,
 	                p->features, p->pred, p->trace, p->sig, p->d_which, p->sl_par, p->tile_order[0],
 	                p->tile_order[1], p->work_ridge, p->grp};
@@ -492,6 +493,8 @@ int ha_plan_create_ex(int32_t nt, const HaPlanSetup *su, const int32_t *slx_idx,
 	ACK(p->work64, (size_t)p->n_slots64 * 4 * p->stride64 * sizeof(double));
 	ACK(p->X, (size_t)p->ld * p->nv_x * sizeof(float));
 	ACK(p->Y, (size_t)p->ld * p->nv_y * sizeof(float));
+	ACK(p->Xs, (size_t)2 * p->ld * p->nv_x * sizeof(uint16_t));
+	ACK(p->Ys, (size_t)2 * p->ld * p->nv_y * sizeof(uint16_t));
 	ACK(p->X64, (size_t)nt * p->nv_x * sizeof(double));
 	ACK(p->Y64, (size_t)nt * p->nv_y * sizeof(double));
 	HCK(hipStreamSynchronize(st));
@@ -567,7 +570,8 @@ fail:
 	return rc;
 }
 
-static int upload_matrix(HaHipPlan *p, const double *M, int32_t nv, bool col_major, float *d32, double *d64) {
+static int upload_matrix(HaHipPlan *p, const double *M, int32_t nv, bool col_major, float *d32, double *d64,
+                         uint16_t *split) {
 	int rc = kHaSuccess;
 	hipStream_t st = g.stream;
 	size_t bytes = (size_t)p->nt * nv * sizeof(double);
@@ -576,7 +580,8 @@ static int upload_matrix(HaHipPlan *p, const double *M, int32_t nv, bool col_maj
 		ACK(p->stage, (size_t)p->nt * nvm * sizeof(double));
 	}
 	HCK(hipMemcpyAsync(p->stage, M, bytes, hipMemcpyHostToDevice, st));
-	KCK(hak_convert_data(p->stage, p->nt, nv, col_major ? 1 : 0, d32, p->ld, d64, st));
+	KCK(hak_convert_data(p->stage, p->nt, nv, col_major ? 1 : 0, d32, p->ld, d64, split,
+	                     split + (size_t)p->ld * nv, st));
 	HCK(hipStreamSynchronize(st));
 	return kHaSuccess;
 fail:
@@ -585,9 +590,9 @@ fail:
 
 int ha_plan_load_xy(HaHipPlan *p, const double *X, const double *Y, bool col_major) {
 	if (!p || !X || !Y) return kHaErrorArg;
-	int rc = upload_matrix(p, X, p->nv_x, col_major, p->X, p->X64);
+	int rc = upload_matrix(p, X, p->nv_x, col_major, p->X, p->X64, p->Xs);
 	if (rc) return rc;
-	rc = upload_matrix(p, Y, p->nv_y, col_major, p->Y, p->Y64);
+	rc = upload_matrix(p, Y, p->nv_y, col_major, p->Y, p->Y64, p->Ys);
 	if (rc) return rc;
 	p->loaded = true;
 	return kHaSuccess;
@@ -601,6 +606,8 @@ static HakLocalArgs local_args(HaHipPlan *p) {
 	HakLocalArgs a;
 	memset(&a, 0, sizeof(a));
 	a.X = p->X; a.Y = p->Y; a.X64 = p->X64; a.Y64 = p->Y64;
+	a.Xh = p->Xs; a.Xl = p->Xs + (size_t)p->ld * p->nv_x;
+	a.Yh = p->Ys; a.Yl = p->Ys + (size_t)p->ld * p->nv_y;
 	a.nt = p->nt; a.ld = p->ld;
 	a.slx_idx = p->slx; a.sly_idx = p->sly; a.sl_off = p->off;
 	a.order = p->order; a.n_order = p->count;
@@ -1162,7 +1169,7 @@ int ha_plan_template(HaHipPlan *p, const double *const *dms, int32_t ns, bool co
 	ACK(stage, (size_t)nt * nv * sizeof(double));
 	for (int32_t s = 0; s < ns; ++s) {
 		HCK(hipMemcpyAsync(stage, dms[s], (size_t)nt * nv * sizeof(double), hipMemcpyHostToDevice, st));
-		KCK(hak_convert_data(stage, nt, nv, col_major ? 1 : 0, data + (size_t)s * nv * ld, ld, NULL, st));
+		KCK(hak_convert_data(stage, nt, nv, col_major ? 1 : 0, data + (size_t)s * nv * ld, ld, NULL, NULL, NULL, st));
 	}
 	ACK(order, (size_t)count * sizeof(int32_t));
 	ACK(status, (size_t)count * sizeof(int32_t));
@@ -1255,11 +1262,11 @@ int ha_hip_plan_evaluate(HaHipPlan *p, const double *X, const double *Y, int32_t
 		const int32_t nmax = nvx > nc ? nvx : nc;
 		ACK(f32, (size_t)ld * nmax * sizeof(float));
 		HCK(hipMemcpyAsync(stage, X, (size_t)nt * nvx * sizeof(double), hipMemcpyHostToDevice, st));
-		rc = hak_convert_data(stage, nt, nvx, col_major ? 1 : 0, f32, ld, dx, st);
+		rc = hak_convert_data(stage, nt, nvx, col_major ? 1 : 0, f32, ld, dx, NULL, NULL, st);
 		if (!rc && r_out) {
 			ACK(dy, (size_t)nt * nc * sizeof(double));
 			HCK(hipMemcpyAsync(stage, Y, (size_t)nt * nc * sizeof(double), hipMemcpyHostToDevice, st));
-			rc = hak_convert_data(stage, nt, nc, col_major ? 1 : 0, f32, ld, dy, st);
+			rc = hak_convert_data(stage, nt, nc, col_major ? 1 : 0, f32, ld, dy, NULL, NULL, st);
 		}
 		HCK(hipStreamSynchronize(st));
 		hipFree(f32);

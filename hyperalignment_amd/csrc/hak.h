@@ -75,13 +75,14 @@ int hak_scatter_one(const int32_t *indptr, const int32_t *indices, double *data,
 /* FP64 nt x nv (col- or row-major) -> FP32 col-major with leading dim ld
  * (rows nt..ld-1 zero) and an FP64 col-major copy with leading dim nt. */
 int hak_convert_data(const double *src, int32_t nt, int32_t nv, int col_major,
-                     float *dst32, int32_t ld, double *dst64, void *stream);
+                     float *dst32, int32_t ld, double *dst64, uint16_t *hi, uint16_t *lo, void *stream);
 
 /* ---- local transforms (k_local.hip) ---- */
 #define HAK_NFEAT 12
 typedef struct {
 	/* inputs */
 	const float *X;  const float *Y;     /* FP32 col-major, leading dim ld (rows >= nt are zero) */
+	const uint16_t *Xh, *Xl, *Yh, *Yl;   /* the same as bf16 hi/lo planes: hi = bf16(x), lo = bf16(x - hi) */
 	const double *X64; const double *Y64;/* FP64 col-major, leading dim nt */
 	int32_t nt, ld;
 	const int32_t *slx_idx; const int32_t *sly_idx; const int32_t *sl_off;

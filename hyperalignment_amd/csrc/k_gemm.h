@@ -19,6 +19,8 @@
 //   OP_F32_GATHER  f32[col(m) * ld + k], col(m) = cols[m] (cols == nullptr:
 //                  col(m) = m): FP32 data, k contiguous
 //   OP_SPLIT_K     hi/lo[m * ld + k]: pre-split planes, k contiguous
+//   OP_SPLIT_GATHER hi/lo[cols[m] * ld + k]: pre-split planes, k contiguous,
+//                  rows gathered (the row offsets are read once per tile)
 //   OP_SPLIT_M     hi/lo[m + k * ld]: pre-split planes, m contiguous
 //   OP_F32_M       f32[m + k * ld]: FP32 data, m contiguous
 // Rows m >= mlimit read as zero; every k < K is in bounds.
@@ -48,7 +50,7 @@ typedef float f32x4_t __attribute__((ext_vector_type(4)));
 #define NT 256  // threads per workgroup (4 waves)
 #define KC 32   // K chunk staged in LDS
 
-enum { OP_F32_GATHER = 0, OP_SPLIT_K = 1, OP_SPLIT_M = 2, OP_F32_M = 3 };
+enum { OP_F32_GATHER = 0, OP_SPLIT_K = 1, OP_SPLIT_M = 2, OP_F32_M = 3, OP_SPLIT_GATHER = 4 };
 enum { PREC_BF16X3 = 0, PREC_F32 = 1 };
 
 // ---- bf16 helpers (round to nearest even) ----
@@ -108,6 +110,14 @@ __device__ __forceinline__ uint16_t ld_b16(const void *base, int64_t ofs) {
     return *reinterpret_cast<const uint16_t *>(static_cast<const char *>(base) + ofs);
 }
 
+// 16-byte load through a global-address-space pointer (a generic pointer
+// would give a flat load, which also waits on the LDS counter)
+__device__ __forceinline__ uint4 gld16(const uint16_t *p) {
+  typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+  typedef __attribute__((address_space(1))) const u32x4_t gu32x4_t;
+  return __builtin_bit_cast(uint4, *(gu32x4_t *)(p));
+}
+
 template <bool COH = false>
 __device__ __forceinline__ float get_split(const SplitM &s, int64_t i) {
   return bf2f(ld_b16<COH>(s.hi, 2 * i)) + bf2f(ld_b16<COH>(s.lo, 2 * i));
@@ -145,6 +155,13 @@ struct Opnd {
 __device__ __forceinline__ Opnd opnd_gather(const float *base, const int32_t *cols, int ld, int mlimit) {
   Opnd o;
   o.kind = OP_F32_GATHER; o.f32 = base; o.cols = cols; o.hi = nullptr; o.lo = nullptr; o.ld = ld; o.mlimit = mlimit;
+  return o;
+}
+// pre-split planes (split_bf16) with rows gathered through cols
+__device__ __forceinline__ Opnd opnd_gather_split(const uint16_t *hi, const uint16_t *lo, const int32_t *cols, int ld,
+                                                  int mlimit) {
+  Opnd o;
+  o.kind = OP_SPLIT_GATHER; o.f32 = nullptr; o.cols = cols; o.hi = hi; o.lo = lo; o.ld = ld; o.mlimit = mlimit;
   return o;
 }
 __device__ __forceinline__ Opnd opnd_f32(const float *base, int kind, int ld, int mlimit) {
@@ -203,7 +220,22 @@ __device__ __forceinline__ void fetch(const Opnd &o, int m0, int kb, Stage<TILE>
   static_assert(!COH || KIND == OP_SPLIT_K || KIND == OP_SPLIT_M, "device-scope loads for split operands only");
   constexpr int N = Stage<TILE>::N;
   const int tid = threadIdx.x;
-  if constexpr (KIND == OP_F32_GATHER) {
+  if constexpr (KIND == OP_SPLIT_GATHER) {
+#pragma unroll
+    for (int it = 0; it < N / 2; ++it) {
+      const int e = tid + it * NT;
+      const int m = e / (KC / 8), kg = e % (KC / 8);
+      const int gm = m0 + m;
+      uint4 h = make_uint4(0, 0, 0, 0), l = h;
+      if (gm < o.mlimit) {
+        const int64_t ofs = (int64_t)o.cols[gm] * o.ld + kb + kg * 8;
+        h = gld16(o.hi + ofs);
+        l = gld16(o.lo + ofs);
+      }
+      st.r[it] = h;
+      st.r[N / 2 + it] = l;
+    }
+  } else if constexpr (KIND == OP_F32_GATHER) {
 #pragma unroll
     for (int it = 0; it < N; ++it) {
       const int e = tid + it * NT;
@@ -256,6 +288,41 @@ __device__ __forceinline__ void fetch(const Opnd &o, int m0, int kb, Stage<TILE>
   }
 }
 
+// OP_SPLIT_GATHER rows of one tile: each thread's element offsets cols[m] * ld
+// (-1: row m outside the operand), the same for every K chunk of the tile
+template <int TILE>
+__device__ __forceinline__ void gather_rows(const Opnd &o, int m0, int64_t *rows) {
+#pragma unroll
+  for (int it = 0; it < Stage<TILE>::N / 2; ++it) {
+    const int gm = m0 + (threadIdx.x + it * NT) / (KC / 8);
+    rows[it] = gm < o.mlimit ? (int64_t)o.cols[gm] * o.ld : -1;
+  }
+}
+template <int TILE>
+__device__ __forceinline__ void fetch_rows(const Opnd &o, const int64_t *rows, int kb, Stage<TILE> &st) {
+  constexpr int N = Stage<TILE>::N;
+#pragma unroll
+  for (int it = 0; it < N / 2; ++it) {
+    const int kg = (threadIdx.x + it * NT) % (KC / 8);
+    uint4 h = make_uint4(0, 0, 0, 0), l = h;
+    if (rows[it] >= 0) {
+      const int64_t ofs = rows[it] + kb + kg * 8;
+      h = gld16(o.hi + ofs);
+      l = gld16(o.lo + ofs);
+    }
+    st.r[it] = h;
+    st.r[N / 2 + it] = l;
+  }
+}
+// one K chunk of an operand inside a tile (gathered rows from their offsets)
+template <int TILE, int KIND, bool COH>
+__device__ __forceinline__ void fetch_in_tile(const Opnd &o, int m0, const int64_t *rows, int kb, Stage<TILE> &st) {
+  if constexpr (KIND == OP_SPLIT_GATHER)
+    fetch_rows<TILE>(o, rows, kb, st);
+  else
+    fetch<TILE, KIND, COH>(o, m0, kb, st);
+}
+
 __device__ __forceinline__ uint32_t pack2(uint16_t a, uint16_t b) { return (uint32_t)a | ((uint32_t)b << 16); }
 
 // FP32 staging view of one operand buffer: k quad q (k = 4q .. 4q+3) of row m
@@ -301,7 +368,7 @@ __device__ __forceinline__ void stash(const Stage<TILE> &st, SRow<TILE> *sh, SRo
       ((uint2 *)&sh[kg >> 1][m])[kg & 1] = make_uint2(pack2(h0, h1), pack2(h2, h3));
       ((uint2 *)&sl[kg >> 1][m])[kg & 1] = make_uint2(pack2(l0, l1), pack2(l2, l3));
     }
-  } else if constexpr (KIND == OP_SPLIT_K) {
+  } else if constexpr (KIND == OP_SPLIT_K || KIND == OP_SPLIT_GATHER) {
 #pragma unroll
     for (int it = 0; it < N / 2; ++it) {
       const int e = tid + it * NT;
@@ -457,9 +524,12 @@ __device__ __forceinline__ void tile_gemm(GemmSmem<TILE> &sm, const Opnd &A, con
       for (int r = 0; r < 16; ++r) acc.v[i][j][r] = 0.f;
   static_assert(KB != OP_SPLIT_M, "only the A operand may be read with m contiguous");
   const int nc = K / KC;
+  int64_t ra[Stage<TILE>::N / 2], rb[Stage<TILE>::N / 2];
+  if constexpr (KA == OP_SPLIT_GATHER) gather_rows<TILE>(A, tm0, ra);
+  if constexpr (KB == OP_SPLIT_GATHER) gather_rows<TILE>(B, tn0, rb);
   if (!have_first) {
-    fetch<TILE, KA, COH>(A, tm0, 0, sa);
-    fetch<TILE, KB, COH>(B, tn0, 0, sb);
+    fetch_in_tile<TILE, KA, COH>(A, tm0, ra, 0, sa);
+    fetch_in_tile<TILE, KB, COH>(B, tn0, rb, 0, sb);
   }
   if constexpr (KA == OP_SPLIT_M)
     stash_m<TILE>(sa, sm.th[0], sm.tl[0]);
@@ -471,8 +541,8 @@ __device__ __forceinline__ void tile_gemm(GemmSmem<TILE> &sm, const Opnd &A, con
     const int cur = c & 1;
     const bool more = c + 1 < nc;
     if (more) {
-      fetch<TILE, KA, COH>(A, tm0, (c + 1) * KC, sa);
-      fetch<TILE, KB, COH>(B, tn0, (c + 1) * KC, sb);
+      fetch_in_tile<TILE, KA, COH>(A, tm0, ra, (c + 1) * KC, sa);
+      fetch_in_tile<TILE, KB, COH>(B, tn0, rb, (c + 1) * KC, sb);
     }
     if constexpr (PREC == PREC_F32)
       mma_chunk_f32<TILE>(sm, cur, acc, bm, bn);

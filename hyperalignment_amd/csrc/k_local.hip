@@ -35,9 +35,11 @@
 template <int TILE>
 __device__ void crosscov(GemmSmem<TILE> &sm, const HakLocalArgs &a, int s, int k, int kp, float *Ab) {
   const int o = a.sl_off[s];
-  Opnd oa = opnd_gather(a.X, a.slx_idx + o, a.ld, k);
-  Opnd ob = opnd_gather(a.Y, a.sly_idx + o, a.ld, k);
-  wg_gemm<TILE, OP_F32_GATHER, OP_F32_GATHER>(sm, oa, ob, kp, a.ld, false, [&](int m, int n, float4 v) {
+  // the pre-split planes of X and Y: the same products as on the FP32 data,
+  // without splitting every chunk while staging it
+  Opnd oa = opnd_gather_split(a.Xh, a.Xl, a.slx_idx + o, a.ld, k);
+  Opnd ob = opnd_gather_split(a.Yh, a.Yl, a.sly_idx + o, a.ld, k);
+  wg_gemm<TILE, OP_SPLIT_GATHER, OP_SPLIT_GATHER>(sm, oa, ob, kp, a.ld, false, [&](int m, int n, float4 v) {
     *reinterpret_cast<float4 *>(Ab + m + (int64_t)n * kp) = v;
   });
 }
@@ -871,8 +873,13 @@ extern "C" int hak_jacobi64(const HakLocalArgs *a, int want_det_fix, double *wor
 // data layout and scaling helpers
 // ---------------------------------------------------------------------------
 
+// FP64 input -> FP32 (leading dim ld), optional FP64 copy, and optional bf16
+// hi/lo planes of the FP32 values split as the bf16x3 staging splits FP32
+// operands (hi = bf16(x), lo = bf16(x - hi)), so a product on the planes equals
+// one on the FP32 data bit for bit
 __global__ void k_convert(const double *__restrict__ src, int32_t nt, int32_t nv, int col_major,
-                          float *__restrict__ d32, int32_t ld, double *__restrict__ d64) {
+                          float *__restrict__ d32, int32_t ld, double *__restrict__ d64, uint16_t *__restrict__ hi,
+                          uint16_t *__restrict__ lo) {
   const int64_t n = (int64_t)ld * nv;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
     int64_t v = e / ld;
@@ -882,16 +889,23 @@ __global__ void k_convert(const double *__restrict__ src, int32_t nt, int32_t nv
       x = col_major ? src[t + v * nt] : src[(int64_t)t * nv + v];
       if (d64) d64[t + v * nt] = x;
     }
-    d32[e] = (float)x;
+    const float f = (float)x;
+    d32[e] = f;
+    if (hi) {
+      const uint16_t h = f2bf(f);
+      hi[e] = h;
+      lo[e] = f2bf(f - bf2f(h));
+    }
   }
 }
 extern "C" int hak_convert_data(const double *src, int32_t nt, int32_t nv, int col_major, float *dst32,
-                                int32_t ld, double *dst64, void *stream) {
+                                int32_t ld, double *dst64, uint16_t *hi, uint16_t *lo, void *stream) {
+  if ((hi == nullptr) != (lo == nullptr)) return (int)hipErrorInvalidValue;
   int64_t n = (int64_t)ld * nv;
   int64_t g = (n + 255) / 256;
   if (g > 65536) g = 65536;
   hipLaunchKernelGGL(k_convert, dim3((int)g), dim3(256), 0, (hipStream_t)stream, src, nt, nv, col_major, dst32,
-                     ld, dst64);
+                     ld, dst64, hi, lo);
   LAUNCH_RET();
 }
 

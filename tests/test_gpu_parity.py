@@ -7,6 +7,7 @@ oracle on T, the bar the reference's GPU target states.
 """
 
 import numpy as np
+import scipy.sparse
 import pytest
 
 import hyperalignment_amd as ha
@@ -199,6 +200,35 @@ def test_csc_api_unweighted_separate_y_lists():
     sy = synth.toy_searchlights(ly, nv=nv)
     ref = O.searchlight_procrustes_csc(X, Y, sx.indices, sx.offsets, m.indptr, m.indices, idx_y=sy.indices)
     assert np.abs(m.data - ref).max() < TOL_64
+
+
+def test_csc_long_columns_global_path():
+    """A caller pattern with five full columns (3,300 rows, longer than the 3,072
+    the scatter holds in LDS): those columns accumulate in global memory, the
+    others in LDS, in one pass; rows outside every searchlight stay zero."""
+    rng = np.random.default_rng(5)
+    nv, nt = 3300, 40
+    X = rng.standard_normal((nt, nv))
+    Y = rng.standard_normal((nt, nv))
+    lx = [np.sort(rng.choice(nv, 12, replace=False)) for _ in range(60)]
+    base = ha.sparse_init(lx, nv)
+    full = {int(lx[i][0]) for i in range(5)}
+    cols, rows = [], []
+    for c in range(nv):
+        r = np.arange(nv) if c in full else base.indices[base.indptr[c]:base.indptr[c + 1]]
+        rows.append(r)
+        cols.append(np.full(len(r), c))
+    rows, cols = np.concatenate(rows), np.concatenate(cols)
+    m = scipy.sparse.csc_matrix((np.zeros(len(rows)), (rows, cols)), shape=(nv, nv))
+    m.sort_indices()
+    assert np.diff(m.indptr).max() == nv
+    ha.set_options(method="jacobi64")
+    ha.searchlight_procrustes_csc(X, Y, lx, m)
+    sx = synth.toy_searchlights(lx, nv=nv)
+    ref = O.searchlight_procrustes_csc(X, Y, sx.indices, sx.offsets, m.indptr, m.indices)
+    assert np.abs(m.data - ref).max() < TOL_64
+    for c in full:
+        assert np.count_nonzero(m.data[m.indptr[c]:m.indptr[c + 1]]) > 0
 
 
 # ---- larger config: sampled columns ----
