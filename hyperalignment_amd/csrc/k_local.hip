@@ -45,10 +45,10 @@ __device__ void crosscov(GemmSmem<TILE> &sm, const HakLocalArgs &a, int s, int k
 }
 
 // Learned-scaling features of A (12 per searchlight), written when a.features != NULL.
+// fro2 = ||A||_F^2 as frob2 gives it.
 __device__ __noinline__ void features_of(const HakLocalArgs &a, int s, int k, int kp, const float *Ab, float *Pv,
-                                         float *feat_sh, double *red) {
-  // Frobenius, trace, diagonal statistics
-  double fro2 = frob2(Ab, kp, red);
+                                         float *feat_sh, double *red, double fro2) {
+  // trace, diagonal statistics
   double tr = 0.0, dabs = 0.0, d2 = 0.0;
   float dmin = 3.0e38f, dmax = 0.f;
   for (int i = threadIdx.x; i < k; i += NT) {
@@ -79,39 +79,46 @@ __device__ __noinline__ void features_of(const HakLocalArgs &a, int s, int k, in
   __syncthreads();
   // power iteration on A^T A with v0 = ones: estimates after 2, 4, 8 steps.
   // A (kp x kp, column-major, zero outside k x k, kp a multiple of 32) is read
-  // as float4 row quads with four columns per batch, so each lane keeps eight
-  // independent 16-byte loads in flight: the loop is bound by memory latency,
-  // not by arithmetic. u = A v splits the columns over the four waves (partial
-  // sums per wave); v = A^T u gives each wave whole columns.
+  // as float4 row quads, four columns per group and FPG groups per batch: a
+  // lane keeps up to 8 FPG independent 16-byte loads in flight (the passes are
+  // bound by memory latency), and sums in the order of one group at a time.
+  // u = A v splits the columns over the four waves (partial sums per wave);
+  // v = A^T u gives each wave whole columns.
   float *v = Pv, *u = Pv + kp, *part = Pv + 2 * kp;  // part: 4 x kp partial sums
   for (int i = threadIdx.x; i < kp; i += NT) v[i] = (i < k) ? 1.0f : 0.f;
   __syncthreads();
   double est[3] = {0, 0, 0};
   int ei = 0;
   const int nq = kp / 4;  // row quads per column
+  constexpr int FPG = 2;
   for (int it = 1; it <= 8; ++it) {
     // u = A v: lane owns row quads q = lane + 64 c
     for (int q0 = 0; q0 < nq; q0 += 128) {
       float4 acc[2] = {make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f)};
       const bool in0 = q0 + lane < nq, in1 = q0 + 64 + lane < nq;
-      for (int j0 = wave * 4; j0 < k; j0 += 16) {
-        float4 x0[4], x1[4];
-        float vj[4];
+      for (int jb = wave * 4; jb < k; jb += 16 * FPG) {
+        // columns j >= k read column 0 (in bounds) and weigh it by 0
+        float4 x0[FPG][4], x1[FPG][4];
+        float vj[FPG][4];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          const int j = j0 + c;
-          vj[c] = (j < k) ? v[j] : 0.f;
-          const float4 *col = reinterpret_cast<const float4 *>(Ab + (int64_t)(j < k ? j : 0) * kp);
-          x0[c] = in0 ? col[q0 + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
-          x1[c] = in1 ? col[q0 + 64 + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
-        }
+        for (int g = 0; g < FPG; ++g)
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          acc[0].x += x0[c].x * vj[c]; acc[0].y += x0[c].y * vj[c];
-          acc[0].z += x0[c].z * vj[c]; acc[0].w += x0[c].w * vj[c];
-          acc[1].x += x1[c].x * vj[c]; acc[1].y += x1[c].y * vj[c];
-          acc[1].z += x1[c].z * vj[c]; acc[1].w += x1[c].w * vj[c];
-        }
+          for (int c = 0; c < 4; ++c) {
+            const int j = jb + 16 * g + c;
+            vj[g][c] = (j < k) ? v[j] : 0.f;
+            const float4 *col = reinterpret_cast<const float4 *>(Ab + (int64_t)(j < k ? j : 0) * kp);
+            x0[g][c] = in0 ? col[q0 + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
+            x1[g][c] = in1 ? col[q0 + 64 + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
+          }
+#pragma unroll
+        for (int g = 0; g < FPG; ++g)
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            acc[0].x += x0[g][c].x * vj[g][c]; acc[0].y += x0[g][c].y * vj[g][c];
+            acc[0].z += x0[g][c].z * vj[g][c]; acc[0].w += x0[g][c].w * vj[g][c];
+            acc[1].x += x1[g][c].x * vj[g][c]; acc[1].y += x1[g][c].y * vj[g][c];
+            acc[1].z += x1[g][c].z * vj[g][c]; acc[1].w += x1[g][c].w * vj[g][c];
+          }
       }
       float4 *pw = reinterpret_cast<float4 *>(part + wave * kp);
       if (in0) pw[q0 + lane] = acc[0];
@@ -127,32 +134,42 @@ __device__ __noinline__ void features_of(const HakLocalArgs &a, int s, int k, in
     nu = wg_sum_f64(nu, red);  // its barriers publish u
     // v = A^T u: wave w takes columns w*4 .. w*4+3, then +16
     double nv = 0.0;
-    for (int j0 = wave * 4; j0 < k; j0 += 16) {
-      float dot[4];
+    for (int jb = wave * 4; jb < k; jb += 16 * FPG) {
+      float dot[FPG][4];
 #pragma unroll
-      for (int c = 0; c < 4; ++c) dot[c] = 0.f;
+      for (int g = 0; g < FPG; ++g)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) dot[g][c] = 0.f;
       for (int q0 = 0; q0 < nq; q0 += 64) {
         const int q = q0 + lane;
         const float4 uq = (q < nq) ? reinterpret_cast<const float4 *>(u)[q] : make_float4(0.f, 0.f, 0.f, 0.f);
-        float4 x[4];
+        float4 x[FPG][4];
+#pragma unroll
+        for (int g = 0; g < FPG; ++g)
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            const int j = jb + 16 * g + c;
+            const float4 *col = reinterpret_cast<const float4 *>(Ab + (int64_t)(j < k ? j : 0) * kp);
+            x[g][c] = (q < nq && j < k) ? col[q] : make_float4(0.f, 0.f, 0.f, 0.f);
+          }
+#pragma unroll
+        for (int g = 0; g < FPG; ++g)
+#pragma unroll
+          for (int c = 0; c < 4; ++c)
+            dot[g][c] += x[g][c].x * uq.x + x[g][c].y * uq.y + x[g][c].z * uq.z + x[g][c].w * uq.w;
+      }
+#pragma unroll
+      for (int g = 0; g < FPG; ++g)
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-          const int j = j0 + c;
-          const float4 *col = reinterpret_cast<const float4 *>(Ab + (int64_t)(j < k ? j : 0) * kp);
-          x[c] = (q < nq && j < k) ? col[q] : make_float4(0.f, 0.f, 0.f, 0.f);
+          float d = dot[g][c];
+          for (int o2 = 32; o2 > 0; o2 >>= 1) d += __shfl_xor(d, o2);
+          const int j = jb + 16 * g + c;
+          if (lane == 0 && j < k) {
+            v[j] = d;
+            nv += (double)d * d;
+          }
         }
-#pragma unroll
-        for (int c = 0; c < 4; ++c) dot[c] += x[c].x * uq.x + x[c].y * uq.y + x[c].z * uq.z + x[c].w * uq.w;
-      }
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        float d = dot[c];
-        for (int o2 = 32; o2 > 0; o2 >>= 1) d += __shfl_xor(d, o2);
-        if (lane == 0 && j0 + c < k) {
-          v[j0 + c] = d;
-          nv += (double)d * d;
-        }
-      }
     }
     nv = wg_sum_f64(nv, red);
     // |A^T A v| / |A v| with |v| = 1 tends to sigma_max from below
@@ -222,7 +239,8 @@ __global__ __launch_bounds__(NT, 2) void k_crosscov(HakLocalArgs a) {
     crosscov<TILE>(sm, a, s, k, kp, A);
     __syncthreads();
     const uint64_t t1 = a.clk ? wall_clock64() : 0;
-    const float fro = (float)sqrt(frob2(A, kp, red));
+    const double fro2 = frob2(A, kp, red);
+    const float fro = (float)sqrt(fro2);
     // plain: X0 = A / ||A||_F (sigma_max(X0) <= 1 always), sigma_min(X0) >= plain_ell assumed
     float scale = fro, ell = a.plain_ell;
     if (!(fro > 0.f) || !isfinite(fro)) {
@@ -232,7 +250,7 @@ __global__ __launch_bounds__(NT, 2) void k_crosscov(HakLocalArgs a) {
         if (a.steps) a.steps[s] = 0;
       }
     } else if (a.features || a.mlp) {
-      features_of(a, s, k, kp, A, scratch, feat_sh, red);
+      features_of(a, s, k, kp, A, scratch, feat_sh, red, fro2);
       if (a.features && threadIdx.x < HAK_NFEAT)
         a.features[(int64_t)s * HAK_NFEAT + threadIdx.x] = feat_sh[threadIdx.x];
       if (a.mlp) {
@@ -621,7 +639,7 @@ __global__ __launch_bounds__(NT) void k_jacobi32(HakLocalArgs a) {
     crosscov<TILE>(sm, a, s, k, kp, w.A);
     __syncthreads();
     if (a.features) {
-      features_of(a, s, k, kp, w.A, w.P, feat_sh, red);
+      features_of(a, s, k, kp, w.A, w.P, feat_sh, red, frob2(w.A, kp, red));
       if (threadIdx.x < HAK_NFEAT) a.features[(int64_t)s * HAK_NFEAT + threadIdx.x] = feat_sh[threadIdx.x];
       __syncthreads();
     }

@@ -30,6 +30,16 @@ __device__ __forceinline__ double frob2(const float *M, int kp, double *red) {
   const int64_t n4 = (int64_t)kp * kp / 4;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   int64_t e = threadIdx.x;
+  // sixteen loads in flight, summed four at a time in the same order
+  for (; e + 15 * NT < n4; e += 16 * NT) {
+    float4 x[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) x[c] = m4[e + c * NT];
+#pragma unroll
+    for (int c = 0; c < 16; ++c)
+      acc[c & 3] += (double)x[c].x * x[c].x + (double)x[c].y * x[c].y + (double)x[c].z * x[c].z +
+                    (double)x[c].w * x[c].w;
+  }
   for (; e + 3 * NT < n4; e += 4 * NT) {
     float4 x[4];
 #pragma unroll
