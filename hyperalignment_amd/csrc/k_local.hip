@@ -341,9 +341,12 @@ __global__ __launch_bounds__(NT, TILE == 128 ? HAK_POLAR_OCC128 : HAK_POLAR_OCC)
 
 // Stage 2 with teams: HAK_GRP workgroups of one XCD (workgroups are dispatched
 // round-robin over the 8 XCDs, so workgroup b runs on XCD b % 8) iterate one
-// searchlight together, sharing panel reads through that XCD's L2. The finishing
-// product goes to the team's G buffer, and Q replaces A in the Q store only after
-// a barrier, so a searchlight the team gives up on still holds A for the
+// searchlight together, sharing panel reads through that XCD's L2. The
+// finishing product writes Q over A in the Q store. Before it starts, the
+// searchlight is marked for the FP64 pass (flag 1, steps -1, published by the
+// barrier ahead of the product), and the mark is cleared once every member has
+// written its tiles: a team that gives up part way leaves a marked searchlight,
+// and one that gives up earlier leaves A intact and steps 0 for the
 // one-workgroup kernel that follows.
 __global__ __launch_bounds__(NT, HAK_POLAR_OCC128) void k_polar_grp(HakLocalArgs a) {
   constexpr int TILE = 128;
@@ -389,11 +392,27 @@ __global__ __launch_bounds__(NT, HAK_POLAR_OCC128) void k_polar_grp(HakLocalArgs
     const int kp = a.qld[s];
     float *Q = a.q + a.qoff[s];
     w.A = Q;
-    float *Qn = w.G;  // free once the finishing step has formed P
+    const bool want_tr = a.sigma_sum != nullptr;
+    double tr = 0.0;
+    // Q over A: each element of A is read (for tr(Q^T A)) and then replaced by
+    // the same thread, and A is not read again for this searchlight
     const PolarOut po = polar_solve<TILE, PREC_BF16X3>(
         sm, a, w, k, kp, fro, scale, ell, a.learned_scale != 0, red,
         a.trace ? a.trace + (int64_t)s * a.max_steps : nullptr, team,
-        [&](int m, int n, float4 v) { *reinterpret_cast<float4 *>(Qn + m + (int64_t)n * kp) = v; });
+        [&](int m, int n, float4 v) {
+          float4 *q4 = reinterpret_cast<float4 *>(Q + m + (int64_t)n * kp);
+          if (want_tr) {
+            const float4 o = *q4;
+            tr += (double)v.x * o.x + (double)v.y * o.y + (double)v.z * o.z + (double)v.w * o.w;
+          }
+          *q4 = v;
+        },
+        [&]() {
+          if (lead) {
+            a.flag[s] = 1;
+            a.steps[s] = -1;
+          }
+        });
     if (po.aborted) break;
     if (!po.done) {
       if (lead) {
@@ -402,28 +421,12 @@ __global__ __launch_bounds__(NT, HAK_POLAR_OCC128) void k_polar_grp(HakLocalArgs
       }
       continue;
     }
-    // Q replaces A: marked first, so a copy cut short sends the searchlight to the FP64 pass
-    if (lead) {
-      a.flag[s] = 1;
-      a.steps[s] = -po.steps;
+    // done: polar_solve's last barrier followed every member's Q tiles
+    if (want_tr) {
+      bool ok;
+      tr = team.sum(tr, red, &ok);
+      if (!ok) break;
     }
-    if (!team.sync()) break;
-    const bool want_tr = a.sigma_sum != nullptr;
-    double tr = 0.0;
-    {
-      // Qn holds what other members wrote: device-scope loads; A (the old Q) is
-      // read and replaced by the same thread
-      float4 *q4 = reinterpret_cast<float4 *>(Q);
-      const int64_t nq = (int64_t)kp * kp / 4;
-      for (int64_t e = team.e0(); e < nq; e += team.es()) {
-        const float4 o = q4[e], v = __builtin_bit_cast(float4, ld_b128<true>(Qn, 16 * e));
-        if (want_tr) tr += (double)v.x * o.x + (double)v.y * o.y + (double)v.z * o.z + (double)v.w * o.w;
-        q4[e] = v;
-      }
-    }
-    bool ok;
-    tr = team.sum(tr, red, &ok);
-    if (!ok) break;
     if (lead) {
       a.flag[s] = 0;
       if (want_tr) a.sigma_sum[s] = tr;

@@ -289,10 +289,17 @@ struct GroupTeam {
 // learned: (scale, ell) came from a prediction; an attempt that does not
 // converge is redone once from (fro, plain_ell), which always holds
 // sigma_max(X0) <= 1. w.X, w.X2, w.G, w.P are overwritten.
-template <int TILE, int PREC, class Team, class Fin>
+// on_fin() runs in every thread once the finishing step is decided, before the
+// team's barrier that precedes the finishing product (so before any fin call
+// of any member).
+struct NoFinMark {
+  __device__ void operator()() const {}
+};
+template <int TILE, int PREC, class Team, class Fin, class OnFin = NoFinMark>
 __device__ __forceinline__ PolarOut polar_solve(GemmSmem<TILE> &sm, const HakLocalArgs &a, const WorkSlots &w,
                                                 int k, int kp, float fro, float scale, float ell, bool learned,
-                                                double *red, float *trace, Team &team, Fin fin) {
+                                                double *red, float *trace, Team &team, Fin fin,
+                                                OnFin on_fin = OnFin()) {
   using M = MatS<PREC>;
   const M G(w.G, kp), P(w.P, kp);
   int step = 0, total = 0;
@@ -354,6 +361,7 @@ __device__ __forceinline__ PolarOut polar_solve(GemmSmem<TILE> &sm, const HakLoc
         break;
       }
       const bool finish = (r < a.tol);
+      if (finish) on_fin();
       int ts = step - restart;
       if (!finish && ts >= a.sched_len[bucket] && r > 0.7 && restarts < 2) {
         // The growth phase of the schedule is over but some singular values lag
