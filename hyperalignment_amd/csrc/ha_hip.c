@@ -662,7 +662,9 @@ static void stats_update(HaHipPlan *p, int method) {
 			if (v < 0) { S->n_unconverged++; v = -v; }
 			S->steps_total += v;
 			S->steps_hist[v < 31 ? v : 31]++;
-			flops += v * 6.0 * k * k * k; /* X^T X, G^2 and X P per step */
+			/* per step: X^T X and G^2 are symmetric (k^3 each, only tiles I <= J
+			 * are computed), X P is a full product (2 k^3) */
+			flops += v * 4.0 * k * k * k;
 		} else {
 			if (v >= 1000) v -= 1000;
 			if (v < 0) { S->n_unconverged++; v = -v; }

@@ -115,7 +115,8 @@ typedef struct {
 	int32_t steps_hist[32];    /* histogram of polar steps (last bin = overflow) */
 	int64_t nnz;
 	double  gemm_flops;        /* algorithmic flops of the polar iteration stage of the last pass
-	                              (6 k^3 per executed step and searchlight) */
+	                              (4 k^3 per executed step and searchlight: X^T X and G^2
+	                              symmetric, k^3 each; X P, 2 k^3) */
 	double  kernel_ms_polar_avg; /* average duration of one polar launch */
 	int32_t polar_launches;
 	double  ms_apply;          /* last ha_hip_plan_evaluate: Yhat = X T */
