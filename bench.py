@@ -34,6 +34,8 @@ WORKLOADS = {
     "c0": "order-4 icosphere (2,562 vertices), 15 mm searchlights, n=200",
     "c1": "order-5 icosphere (10,242 vertices), 20 mm searchlights, n=300",
     "c2": "order-6 icosphere (40,962 vertices), 20 mm searchlights, n=500 (full hemisphere)",
+    "c2tail": "order-6 icosphere (40,962 vertices), per-centre radii for sizes 37-530 with a long tail "
+              "(median 90), n=500",
 }
 
 

@@ -234,11 +234,12 @@ struct HaHipPlan {
 	int32_t bucket_tile[HA_TILE_BUCKETS];
 	bool tuned;
 	int32_t tuned_opt;  /* the tile option the choice was made under */
-	int32_t *tile_order[2]; int32_t tile_count[2];
+	int32_t *tile_order[3]; int32_t tile_count[3];  /* 64, 128, rank-deficient (FP32 products) */
 	/* captured pass (use_graph) and the arguments it was captured with */
 	hipGraphExec_t gexec;
 	struct { HakLocalArgs a; int method; bool refl, scal, zero; int32_t team; uint32_t team_spin; } gkey;
 	int32_t tuned_team;
+	int32_t tuned_fb;   /* the fp64_fallback option the lists were built under */
 	bool loaded;
 };
 
@@ -249,7 +250,7 @@ static void plan_free(HaHipPlan *p) {
 	                p->qld, p->qoff, p->q, p->q64, p->X, p->Y, p->X64, p->Y64, p->Xs, p->Ys, p->stage, p->work, p->work64,
 	                p->flag, p->steps, p->queues, p->order, p->sigma_sum, p->var_sum, p->scale,
 	                p->features, p->pred, p->trace, p->sig, p->d_which, p->sl_par, p->tile_order[0],
-	                p->tile_order[1], p->work_ridge, p->grp};
+	                p->tile_order[1], p->tile_order[2], p->work_ridge, p->grp};
 	for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i)
 		if (ptrs[i]) hipFree(ptrs[i]);
 	if (p->gexec) hipGraphExecDestroy(p->gexec);
@@ -744,17 +745,21 @@ static int kp_bucket(int32_t kp) {
 static int build_tile_orders(HaHipPlan *p) {
 	int rc = kHaSuccess;
 	hipStream_t st = g.stream;
-	int32_t *h = (int32_t *)malloc((size_t)p->count * 2 * sizeof(int32_t));
+	int32_t *h = (int32_t *)malloc((size_t)p->count * 3 * sizeof(int32_t));
 	int32_t *ord = (int32_t *)malloc((size_t)p->count * sizeof(int32_t));
 	if (!h || !ord) { free(h); free(ord); return kHaErrorAlloc; }
 	HCK(hipMemcpy(ord, p->order, (size_t)p->count * sizeof(int32_t), hipMemcpyDeviceToHost));
-	int32_t n[2] = {0, 0};
+	int32_t n[3] = {0, 0, 0};
 	for (int32_t i = 0; i < p->count; ++i) {
 		const int32_t s = ord[i];
-		const int t = p->bucket_tile[kp_bucket((p->h_k[s] + 31) / 32 * 32)] == 128 ? 1 : 0;
+		/* rank deficient (stage 1 starts them from the smallest bucket): FP32 products */
+		const int t = (g.opt.fp64_fallback && p->h_k[s] >= p->nt)
+		                  ? 2
+		                  : (p->bucket_tile[kp_bucket((p->h_k[s] + 31) / 32 * 32)] == 128 ? 1 : 0);
 		h[(size_t)t * p->count + n[t]++] = s;
 	}
-	for (int t = 0; t < 2; ++t) {
+	p->tuned_fb = g.opt.fp64_fallback;
+	for (int t = 0; t < 3; ++t) {
 		if (!p->tile_order[t]) ACK(p->tile_order[t], (size_t)p->count * sizeof(int32_t));
 		if (n[t]) HCK(hipMemcpyAsync(p->tile_order[t], h + (size_t)t * p->count, (size_t)n[t] * sizeof(int32_t),
 		                             hipMemcpyHostToDevice, st));
@@ -851,7 +856,8 @@ fail:
 int ha_hip_plan_tiles(HaHipPlan *p, int32_t *bucket_tile) {
 	if (!p || !bucket_tile) return kHaErrorArg;
 	if (!p->loaded) { ERR("plan has no data loaded\n"); return kHaErrorArg; }
-	if (!p->tuned || p->tuned_opt != g.opt.tile || p->tuned_team != g.opt.team) {
+	if (!p->tuned || p->tuned_opt != g.opt.tile || p->tuned_team != g.opt.team ||
+	    p->tuned_fb != g.opt.fp64_fallback) {
 		int rc = tune_tiles(p);
 		if (rc) return rc;
 	}
@@ -899,15 +905,17 @@ static int enqueue_pass(HaHipPlan *p, HakLocalArgs a, int method, bool isReflect
 			KCK(hak_crosscov(&c, p->n_slots, st));
 		}
 		if (timed) HCK(hipEventRecord(p->ev[6], st));
-		/* one persistent launch per tile over its searchlights (k descending) */
-		for (int t = 0; t < 2; ++t) {
+		/* one persistent launch per tile over its searchlights (k descending), and
+		 * one with FP32 products over the rank-deficient ones */
+		for (int t = 0; t < 3; ++t) {
 			if (!p->tile_count[t]) continue;
 			HakLocalArgs b = a;
-			b.tile = t ? 128 : 64;
+			b.tile = t == 1 ? 128 : 64;
+			b.prec_f32 = t == 2;
 			b.order = p->tile_order[t];
 			b.n_order = p->tile_count[t];
-			b.queue = p->queues + 4 + t;
-			if (t) {
+			b.queue = p->queues + (t == 2 ? 7 : 4 + t);
+			if (t == 1) {
 				rc = launch_polar128(p, b, st);
 				if (rc) goto fail;
 			} else {
@@ -984,7 +992,8 @@ int ha_plan_run(HaHipPlan *p, bool isReflection, bool isScaling, bool zero_data,
 		rc = ensure_teams(p);
 		if (rc) return rc;
 	}
-	if (is_polar && (!p->tuned || p->tuned_opt != g.opt.tile || p->tuned_team != g.opt.team)) {
+	if (is_polar && (!p->tuned || p->tuned_opt != g.opt.tile || p->tuned_team != g.opt.team ||
+	                 p->tuned_fb != g.opt.fp64_fallback)) {
 		rc = tune_tiles(p);
 		if (rc) return rc;
 	}
@@ -1029,7 +1038,8 @@ int ha_plan_run(HaHipPlan *p, bool isReflection, bool isScaling, bool zero_data,
 		p->stats.ms_scatter = t[4] + t[3];
 		p->stats.ms_total = tot;
 		p->stats.kernel_ms_polar_avg = ti;
-		p->stats.polar_launches = is_polar ? (p->tile_count[0] > 0) + (p->tile_count[1] > 0) : 1;
+		p->stats.polar_launches =
+		    is_polar ? (p->tile_count[0] > 0) + (p->tile_count[1] > 0) + (p->tile_count[2] > 0) : 1;
 		debug_report(p, &a, clk, nclk);
 	} else {
 		/* replay the captured pass when nothing it depends on changed */

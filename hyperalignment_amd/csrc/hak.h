@@ -119,7 +119,8 @@ typedef struct {
 	float *sig_out;                      /* Jacobi paths: per searchlight (sigma_max, sigma_min) (may be NULL) */
 	int32_t tile;
 	int32_t fallback_enabled;
-	int32_t rank_check_nt;               /* flag k >= nt as rank deficient */
+	int32_t rank_check_nt;               /* k >= nt: rank deficient, started from the smallest bucket */
+	int32_t prec_f32;                    /* hak_polar: FP32 products (PREC_F32) instead of bf16x3 */
 	/* teams of HAK_GRP workgroups per searchlight (k_polar_grp) */
 	void *grp;                           /* per team HAK_GRP_STATE_BYTES of state, zeroed before the launch */
 	uint32_t grp_spin;                   /* polls of a team barrier before it gives up (0: default) */

@@ -227,27 +227,31 @@ __global__ __launch_bounds__(NT, 2) void k_crosscov(HakLocalArgs a) {
     const int k = a.sl_off[s + 1] - a.sl_off[s];
     const int kp = a.qld[s];
     float *A = a.q + a.qoff[s];
-    if (a.rank_check_nt && k >= a.nt) {
-      if (threadIdx.x == 0) {
-        a.flag[s] = 1;
-        if (a.steps) a.steps[s] = 0;
-        a.sl_par[3 * s] = 0.f;
-      }
-      continue;
-    }
+    // k >= nt: z-scored columns leave A of rank <= nt - 1 < k, so its polar
+    // factor is not unique and its smallest singular values are rounding noise
+    // (a few 1e-9 of ||A||_F). The plain scale with the schedule of the smallest
+    // bucket takes that noise to 1 as well: one orthogonal completion, a valid
+    // Procrustes solution. One that does not converge goes to the FP64 path.
+    const bool deficient = a.rank_check_nt && k >= a.nt;
     const uint64_t t0 = a.clk ? wall_clock64() : 0;
     crosscov<TILE>(sm, a, s, k, kp, A);
     __syncthreads();
     const uint64_t t1 = a.clk ? wall_clock64() : 0;
     const double fro2 = frob2(A, kp, red);
     const float fro = (float)sqrt(fro2);
-    // plain: X0 = A / ||A||_F (sigma_max(X0) <= 1 always), sigma_min(X0) >= plain_ell assumed
-    float scale = fro, ell = a.plain_ell;
+    // plain: X0 = A / ||A||_F (sigma_max(X0) <= 1 always), sigma_min(X0) >= plain_ell
+    // assumed (0 for a rank-deficient A: the smallest bucket's schedule)
+    float scale = fro, ell = deficient ? 0.f : a.plain_ell;
     if (!(fro > 0.f) || !isfinite(fro)) {
       scale = 0.f;
       if (threadIdx.x == 0) {
         a.flag[s] = 1;
         if (a.steps) a.steps[s] = 0;
+      }
+    } else if (deficient) {
+      if (a.pred_out && threadIdx.x == 0) {  // no prediction is made for it
+        a.pred_out[2 * s] = __int_as_float(0x7fc00000);
+        a.pred_out[2 * s + 1] = __int_as_float(0x7fc00000);
       }
     } else if (a.features || a.mlp) {
       features_of(a, s, k, kp, A, scratch, feat_sh, red, fro2);
@@ -287,7 +291,7 @@ __global__ __launch_bounds__(NT, 2) void k_crosscov(HakLocalArgs a) {
 // is read before the same thread writes it). Without the gather and the
 // learned prologue the kernel fits HAK_POLAR_OCC workgroups per CU, so that
 // many slots' operand loads are in flight per CU.
-template <int TILE>
+template <int TILE, int PREC>
 __global__ __launch_bounds__(NT, TILE == 128 ? HAK_POLAR_OCC128 : HAK_POLAR_OCC) void k_polar(HakLocalArgs a) {
   __shared__ GemmSmem<TILE> sm;
   __shared__ double red[4];
@@ -317,7 +321,7 @@ __global__ __launch_bounds__(NT, TILE == 128 ? HAK_POLAR_OCC128 : HAK_POLAR_OCC)
     double tr = 0.0;
     const bool want_tr = a.sigma_sum != nullptr;
     SoloTeam team;
-    const PolarOut po = polar_solve<TILE, PREC_BF16X3>(sm, a, w, k, kp, fro, scale, ell, a.learned_scale != 0,
+    const PolarOut po = polar_solve<TILE, PREC>(sm, a, w, k, kp, fro, scale, ell, a.learned_scale != 0,
                                                        red, a.trace ? a.trace + (int64_t)s * a.max_steps : nullptr,
                                                        team, [&](int m, int n, float4 v) {
                                                          float4 *q4 = reinterpret_cast<float4 *>(Q + m + (int64_t)n * kp);
@@ -451,12 +455,18 @@ extern "C" int hak_crosscov(const HakLocalArgs *a, int n_slots, void *stream) {
   LAUNCH_RET();
 }
 
+// prec_f32: FP32 products on 64 x 64 tiles, for rank-deficient searchlights
+// (k >= nt): their rounding-noise singular values are raised from about 1e-9
+// to 1 over a long schedule, which carries the bf16x3 products' 2^-16 error
+// into the range part of Q (about 5e-5 against 1e-6 with FP32 products).
 extern "C" int hak_polar(const HakLocalArgs *a, void *stream) {
   if (a->n_order <= 0) return 0;
-  if (a->tile == 64)
-    hipLaunchKernelGGL(k_polar<64>, dim3(a->n_slots), dim3(NT), 0, (hipStream_t)stream, *a);
+  if (a->prec_f32)
+    hipLaunchKernelGGL((k_polar<64, PREC_F32>), dim3(a->n_slots), dim3(NT), 0, (hipStream_t)stream, *a);
+  else if (a->tile == 64)
+    hipLaunchKernelGGL((k_polar<64, PREC_BF16X3>), dim3(a->n_slots), dim3(NT), 0, (hipStream_t)stream, *a);
   else
-    hipLaunchKernelGGL(k_polar<128>, dim3(a->n_slots), dim3(NT), 0, (hipStream_t)stream, *a);
+    hipLaunchKernelGGL((k_polar<128, PREC_BF16X3>), dim3(a->n_slots), dim3(NT), 0, (hipStream_t)stream, *a);
   LAUNCH_RET();
 }
 

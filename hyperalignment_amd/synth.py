@@ -111,8 +111,9 @@ def icosphere_mesh(order: int, radius: float = SPHERE_RADIUS_MM):
     return v * radius, f.astype(np.int32)
 
 
-def _neighbourhoods(coords: np.ndarray, radius_mm: float):
-    """Great-circle neighbourhoods: for each vertex, sorted ids with d < radius."""
+def _neighbourhoods(coords: np.ndarray, radius_mm: float, per_centre: np.ndarray | None = None):
+    """Great-circle neighbourhoods: for each vertex, sorted ids with d < radius
+    (d < per_centre[c] for centre c when given, every value <= radius_mm)."""
     from scipy.spatial import cKDTree
 
     R = float(np.linalg.norm(coords[0]))
@@ -124,7 +125,7 @@ def _neighbourhoods(coords: np.ndarray, radius_mm: float):
     centre = np.repeat(np.arange(len(lists), dtype=np.int64), sizes)
     cosang = np.einsum("ij,ij->i", coords[flat], coords[centre]) / (R * R)
     d = R * np.arccos(np.clip(cosang, -1.0, 1.0))
-    keep = d < radius_mm
+    keep = d < (radius_mm if per_centre is None else per_centre[centre])
     flat, centre, d = flat[keep], centre[keep], d[keep]
     sizes = np.bincount(centre, minlength=len(lists))
     return flat, sizes, d
@@ -156,9 +157,26 @@ class Searchlights:
         return [self.dists[o[s]:o[s + 1]] for s in range(self.count)]
 
 
-def searchlights(order: int, radius_mm: float) -> tuple[np.ndarray, Searchlights]:
+def tail_radii(nv: int, sphere_radius: float, seed: int, kmin: float = 50.0, kmax: float = 500.0):
+    """Per-centre searchlight radii for sizes with a long tail: target size
+    k = kmin (kmax / kmin)^(u^2), u uniform per centre (median about 90 for
+    50-500), and the radius of a disc of k vertices' area, 4 pi R^2 / nv each."""
+    u = uniform(seed, nv)
+    k = kmin * (kmax / kmin) ** (u * u)
+    area = 4.0 * np.pi * sphere_radius ** 2 / nv
+    return np.sqrt(k * area / np.pi)
+
+
+def searchlights(order: int, radius_mm: float, tail_seed: int | None = None) -> tuple[np.ndarray, Searchlights]:
+    """Searchlights of every vertex: great-circle radius radius_mm, or with
+    tail_seed the per-centre radii of tail_radii (capped at radius_mm, which
+    stays the radius of the distance weights)."""
     coords = icosphere(order)
-    flat, sizes, d = _neighbourhoods(coords, radius_mm)
+    per = None
+    if tail_seed is not None:
+        R = float(np.linalg.norm(coords[0]))
+        per = np.minimum(tail_radii(coords.shape[0], R, tail_seed), radius_mm)
+    flat, sizes, d = _neighbourhoods(coords, radius_mm, per)
     offsets = np.zeros(len(sizes) + 1, dtype=np.int64)
     np.cumsum(sizes, out=offsets[1:])
     if offsets[-1] >= 2 ** 31:
@@ -208,11 +226,11 @@ _SL_CACHE: dict = {}
 _MIX_CACHE: dict = {}
 
 
-def _cached_searchlights(order, radius):
-    key = (order, float(radius))
+def _cached_searchlights(order, radius, tail_seed=None):
+    key = (order, float(radius), tail_seed)
     if key not in _SL_CACHE:
         _SL_CACHE.clear()
-        _SL_CACHE[key] = searchlights(order, radius)
+        _SL_CACHE[key] = searchlights(order, radius, tail_seed)
     return _SL_CACHE[key]
 
 
@@ -230,6 +248,9 @@ CONFIGS = {
     "c0": dict(order=4, radius=15.0, n=200, dtype="float64", noise=0.3),
     "c1": dict(order=5, radius=20.0, n=300, dtype="float32", noise=0.3),
     "c2": dict(order=6, radius=20.0, n=500, dtype="float32", noise=0.3),
+    # c2 with uneven searchlights: per-centre radii for sizes of about 50-500
+    # with a long tail (median about 90), every kp bucket populated
+    "c2tail": dict(order=6, radius=23.0, n=500, dtype="float32", noise=0.3, tail_seed=2024),
 }
 
 
@@ -245,7 +266,7 @@ def make_workload(name: str, seed: int = 0, n: int | None = None, noise: float |
         cfg["order"] = order
     if radius is not None:
         cfg["radius"] = radius
-    coords, sls = _cached_searchlights(cfg["order"], cfg["radius"])
+    coords, sls = _cached_searchlights(cfg["order"], cfg["radius"], cfg.get("tail_seed"))
     nv = sls.nv
     X = zscore_columns(normal_matrix(stream_seed(seed, 1), cfg["n"], nv))
     M = _cached_mixing(cfg["order"], cfg["radius"], stream_seed(seed, 2))

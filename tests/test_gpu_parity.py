@@ -275,10 +275,13 @@ def test_rank_deficient_local():
     _check_rank_deficient(X, Y, ha.procrustes(X, Y), 1e-9)
 
 
-@pytest.mark.parametrize("method", ["polar", "jacobi32"])
-def test_rank_deficient_searchlights(method):
-    """k >= n_samples routes to the FP64 fallback; every local transform is an
-    optimal orthogonal Procrustes solution."""
+@pytest.mark.parametrize("method,tol", [("polar", TOL_T), ("jacobi32", 2e-6)])
+def test_rank_deficient_searchlights(method, tol):
+    """k >= n_samples: the polar path iterates from the smallest bucket's
+    schedule, which takes the rounding-noise singular values to 1 as well (one
+    orthogonal completion), and the FP32 Jacobi path hands them to the FP64
+    fallback. Every local transform is an optimal orthogonal Procrustes
+    solution, to the accuracy of the path that made it."""
     rng = np.random.default_rng(12)
     nv, nt = 60, 10
     X = rng.standard_normal((nt, nv))
@@ -290,10 +293,11 @@ def test_rank_deficient_searchlights(method):
     ha.set_options(method=method)
     plan.execute()
     st, fl = plan.steps()
-    assert fl.all()
+    print(f"[rank] {method}: fallback {int(fl.sum())} of {len(lists)}, steps {st.tolist()}", flush=True)
+    assert fl.all() if method == "jacobi32" else not fl.any()
     for s, T in enumerate(plan.local(np.arange(len(lists)))):
         l = lists[s]
-        _check_rank_deficient(X[:, l], Y[:, l], T.astype(np.float64), 2e-6)
+        _check_rank_deficient(X[:, l], Y[:, l], T.astype(np.float64), tol)
     plan.close()
 
 
