@@ -977,6 +977,32 @@ static void debug_report(HaHipPlan *p, const HakLocalArgs *a, uint64_t *clk, int
 	}
 }
 
+/* More FP64 slots (k_jacobi64 fits four workgroups per CU) for passes that run
+ * every searchlight through the FP64 path; at most 8 GiB of workspace. Called
+ * before any capture: a captured pass holding the old workspace is dropped. An
+ * allocation that fails leaves the slots as they were. */
+static int ensure_work64(HaHipPlan *p, int32_t want) {
+	if (want > p->count) want = p->count;
+	const size_t per = (size_t)4 * p->stride64 * sizeof(double);
+	const int64_t cap = per ? (int64_t)(((size_t)8 << 30) / per) : want;
+	if (want > cap) want = (int32_t)cap;
+	if (want <= p->n_slots64) return kHaSuccess;
+	double *w = NULL;
+	if (hipMalloc((void **)&w, (size_t)want * per) != hipSuccess) {
+		(void)hipGetLastError();
+		return kHaSuccess;
+	}
+	if (hipStreamSynchronize(g.stream) != hipSuccess) {
+		hipFree(w);
+		return kHaErrorInternal;
+	}
+	if (p->gexec) { hipGraphExecDestroy(p->gexec); p->gexec = NULL; }
+	hipFree(p->work64);
+	p->work64 = w;
+	p->n_slots64 = want;
+	return kHaSuccess;
+}
+
 int ha_plan_run(HaHipPlan *p, bool isReflection, bool isScaling, bool zero_data, int method_in) {
 	int rc = kHaSuccess;
 	uint64_t *clk = NULL;
@@ -990,6 +1016,10 @@ int ha_plan_run(HaHipPlan *p, bool isReflection, bool isScaling, bool zero_data,
 	const bool is_polar = isReflection && (method == kHaHipMethodPolar || method == kHaHipMethodPolarLearned);
 	if (is_polar) {
 		rc = ensure_teams(p);
+		if (rc) return rc;
+	}
+	if (!isReflection || method == kHaHipMethodJacobi64) {
+		rc = ensure_work64(p, 4 * g.n_cu);
 		if (rc) return rc;
 	}
 	if (is_polar && (!p->tuned || p->tuned_opt != g.opt.tile || p->tuned_team != g.opt.team ||
