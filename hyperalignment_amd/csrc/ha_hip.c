@@ -221,6 +221,7 @@ struct HaHipPlan {
 	float *work; int64_t slot_stride; int32_t n_slots, n_slots_polar;
 	float *sl_par;
 	double *work64; int64_t stride64; int32_t n_slots64;
+	int32_t *iota;   /* 0, 1, ..., kpmax - 1 (the reflection fix's A^T A product) */
 	double *work_ridge; int32_t n_wg_ridge;  /* ridge: 2 x stride64 doubles per workgroup */
 	void *grp; int32_t n_teams;               /* polar teams (128 x 128 tiles): state per team */
 	int32_t *flag, *steps, *queues, *order;
@@ -250,7 +251,7 @@ static void plan_free(HaHipPlan *p) {
 	                p->qld, p->qoff, p->q, p->q64, p->X, p->Y, p->X64, p->Y64, p->Xs, p->Ys, p->stage, p->work, p->work64,
 	                p->flag, p->steps, p->queues, p->order, p->sigma_sum, p->var_sum, p->scale,
 	                p->features, p->pred, p->trace, p->sig, p->d_which, p->sl_par, p->tile_order[0],
-	                p->tile_order[1], p->tile_order[2], p->work_ridge, p->grp};
+	                p->tile_order[1], p->tile_order[2], p->work_ridge, p->grp, p->iota};
 	for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i)
 		if (ptrs[i]) hipFree(ptrs[i]);
 	if (p->gexec) hipGraphExecDestroy(p->gexec);
@@ -492,6 +493,15 @@ int ha_plan_create_ex(int32_t nt, const HaPlanSetup *su, const int32_t *slx_idx,
 	if (p->n_slots64 > count) p->n_slots64 = count;
 	if (p->n_slots64 > 128) p->n_slots64 = 128;
 	ACK(p->work64, (size_t)p->n_slots64 * 4 * p->stride64 * sizeof(double));
+	ACK(p->iota, (size_t)p->kpmax * sizeof(int32_t));
+	{
+		int32_t *h_iota = (int32_t *)malloc((size_t)p->kpmax * sizeof(int32_t));
+		if (!h_iota) { rc = kHaErrorAlloc; goto fail; }
+		for (int32_t i = 0; i < p->kpmax; ++i) h_iota[i] = i;
+		const hipError_t e_ = hipMemcpy(p->iota, h_iota, (size_t)p->kpmax * sizeof(int32_t), hipMemcpyHostToDevice);
+		free(h_iota);
+		if (e_ != hipSuccess) { rc = kHaErrorInternal; goto fail; }
+	}
 	ACK(p->X, (size_t)p->ld * p->nv_x * sizeof(float));
 	ACK(p->Y, (size_t)p->ld * p->nv_y * sizeof(float));
 	ACK(p->Xs, (size_t)2 * p->ld * p->nv_x * sizeof(uint16_t));
@@ -884,9 +894,11 @@ static int enqueue_pass(HaHipPlan *p, HakLocalArgs a, int method, bool isReflect
 		KCK(hak_weights(p->slx, p->dist, p->radius, p->total, optr, occ, p->nv_x, p->wsum, p->w, st));
 	}
 	if (timed) HCK(hipEventRecord(p->ev[1], st));
-	/* local transforms */
+	/* local transforms; without reflections the polar path fixes det(Q) < 0
+	 * afterwards (k_detfix), the FP32 Jacobi path leaves it to the FP64 one */
 	const bool det_fix = !isReflection;
-	if (det_fix || method == kHaHipMethodJacobi64) {
+	const bool polar_m = method == kHaHipMethodPolar || method == kHaHipMethodPolarLearned;
+	if (method == kHaHipMethodJacobi64 || (det_fix && !polar_m)) {
 		a.flag = NULL;
 		a.queue = p->queues + 1;
 		KCK(hak_jacobi64(&a, det_fix ? 1 : 0, p->work64, p->stride64, p->n_slots64, p->count, st));
@@ -924,11 +936,16 @@ static int enqueue_pass(HaHipPlan *p, HakLocalArgs a, int method, bool isReflect
 			}
 		}
 	}
+	if (det_fix && polar_m) {
+		HakLocalArgs b = a;
+		b.queue = p->queues + 0;
+		KCK(hak_detfix(&b, p->work64, p->stride64, p->iota, p->n_slots64, st));
+	}
 	if (timed) HCK(hipEventRecord(p->ev[2], st));
-	if (!det_fix && method != kHaHipMethodJacobi64 && g.opt.fp64_fallback) {
+	if (method != kHaHipMethodJacobi64 && (polar_m || !det_fix) && g.opt.fp64_fallback) {
 		HakLocalArgs b = a;
 		b.queue = p->queues + 2;
-		KCK(hak_jacobi64(&b, 0, p->work64, p->stride64, p->n_slots64, p->count, st));
+		KCK(hak_jacobi64(&b, det_fix ? 1 : 0, p->work64, p->stride64, p->n_slots64, p->count, st));
 	}
 	if (timed) HCK(hipEventRecord(p->ev[3], st));
 	if (isScaling) {
@@ -1013,7 +1030,7 @@ int ha_plan_run(HaHipPlan *p, bool isReflection, bool isScaling, bool zero_data,
 	const int method = resolve_method(p, method_in);
 	if (method < 0) return kHaErrorArg;
 	hipStream_t st = g.stream;
-	const bool is_polar = isReflection && (method == kHaHipMethodPolar || method == kHaHipMethodPolarLearned);
+	const bool is_polar = method == kHaHipMethodPolar || method == kHaHipMethodPolarLearned;
 	if (is_polar) {
 		rc = ensure_teams(p);
 		if (rc) return rc;

@@ -151,6 +151,14 @@ int hak_jacobi32(const HakLocalArgs *a, void *stream);
 int hak_jacobi64(const HakLocalArgs *a, int want_det_fix, double *work64, int64_t stride64,
                  int32_t n_slots, int32_t count, void *stream);
 
+/* Reflection fix after the polar iteration (k_detfix.hip): for every
+ * searchlight of a->order with flag 0 whose Q has det < 0, Q <- Q - 2 (Q v) v^T
+ * with v the smallest right singular vector of the FP64 A (a->sigma_sum, when
+ * set, loses 2 sigma_min); a failed Cholesky sets the flag for the FP64 path.
+ * work64: n_wg * 4 * stride64 doubles; iota: 0, 1, ..., kpmax - 1. */
+int hak_detfix(const HakLocalArgs *a, double *work64, int64_t stride64, const int32_t *iota, int n_wg,
+               void *stream);
+
 /* ---- spectral-scaling MLP (k_mlp.hip) ---- */
 /* Y[N][2] = MLP(X[N][12]) with the given parameters and input normalisation. */
 int hak_mlp_forward(const float *params, const float *norm, const float *X, int64_t N, float *Y, void *stream);

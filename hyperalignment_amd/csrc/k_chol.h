@@ -1,0 +1,200 @@
+// k_chol.h -- FP64 building blocks on 32 x 32 LDS tiles for one 256-thread
+// workgroup: gathered cross-products over the time points, a blocked
+// Cholesky factorization, and blocked triangular solves with kp right-hand
+// sides (kp a multiple of 32). Used by the searchlight ridge (k_ridge.hip)
+// and the reflection fix of the polar path (k_detfix.hip).
+#ifndef K_CHOL_H
+#define K_CHOL_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <math.h>
+
+namespace kchol {
+
+constexpr int RT = 256;  // threads per workgroup
+constexpr int B = 32;    // block size
+
+// out[m + n kp] = sum_t P[cols_p[m]] Q[cols_q[n]] over t < nt (m, n < k; zero
+// elsewhere) + diag on m == n < k and 1 on m == n >= k.
+// lower: only the tiles on and below the diagonal (the Cholesky reads no others).
+static __device__ void crossprod64(const double *__restrict__ P, const int32_t *cols_p, const double *__restrict__ Q,
+                            const int32_t *cols_q, int nt, int k, int kp, double diag, double pad_diag, double *out,
+                            bool lower, double (*xs)[B + 1], double (*ys)[B + 1]) {
+  const int mi = threadIdx.x & 31, ni = threadIdx.x >> 5;  // ni in 0..7
+  for (int tm = 0; tm < kp; tm += B)
+    for (int tn = 0; tn < (lower ? tm + B : kp); tn += B) {
+      double acc[4] = {0.0, 0.0, 0.0, 0.0};
+      for (int t0 = 0; t0 < nt; t0 += B) {
+        for (int e = threadIdx.x; e < B * B; e += RT) {
+          const int c = e >> 5, t = t0 + (e & 31);
+          const int gm = tm + c, gn = tn + c;
+          xs[c][e & 31] = (gm < k && t < nt) ? P[(int64_t)cols_p[gm] * nt + t] : 0.0;
+          ys[c][e & 31] = (gn < k && t < nt) ? Q[(int64_t)cols_q[gn] * nt + t] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int t = 0; t < B; ++t) {
+          const double xv = xs[mi][t];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) acc[q] += xv * ys[ni + 8 * q][t];
+        }
+        __syncthreads();
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int m = tm + mi, n = tn + ni + 8 * q;
+        double v = acc[q];
+        if (m == n) v += (m < k) ? diag : pad_diag;
+        out[m + (int64_t)n * kp] = v;
+      }
+    }
+  __syncthreads();
+}
+
+// dst(r, n) -= sum_c A(r, c) Bm(c, n) for a 32 x 32 tile; A(r, c) = a[r + c * lda]
+// (or a[c + r * lda] when ta), Bm(c, n) = b[c + n * ldb] (or b[n + c * ldb] when tb).
+static __device__ __forceinline__ void tile_sub(double *dst, int ldd, const double *a, int lda, bool ta, const double *b,
+                                         int ldb, bool tb, int kc, double (*as)[B + 1], double (*bs)[B + 1]) {
+  for (int e = threadIdx.x; e < B * B; e += RT) {
+    const int i = e & 31, c = e >> 5;
+    as[i][c] = c < kc ? (ta ? a[c + (int64_t)i * lda] : a[i + (int64_t)c * lda]) : 0.0;
+    bs[c][i] = c < kc ? (tb ? b[i + (int64_t)c * ldb] : b[c + (int64_t)i * ldb]) : 0.0;
+  }
+  __syncthreads();
+  const int ri = threadIdx.x & 31, nj = threadIdx.x >> 5;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int n = nj + 8 * q;
+    double s = 0.0;
+#pragma unroll 8
+    for (int c = 0; c < B; ++c) s += as[ri][c] * bs[c][n];
+    dst[ri + (int64_t)n * ldd] -= s;
+  }
+  __syncthreads();
+}
+
+// cs(:, n) <- D^-1 cs(:, n) for the 32 right-hand sides n, D = d lower
+// triangular (one thread per right-hand side).
+static __device__ __forceinline__ void forward_sub(double (*d)[B + 1], double (*cs)[B + 1]) {
+  if (threadIdx.x < B) {
+    const int n = threadIdx.x;
+    for (int i = 0; i < B; ++i) {
+      double v = cs[i][n];
+      for (int q = 0; q < i; ++q) v -= d[i][q] * cs[q][n];
+      cs[i][n] = v / d[i][i];
+    }
+  }
+  __syncthreads();
+}
+
+// cs(:, n) <- D^-T cs(:, n), D = d lower triangular.
+static __device__ __forceinline__ void backward_sub_t(double (*d)[B + 1], double (*cs)[B + 1]) {
+  if (threadIdx.x < B) {
+    const int n = threadIdx.x;
+    for (int i = B - 1; i >= 0; --i) {
+      double v = cs[i][n];
+      for (int q = i + 1; q < B; ++q) v -= d[q][i] * cs[q][n];  // D^T(i, q) = D(q, i)
+      cs[i][n] = v / d[i][i];
+    }
+  }
+  __syncthreads();
+}
+
+// In-place lower Cholesky of the kp x kp block G (kp a multiple of 32). Returns
+// false if a pivot is not positive (G not positive definite).
+static __device__ bool cholesky64(double *G, int kp, double (*d)[B + 1], double (*cs)[B + 1], double (*as)[B + 1],
+                           double (*bs)[B + 1], int32_t *ok_sh) {
+  if (threadIdx.x == 0) *ok_sh = 1;
+  for (int j0 = 0; j0 < kp; j0 += B) {
+    // diagonal block
+    for (int e = threadIdx.x; e < B * B; e += RT) d[e & 31][e >> 5] = G[(j0 + (e & 31)) + (int64_t)(j0 + (e >> 5)) * kp];
+    __syncthreads();
+    for (int c = 0; c < B; ++c) {
+      if (threadIdx.x == 0) {
+        const double p = d[c][c];
+        if (!(p > 0.0)) *ok_sh = 0;
+        d[c][c] = sqrt(p > 0.0 ? p : 1.0);
+      }
+      __syncthreads();
+      for (int r = c + 1 + threadIdx.x; r < B; r += RT) d[r][c] /= d[c][c];
+      __syncthreads();
+      const int n = B - c - 1;
+      for (int e = threadIdx.x; e < n * n; e += RT) {
+        const int r = c + 1 + e % n, s = c + 1 + e / n;
+        if (s <= r) d[r][s] -= d[r][c] * d[s][c];
+      }
+      __syncthreads();
+    }
+    for (int e = threadIdx.x; e < B * B; e += RT) {
+      const int r = e & 31, s = e >> 5;
+      G[(j0 + r) + (int64_t)(j0 + s) * kp] = s <= r ? d[r][s] : 0.0;
+    }
+    __syncthreads();
+    // panel below: L21 = A21 L11^-T, i.e. L11 L21(r, :)^T = A21(r, :)^T for each
+    // row r, solved 32 rows at a time with the rows as right-hand sides in LDS
+    for (int r0 = j0 + B; r0 < kp; r0 += B) {
+      for (int e = threadIdx.x; e < B * B; e += RT) cs[e >> 5][e & 31] = G[(r0 + (e & 31)) + (int64_t)(j0 + (e >> 5)) * kp];
+      __syncthreads();
+      forward_sub(d, cs);
+      for (int e = threadIdx.x; e < B * B; e += RT) G[(r0 + (e & 31)) + (int64_t)(j0 + (e >> 5)) * kp] = cs[e >> 5][e & 31];
+      __syncthreads();
+    }
+    // trailing update of the lower part: A22 -= L21 L21^T, 32 x 32 tiles
+    for (int s0 = j0 + B; s0 < kp; s0 += B)
+      for (int r0 = s0; r0 < kp; r0 += B)
+        tile_sub(G + r0 + (int64_t)s0 * kp, kp, G + r0 + (int64_t)j0 * kp, kp, false, G + s0 + (int64_t)j0 * kp, kp,
+                 true, B, as, bs);
+  }
+  __syncthreads();
+  return *ok_sh != 0;
+}
+
+// C <- L^-1 C (lower L, kp x kp; C kp x kp), blocked forward substitution.
+static __device__ void trsm_lower64(const double *L, double *C, int kp, double (*d)[B + 1], double (*cs)[B + 1],
+                             double (*as)[B + 1], double (*bs)[B + 1]) {
+  for (int j0 = 0; j0 < kp; j0 += B) {
+    for (int e = threadIdx.x; e < B * B; e += RT) d[e & 31][e >> 5] = L[(j0 + (e & 31)) + (int64_t)(j0 + (e >> 5)) * kp];
+    __syncthreads();
+    // the diagonal block for every column of C, 32 columns at a time through LDS
+    for (int n0 = 0; n0 < kp; n0 += B) {
+      for (int e = threadIdx.x; e < B * B; e += RT) cs[e & 31][e >> 5] = C[(j0 + (e & 31)) + (int64_t)(n0 + (e >> 5)) * kp];
+      __syncthreads();
+      forward_sub(d, cs);
+      for (int e = threadIdx.x; e < B * B; e += RT) C[(j0 + (e & 31)) + (int64_t)(n0 + (e >> 5)) * kp] = cs[e & 31][e >> 5];
+      __syncthreads();
+    }
+    // rows below: C(r, :) -= L(r, j0..j0+31) C(j0..j0+31, :)
+    for (int r0 = j0 + B; r0 < kp; r0 += B)
+      for (int n0 = 0; n0 < kp; n0 += B)
+        tile_sub(C + r0 + (int64_t)n0 * kp, kp, L + r0 + (int64_t)j0 * kp, kp, false, C + j0 + (int64_t)n0 * kp, kp,
+                 false, B, as, bs);
+  }
+  __syncthreads();
+}
+
+// C <- L^-T C, blocked backward substitution.
+static __device__ void trsm_lower_t64(const double *L, double *C, int kp, double (*d)[B + 1], double (*cs)[B + 1],
+                               double (*as)[B + 1], double (*bs)[B + 1]) {
+  for (int j0 = kp - B; j0 >= 0; j0 -= B) {
+    for (int e = threadIdx.x; e < B * B; e += RT) d[e & 31][e >> 5] = L[(j0 + (e & 31)) + (int64_t)(j0 + (e >> 5)) * kp];
+    __syncthreads();
+    for (int n0 = 0; n0 < kp; n0 += B) {
+      for (int e = threadIdx.x; e < B * B; e += RT) cs[e & 31][e >> 5] = C[(j0 + (e & 31)) + (int64_t)(n0 + (e >> 5)) * kp];
+      __syncthreads();
+      backward_sub_t(d, cs);
+      for (int e = threadIdx.x; e < B * B; e += RT) C[(j0 + (e & 31)) + (int64_t)(n0 + (e >> 5)) * kp] = cs[e & 31][e >> 5];
+      __syncthreads();
+    }
+    // rows above: C(r, :) -= L^T(r, j0..j0+31) C(j0..j0+31, :), L^T(r, c) = L(c, r)
+    for (int r0 = 0; r0 < j0; r0 += B)
+      for (int n0 = 0; n0 < kp; n0 += B)
+        tile_sub(C + r0 + (int64_t)n0 * kp, kp, L + j0 + (int64_t)r0 * kp, kp, true, C + j0 + (int64_t)n0 * kp, kp,
+                 false, B, as, bs);
+  }
+  __syncthreads();
+}
+
+}  // namespace kchol
+
+#endif  // K_CHOL_H

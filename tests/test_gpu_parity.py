@@ -145,13 +145,45 @@ def test_dense_accumulates(c0):
     assert np.abs(T - ref).max() < TOL_64
 
 
-def test_reflection_false_c0():
+@pytest.mark.parametrize("method,tol", [("polar", TOL_T), ("jacobi64", TOL_64)])
+@pytest.mark.parametrize("scaling", [False, True])
+def test_reflection_false_c0(method, tol, scaling):
+    """isReflection = false: the polar path fixes det(Q) < 0 after the iteration
+    (LU sign of Q, smallest right singular vector of the FP64 A, rank-1 flip),
+    the FP64 path inside its Jacobi SVD; both against LAPACK's dgesdd + LU sign."""
     w = synth.make_workload("c0")
     s = w.sls
-    ha.set_options(method="polar")
-    T = ha.searchlight_procrustes_flat(w.X, w.Y, s.indices, s.offsets, s.dists, s.radius, reflection=False)
-    ref = O.searchlight_procrustes_dense(w.X, w.Y, s.indices, s.offsets, s.dists, s.radius, reflection=False)
-    assert np.abs(T - ref).max() < TOL_64
+    ha.set_options(method=method)
+    T = ha.searchlight_procrustes_flat(w.X, w.Y, s.indices, s.offsets, s.dists, s.radius, reflection=False,
+                                       scaling=scaling)
+    ref = O.searchlight_procrustes_dense(w.X, w.Y, s.indices, s.offsets, s.dists, s.radius, reflection=False,
+                                         scaling=scaling)
+    assert np.abs(T - ref).max() < tol
+
+
+def test_reflection_false_c1_local():
+    """The polar path's reflection fix at kp 96-128: every local transform has
+    det = +1 and matches the FP64 path's to the polar bar."""
+    w = synth.make_workload("c1")
+    which = np.arange(0, w.sls.count, 61)[:96]
+    from hyperalignment_amd import learn
+    idx, off, d = learn.sub_searchlights(w.sls, which)
+    out = {}
+    for method in ("polar", "jacobi64"):
+        plan = ha.Plan(w.X.shape[0], w.sls.nv, idx, off, d, w.sls.radius)
+        plan.load(w.X, w.Y)
+        ha.set_options(method=method)
+        plan.execute(reflection=False)
+        out[method] = plan.local(np.arange(len(which)))
+        st, fl = plan.steps()
+        if method == "polar":
+            assert not fl.any()
+        plan.close()
+    for Tp, Tj in zip(out["polar"], out["jacobi64"]):
+        dp = np.linalg.det(Tp.astype(np.float64))
+        assert abs(dp - 1.0) < 1e-3
+        assert np.abs(Tp - Tj).max() < TOL_T
+    print(f"[refl] {len(which)} searchlights checked", flush=True)
 
 
 @pytest.mark.parametrize("method", ["jacobi32", "polar"])
