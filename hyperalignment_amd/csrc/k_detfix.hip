@@ -43,56 +43,100 @@ __device__ __forceinline__ double wg_sum(double v, double *red) {
 }
 
 // sign of det of the k x k block of M (FP32, column-major, leading dim kp),
-// destroyed: right-looking LU with partial pivoting, the lowest row index on
-// ties; 0 for an exactly singular block.
-__device__ int lu_sign(float *M, int k, int kp, float *pmax, int *pidx, int *piv_sh, int *sign_sh) {
+// destroyed: blocked right-looking LU with partial pivoting (the lowest row
+// index on ties), 32-column panels: each panel factored in place with whole
+// rows swapped as pivots are chosen, the U block beside it solved one column
+// per thread, and the trailing matrix updated once per panel in 32 x 32 LDS
+// tiles. 0 for an exactly singular block.
+__device__ int lu_sign(float *M, int k, int kp, float *pmax, int *pidx, int *piv_sh, int *sign_sh,
+                       float (*ls)[B + 1], float (*us)[B + 1]) {
   if (threadIdx.x == 0) *sign_sh = 1;
   __syncthreads();
-  for (int c = 0; c < k; ++c) {
-    float best = -1.f;
-    int bi = c;
-    for (int r = c + threadIdx.x; r < k; r += RT) {
-      const float v = fabsf(M[r + (int64_t)c * kp]);
-      if (v > best) { best = v; bi = r; }
+  for (int j0 = 0; j0 < k; j0 += B) {
+    const int nb = k - j0 < B ? k - j0 : B, je = j0 + nb;
+    // panel: columns j0 .. je-1, rows j0 .. k-1
+    for (int c = j0; c < je; ++c) {
+      float best = -1.f;
+      int bi = c;
+      for (int r = c + threadIdx.x; r < k; r += RT) {
+        const float v = fabsf(M[r + (int64_t)c * kp]);
+        if (v > best) { best = v; bi = r; }
+      }
+      for (int o = 32; o > 0; o >>= 1) {
+        const float ob = __shfl_xor(best, o);
+        const int oi = __shfl_xor(bi, o);
+        if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+      }
+      const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+      if (lane == 0) { pmax[wave] = best; pidx[wave] = bi; }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        float b0 = pmax[0];
+        int i0 = pidx[0];
+        for (int q = 1; q < 4; ++q)
+          if (pmax[q] > b0 || (pmax[q] == b0 && pidx[q] < i0)) { b0 = pmax[q]; i0 = pidx[q]; }
+        *piv_sh = i0;
+        if (i0 != c) *sign_sh = -*sign_sh;
+      }
+      __syncthreads();
+      const int pr = *piv_sh;
+      if (pr != c)
+        for (int j = threadIdx.x; j < k; j += RT) {
+          const float t = M[c + (int64_t)j * kp];
+          M[c + (int64_t)j * kp] = M[pr + (int64_t)j * kp];
+          M[pr + (int64_t)j * kp] = t;
+        }
+      __syncthreads();
+      const float d = M[c + (int64_t)c * kp];
+      if (d == 0.f) {
+        if (threadIdx.x == 0) *sign_sh = 0;
+        __syncthreads();
+        return 0;
+      }
+      if (threadIdx.x == 0 && d < 0.f) *sign_sh = -*sign_sh;
+      // multipliers below the pivot, then the rest of the panel
+      for (int r = c + 1 + threadIdx.x; r < k; r += RT) M[r + (int64_t)c * kp] /= d;
+      __syncthreads();
+      const int nr = k - c - 1, ncol = je - c - 1;
+      for (int64_t e = threadIdx.x; e < (int64_t)nr * ncol; e += RT) {
+        const int r = c + 1 + (int)(e % nr), j = c + 1 + (int)(e / nr);
+        M[r + (int64_t)j * kp] -= M[r + (int64_t)c * kp] * M[c + (int64_t)j * kp];
+      }
+      __syncthreads();
     }
-    for (int o = 32; o > 0; o >>= 1) {
-      const float ob = __shfl_xor(best, o);
-      const int oi = __shfl_xor(bi, o);
-      if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
-    }
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0) { pmax[wave] = best; pidx[wave] = bi; }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      float b0 = pmax[0];
-      int i0 = pidx[0];
-      for (int q = 1; q < 4; ++q)
-        if (pmax[q] > b0 || (pmax[q] == b0 && pidx[q] < i0)) { b0 = pmax[q]; i0 = pidx[q]; }
-      *piv_sh = i0;
-      if (i0 != c) *sign_sh = -*sign_sh;
-    }
-    __syncthreads();
-    const int pr = *piv_sh;
-    if (pr != c)
-      for (int j = threadIdx.x; j < k; j += RT) {
-        const float t = M[c + (int64_t)j * kp];
-        M[c + (int64_t)j * kp] = M[pr + (int64_t)j * kp];
-        M[pr + (int64_t)j * kp] = t;
+    if (je >= k) break;
+    // U block: rows j0 .. je-1, columns je .. k-1, unit lower L11 (one column per thread)
+    for (int j = je + threadIdx.x; j < k; j += RT)
+      for (int i = 1; i < nb; ++i) {
+        float v = M[(j0 + i) + (int64_t)j * kp];
+        for (int p = 0; p < i; ++p) v -= M[(j0 + i) + (int64_t)(j0 + p) * kp] * M[(j0 + p) + (int64_t)j * kp];
+        M[(j0 + i) + (int64_t)j * kp] = v;
       }
     __syncthreads();
-    const float d = M[c + (int64_t)c * kp];
-    if (d == 0.f) {
-      if (threadIdx.x == 0) *sign_sh = 0;
-      __syncthreads();
-      return 0;
-    }
-    if (threadIdx.x == 0 && d < 0.f) *sign_sh = -*sign_sh;
-    const int n = k - c - 1;
-    for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += RT) {
-      const int r = c + 1 + (int)(e % n), j = c + 1 + (int)(e / n);
-      M[r + (int64_t)j * kp] -= M[r + (int64_t)c * kp] / d * M[c + (int64_t)j * kp];
-    }
-    __syncthreads();
+    // trailing matrix: M(r, j) -= sum_p L(r, j0 + p) U(j0 + p, j), 32 x 32 tiles
+    const int ri = threadIdx.x & 31, nj = threadIdx.x >> 5;
+    for (int r0 = je; r0 < k; r0 += B)
+      for (int c0 = je; c0 < k; c0 += B) {
+        for (int e = threadIdx.x; e < B * B; e += RT) {
+          const int a = e & 31, b = e >> 5;  // row / column inside the tile, panel index
+          ls[a][b] = (r0 + a < k && b < nb) ? M[(r0 + a) + (int64_t)(j0 + b) * kp] : 0.f;
+          us[b][a] = (c0 + a < k && b < nb) ? M[(j0 + b) + (int64_t)(c0 + a) * kp] : 0.f;
+        }
+        __syncthreads();
+        if (r0 + ri < k) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int j = c0 + nj + 8 * q;
+            if (j < k) {
+              float acc = 0.f;
+#pragma unroll 8
+              for (int p = 0; p < B; ++p) acc += ls[ri][p] * us[p][nj + 8 * q];
+              M[(r0 + ri) + (int64_t)j * kp] -= acc;
+            }
+          }
+        }
+        __syncthreads();
+      }
   }
   const int sgn = *sign_sh;
   __syncthreads();
@@ -224,7 +268,8 @@ __global__ __launch_bounds__(RT, 4) void k_detfix(HakLocalArgs a, double *work64
     float *Q = a.q + a.qoff[s];
     for (int64_t e = threadIdx.x; e < (int64_t)kp * kp; e += RT) M[e] = Q[e];
     __syncthreads();
-    const int sgn = lu_sign(M, k, kp, pmax, pidx, &piv_sh, &sign_sh);
+    const int sgn = lu_sign(M, k, kp, pmax, pidx, &piv_sh, &sign_sh, reinterpret_cast<float (*)[B + 1]>(&t0[0][0]),
+                            reinterpret_cast<float (*)[B + 1]>(&t1[0][0]));
     if (sgn >= 0) continue;
     // G = A64^T A64 (lower part; identity on the padding) and its Cholesky factor
     crossprod64(a.X64, a.slx_idx + o, a.Y64, a.sly_idx + o, a.nt, k, kp, 0.0, 1.0, A64, false, t0, t1);
