@@ -71,8 +71,9 @@ def cpu_baseline(w, seconds, threads):
 def comparator(w, args, every):
     """The plain paths the learned one is measured against, on the same inputs:
     the plain polar iteration (fixed scale ||A||_F and the 1e-7 schedule) over
-    the whole workload, and the FP32 one-sided Jacobi SVD path on every
-    `every`-th searchlight, its rate extrapolated per searchlight."""
+    the whole workload, the default method without reflections, and the FP32
+    one-sided Jacobi SVD path on every `every`-th searchlight, its rate
+    extrapolated per searchlight."""
     from hyperalignment_amd import learn
     s = w.sls
     nt = w.X.shape[0]
@@ -90,6 +91,19 @@ def comparator(w, args, every):
                           "fp64_fallback": st["n_fallback"]}
     plan.close()
     print(f"[bench] plain polar {1e3 * dt:.1f} ms", file=sys.stderr, flush=True)
+    # the default method without reflections (isReflection = false): the polar
+    # pass plus the det(Q) < 0 fix
+    ha.set_options(method=args.method)
+    plan = ha.Plan(nt, s.nv, s.indices, s.offsets, s.dists, s.radius)
+    plan.load(w.X, w.Y)
+    plan.execute(reflection=False)
+    t0 = time.perf_counter()
+    plan.execute(reflection=False)
+    dt = time.perf_counter() - t0
+    st = plan.stats()
+    res["no_reflection"] = {"value": s.count / dt, "hemisphere_s": dt, "fp64_fallback": st["n_fallback"]}
+    plan.close()
+    print(f"[bench] without reflections {1e3 * dt:.1f} ms", file=sys.stderr, flush=True)
     which = np.arange(0, s.count, every)
     idx, off, d = learn.sub_searchlights(s, which)
     ha.set_options(method="jacobi32")
