@@ -15,16 +15,38 @@ namespace kchol {
 constexpr int RT = 256;  // threads per workgroup
 constexpr int B = 32;    // block size
 
+typedef double f64x4_t __attribute__((ext_vector_type(4)));
+
+// 32 x 32 product on v_mfma_f64_16x16x4f64: wave w takes the 16 x 16 quadrant
+// (rows 16 (w >> 1), columns 16 (w & 1)); lane l supplies A(l % 16, l / 16) and
+// B(l / 16, l % 16) of each K = 4 step and holds D(l / 16 + 4 r, l % 16), r < 4.
+// acc += sum_c X(i, c) Y(c, j) over c < 32 with X(i, c) = xa[i][c] and
+// Y(c, j) = yb(c, j) (yb(c, j) = yt[j][c] when ytr, else yt[c][j]).
+static __device__ __forceinline__ f64x4_t mfma32(const double (*xa)[B + 1], const double (*yt)[B + 1], bool ytr,
+                                          f64x4_t acc) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = (wave >> 1) * 16, wn = (wave & 1) * 16;
+  const int i = lane & 15, kq = lane >> 4;
+#pragma unroll
+  for (int kk = 0; kk < B; kk += 4) {
+    const double av = xa[wm + i][kk + kq];
+    const double bv = ytr ? yt[wn + i][kk + kq] : yt[kk + kq][wn + i];
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+  }
+  return acc;
+}
+
 // out[m + n kp] = sum_t P[cols_p[m]] Q[cols_q[n]] over t < nt (m, n < k; zero
 // elsewhere) + diag on m == n < k and 1 on m == n >= k.
 // lower: only the tiles on and below the diagonal (the Cholesky reads no others).
 static __device__ void crossprod64(const double *__restrict__ P, const int32_t *cols_p, const double *__restrict__ Q,
                             const int32_t *cols_q, int nt, int k, int kp, double diag, double pad_diag, double *out,
                             bool lower, double (*xs)[B + 1], double (*ys)[B + 1]) {
-  const int mi = threadIdx.x & 31, ni = threadIdx.x >> 5;  // ni in 0..7
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = (wave >> 1) * 16, wn = (wave & 1) * 16;
   for (int tm = 0; tm < kp; tm += B)
     for (int tn = 0; tn < (lower ? tm + B : kp); tn += B) {
-      double acc[4] = {0.0, 0.0, 0.0, 0.0};
+      f64x4_t acc = {0.0, 0.0, 0.0, 0.0};
       for (int t0 = 0; t0 < nt; t0 += B) {
         for (int e = threadIdx.x; e < B * B; e += RT) {
           const int c = e >> 5, t = t0 + (e & 31);
@@ -33,18 +55,13 @@ static __device__ void crossprod64(const double *__restrict__ P, const int32_t *
           ys[c][e & 31] = (gn < k && t < nt) ? Q[(int64_t)cols_q[gn] * nt + t] : 0.0;
         }
         __syncthreads();
-#pragma unroll 4
-        for (int t = 0; t < B; ++t) {
-          const double xv = xs[mi][t];
-#pragma unroll
-          for (int q = 0; q < 4; ++q) acc[q] += xv * ys[ni + 8 * q][t];
-        }
+        acc = mfma32(xs, ys, true, acc);  // Y(t, n) = ys[n][t]
         __syncthreads();
       }
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int m = tm + mi, n = tn + ni + 8 * q;
-        double v = acc[q];
+      for (int r = 0; r < 4; ++r) {
+        const int m = tm + wm + (lane >> 4) + 4 * r, n = tn + wn + (lane & 15);
+        double v = acc[r];
         if (m == n) v += (m < k) ? diag : pad_diag;
         out[m + (int64_t)n * kp] = v;
       }
@@ -62,14 +79,14 @@ static __device__ __forceinline__ void tile_sub(double *dst, int ldd, const doub
     bs[c][i] = c < kc ? (tb ? b[i + (int64_t)c * ldb] : b[c + (int64_t)i * ldb]) : 0.0;
   }
   __syncthreads();
-  const int ri = threadIdx.x & 31, nj = threadIdx.x >> 5;
+  f64x4_t acc = {0.0, 0.0, 0.0, 0.0};
+  acc = mfma32(as, bs, false, acc);  // Bm(c, n) = bs[c][n]
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = (wave >> 1) * 16, wn = (wave & 1) * 16;
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const int n = nj + 8 * q;
-    double s = 0.0;
-#pragma unroll 8
-    for (int c = 0; c < B; ++c) s += as[ri][c] * bs[c][n];
-    dst[ri + (int64_t)n * ldd] -= s;
+  for (int r = 0; r < 4; ++r) {
+    const int i = wm + (lane >> 4) + 4 * r, n = wn + (lane & 15);
+    dst[i + (int64_t)n * ldd] -= acc[r];
   }
   __syncthreads();
 }
