@@ -47,14 +47,36 @@ static __device__ void crossprod64(const double *__restrict__ P, const int32_t *
   for (int tm = 0; tm < kp; tm += B)
     for (int tn = 0; tn < (lower ? tm + B : kp); tn += B) {
       f64x4_t acc = {0.0, 0.0, 0.0, 0.0};
+      // each thread stages the same four rows of each operand in every chunk:
+      // their offsets once per tile, and the next chunk's values loaded into
+      // registers while the current chunk's MFMAs run
+      constexpr int U = B * B / RT;
+      int64_t ox[U], oy[U];
+      double px[U], py[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int c = (threadIdx.x + u * RT) >> 5;
+        ox[u] = tm + c < k ? (int64_t)cols_p[tm + c] * nt : -1;
+        oy[u] = tn + c < k ? (int64_t)cols_q[tn + c] * nt : -1;
+      }
+      auto load = [&](int t0) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int t = t0 + ((threadIdx.x + u * RT) & 31);
+          px[u] = (ox[u] >= 0 && t < nt) ? P[ox[u] + t] : 0.0;
+          py[u] = (oy[u] >= 0 && t < nt) ? Q[oy[u] + t] : 0.0;
+        }
+      };
+      load(0);
       for (int t0 = 0; t0 < nt; t0 += B) {
-        for (int e = threadIdx.x; e < B * B; e += RT) {
-          const int c = e >> 5, t = t0 + (e & 31);
-          const int gm = tm + c, gn = tn + c;
-          xs[c][e & 31] = (gm < k && t < nt) ? P[(int64_t)cols_p[gm] * nt + t] : 0.0;
-          ys[c][e & 31] = (gn < k && t < nt) ? Q[(int64_t)cols_q[gn] * nt + t] : 0.0;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int e = threadIdx.x + u * RT;
+          xs[e >> 5][e & 31] = px[u];
+          ys[e >> 5][e & 31] = py[u];
         }
         __syncthreads();
+        if (t0 + B < nt) load(t0 + B);
         acc = mfma32(xs, ys, true, acc);  // Y(t, n) = ys[n][t]
         __syncthreads();
       }
