@@ -189,49 +189,66 @@ static __device__ bool cholesky64(double *G, int kp, double (*d)[B + 1], double 
   return *ok_sh != 0;
 }
 
-// C <- L^-1 C (lower L, kp x kp; C kp x kp), blocked forward substitution.
-static __device__ void trsm_lower64(const double *L, double *C, int kp, double (*d)[B + 1], double (*cs)[B + 1],
-                             double (*as)[B + 1], double (*bs)[B + 1]) {
-  for (int j0 = 0; j0 < kp; j0 += B) {
-    for (int e = threadIdx.x; e < B * B; e += RT) d[e & 31][e >> 5] = L[(j0 + (e & 31)) + (int64_t)(j0 + (e >> 5)) * kp];
-    __syncthreads();
-    // the diagonal block for every column of C, 32 columns at a time through LDS
-    for (int n0 = 0; n0 < kp; n0 += B) {
-      for (int e = threadIdx.x; e < B * B; e += RT) cs[e & 31][e >> 5] = C[(j0 + (e & 31)) + (int64_t)(n0 + (e >> 5)) * kp];
-      __syncthreads();
-      forward_sub(d, cs);
-      for (int e = threadIdx.x; e < B * B; e += RT) C[(j0 + (e & 31)) + (int64_t)(n0 + (e >> 5)) * kp] = cs[e & 31][e >> 5];
-      __syncthreads();
-    }
-    // rows below: C(r, :) -= L(r, j0..j0+31) C(j0..j0+31, :)
-    for (int r0 = j0 + B; r0 < kp; r0 += B)
-      for (int n0 = 0; n0 < kp; n0 += B)
-        tile_sub(C + r0 + (int64_t)n0 * kp, kp, L + r0 + (int64_t)j0 * kp, kp, false, C + j0 + (int64_t)n0 * kp, kp,
-                 false, B, as, bs);
-  }
-  __syncthreads();
-}
-
-// C <- L^-T C, blocked backward substitution.
-static __device__ void trsm_lower_t64(const double *L, double *C, int kp, double (*d)[B + 1], double (*cs)[B + 1],
+// C <- L^-1 C (trans false) or C <- L^-T C (trans true), L lower triangular
+// and C kp x kp (kp a multiple of 32), left-looking by 32 x 32 blocks: block
+// (r0, n0) of the solution is X = D^-1 (C(r0, n0) - sum_j S(j) X(j, n0)) with
+// D the diagonal block of L (D^T when trans), S(j) = L(r0, j) over the blocks j
+// before r0 (L(j, r0)^T over the blocks after it when trans). The sum runs on
+// MFMA tiles with the next block's operands loaded into registers while the
+// current one's MFMAs run; blocks go in the order that leaves every X(j, n0)
+// final before it is read, and X overwrites C.
+static __device__ void trsm_ll(const double *L, double *C, int kp, bool trans, double (*d)[B + 1], double (*cs)[B + 1],
                                double (*as)[B + 1], double (*bs)[B + 1]) {
-  for (int j0 = kp - B; j0 >= 0; j0 -= B) {
-    for (int e = threadIdx.x; e < B * B; e += RT) d[e & 31][e >> 5] = L[(j0 + (e & 31)) + (int64_t)(j0 + (e >> 5)) * kp];
-    __syncthreads();
+  constexpr int U = B * B / RT;
+  const int nb = kp / B;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = (wave >> 1) * 16, wn = (wave & 1) * 16;
+  for (int bi = 0; bi < nb; ++bi) {
+    const int r0 = (trans ? nb - 1 - bi : bi) * B;
+    for (int e = threadIdx.x; e < B * B; e += RT) d[e & 31][e >> 5] = L[(r0 + (e & 31)) + (int64_t)(r0 + (e >> 5)) * kp];
+    const int jbeg = trans ? r0 + B : 0, jend = trans ? kp : r0;
     for (int n0 = 0; n0 < kp; n0 += B) {
-      for (int e = threadIdx.x; e < B * B; e += RT) cs[e & 31][e >> 5] = C[(j0 + (e & 31)) + (int64_t)(n0 + (e >> 5)) * kp];
+      f64x4_t acc = {0.0, 0.0, 0.0, 0.0};
+      double pa[U], pb[U];
+      auto load = [&](int j0) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int e = threadIdx.x + u * RT, lo = e & 31, hi = e >> 5;
+          // A(i, c): L(r0 + i, j0 + c), or L(j0 + c, r0 + i) when trans; both read down a column
+          pa[u] = trans ? L[(j0 + lo) + (int64_t)(r0 + hi) * kp] : L[(r0 + lo) + (int64_t)(j0 + hi) * kp];
+          pb[u] = C[(j0 + lo) + (int64_t)(n0 + hi) * kp];  // B(c, n) = X(j0 + c, n0 + n)
+        }
+      };
+      if (jbeg < jend) load(jbeg);
+      for (int j0 = jbeg; j0 < jend; j0 += B) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int e = threadIdx.x + u * RT, lo = e & 31, hi = e >> 5;
+          if (trans)
+            as[hi][lo] = pa[u];  // (i, c) = (hi, lo)
+          else
+            as[lo][hi] = pa[u];  // (i, c) = (lo, hi)
+          bs[lo][hi] = pb[u];
+        }
+        __syncthreads();
+        if (j0 + B < jend) load(j0 + B);
+        acc = mfma32(as, bs, false, acc);
+        __syncthreads();
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = wm + (lane >> 4) + 4 * r, n = wn + (lane & 15);
+        cs[i][n] = C[(r0 + i) + (int64_t)(n0 + n) * kp] - acc[r];
+      }
       __syncthreads();
-      backward_sub_t(d, cs);
-      for (int e = threadIdx.x; e < B * B; e += RT) C[(j0 + (e & 31)) + (int64_t)(n0 + (e >> 5)) * kp] = cs[e & 31][e >> 5];
+      if (trans)
+        backward_sub_t(d, cs);
+      else
+        forward_sub(d, cs);
+      for (int e = threadIdx.x; e < B * B; e += RT) C[(r0 + (e & 31)) + (int64_t)(n0 + (e >> 5)) * kp] = cs[e & 31][e >> 5];
       __syncthreads();
     }
-    // rows above: C(r, :) -= L^T(r, j0..j0+31) C(j0..j0+31, :), L^T(r, c) = L(c, r)
-    for (int r0 = 0; r0 < j0; r0 += B)
-      for (int n0 = 0; n0 < kp; n0 += B)
-        tile_sub(C + r0 + (int64_t)n0 * kp, kp, L + j0 + (int64_t)r0 * kp, kp, true, C + j0 + (int64_t)n0 * kp, kp,
-                 false, B, as, bs);
   }
-  __syncthreads();
 }
 
 }  // namespace kchol

@@ -8,7 +8,7 @@
 // FP64 products over the time points), a blocked Cholesky G = L L^T in place
 // (32-column panels: unblocked factor of the diagonal block in LDS, the panel
 // below solved 32 rows at a time in LDS, 32 x 32-tiled update of the trailing
-// matrix), then C <- L^-T L^-1 C by blocked substitution. Padding rows and
+// matrix), then C <- L^-T L^-1 C by left-looking blocked substitution. Padding rows and
 // columns k..kp-1 carry an identity diagonal, so the padded system stays SPD
 // and its solution is zero there. The betas (k x k) go to the searchlight's
 // place in the Q store (FP32; FP64 too when a.q64 is set) for the
@@ -51,8 +51,8 @@ __global__ __launch_bounds__(RT, HAK_RIDGE_OCC) void k_ridge64(HakLocalArgs a, d
     crossprod64(a.X64, a.slx_idx + o, a.X64, a.slx_idx + o, a.nt, k, kp, alpha, 1.0, Gm, true, t0, t1);
     crossprod64(a.X64, a.slx_idx + o, a.Y64, a.sly_idx + o, a.nt, k, kp, 0.0, 0.0, Cm, false, t0, t1);
     const bool ok = cholesky64(Gm, kp, t0, t3, t1, t2, &ok_sh);
-    trsm_lower64(Gm, Cm, kp, t0, t3, t1, t2);
-    trsm_lower_t64(Gm, Cm, kp, t0, t3, t1, t2);
+    trsm_ll(Gm, Cm, kp, false, t0, t3, t1, t2);
+    trsm_ll(Gm, Cm, kp, true, t0, t3, t1, t2);
     float *Q = a.q + a.qoff[s];
     double *Q64 = a.q64 ? a.q64 + a.qoff[s] : nullptr;
     for (int64_t e = threadIdx.x; e < (int64_t)kp * kp; e += RT) {
