@@ -91,28 +91,6 @@ static __device__ void crossprod64(const double *__restrict__ P, const int32_t *
   __syncthreads();
 }
 
-// dst(r, n) -= sum_c A(r, c) Bm(c, n) for a 32 x 32 tile; A(r, c) = a[r + c * lda]
-// (or a[c + r * lda] when ta), Bm(c, n) = b[c + n * ldb] (or b[n + c * ldb] when tb).
-static __device__ __forceinline__ void tile_sub(double *dst, int ldd, const double *a, int lda, bool ta, const double *b,
-                                         int ldb, bool tb, int kc, double (*as)[B + 1], double (*bs)[B + 1]) {
-  for (int e = threadIdx.x; e < B * B; e += RT) {
-    const int i = e & 31, c = e >> 5;
-    as[i][c] = c < kc ? (ta ? a[c + (int64_t)i * lda] : a[i + (int64_t)c * lda]) : 0.0;
-    bs[c][i] = c < kc ? (tb ? b[i + (int64_t)c * ldb] : b[c + (int64_t)i * ldb]) : 0.0;
-  }
-  __syncthreads();
-  f64x4_t acc = {0.0, 0.0, 0.0, 0.0};
-  acc = mfma32(as, bs, false, acc);  // Bm(c, n) = bs[c][n]
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int wm = (wave >> 1) * 16, wn = (wave & 1) * 16;
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int i = wm + (lane >> 4) + 4 * r, n = wn + (lane & 15);
-    dst[i + (int64_t)n * ldd] -= acc[r];
-  }
-  __syncthreads();
-}
-
 // cs(:, n) <- D^-1 cs(:, n) for the 32 right-hand sides n, D = d lower
 // triangular (one thread per right-hand side).
 static __device__ __forceinline__ void forward_sub(double (*d)[B + 1], double (*cs)[B + 1]) {
@@ -141,49 +119,86 @@ static __device__ __forceinline__ void backward_sub_t(double (*d)[B + 1], double
 }
 
 // In-place lower Cholesky of the kp x kp block G (kp a multiple of 32). Returns
-// false if a pivot is not positive (G not positive definite).
+// false if a pivot is not positive (G not positive definite). Left-looking by
+// 32-column panels: each block (r0, j0) of the panel first takes its update
+// -sum_p L(r0, p) L(j0, p)^T over the final panels p before j0 (one pipelined
+// MFMA accumulation, the next block's operands loaded into registers while the
+// current one's MFMAs run), then the diagonal block is factored in LDS and the
+// blocks below it are solved 32 rows at a time with the rows as right-hand
+// sides. Only the lower triangle of G is read.
 static __device__ bool cholesky64(double *G, int kp, double (*d)[B + 1], double (*cs)[B + 1], double (*as)[B + 1],
-                           double (*bs)[B + 1], int32_t *ok_sh) {
+                                  double (*bs)[B + 1], int32_t *ok_sh) {
+  constexpr int U = B * B / RT;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = (wave >> 1) * 16, wn = (wave & 1) * 16;
   if (threadIdx.x == 0) *ok_sh = 1;
   for (int j0 = 0; j0 < kp; j0 += B) {
-    // diagonal block
-    for (int e = threadIdx.x; e < B * B; e += RT) d[e & 31][e >> 5] = G[(j0 + (e & 31)) + (int64_t)(j0 + (e >> 5)) * kp];
-    __syncthreads();
-    for (int c = 0; c < B; ++c) {
-      if (threadIdx.x == 0) {
-        const double p = d[c][c];
-        if (!(p > 0.0)) *ok_sh = 0;
-        d[c][c] = sqrt(p > 0.0 ? p : 1.0);
+    for (int r0 = j0; r0 < kp; r0 += B) {
+      if (j0 > 0) {
+        f64x4_t acc = {0.0, 0.0, 0.0, 0.0};
+        double pa[U], pb[U];
+        auto load = [&](int p0) {
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const int e = threadIdx.x + u * RT, lo = e & 31, hi = e >> 5;
+            pa[u] = G[(r0 + lo) + (int64_t)(p0 + hi) * kp];  // A(i, c) = L(r0 + i, p0 + c)
+            pb[u] = G[(j0 + lo) + (int64_t)(p0 + hi) * kp];  // B(c, n) = L(j0 + n, p0 + c)
+          }
+        };
+        load(0);
+        for (int p0 = 0; p0 < j0; p0 += B) {
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const int e = threadIdx.x + u * RT, lo = e & 31, hi = e >> 5;
+            as[lo][hi] = pa[u];
+            bs[hi][lo] = pb[u];
+          }
+          __syncthreads();
+          if (p0 + B < j0) load(p0 + B);
+          acc = mfma32(as, bs, false, acc);
+          __syncthreads();
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = wm + (lane >> 4) + 4 * r, n = wn + (lane & 15);
+          G[(r0 + i) + (int64_t)(j0 + n) * kp] -= acc[r];
+        }
+        __syncthreads();
       }
-      __syncthreads();
-      for (int r = c + 1 + threadIdx.x; r < B; r += RT) d[r][c] /= d[c][c];
-      __syncthreads();
-      const int n = B - c - 1;
-      for (int e = threadIdx.x; e < n * n; e += RT) {
-        const int r = c + 1 + e % n, s = c + 1 + e / n;
-        if (s <= r) d[r][s] -= d[r][c] * d[s][c];
+      if (r0 == j0) {
+        // diagonal block
+        for (int e = threadIdx.x; e < B * B; e += RT) d[e & 31][e >> 5] = G[(j0 + (e & 31)) + (int64_t)(j0 + (e >> 5)) * kp];
+        __syncthreads();
+        for (int c = 0; c < B; ++c) {
+          if (threadIdx.x == 0) {
+            const double p = d[c][c];
+            if (!(p > 0.0)) *ok_sh = 0;
+            d[c][c] = sqrt(p > 0.0 ? p : 1.0);
+          }
+          __syncthreads();
+          for (int r = c + 1 + threadIdx.x; r < B; r += RT) d[r][c] /= d[c][c];
+          __syncthreads();
+          const int n = B - c - 1;
+          for (int e = threadIdx.x; e < n * n; e += RT) {
+            const int r = c + 1 + e % n, q = c + 1 + e / n;
+            if (q <= r) d[r][q] -= d[r][c] * d[q][c];
+          }
+          __syncthreads();
+        }
+        for (int e = threadIdx.x; e < B * B; e += RT) {
+          const int r = e & 31, q = e >> 5;
+          G[(j0 + r) + (int64_t)(j0 + q) * kp] = q <= r ? d[r][q] : 0.0;
+        }
+        __syncthreads();
+      } else {
+        // L(r0, j0) = G(r0, j0) L11^-T: L11 L(r0, :)^T = G(r0, :)^T, the 32 rows as right-hand sides
+        for (int e = threadIdx.x; e < B * B; e += RT) cs[e >> 5][e & 31] = G[(r0 + (e & 31)) + (int64_t)(j0 + (e >> 5)) * kp];
+        __syncthreads();
+        forward_sub(d, cs);
+        for (int e = threadIdx.x; e < B * B; e += RT) G[(r0 + (e & 31)) + (int64_t)(j0 + (e >> 5)) * kp] = cs[e >> 5][e & 31];
+        __syncthreads();
       }
-      __syncthreads();
     }
-    for (int e = threadIdx.x; e < B * B; e += RT) {
-      const int r = e & 31, s = e >> 5;
-      G[(j0 + r) + (int64_t)(j0 + s) * kp] = s <= r ? d[r][s] : 0.0;
-    }
-    __syncthreads();
-    // panel below: L21 = A21 L11^-T, i.e. L11 L21(r, :)^T = A21(r, :)^T for each
-    // row r, solved 32 rows at a time with the rows as right-hand sides in LDS
-    for (int r0 = j0 + B; r0 < kp; r0 += B) {
-      for (int e = threadIdx.x; e < B * B; e += RT) cs[e >> 5][e & 31] = G[(r0 + (e & 31)) + (int64_t)(j0 + (e >> 5)) * kp];
-      __syncthreads();
-      forward_sub(d, cs);
-      for (int e = threadIdx.x; e < B * B; e += RT) G[(r0 + (e & 31)) + (int64_t)(j0 + (e >> 5)) * kp] = cs[e >> 5][e & 31];
-      __syncthreads();
-    }
-    // trailing update of the lower part: A22 -= L21 L21^T, 32 x 32 tiles
-    for (int s0 = j0 + B; s0 < kp; s0 += B)
-      for (int r0 = s0; r0 < kp; r0 += B)
-        tile_sub(G + r0 + (int64_t)s0 * kp, kp, G + r0 + (int64_t)j0 * kp, kp, false, G + s0 + (int64_t)j0 * kp, kp,
-                 true, B, as, bs);
   }
   __syncthreads();
   return *ok_sh != 0;
