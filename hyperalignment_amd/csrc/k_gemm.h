@@ -500,13 +500,52 @@ __device__ __forceinline__ void mma_chunk_f32(GemmSmem<TILE> &sm, int buf, Acc<T
   }
 }
 
+// acc += dadd * A(m, tn0 + n) for the columns n of the tile that the K chunk in
+// buffer buf covers (kn = its first k minus tn0, a multiple of 32): over all
+// chunks the tile becomes that of A (B + dadd I), from the A rows already in
+// LDS. Only the waves whose 32-column block is kn do work.
+template <int TILE, int KA, int PREC>
+__device__ __forceinline__ void add_shift_chunk(GemmSmem<TILE> &sm, int buf, int kn, float dadd, Acc<TILE> &acc) {
+  static_assert(KA == OP_SPLIT_K || (PREC == PREC_F32 && KA == OP_F32_GATHER), "k-contiguous A operand");
+  constexpr int W = TILE / 64;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm0 = (wave >> 1) * (TILE / 2), wn0 = (wave & 1) * (TILE / 2);
+#pragma unroll
+  for (int j = 0; j < W; ++j) {
+    if (wn0 + j * 32 != kn) continue;
+#pragma unroll
+    for (int i = 0; i < W; ++i) {
+      const int m = wm0 + i * 32 + (lane & 31);
+      // element r of the block is column (r / 4) * 8 + (lane / 32) * 4 + r % 4
+#pragma unroll
+      for (int g = 0; g < KC / 8; ++g) {
+        float v[4];
+        if constexpr (PREC == PREC_F32) {
+          const float4 q = *f32_quad<TILE>(sm.ah[buf], sm.al[buf], 2 * g + (lane >> 5), m);
+          v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+        } else {
+          const uint2 h = reinterpret_cast<const uint2 *>(&sm.ah[buf][g][m])[lane >> 5];
+          const uint2 l = reinterpret_cast<const uint2 *>(&sm.al[buf][g][m])[lane >> 5];
+          v[0] = bf2f((uint16_t)h.x) + bf2f((uint16_t)l.x);
+          v[1] = bf2f((uint16_t)(h.x >> 16)) + bf2f((uint16_t)(l.x >> 16));
+          v[2] = bf2f((uint16_t)h.y) + bf2f((uint16_t)l.y);
+          v[3] = bf2f((uint16_t)(h.y >> 16)) + bf2f((uint16_t)(l.y >> 16));
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) acc.v[i][j][4 * g + t] += dadd * v[t];
+      }
+    }
+  }
+}
+
 // One output tile over K. sa, sb: the stage registers; with have_first the
 // caller has already fetched chunk 0 of this tile into them (during the
-// previous tile's epilogue), so its load latency is hidden.
+// previous tile's epilogue), so its load latency is hidden. dadd != 0: the
+// tile of A (B + dadd I) instead (square products, k-contiguous A).
 template <int TILE, int KA, int KB, int PREC = PREC_BF16X3, bool COH = false>
 __device__ __forceinline__ void tile_gemm(GemmSmem<TILE> &sm, const Opnd &A, const Opnd &B, int tm0, int tn0,
                                           int K, Acc<TILE> &acc, Stage<TILE> &sa, Stage<TILE> &sb,
-                                          bool have_first) {
+                                          bool have_first, float dadd = 0.f) {
   constexpr int W = TILE / 64;
   const int wave = threadIdx.x >> 6;
   const int wm0 = (wave >> 1) * (TILE / 2), wn0 = (wave & 1) * (TILE / 2);
@@ -548,6 +587,10 @@ __device__ __forceinline__ void tile_gemm(GemmSmem<TILE> &sm, const Opnd &A, con
       mma_chunk_f32<TILE>(sm, cur, acc, bm, bn);
     else
       mma_chunk<TILE, KA>(sm, cur, acc, bm, bn);
+    if constexpr (KA == OP_SPLIT_K || (PREC == PREC_F32 && KA == OP_F32_GATHER)) {
+      const int kn = c * KC - tn0;
+      if (dadd != 0.f && kn >= 0 && kn < TILE) add_shift_chunk<TILE, KA, PREC>(sm, cur, kn, dadd, acc);
+    }
     if (more) {
       if constexpr (KA == OP_SPLIT_M)
         stash_m<TILE>(sa, sm.th[cur ^ 1], sm.tl[cur ^ 1]);
@@ -591,7 +634,7 @@ __device__ __forceinline__ void acc_to_lds(const Acc<TILE> &acc, float *ctile) {
 // workgroups work on tiles of the same row and read its row panel together.
 template <int TILE, int KA, int KB, int PREC = PREC_BF16X3, bool COH = false, class Epi>
 __device__ void wg_gemm_mn(GemmSmem<TILE> &sm, const Opnd &A, const Opnd &B, int M, int N, int K, bool sym,
-                           Epi epi, int nw = 1, int w = 0) {
+                           Epi epi, int nw = 1, int w = 0, float dadd = 0.f) {
   const int ntm = (M + TILE - 1) / TILE, ntn = (N + TILE - 1) / TILE;
   constexpr int LD = TILE + 1;
   // this workgroup's tiles in row-major order (only I <= J when sym)
@@ -610,7 +653,7 @@ __device__ void wg_gemm_mn(GemmSmem<TILE> &sm, const Opnd &A, const Opnd &B, int
   for (bool more = next_tile(I, J); more;) {
     {
       Acc<TILE> acc;
-      tile_gemm<TILE, KA, KB, PREC, COH>(sm, A, B, I * TILE, J * TILE, K, acc, sa, sb, have_first);
+      tile_gemm<TILE, KA, KB, PREC, COH>(sm, A, B, I * TILE, J * TILE, K, acc, sa, sb, have_first, dadd);
       acc_to_lds<TILE>(acc, sm.ctile);
     }
     // the next tile's first K chunk is loaded while this tile is delivered
@@ -653,11 +696,11 @@ __device__ void wg_gemm_mn(GemmSmem<TILE> &sm, const Opnd &A, const Opnd &B, int
   }
 }
 
-// Square kp x kp product (see wg_gemm_mn).
+// Square kp x kp product (see wg_gemm_mn); dadd != 0: A (B + dadd I).
 template <int TILE, int KA, int KB, int PREC = PREC_BF16X3, bool COH = false, class Epi>
 __device__ __forceinline__ void wg_gemm(GemmSmem<TILE> &sm, const Opnd &A, const Opnd &B, int kp, int K, bool sym,
-                                        Epi epi, int nw = 1, int w = 0) {
-  wg_gemm_mn<TILE, KA, KB, PREC, COH>(sm, A, B, kp, kp, K, sym, epi, nw, w);
+                                        Epi epi, int nw = 1, int w = 0, float dadd = 0.f) {
+  wg_gemm_mn<TILE, KA, KB, PREC, COH>(sm, A, B, kp, kp, K, sym, epi, nw, w, dadd);
 }
 
 #endif  // K_GEMM_H
