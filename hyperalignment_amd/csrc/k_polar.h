@@ -288,7 +288,8 @@ struct GroupTeam {
 // the kp x kp block (m a multiple of 4).
 // learned: (scale, ell) came from a prediction; an attempt that does not
 // converge is redone once from (fro, plain_ell), which always holds
-// sigma_max(X0) <= 1. w.X, w.X2, w.G, w.P are overwritten.
+// sigma_max(X0) <= 1. w.X, w.G, w.P are overwritten (w.X2 is not used: each
+// step's X P goes over G, dead by then, and the two buffers trade roles).
 // on_fin() runs in every thread once the finishing step is decided, before the
 // team's barrier that precedes the finishing product (so before any fin call
 // of any member).
@@ -301,7 +302,7 @@ __device__ __forceinline__ PolarOut polar_solve(GemmSmem<TILE> &sm, const HakLoc
                                                 double *red, float *trace, Team &team, Fin fin,
                                                 OnFin on_fin = OnFin()) {
   using M = MatS<PREC>;
-  const M G(w.G, kp), P(w.P, kp);
+  const M P(w.P, kp);
   int step = 0, total = 0;
   bool done = false;
   constexpr int nw = Team::nw;
@@ -325,10 +326,11 @@ __device__ __forceinline__ PolarOut polar_solve(GemmSmem<TILE> &sm, const HakLoc
     const float inv = 1.0f / scale;
     M(w.X, kp).load_scaled(w.A, kp, inv, team.e0(), team.es());
     if (!team.sync()) return PolarOut{total + step, false, true};
-    float *Xb = w.X, *Xnb = w.X2;
+    float *Xb = w.X, *Gb = w.G;
     int restart = 0, restarts = 0;
     for (step = 0; step < a.max_steps; ++step) {
-      const M X(Xb, kp), Xn(Xnb, kp);
+      // Xn goes over G: after the barrier that follows P, nothing reads G
+      const M X(Xb, kp), G(Gb, kp), Xn(Gb, kp);
       // G = X^T X and r = ||G - I||_F over the k x k block
       double r2 = 0.0;
       {
@@ -433,8 +435,8 @@ __device__ __forceinline__ PolarOut polar_solve(GemmSmem<TILE> &sm, const HakLoc
         break;
       }
       float *t = Xb;
-      Xb = Xnb;
-      Xnb = t;
+      Xb = Gb;
+      Gb = t;
     }
     total += step;
   }
