@@ -388,11 +388,13 @@ __device__ __forceinline__ PolarOut polar_solve(GemmSmem<TILE> &sm, const HakLoc
       } else {
         ca = 1.875f; cb = -1.25f; cc = 0.375f;  // quintic Newton-Schulz
       }
-      // P = ca I + cb G + cc G^2, zero outside the k x k block. With cc != 0 the
-      // product is G (G + (cb / cc) I), whose G term comes from the operand rows
-      // already staged in LDS, so the epilogue reads nothing; a cc so small that
-      // cb / cc overflows leaves G^2 below the rounding of cb G.
-      if (cc != 0.f && isfinite(cb / cc)) {
+      // P = ca I + cb G + cc G^2, zero outside the k x k block. In a team, with
+      // cc != 0, the product is G (G + (cb / cc) I), whose G term comes from the
+      // operand rows already staged in LDS, so the epilogue reads nothing (a cc
+      // so small that cb / cc overflows leaves G^2 below the rounding of cb G);
+      // one workgroup alone reads G back in the epilogue, which measured faster
+      // there.
+      if (Team::nw > 1 && cc != 0.f && isfinite(cb / cc)) {
         const Opnd og = G.opnd(M::KK, kp, kp);
         wg_gemm<TILE, M::KK, M::KK, PREC, Team::coh>(
             sm, og, og, kp, kp, true,
@@ -407,6 +409,23 @@ __device__ __forceinline__ PolarOut polar_solve(GemmSmem<TILE> &sm, const HakLoc
               P.put4(m + n * kp, pv);
             },
             nw, tw, cb / cc);
+      } else if (cc != 0.f) {
+        const Opnd og = G.opnd(M::KK, kp, kp);
+        wg_gemm<TILE, M::KK, M::KK, PREC, Team::coh>(
+            sm, og, og, kp, kp, true,
+            [&](int m, int n, float4 v) {
+              const int i = m + n * kp;
+              float4 pv = make_float4(0.f, 0.f, 0.f, 0.f);
+              if (n < k && m < k) {
+                const float4 g = G.template get4<Team::coh>(i);
+                pv.x = (m == n ? ca : 0.f) + cb * g.x + cc * v.x;
+                pv.y = (m + 1 < k) ? (m + 1 == n ? ca : 0.f) + cb * g.y + cc * v.y : 0.f;
+                pv.z = (m + 2 < k) ? (m + 2 == n ? ca : 0.f) + cb * g.z + cc * v.z : 0.f;
+                pv.w = (m + 3 < k) ? (m + 3 == n ? ca : 0.f) + cb * g.w + cc * v.w : 0.f;
+              }
+              P.put4(i, pv);
+            },
+            nw, tw);
       } else {
         for (int64_t e = team.e0(); e < (int64_t)kp * kp; e += team.es()) {
           const int m = (int)(e % kp), n = (int)(e / kp);
