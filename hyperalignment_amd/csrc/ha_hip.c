@@ -932,6 +932,10 @@ static int enqueue_pass(HaHipPlan *p, HakLocalArgs a, int method, bool isReflect
 				if (rc) goto fail;
 			} else {
 				b.n_slots = p->n_slots_polar;
+				/* the rank-deficient searchlights are few and long: one
+				 * workgroup per CU, so no two of them share one (which made
+				 * the launch's time vary by a third from pass to pass) */
+				if (t == 2 && b.n_slots > g.n_cu) b.n_slots = g.n_cu;
 				KCK(hak_polar(&b, st));
 			}
 		}
