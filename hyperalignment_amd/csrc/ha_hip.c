@@ -471,7 +471,7 @@ int ha_plan_create_ex(int32_t nt, const HaPlanSetup *su, const int32_t *slx_idx,
 	ACK(p->q, (size_t)p->q_elems * sizeof(float));
 	ACK(p->flag, (size_t)count * sizeof(int32_t));
 	ACK(p->steps, (size_t)count * sizeof(int32_t));
-	ACK(p->queues, 8 * sizeof(int32_t));
+	ACK(p->queues, 16 * sizeof(int32_t));
 	ACK(p->sigma_sum, (size_t)count * sizeof(double));
 	ACK(p->var_sum, (size_t)count * sizeof(double));
 	ACK(p->scale, (size_t)count * sizeof(double));
@@ -792,7 +792,7 @@ static int time_tile(HaHipPlan *p, const int32_t *sel, int32_t n, int tile, floa
 	a.order = sel;
 	a.n_order = n;
 	a.tile = tile;
-	KCK(hak_fill_i32(p->queues, 8, 0, st));
+	KCK(hak_fill_i32(p->queues, 16, 0, st));
 	KCK(hak_fill_i32(p->flag, p->count, 0, st));
 	KCK(hak_fill_i32(p->steps, p->count, 0, st));
 	a.queue = p->queues + 3;
@@ -883,7 +883,7 @@ static int enqueue_pass(HaHipPlan *p, HakLocalArgs a, int method, bool isReflect
                         bool zero_data, hipStream_t st, bool timed) {
 	int rc = kHaSuccess;
 	if (timed) HCK(hipEventRecord(p->ev[0], st));
-	KCK(hak_fill_i32(p->queues, 8, 0, st));
+	KCK(hak_fill_i32(p->queues, 16, 0, st));
 	KCK(hak_fill_i32(p->flag, p->count, 0, st));
 	KCK(hak_fill_i32(p->steps, p->count, 0, st));
 	if (zero_data) KCK(hak_fill_f64(p->data, p->nnz, 0.0, st));
@@ -932,9 +932,21 @@ static int enqueue_pass(HaHipPlan *p, HakLocalArgs a, int method, bool isReflect
 				if (rc) goto fail;
 			} else {
 				b.n_slots = p->n_slots_polar;
-				/* the rank-deficient searchlights are few and long: one
-				 * workgroup per CU, so no two of them share one (which made
-				 * the launch's time vary by a third from pass to pass) */
+				b.skip_done = 0;
+				if (t == 2 && g.opt.team && p->grp) {
+					/* the rank-deficient searchlights are few and long: teams
+					 * with FP32 products first, then one workgroup each for
+					 * what a team left */
+					KCK(hak_fill_i32((int32_t *)p->grp, (int64_t)p->n_teams * (HAK_GRP_STATE_BYTES / 4), 0, st));
+					HakLocalArgs tt = b;
+					tt.grp = p->grp;
+					tt.grp_spin = g.opt.team_spin;
+					tt.queue = p->queues + 8;
+					KCK(hak_polar_grp(&tt, p->n_teams, st));
+					b.skip_done = 1;
+				}
+				/* one workgroup per CU at most for them, so that no two share
+				 * one (which made the launch's time vary by a third) */
 				if (t == 2 && b.n_slots > g.n_cu) b.n_slots = g.n_cu;
 				KCK(hak_polar(&b, st));
 			}
@@ -1411,7 +1423,7 @@ int ha_plan_local64(HaHipPlan *p, bool isReflection, bool isScaling, double *out
 	a.sigma_sum = isScaling ? p->sigma_sum : NULL;
 	ACK(q64, (size_t)p->q_elems * sizeof(double));
 	a.q64 = q64;
-	KCK(hak_fill_i32(p->queues, 8, 0, st));
+	KCK(hak_fill_i32(p->queues, 16, 0, st));
 	KCK(hak_jacobi64(&a, isReflection ? 0 : 1, p->work64, p->stride64, p->n_slots64, p->count, st));
 	if (isScaling) {
 		KCK(hak_var_sum(p->X64, p->nt, p->slx, p->off, p->count, p->var_sum, st));
@@ -1467,7 +1479,7 @@ int ha_plan_ridge(HaHipPlan *p, double alpha, bool zero_data, double *betas64) {
 		a.q64 = q64;
 	}
 	HCK(hipEventRecord(p->ev[0], st));
-	KCK(hak_fill_i32(p->queues, 8, 0, st));
+	KCK(hak_fill_i32(p->queues, 16, 0, st));
 	KCK(hak_fill_i32(p->flag, p->count, 0, st));
 	KCK(hak_fill_i32(p->steps, p->count, 0, st));
 	if (!betas64) {
@@ -1589,7 +1601,7 @@ int ha_hip_plan_spectra(HaHipPlan *p, const int32_t *which, int32_t n, float *fe
 	a.fallback_enabled = 0;
 	a.rank_check_nt = 0;
 	a.queue = p->queues + 4;
-	KCK(hak_fill_i32(p->queues, 8, 0, st));
+	KCK(hak_fill_i32(p->queues, 16, 0, st));
 	KCK(hak_jacobi32(&a, st));
 	hf = (float *)malloc((size_t)p->count * HAK_NFEAT * sizeof(float));
 	hs = (float *)malloc((size_t)p->count * 2 * sizeof(float));

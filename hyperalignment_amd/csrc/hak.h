@@ -136,7 +136,8 @@ int hak_crosscov(const HakLocalArgs *a, int n_slots, void *stream);
 /* Stage 2: the polar iteration, a->n_slots workgroups, 4 slot buffers each. */
 int hak_polar(const HakLocalArgs *a, void *stream);
 /* Stage 2 with teams of HAK_GRP workgroups of one XCD per searchlight (128 x 128
- * tiles): n_teams teams (a multiple of 8), 4 slot buffers per team, a->grp holds
+ * bf16x3 tiles, or 64 x 64 FP32 tiles when a->prec_f32): n_teams teams (a
+ * multiple of 8), 4 slot buffers per team, a->grp holds
  * n_teams states. A searchlight it finishes gets steps[s] != 0; one it leaves
  * (a team barrier gave up) keeps steps[s] == 0 and A in the Q store, for a
  * following hak_polar with skip_done; one cut short while Q was being written is

@@ -103,6 +103,13 @@ __device__ __forceinline__ uint2 ld_b64(const void *base, int64_t ofs) {
     return *reinterpret_cast<const uint2 *>(static_cast<const char *>(base) + ofs);
 }
 template <bool COH>
+__device__ __forceinline__ uint32_t ld_b32(const void *base, int64_t ofs) {
+  if constexpr (COH)
+    return __builtin_amdgcn_raw_buffer_load_b32(dev_rsrc(base), (int)ofs, 0, CPOL_SC1);
+  else
+    return *reinterpret_cast<const uint32_t *>(static_cast<const char *>(base) + ofs);
+}
+template <bool COH>
 __device__ __forceinline__ uint16_t ld_b16(const void *base, int64_t ofs) {
   if constexpr (COH)
     return __builtin_amdgcn_raw_buffer_load_b16(dev_rsrc(base), (int)ofs, 0, CPOL_SC1);
@@ -213,11 +220,11 @@ struct Stage {
   uint4 r[N];
 };
 
-// COH: device-scope loads (split operands only), for operands that other
-// workgroups of the launch wrote.
+// COH: device-scope loads, for operands that other workgroups of the launch
+// wrote (not for gathered split rows; byte offsets below 2^31).
 template <int TILE, int KIND, bool COH = false>
 __device__ __forceinline__ void fetch(const Opnd &o, int m0, int kb, Stage<TILE> &st) {
-  static_assert(!COH || KIND == OP_SPLIT_K || KIND == OP_SPLIT_M, "device-scope loads for split operands only");
+  static_assert(!COH || KIND != OP_SPLIT_GATHER, "no device-scope loads for gathered split rows");
   constexpr int N = Stage<TILE>::N;
   const int tid = threadIdx.x;
   if constexpr (KIND == OP_SPLIT_GATHER) {
@@ -242,7 +249,7 @@ __device__ __forceinline__ void fetch(const Opnd &o, int m0, int kb, Stage<TILE>
       const int m = e / (KC / 4), kg = e % (KC / 4);
       const int gm = m0 + m;
       uint4 v = make_uint4(0, 0, 0, 0);
-      if (gm < o.mlimit) v = *(const uint4 *)(o.f32 + (int64_t)(o.cols ? o.cols[gm] : gm) * o.ld + kb + kg * 4);
+      if (gm < o.mlimit) v = ld_b128<COH>(o.f32, 4 * ((int64_t)(o.cols ? o.cols[gm] : gm) * o.ld + kb + kg * 4));
       st.r[it] = v;
     }
   } else if constexpr (KIND == OP_F32_M) {
@@ -252,7 +259,7 @@ __device__ __forceinline__ void fetch(const Opnd &o, int m0, int kb, Stage<TILE>
       const int kk = e / (TILE / 4), mq = e % (TILE / 4);
       const int gm = m0 + mq * 4;
       uint4 v = make_uint4(0, 0, 0, 0);
-      if (gm < o.mlimit) v = *(const uint4 *)(o.f32 + gm + (int64_t)(kb + kk) * o.ld);  // mlimit % 4 == 0
+      if (gm < o.mlimit) v = ld_b128<COH>(o.f32, 4 * (gm + (int64_t)(kb + kk) * o.ld));  // mlimit % 4 == 0
       st.r[it] = v;
     }
   } else if constexpr (KIND == OP_SPLIT_K) {

@@ -352,8 +352,8 @@ __global__ __launch_bounds__(NT, TILE == 128 ? HAK_POLAR_OCC128 : HAK_POLAR_OCC)
 // written its tiles: a team that gives up part way leaves a marked searchlight,
 // and one that gives up earlier leaves A intact and steps 0 for the
 // one-workgroup kernel that follows.
+template <int TILE, int PREC>
 __global__ __launch_bounds__(NT, HAK_POLAR_OCC128) void k_polar_grp(HakLocalArgs a) {
-  constexpr int TILE = 128;
   __shared__ GemmSmem<TILE> sm;
   __shared__ double red[4];
   __shared__ int ok_sh;
@@ -400,7 +400,7 @@ __global__ __launch_bounds__(NT, HAK_POLAR_OCC128) void k_polar_grp(HakLocalArgs
     double tr = 0.0;
     // Q over A: each element of A is read (for tr(Q^T A)) and then replaced by
     // the same thread, and A is not read again for this searchlight
-    const PolarOut po = polar_solve<TILE, PREC_BF16X3>(
+    const PolarOut po = polar_solve<TILE, PREC>(
         sm, a, w, k, kp, fro, scale, ell, a.learned_scale != 0, red,
         a.trace ? a.trace + (int64_t)s * a.max_steps : nullptr, team,
         [&](int m, int n, float4 v) {
@@ -442,7 +442,11 @@ __global__ __launch_bounds__(NT, HAK_POLAR_OCC128) void k_polar_grp(HakLocalArgs
 extern "C" int hak_polar_grp(const HakLocalArgs *a, int n_teams, void *stream) {
   if (a->n_order <= 0) return 0;
   if (n_teams <= 0 || n_teams % 8 != 0 || !a->grp || !a->steps || !a->flag) return (int)hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_polar_grp, dim3(n_teams * HAK_GRP), dim3(NT), 0, (hipStream_t)stream, *a);
+  if (a->prec_f32)
+    hipLaunchKernelGGL((k_polar_grp<64, PREC_F32>), dim3(n_teams * HAK_GRP), dim3(NT), 0, (hipStream_t)stream, *a);
+  else
+    hipLaunchKernelGGL((k_polar_grp<128, PREC_BF16X3>), dim3(n_teams * HAK_GRP), dim3(NT), 0, (hipStream_t)stream,
+                       *a);
   LAUNCH_RET();
 }
 

@@ -156,14 +156,13 @@ struct MatS<PREC_F32> {
   __device__ __forceinline__ void put(int64_t i, float v) const { f[i] = v; }
   template <bool COH = false>
   __device__ __forceinline__ float get(int64_t i) const {
-    static_assert(!COH, "FP32 storage is not shared between workgroups");
-    return f[i];
+    return __uint_as_float(ld_b32<COH>(f, 4 * i));
   }
   __device__ __forceinline__ void put4(int64_t i, float4 v) const { *reinterpret_cast<float4 *>(f + i) = v; }
   template <bool COH = false>
   __device__ __forceinline__ float4 get4(int64_t i) const {
-    static_assert(!COH, "FP32 storage is not shared between workgroups");
-    return *reinterpret_cast<const float4 *>(f + i);
+    const uint4 u = ld_b128<COH>(f, 4 * i);
+    return make_float4(__uint_as_float(u.x), __uint_as_float(u.y), __uint_as_float(u.z), __uint_as_float(u.w));
   }
   __device__ __forceinline__ Opnd opnd(int kind, int ld, int mlimit) const { return opnd_f32(f, kind, ld, mlimit); }
   __device__ __forceinline__ void load_scaled(const float *A, int kp, float v, int64_t e0 = -1,
