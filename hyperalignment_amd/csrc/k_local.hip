@@ -351,7 +351,8 @@ __global__ __launch_bounds__(NT, TILE == 128 ? HAK_POLAR_OCC128 : HAK_POLAR_OCC)
 // barrier ahead of the product), and the mark is cleared once every member has
 // written its tiles: a team that gives up part way leaves a marked searchlight,
 // and one that gives up earlier leaves A intact and steps 0 for the
-// one-workgroup kernel that follows.
+// one-workgroup kernel that follows. Two instances: 128 x 128 bf16x3 tiles for
+// the 128-tile searchlights, 64 x 64 FP32 tiles for the rank-deficient ones.
 template <int TILE, int PREC>
 __global__ __launch_bounds__(NT, HAK_POLAR_OCC128) void k_polar_grp(HakLocalArgs a) {
   __shared__ GemmSmem<TILE> sm;
