@@ -354,7 +354,7 @@ __global__ __launch_bounds__(NT, TILE == 128 ? HAK_POLAR_OCC128 : HAK_POLAR_OCC)
 // one-workgroup kernel that follows. Two instances: 128 x 128 bf16x3 tiles for
 // the 128-tile searchlights, 64 x 64 FP32 tiles for the rank-deficient ones.
 template <int TILE, int PREC>
-__global__ __launch_bounds__(NT, HAK_POLAR_OCC128) void k_polar_grp(HakLocalArgs a) {
+__global__ __launch_bounds__(NT, TILE == 128 ? HAK_POLAR_OCC128 : HAK_POLAR_OCC) void k_polar_grp(HakLocalArgs a) {
   __shared__ GemmSmem<TILE> sm;
   __shared__ double red[4];
   __shared__ int ok_sh;
