@@ -203,7 +203,7 @@ struct SoloTeam {
   }
 };
 
-#define HAK_GRP_SPIN (1u << 18)  // polls of a team barrier before it gives up (a fraction of a second)
+#define HAK_GRP_SPIN (1u << 18)  // polls of a team barrier before it gives up (each an L2 round trip: about 0.1 s)
 
 // Per-team state in global memory, zeroed before the launch.
 struct GrpState {
@@ -252,7 +252,6 @@ struct GroupTeam {
       uint32_t n = 0;
       int ok = 1;
       while (__hip_atomic_load(&st->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-        __builtin_amdgcn_s_sleep(1);
         if (++n > limit) {
           ok = 0;
           break;
