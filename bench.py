@@ -262,7 +262,10 @@ def main():
                          f"estimate {s.count / rate:.1f} s"}
         loc = plan.local(which)
         errs = [float(np.abs(a - b).max()) for a, b in zip(loc, Ts)]
-        parity = {"local_maxabs": max(errs), "local_median": float(np.median(errs))}
+        fro = [float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(loc, Ts)]
+        parity = {"local_maxabs": max(errs), "local_median": float(np.median(errs)),
+                  "local_rel_frobenius_max": max(fro), "local_rel_frobenius_median": float(np.median(fro)),
+                  "searchlights": len(errs)}
 
     out = {
         "metric": "searchlights/s (full-hemisphere searchlight Procrustes pass)",
@@ -282,6 +285,10 @@ def main():
                    "pattern_ms": st["ms_pattern"], "nnz": st["nnz"], "fp64_fallback": st["n_fallback"],
                    "unconverged": st["n_unconverged"],
                    "mean_steps": st["steps_total"] / max(1, s.count),
+                   "steps_hist": {str(i): int(v) for i, v in enumerate(st["steps_hist"]) if v},
+                   "orthogonality_check": {"checked": st["n_orth_checked"], "sent_to_fp64": st["n_orth_flagged"],
+                                           "max_fro": st["orth_max"], "ms": st["ms_orth"]},
+                   "teams": {"gave_up": st["n_team_giveup"], "searchlights_handed_back": st["n_team_handback"]},
                    "stage_ms": {k: st[k] for k in ("ms_weights", "ms_crosscov", "ms_polar", "ms_fallback",
                                                    "ms_scatter")},
                    "iteration_tile": tiles, "parity_vs_fp64": parity, "compare": compare,

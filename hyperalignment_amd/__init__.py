@@ -59,7 +59,7 @@ class HaHipOptions(ctypes.Structure):
                 ("plain_ell", ctypes.c_double), ("tile", ctypes.c_int32), ("use_graph", ctypes.c_int32),
                 ("chunk", ctypes.c_int32), ("verbose", ctypes.c_int32),
                 ("learned_margin_scale", ctypes.c_float), ("learned_margin_ell", ctypes.c_float),
-                ("team", ctypes.c_int32), ("team_spin", ctypes.c_uint32)]
+                ("team", ctypes.c_int32), ("team_spin", ctypes.c_uint32), ("orth_tol", ctypes.c_double)]
 
 
 class HaHipStats(ctypes.Structure):
@@ -71,7 +71,10 @@ class HaHipStats(ctypes.Structure):
                 ("steps_hist", ctypes.c_int32 * 32), ("nnz", ctypes.c_int64),
                 ("gemm_flops", ctypes.c_double), ("kernel_ms_polar_avg", ctypes.c_double),
                 ("polar_launches", ctypes.c_int32), ("ms_apply", ctypes.c_double),
-                ("ms_corr", ctypes.c_double)]
+                ("ms_corr", ctypes.c_double), ("n_orth_checked", ctypes.c_int64),
+                ("n_orth_flagged", ctypes.c_int64), ("orth_max", ctypes.c_double),
+                ("n_team_giveup", ctypes.c_int64), ("n_team_handback", ctypes.c_int64),
+                ("ms_orth", ctypes.c_double)]
 
     def as_dict(self):
         d = {name: getattr(self, name) for name, _ in self._fields_}
@@ -121,6 +124,7 @@ _SIGS = {
     "ha_hip_plan_download": (ctypes.c_int, [ctypes.c_void_p, _P(_i32), _P(_i32), _P(_d)]),
     "ha_hip_plan_download_local": (ctypes.c_int, [ctypes.c_void_p, _P(_i32), _i32, _P(_f)]),
     "ha_hip_plan_stats": (ctypes.c_int, [ctypes.c_void_p, _P(HaHipStats)]),
+    "ha_hip_plan_orthogonality": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _P(_d)]),
     "ha_hip_plan_spectra": (ctypes.c_int, [ctypes.c_void_p, _P(_i32), _i32, _P(_f), _P(_f)]),
     "ha_hip_plan_steps": (ctypes.c_int, [ctypes.c_void_p, _P(_i32), _P(_i32)]),
     "ha_hip_plan_predictions": (ctypes.c_int, [ctypes.c_void_p, _P(_f)]),
@@ -729,6 +733,14 @@ class Plan:
         t = np.zeros(6, dtype=np.int32)
         _check("ha_hip_plan_tiles", lib().ha_hip_plan_tiles(self.h, _ptr(t, ctypes.c_int32)))
         return t
+
+    def orthogonality(self, exact=True):
+        """||Q_s^T Q_s - I||_F of every local transform in FP64 on the GPU (exact:
+        every Gram entry; else the four-probe estimate the pass uses)."""
+        out = np.zeros(self.count, dtype=np.float64)
+        _check("ha_hip_plan_orthogonality", lib().ha_hip_plan_orthogonality(self.h, 1 if exact else 0,
+                                                                            _ptr(out, ctypes.c_double)))
+        return out
 
     def stats(self):
         s = HaHipStats()

@@ -235,6 +235,8 @@ int ha_searchlight_procrustes_hip_csc(const TMat *X, const TMat *Y, const TSearc
 	if (X->rows != Y->rows || X->rows <= 0) return kHaErrorArg;
 	const TSearchlights *sy = sls_Y ? sls_Y : sls_X;
 	if (sy->count != sls_X->count) return kHaErrorArg;
+	if (mat->cols > Y->cols) { ERR("CSC pattern has more columns than Y vertices\n"); return kHaErrorArg; }
+	if (ha_check_csc(mat->cols, X->cols, mat->indptr, mat->indices, mat->nnz)) return kHaErrorArg;
 	for (int32_t s = 0; s < sls_X->count; ++s)
 		if (sy->sizes[s] != sls_X->sizes[s]) {
 			ERR("searchlight %d has %d X and %d Y vertices; this backend needs equal sizes\n", s,
@@ -296,6 +298,8 @@ int ha_searchlight_ridge(const TMat *X, const TMat *Y, const TSearchlights *sls_
 	if (X->rows != Y->rows || X->rows <= 0) return kHaErrorArg;
 	const TSearchlights *sy = sls_Y ? sls_Y : sls_X;
 	if (sy->count != sls_X->count) return kHaErrorArg;
+	if (mat->cols > Y->cols) { ERR("CSC pattern has more columns than Y vertices\n"); return kHaErrorArg; }
+	if (ha_check_csc(mat->cols, X->cols, mat->indptr, mat->indices, mat->nnz)) return kHaErrorArg;
 	for (int32_t s = 0; s < sls_X->count; ++s)
 		if (sy->sizes[s] != sls_X->sizes[s]) {
 			ERR("searchlight %d has %d X and %d Y vertices; this backend needs equal sizes\n", s,

@@ -50,6 +50,7 @@ static struct {
 		.learned_margin_ell = 1.0f,
 		.team = 1,
 		.team_spin = 0,
+		.orth_tol = 1e-3,
 	},
 	.host_threads = 0,
 };
@@ -226,6 +227,8 @@ struct HaHipPlan {
 	void *grp; int32_t n_teams;               /* polar teams (128 x 128 tiles): state per team */
 	int32_t *flag, *steps, *queues, *order;
 	double *sigma_sum, *var_sum, *scale;
+	double *orth;        /* per searchlight FP64 orthogonality of Q (the pass's check) */
+	bool orth_ran;       /* the last pass ran the check */
 	float *features, *pred, *trace, *sig;
 	int32_t *d_which;
 	HaHipStats stats;
@@ -251,7 +254,7 @@ static void plan_free(HaHipPlan *p) {
 	                p->qld, p->qoff, p->q, p->q64, p->X, p->Y, p->X64, p->Y64, p->Xs, p->Ys, p->stage, p->work, p->work64,
 	                p->flag, p->steps, p->queues, p->order, p->sigma_sum, p->var_sum, p->scale,
 	                p->features, p->pred, p->trace, p->sig, p->d_which, p->sl_par, p->tile_order[0],
-	                p->tile_order[1], p->tile_order[2], p->work_ridge, p->grp, p->iota};
+	                p->tile_order[1], p->tile_order[2], p->work_ridge, p->grp, p->iota, p->orth};
 	for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i)
 		if (ptrs[i]) hipFree(ptrs[i]);
 	if (p->gexec) hipGraphExecDestroy(p->gexec);
@@ -525,15 +528,45 @@ int ha_hip_plan_create(int32_t nt, int32_t nv, const int32_t *sl_indices, const 
 	return ha_plan_create_ex(nt, &su, sl_indices, NULL, sl_offsets, sl_dists, count, radius, plan_out);
 }
 
+/* A caller's CSC pattern must be well formed before the scatter trusts it
+ * (data + indptr[c] + position is written on the GPU): indptr starts at 0, never
+ * decreases and ends within nnz; each column's rows are strictly increasing and
+ * below nrows. O(ncols + nnz) on the host. */
+int ha_check_csc(int32_t ncols, int32_t nrows, const int32_t *indptr, const int32_t *indices, int64_t nnz) {
+	if (ncols < 0 || nnz < 0 || !indptr || (nnz > 0 && !indices)) {
+		ERR("CSC pattern: missing arrays or negative sizes\n");
+		return kHaErrorArg;
+	}
+	if (indptr[0] != 0) { ERR("CSC pattern: indptr[0] = %d, not 0\n", indptr[0]); return kHaErrorArg; }
+	for (int32_t c = 0; c < ncols; ++c) {
+		const int32_t b = indptr[c], e = indptr[c + 1];
+		if (e < b || (int64_t)e > nnz) {
+			ERR("CSC pattern: column %d spans [%d, %d) outside [0, nnz = %lld)\n", c, b, e, (long long)nnz);
+			return kHaErrorArg;
+		}
+		for (int32_t q = b; q < e; ++q) {
+			const int32_t r = indices[q];
+			if (r < 0 || r >= nrows || (q > b && r <= indices[q - 1])) {
+				ERR("CSC pattern: column %d has row %d at entry %d (rows must be increasing and below %d)\n",
+				    c, r, q, nrows);
+				return kHaErrorArg;
+			}
+		}
+	}
+	return kHaSuccess;
+}
+
 int ha_plan_set_pattern(HaHipPlan *p, int32_t ncols, const int32_t *indptr, const int32_t *indices, int64_t nnz,
                         const double *data) {
 	int rc = kHaSuccess;
 	hipStream_t st = g.stream;
+	if (ncols > p->nv_y) { ERR("pattern has more columns than Y vertices\n"); return kHaErrorArg; }
+	rc = ha_check_csc(ncols, p->nv_x, indptr, indices, nnz);
+	if (rc) return rc;
 	if (p->indptr) { hipFree(p->indptr); p->indptr = NULL; }
 	if (p->indptr64) { hipFree(p->indptr64); p->indptr64 = NULL; }
 	if (p->indices) { hipFree(p->indices); p->indices = NULL; }
 	if (p->data) { hipFree(p->data); p->data = NULL; }
-	if (ncols > p->nv_y) { ERR("pattern has more columns than Y vertices\n"); return kHaErrorArg; }
 	p->ncols = ncols;
 	p->nnz = nnz;
 	int64_t *h64 = (int64_t *)malloc((size_t)(ncols + 1) * sizeof(int64_t));
@@ -638,6 +671,7 @@ static HakLocalArgs local_args(HaHipPlan *p) {
 	a.rank_check_nt = g.opt.fp64_fallback;
 	a.mlp_margin_scale = g.opt.learned_margin_scale;
 	a.mlp_margin_ell = g.opt.learned_margin_ell;
+	a.team_count = p->queues + 12;  /* zeroed with the queues at the start of a pass */
 	return a;
 }
 
@@ -685,6 +719,24 @@ static void stats_update(HaHipPlan *p, int method) {
 	}
 	S->gemm_flops = flops;
 	S->nnz = p->nnz;
+	S->n_orth_checked = 0; S->n_orth_flagged = 0; S->orth_max = 0;
+	if (p->orth_ran) {
+		double *o = (double *)malloc((size_t)p->count * sizeof(double));
+		if (o && hipMemcpy(o, p->orth, (size_t)p->count * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess) {
+			for (int32_t s = 0; s < p->count; ++s) {
+				if (o[s] < 0) continue;  /* already on the FP64 path when the check ran */
+				S->n_orth_checked++;
+				if (!(o[s] <= g.opt.orth_tol)) S->n_orth_flagged++;
+				if (o[s] > S->orth_max || !isfinite(o[s])) S->orth_max = o[s];
+			}
+		}
+		free(o);
+	}
+	int32_t tc[2] = {0, 0};
+	if (hipMemcpy(tc, p->queues + 12, sizeof(tc), hipMemcpyDeviceToHost) == hipSuccess) {
+		S->n_team_giveup = tc[0];
+		S->n_team_handback = tc[1];
+	}
 	free(st);
 	free(fl);
 }
@@ -763,7 +815,7 @@ static int build_tile_orders(HaHipPlan *p) {
 	for (int32_t i = 0; i < p->count; ++i) {
 		const int32_t s = ord[i];
 		/* rank deficient (stage 1 starts them from the smallest bucket): FP32 products */
-		const int t = (g.opt.fp64_fallback && p->h_k[s] >= p->nt)
+		const int t = (g.opt.fp64_fallback && HAK_NEAR_DEFICIENT(p->h_k[s], p->nt))
 		                  ? 2
 		                  : (p->bucket_tile[kp_bucket((p->h_k[s] + 31) / 32 * 32)] == 128 ? 1 : 0);
 		h[(size_t)t * p->count + n[t]++] = s;
@@ -877,6 +929,16 @@ int ha_hip_plan_tiles(HaHipPlan *p, int32_t *bucket_tile) {
 
 /* ---- one pass ---- */
 
+/* The pass's orthogonality check: on for polar passes with orth_tol > 0 when
+ * the plan's searchlights fit the probe kernel's LDS. */
+static bool orth_check_on(const HaHipPlan *p) {
+	return g.opt.orth_tol > 0 && p->orth && p->kpmax <= HAK_ORTH_PROBE_MAX_KP;
+}
+static int32_t orth_workgroups(const HaHipPlan *p) {
+	const int32_t n = 8 * g.n_cu;
+	return p->count < n ? p->count : n;
+}
+
 /* Enqueues the whole pass on st: no allocation, no host synchronisation, so
  * it can be captured into a graph. timed: records the stage events. */
 static int enqueue_pass(HaHipPlan *p, HakLocalArgs a, int method, bool isReflection, bool isScaling,
@@ -957,6 +1019,12 @@ static int enqueue_pass(HaHipPlan *p, HakLocalArgs a, int method, bool isReflect
 		b.queue = p->queues + 0;
 		KCK(hak_detfix(&b, p->work64, p->stride64, p->iota, p->n_slots64, st));
 	}
+	/* FP64 orthogonality of every local transform the iteration wrote; those
+	 * above orth_tol are flagged for the FP64 path below */
+	if (timed) HCK(hipEventRecord(p->ev[7], st));
+	if (polar_m && orth_check_on(p))
+		KCK(hak_orth(p->q, p->qoff, p->qld, p->off, NULL, p->count, p->kpmax, 0, p->orth, p->flag, g.opt.orth_tol,
+		             orth_workgroups(p), st));
 	if (timed) HCK(hipEventRecord(p->ev[2], st));
 	if (method != kHaHipMethodJacobi64 && (polar_m || !det_fix) && g.opt.fp64_fallback) {
 		HakLocalArgs b = a;
@@ -1060,6 +1128,7 @@ int ha_plan_run(HaHipPlan *p, bool isReflection, bool isScaling, bool zero_data,
 		rc = tune_tiles(p);
 		if (rc) return rc;
 	}
+	if (is_polar && g.opt.orth_tol > 0 && !p->orth) ACK(p->orth, (size_t)p->count * sizeof(double));
 	HakLocalArgs a = local_args(p);
 	if (isScaling) a.sigma_sum = p->sigma_sum;
 	if (method == kHaHipMethodPolarLearned) {
@@ -1088,19 +1157,22 @@ int ha_plan_run(HaHipPlan *p, bool isReflection, bool isScaling, bool zero_data,
 		HCK(hipStreamSynchronize(st));
 		float t[5];
 		for (int i = 0; i < 5; ++i) hipEventElapsedTime(&t[i], p->ev[i], p->ev[i + 1]);
-		float tot = 0, tc = 0, ti = t[1];
+		float tot = 0, tc = 0, ti = t[1], tk = t[1], to = 0;
 		hipEventElapsedTime(&tot, p->ev[0], p->ev[5]);
 		if (is_polar) {
 			hipEventElapsedTime(&tc, p->ev[1], p->ev[6]);
 			hipEventElapsedTime(&ti, p->ev[6], p->ev[2]);
+			hipEventElapsedTime(&tk, p->ev[6], p->ev[7]);  /* iteration (+ reflection fix) */
+			hipEventElapsedTime(&to, p->ev[7], p->ev[2]);  /* orthogonality check */
 		}
 		p->stats.ms_weights = t[0];
 		p->stats.ms_crosscov = tc;  /* polar path: gather + cross-covariance (+ features) stage */
 		p->stats.ms_polar = ti;     /* local solver (the iteration stage on the polar path) */
+		p->stats.ms_orth = to;
 		p->stats.ms_fallback = t[2];
 		p->stats.ms_scatter = t[4] + t[3];
 		p->stats.ms_total = tot;
-		p->stats.kernel_ms_polar_avg = ti;
+		p->stats.kernel_ms_polar_avg = tk;
 		p->stats.polar_launches =
 		    is_polar ? (p->tile_count[0] > 0) + (p->tile_count[1] > 0) + (p->tile_count[2] > 0) : 1;
 		debug_report(p, &a, clk, nclk);
@@ -1134,11 +1206,20 @@ int ha_plan_run(HaHipPlan *p, bool isReflection, bool isScaling, bool zero_data,
 		float tot = 0;
 		hipEventElapsedTime(&tot, p->ev[0], p->ev[5]);
 		memset(&p->stats.ms_weights, 0, sizeof(double) * 5);  /* ms_weights .. ms_scatter: not split */
+		p->stats.ms_orth = 0;
 		p->stats.ms_total = tot;
 		p->stats.kernel_ms_polar_avg = 0;
 		p->stats.polar_launches = 0;
 	}
+	p->orth_ran = is_polar && orth_check_on(p);
 	stats_update(p, method);
+	if (!g.opt.fp64_fallback && p->stats.n_fallback > 0 && method != kHaHipMethodJacobi64) {
+		/* flagged searchlights hold no valid transform without the FP64 pass,
+		 * and the scatter has added them into T */
+		ERR("%lld searchlights need the FP64 path (unconverged, rank deficient or failed checks), "
+		    "which the fp64_fallback option switches off\n", (long long)p->stats.n_fallback);
+		rc = kHaErrorSvd;
+	}
 	if (g.opt.verbose)
 		fprintf(stderr, "ha_hip: pass %.3f ms (weights %.3f, cross-cov %.3f, local %.3f, fp64 %.3f, scatter %.3f)%s; "
 		        "fallback %lld\n", p->stats.ms_total, p->stats.ms_weights, p->stats.ms_crosscov, p->stats.ms_polar,
@@ -1206,6 +1287,27 @@ int ha_hip_plan_stats(const HaHipPlan *p, HaHipStats *s) {
 	if (!p || !s) return kHaErrorArg;
 	*s = p->stats;
 	return kHaSuccess;
+}
+
+int ha_hip_plan_orthogonality(HaHipPlan *p, int exact, double *out) {
+	if (!p || !out) return kHaErrorArg;
+	if (!p->loaded || !p->q) { ERR("plan has no local transforms yet\n"); return kHaErrorArg; }
+	if (!exact && p->kpmax > HAK_ORTH_PROBE_MAX_KP) {
+		ERR("the probe check holds searchlights up to kp %d (plan: %d); use the exact check\n",
+		    HAK_ORTH_PROBE_MAX_KP, p->kpmax);
+		return kHaErrorArg;
+	}
+	int rc = kHaSuccess;
+	hipStream_t st = g.stream;
+	double *d = NULL;
+	ACK(d, (size_t)p->count * sizeof(double));
+	KCK(hak_orth(p->q, p->qoff, p->qld, p->off, NULL, p->count, p->kpmax, exact ? 1 : 0, d, NULL, 0.0,
+	             exact ? (p->count < 65535 ? p->count : 65535) : orth_workgroups(p), st));
+	HCK(hipMemcpyAsync(out, d, (size_t)p->count * sizeof(double), hipMemcpyDeviceToHost, st));
+	HCK(hipStreamSynchronize(st));
+fail:
+	if (d) hipFree(d);
+	return rc;
 }
 
 /* Searchlight Procrustes template over the plan's searchlights (see

@@ -37,6 +37,10 @@ typedef struct {
 int ha_plan_create_ex(int32_t nt, const HaPlanSetup *setup,
                       const int32_t *slx_idx, const int32_t *sly_idx, const int32_t *sl_off,
                       const double *sl_dists, int32_t count, double radius, HaHipPlan **out);
+/* Host check of a caller's CSC pattern (indptr from 0, non-decreasing, within
+ * nnz; rows strictly increasing per column and below nrows): kHaErrorArg with a
+ * message when it is malformed. */
+int ha_check_csc(int32_t ncols, int32_t nrows, const int32_t *indptr, const int32_t *indices, int64_t nnz);
 /* Uploads an external CSC pattern (and its current data) into the plan. */
 int ha_plan_set_pattern(HaHipPlan *p, int32_t ncols, const int32_t *indptr, const int32_t *indices,
                         int64_t nnz, const double *data);

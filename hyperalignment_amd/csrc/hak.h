@@ -119,13 +119,21 @@ typedef struct {
 	float *sig_out;                      /* Jacobi paths: per searchlight (sigma_max, sigma_min) (may be NULL) */
 	int32_t tile;
 	int32_t fallback_enabled;
-	int32_t rank_check_nt;               /* k >= nt: rank deficient, started from the smallest bucket */
+	int32_t rank_check_nt;               /* HAK_NEAR_DEFICIENT(k, nt): started from the smallest bucket,
+	                                        FP32 products */
 	int32_t prec_f32;                    /* hak_polar: FP32 products (PREC_F32) instead of bf16x3 */
 	/* teams of HAK_GRP workgroups per searchlight (k_polar_grp) */
 	void *grp;                           /* per team HAK_GRP_STATE_BYTES of state, zeroed before the launch */
 	uint32_t grp_spin;                   /* polls of a team barrier before it gives up (0: default) */
 	int32_t skip_done;                   /* polar: skip searchlights whose steps[] is already nonzero */
+	int32_t *team_count;                 /* [0] teams whose barrier gave up, [1] searchlights the
+	                                        one-workgroup kernel took over (skip_done); may be NULL */
 } HakLocalArgs;
+
+/* Searchlights the polar path treats as rank deficient: z-scored columns span
+ * at most nt - 1 dimensions, so A = X_sl^T Y_sl is singular for k >= nt and,
+ * for k = nt - 1, full rank but as ill-conditioned as the samples allow. */
+#define HAK_NEAR_DEFICIENT(k, nt) ((k) >= (nt) - 1)
 
 /* Workgroups per CU of the polar iteration stage (launch bound and slot count). */
 #define HAK_POLAR_OCC 3
@@ -159,6 +167,18 @@ int hak_jacobi64(const HakLocalArgs *a, int want_det_fix, double *work64, int64_
  * work64: n_wg * 4 * stride64 doubles; iota: 0, 1, ..., kpmax - 1. */
 int hak_detfix(const HakLocalArgs *a, double *work64, int64_t stride64, const int32_t *iota, int n_wg,
                void *stream);
+
+/* ---- FP64 orthogonality check of the local transforms (k_orth.hip) ---- */
+/* kp limit of the probe check: kpmax x 4 doubles and kpmax sign bytes of LDS */
+#define HAK_ORTH_PROBE_MAX_KP 1920
+/* out[s] = ||Q_s^T Q_s - I||_F in FP64 for the searchlights order[0..n)
+ * (identity when NULL): every Gram entry when exact, else the four-probe
+ * estimate sqrt(mean_p ||Q^T Q z_p - z_p||^2). With flag set, searchlights
+ * already flagged get out[s] = -1 and those with out[s] > tol (or not finite)
+ * are flagged. n_wg workgroups (0: one per searchlight); kpmax bounds qld. */
+int hak_orth(const float *q, const int64_t *qoff, const int32_t *qld, const int32_t *sl_off,
+             const int32_t *order, int32_t n, int32_t kpmax, int exact, double *out, int32_t *flag,
+             double tol, int32_t n_wg, void *stream);
 
 /* ---- spectral-scaling MLP (k_mlp.hip) ---- */
 /* Y[N][2] = MLP(X[N][12]) with the given parameters and input normalisation. */

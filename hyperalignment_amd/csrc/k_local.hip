@@ -231,8 +231,12 @@ __global__ __launch_bounds__(NT, 2) void k_crosscov(HakLocalArgs a) {
     // factor is not unique and its smallest singular values are rounding noise
     // (a few 1e-9 of ||A||_F). The plain scale with the schedule of the smallest
     // bucket takes that noise to 1 as well: one orthogonal completion, a valid
-    // Procrustes solution. One that does not converge goes to the FP64 path.
-    const bool deficient = a.rank_check_nt && k >= a.nt;
+    // Procrustes solution. k = nt - 1 fills the rank the samples allow: A is
+    // full rank but as ill-conditioned as the data get, and takes the same
+    // route (bf16x3 products measured 1.4e-4 from the FP64 factor there, FP32
+    // products stay within the bar). One that does not converge goes to the
+    // FP64 path.
+    const bool deficient = a.rank_check_nt && HAK_NEAR_DEFICIENT(k, a.nt);
     const uint64_t t0 = a.clk ? wall_clock64() : 0;
     crosscov<TILE>(sm, a, s, k, kp, A);
     __syncthreads();
@@ -312,6 +316,7 @@ __global__ __launch_bounds__(NT, TILE == 128 ? HAK_POLAR_OCC128 : HAK_POLAR_OCC)
     const float scale = a.sl_par[3 * s];
     if (!(scale > 0.f)) continue;  // left to the FP64 path by stage 1
     if (a.skip_done && a.steps[s] != 0) continue;  // finished (or given up) by the team kernel
+    if (a.skip_done && threadIdx.x == 0 && a.team_count) atomicAdd(&a.team_count[1], 1);  // taken over
     const float ell = a.sl_par[3 * s + 1], fro = a.sl_par[3 * s + 2];
     const int k = a.sl_off[s + 1] - a.sl_off[s];
     const int kp = a.qld[s];
@@ -361,7 +366,12 @@ __global__ __launch_bounds__(NT, TILE == 128 ? HAK_POLAR_OCC128 : HAK_POLAR_OCC)
   const int xcd = blockIdx.x & 7, j = blockIdx.x >> 3;
   const int team_id = (j / HAK_GRP) * 8 + xcd, member = j % HAK_GRP;
   GrpState *st = reinterpret_cast<GrpState *>(a.grp) + team_id;
-  GroupTeam team{st, member, 0u, a.grp_spin ? a.grp_spin : HAK_GRP_SPIN, &ok_sh};
+  // wait budget: grp_spin microseconds (100 ticks of the 100 MHz clock each)
+  GroupTeam team{st, member, 0u, 100ull * (a.grp_spin ? a.grp_spin : HAK_GRP_WAIT_US), &ok_sh};
+  // counts this team once in team_count[0] if its barrier wait runs out
+  auto gave_up = [&]() {
+    if (member == 0 && threadIdx.x == 0 && a.team_count) atomicAdd(&a.team_count[0], 1);
+  };
   WorkSlots w;
   {
     float *b = a.work + (int64_t)team_id * 4 * a.slot_stride;
@@ -377,7 +387,10 @@ __global__ __launch_bounds__(NT, TILE == 128 ? HAK_POLAR_OCC128 : HAK_POLAR_OCC)
     asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(xcc));
     __hip_atomic_fetch_or(&st->xcc_mask, 1u << (xcc & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
-  if (!team.sync()) return;
+  if (!team.sync()) {
+    gave_up();
+    return;
+  }
   if (__popc(__hip_atomic_load(&st->xcc_mask, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) != 1)
     return;  // members on different XCDs: every searchlight is left to the one-workgroup kernel
   // Turn t takes its searchlight from item[t & 1]: a member that skips a
@@ -386,7 +399,10 @@ __global__ __launch_bounds__(NT, TILE == 128 ? HAK_POLAR_OCC128 : HAK_POLAR_OCC)
   // next turn's barrier, which every member reaches after its read.
   for (int turn = 0;; ++turn) {
     if (lead) st->item[turn & 1] = atomicAdd(a.queue, 1);
-    if (!team.sync()) break;
+    if (!team.sync()) {
+      gave_up();
+      break;
+    }
     const int it = __hip_atomic_load(&st->item[turn & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (it >= a.n_order) break;
     const int s = a.order ? a.order[it] : it;
@@ -418,7 +434,10 @@ __global__ __launch_bounds__(NT, TILE == 128 ? HAK_POLAR_OCC128 : HAK_POLAR_OCC)
             a.steps[s] = -1;
           }
         });
-    if (po.aborted) break;
+    if (po.aborted) {
+      gave_up();
+      break;
+    }
     if (!po.done) {
       if (lead) {
         a.flag[s] = 1;  // budget exhausted or non-finite: the FP64 pass writes Q
@@ -430,7 +449,10 @@ __global__ __launch_bounds__(NT, TILE == 128 ? HAK_POLAR_OCC128 : HAK_POLAR_OCC)
     if (want_tr) {
       bool ok;
       tr = team.sum(tr, red, &ok);
-      if (!ok) break;
+      if (!ok) {
+        gave_up();
+        break;
+      }
     }
     if (lead) {
       a.flag[s] = 0;
@@ -461,7 +483,7 @@ extern "C" int hak_crosscov(const HakLocalArgs *a, int n_slots, void *stream) {
 }
 
 // prec_f32: FP32 products on 64 x 64 tiles, for rank-deficient searchlights
-// (k >= nt): their rounding-noise singular values are raised from about 1e-9
+// (k >= nt - 1, HAK_NEAR_DEFICIENT): their rounding-noise singular values are raised from about 1e-9
 // to 1 over a long schedule, which carries the bf16x3 products' 2^-16 error
 // into the range part of Q (about 5e-5 against 1e-6 with FP32 products).
 extern "C" int hak_polar(const HakLocalArgs *a, void *stream) {

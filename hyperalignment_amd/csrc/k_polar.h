@@ -203,7 +203,7 @@ struct SoloTeam {
   }
 };
 
-#define HAK_GRP_SPIN (1u << 18)  // polls of a team barrier before it gives up (each an L2 round trip: about 0.1 s)
+#define HAK_GRP_WAIT_US 100000u  // microseconds a team barrier waits before it gives up
 
 // Per-team state in global memory, zeroed before the launch.
 struct GrpState {
@@ -223,7 +223,7 @@ struct GroupTeam {
   GrpState *st;
   int member;
   uint32_t gen;     // barriers passed by this workgroup
-  uint32_t limit;
+  uint64_t limit;   // wait budget of one barrier in ticks of the 100 MHz wall clock
   int *ok_sh;       // LDS flag broadcast by thread 0
   __device__ int member_id() const { return member; }
   __device__ bool leader() const { return member == 0; }
@@ -249,12 +249,15 @@ struct GroupTeam {
     if (threadIdx.x == 0) {
       __hip_atomic_fetch_add(&st->count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       const uint32_t target = gen * HAK_GRP;
-      uint32_t n = 0;
       int ok = 1;
-      while (__hip_atomic_load(&st->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-        if (++n > limit) {
-          ok = 0;
-          break;
+      if (__hip_atomic_load(&st->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        // bounded by time, not by polls: the wait ends after limit ticks whatever the poll rate
+        const uint64_t t0 = wall_clock64();
+        while (__hip_atomic_load(&st->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+          if (wall_clock64() - t0 > limit) {
+            ok = 0;
+            break;
+          }
         }
       }
       *ok_sh = ok;

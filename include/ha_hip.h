@@ -62,8 +62,12 @@ typedef struct {
 	int32_t team;            /* 1 (default): with 128 x 128 tiles, 4 workgroups of one XCD
 	                            iterate each searchlight together and share panel reads
 	                            through L2; 0: one workgroup per searchlight */
-	uint32_t team_spin;      /* polls of a team barrier before the team gives its searchlight
-	                            back to the one-workgroup kernel (0: default, about 0.1 s) */
+	uint32_t team_spin;      /* microseconds a team barrier waits before the team gives its
+	                            searchlights back to the one-workgroup kernel (0: default 100000) */
+	double  orth_tol;        /* polar paths: every local transform's FP64 orthogonality
+	                            ||Q^T Q - I||_F (four-probe estimate) is checked after the
+	                            iteration; above orth_tol it goes to the FP64 path (default
+	                            1e-3; 0 switches the check off) */
 } HaHipOptions;
 
 void ha_hip_get_options(HaHipOptions *opt);
@@ -121,6 +125,12 @@ typedef struct {
 	int32_t polar_launches;
 	double  ms_apply;          /* last ha_hip_plan_evaluate: Yhat = X T */
 	double  ms_corr;           /* and the vertex-wise correlation */
+	int64_t n_orth_checked;    /* local transforms whose FP64 orthogonality was checked in the pass */
+	int64_t n_orth_flagged;    /* of those, sent to the FP64 path (estimate above orth_tol) */
+	double  orth_max;          /* largest ||Q^T Q - I||_F estimate among the checked ones */
+	int64_t n_team_giveup;     /* teams whose barrier wait ran out (their work went back) */
+	int64_t n_team_handback;   /* searchlights the one-workgroup kernel took over from teams */
+	double  ms_orth;           /* the orthogonality check (part of ms_polar) */
 } HaHipStats;
 
 /* Uploads the flat searchlight arrays (layout of hyperalignment_c.py:161-169)
@@ -170,6 +180,10 @@ int ha_hip_plan_download(const HaHipPlan *plan, int32_t *indptr, int32_t *indice
 int ha_hip_plan_download_local(const HaHipPlan *plan, const int32_t *which, int32_t n_which,
                                float *out);
 int ha_hip_plan_stats(const HaHipPlan *plan, HaHipStats *stats);
+/* FP64 orthogonality of every local transform in the plan's store (after a
+ * pass): out[s] = ||Q_s^T Q_s - I||_F, from every entry of the Gram matrix
+ * when exact, else the four-probe estimate the pass itself uses. */
+int ha_hip_plan_orthogonality(HaHipPlan *plan, int exact, double *out);
 
 /* Features and (sigma_max, sigma_min) of the chosen searchlights from the FP32
  * Jacobi SVD path (training labels of the learned scaling). features: n x 12,

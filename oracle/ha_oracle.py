@@ -391,17 +391,27 @@ def template_procrustes(dms, reflection=True, scaling=False, zscore_common=True,
 
 
 def searchlight_template(dms, idx, offsets, dists, radius, reflection=True, scaling=False,
-                         zscore_common=True, level2_iter=2, which=None):
+                         zscore_common=True, level2_iter=2, which=None, n_jobs=1):
     """Searchlight template: local templates (csrc/ha_template.c:14-114 on the
     searchlight's columns) combined with per-vertex weights as
-    src/hyperalignment/searchlight.py:215-236 (tpl[:, sl] += local * w)."""
+    src/hyperalignment/searchlight.py:215-236 (tpl[:, sl] += local * w), added
+    in searchlight order (n_jobs only spreads the local templates over threads)."""
     nv = dms[0].shape[1]
     w = searchlight_weights_flat(idx, offsets, dists, radius, nv)
     tpl = np.zeros_like(np.asarray(dms[0], dtype=np.float64))
     sl = _lists(idx, offsets)
-    which = range(len(sl)) if which is None else which
-    for s in which:
-        loc = template_procrustes([d[:, sl[s]] for d in dms], reflection, scaling, zscore_common, level2_iter)
+    which = list(range(len(sl)) if which is None else which)
+
+    def one(s):
+        return template_procrustes([d[:, sl[s]] for d in dms], reflection, scaling, zscore_common, level2_iter)
+
+    if n_jobs == 1:
+        locs = [one(s) for s in which]
+    else:
+        from concurrent.futures import ThreadPoolExecutor
+        with ThreadPoolExecutor(n_jobs) as ex:
+            locs = list(ex.map(one, which))
+    for s, loc in zip(which, locs):
         tpl[:, sl[s]] += loc * w[offsets[s]:offsets[s + 1]][None, :]
     return tpl
 

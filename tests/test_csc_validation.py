@@ -1,0 +1,57 @@
+"""A caller's CSC pattern is checked on the host before anything reaches the
+GPU: a malformed indptr or row list makes the CSC entry points return
+kHaErrorArg (-3) instead of letting the scatter write out of bounds. No GPU
+is needed: the check runs before any device call."""
+
+import types
+
+import numpy as np
+import pytest
+
+import hyperalignment_amd as ha
+
+
+def _mat(indptr, indices, nrows, ncols):
+    indptr = np.asarray(indptr, dtype=np.int32)
+    indices = np.asarray(indices, dtype=np.int32)
+    return types.SimpleNamespace(indptr=indptr, indices=indices, data=np.zeros(len(indices)),
+                                 shape=(nrows, ncols))
+
+
+# three vertices, one searchlight over all of them; the good pattern is full
+GOOD = ([0, 3, 6, 9], [0, 1, 2] * 3)
+BAD = {
+    "indptr_not_from_zero": ([1, 3, 6, 9], [0, 1, 2] * 3),
+    "indptr_decreasing": ([0, 3, 2, 9], [0, 1, 2] * 3),
+    "indptr_past_nnz": ([0, 3, 6, 12], [0, 1, 2] * 3),
+    "row_out_of_range": ([0, 3, 6, 9], [0, 1, 2, 0, 1, 3, 0, 1, 2]),
+    "negative_row": ([0, 3, 6, 9], [0, 1, 2, -1, 1, 2, 0, 1, 2]),
+    "rows_unsorted": ([0, 3, 6, 9], [0, 2, 1, 0, 1, 2, 0, 1, 2]),
+    "rows_repeated": ([0, 3, 6, 9], [0, 1, 1, 0, 1, 2, 0, 1, 2]),
+}
+
+
+@pytest.mark.parametrize("case", sorted(BAD))
+@pytest.mark.parametrize("entry", ["procrustes", "ridge"])
+def test_malformed_pattern_is_rejected(case, entry):
+    rng = np.random.default_rng(0)
+    X = rng.standard_normal((8, 3))
+    Y = rng.standard_normal((8, 3))
+    m = _mat(*BAD[case], 3, 3)
+    fn = ha.searchlight_procrustes_csc if entry == "procrustes" else ha.searchlight_ridge_csc
+    with pytest.raises(ha.HaError) as ei:
+        fn(X, Y, [[0, 1, 2]], m)
+    assert ei.value.rc == -3, case
+
+
+def test_good_pattern_passes_the_check():
+    """The good pattern is not rejected as an argument error (without a GPU the
+    call then fails at device set-up instead)."""
+    rng = np.random.default_rng(0)
+    X = rng.standard_normal((8, 3))
+    Y = rng.standard_normal((8, 3))
+    m = _mat(*GOOD, 3, 3)
+    try:
+        ha.searchlight_procrustes_csc(X, Y, [[0, 1, 2]], m)
+    except ha.HaError as e:
+        assert e.rc != -3

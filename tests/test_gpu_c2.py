@@ -1,0 +1,212 @@
+"""The full-hemisphere workload (order-6 icosphere, 40,962 searchlights of
+k 357-441, n = 500, float32 data) on the GPU against the CPU FP64 oracle,
+through the C-ABI, with the default method (learned bf16x3 polar iteration).
+
+Bars (csrc/ha_searchlight.c:950-1106, csrc/ha_sparse.c:31-112 for what is
+compared):
+  * CSC indptr / indices: bit-exact against the oracle's pattern;
+  * local transforms and CSC columns: max |T - T_fp64| <= 1e-4, and the
+    relative Frobenius error ||T - T_fp64||_F / ||T_fp64||_F <= 2e-4 per local
+    transform and per column (a bf16x3 transform measures about 4e-5 in a CPU
+    emulation of the iteration against an exact SVD);
+  * Q^T Q = I for every one of the 40,962 local transforms: the exact FP64
+    ||Q^T Q - I||_F of each is at most 1e-3 (the pass's own check sends larger
+    ones to the FP64 path; typical values are about 1e-4).
+Also here: one cell of the learned-scaling evaluation grid (seed 101, n = 500,
+noise 1.0) against the oracle, reflection-free local transforms on 64
+searchlights against LAPACK + LU sign, and the FP32 Jacobi comparator on
+sampled columns of the 10,242-vertex workload.
+"""
+
+import numpy as np
+import pytest
+
+import hyperalignment_amd as ha
+from hyperalignment_amd import learn, synth
+from oracle import ha_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL_MAX = 1e-4
+TOL_FRO = 2e-4
+TOL_ORTH = 1e-3
+N_LOCAL = 200
+JOBS = 16
+
+
+@pytest.fixture(scope="module", autouse=True)
+def hip():
+    ha.init()
+    learn.install(learn.MODEL_PATH)
+    ha.set_options(method="auto")
+    yield
+    ha.set_options(method="auto")
+
+
+def _pass(w, reflection=True, which=None):
+    s = w.sls
+    if which is None:
+        plan = ha.Plan(w.X.shape[0], s.nv, s.indices, s.offsets, s.dists, s.radius)
+    else:
+        idx, off, d = learn.sub_searchlights(s, which)
+        plan = ha.Plan(w.X.shape[0], s.nv, idx, off, d, s.radius)
+    plan.load(w.X, w.Y)
+    plan.execute(reflection=reflection)
+    return plan
+
+
+def _oracle_local(w, which, reflection=True):
+    from concurrent.futures import ThreadPoolExecutor
+    lists = w.sls.lists()
+
+    def one(i):
+        sl = lists[i]
+        return O.procrustes(w.X[:, sl], w.Y[:, sl], reflection, False)
+
+    with ThreadPoolExecutor(JOBS) as ex:
+        return list(ex.map(one, list(which)))
+
+
+def _check_local(got, ref, label):
+    worst_max, worst_fro = 0.0, 0.0
+    for g, r in zip(got, ref):
+        d = g.astype(np.float64) - r
+        worst_max = max(worst_max, float(np.abs(d).max()))
+        worst_fro = max(worst_fro, float(np.linalg.norm(d) / np.linalg.norm(r)))
+    print(f"[{label}] {len(got)} local transforms: max-abs {worst_max:.3e}, rel Frobenius {worst_fro:.3e}",
+          flush=True)
+    assert worst_max <= TOL_MAX, worst_max
+    assert worst_fro <= TOL_FRO, worst_fro
+
+
+@pytest.fixture(scope="module")
+def c2():
+    w = synth.make_workload("c2")
+    plan = _pass(w)
+    st = plan.stats()
+    print(f"[c2] pass {st['ms_total']:.1f} ms, mean steps {st['steps_total'] / w.sls.count:.2f}, "
+          f"fp64 {st['n_fallback']}, orth checked {st['n_orth_checked']} flagged {st['n_orth_flagged']} "
+          f"max {st['orth_max']:.2e}", flush=True)
+    yield w, plan
+    plan.close()
+
+
+def test_c2_pattern_bitexact(c2):
+    w, plan = c2
+    indptr, indices, _ = plan.csc(with_data=False)
+    p, i = O.sparse_init(w.sls.indices, w.sls.offsets, w.sls.nv)
+    assert len(indices) == len(i) == 64553682
+    assert np.array_equal(indptr.astype(np.int64), p)
+    assert np.array_equal(indices, i)
+
+
+def test_c2_local_transforms(c2):
+    w, plan = c2
+    which = np.sort(np.random.default_rng(2).choice(w.sls.count, N_LOCAL, replace=False))
+    _check_local(plan.local(which), _oracle_local(w, which), "c2")
+
+
+def test_c2_columns(c2):
+    """Eight CSC columns: the vertices of one searchlight, whose columns are
+    sums over the about 500 searchlights that hold any of them."""
+    w, plan = c2
+    s = w.sls
+    indptr, indices, data = plan.csc()
+    cols = [int(c) for c in s.indices[s.offsets[20000]:s.offsets[20001]][::40][:8]]
+    assert len(cols) == 8
+    wts = O.searchlight_weights_flat(s.indices, s.offsets, s.dists, s.radius, s.nv)
+    ref = O.searchlight_procrustes_csc(w.X, w.Y, s.indices, s.offsets, indptr, indices, weights=wts,
+                                       which_cols=cols, n_jobs=JOBS)
+    worst_max, worst_fro = 0.0, 0.0
+    for c in cols:
+        sl = slice(indptr[c], indptr[c + 1])
+        d = data[sl] - ref[sl]
+        worst_max = max(worst_max, float(np.abs(d).max()))
+        worst_fro = max(worst_fro, float(np.linalg.norm(d) / np.linalg.norm(ref[sl])))
+    print(f"[c2 columns] {cols}: max-abs {worst_max:.3e}, rel Frobenius {worst_fro:.3e}", flush=True)
+    assert worst_max <= TOL_MAX and worst_fro <= TOL_FRO
+
+
+def test_c2_orthogonality_every_transform(c2):
+    w, plan = c2
+    e = plan.orthogonality(exact=True)
+    assert e.shape == (w.sls.count,) and np.all(np.isfinite(e))
+    print(f"[c2 orth] exact ||Q^T Q - I||_F: max {e.max():.3e}, median {np.median(e):.3e}", flush=True)
+    assert e.max() <= TOL_ORTH
+    st = plan.stats()
+    assert st["n_orth_checked"] + st["n_fallback"] >= w.sls.count
+    # the pass's four-probe estimate agrees with the exact value within a factor of 3
+    p = plan.orthogonality(exact=False)
+    ok = e > 1e-6
+    ratio = p[ok] / e[ok]
+    assert np.median(ratio) > 0.7 and np.median(ratio) < 1.4
+    assert ratio.min() > 1.0 / 3.0 and ratio.max() < 3.0
+
+
+def test_c2_no_reflection_local():
+    """isReflection = false on 64 C2 searchlights: every local transform has
+    det = +1 and matches LAPACK dgesdd + LU sign + rank-1 flip (the oracle,
+    csrc/ha_searchlight.c:612-632)."""
+    w = synth.make_workload("c2")
+    which = np.arange(0, w.sls.count, w.sls.count // 64)[:64]
+    plan = _pass(w, reflection=False, which=which)
+    try:
+        got = plan.local(np.arange(len(which)))
+    finally:
+        plan.close()
+    ref = _oracle_local(w, which, reflection=False)
+    for g in got:
+        assert abs(np.linalg.det(g.astype(np.float64)) - 1.0) < 1e-3
+    _check_local(got, ref, "c2 no reflection")
+
+
+def test_c3_cell_seed101_noise1():
+    """One held-out cell of the learned-scaling evaluation grid (seed 101,
+    n = 500, noise 1.0) on the full hemisphere, against the oracle."""
+    w = synth.make_workload("c2", seed=101, n=500, noise=1.0)
+    plan = _pass(w)
+    try:
+        st = plan.stats()
+        which = np.sort(np.random.default_rng(3).choice(w.sls.count, N_LOCAL, replace=False))
+        got = plan.local(which)
+        print(f"[c3 cell] mean steps {st['steps_total'] / w.sls.count:.2f}, fp64 {st['n_fallback']}", flush=True)
+    finally:
+        plan.close()
+    _check_local(got, _oracle_local(w, which), "c3 seed 101 noise 1.0")
+
+
+def test_c1_jacobi32_columns():
+    """The FP32 Jacobi comparator at its own workload (10,242 vertices, n = 300)
+    against the oracle on sampled CSC columns."""
+    w = synth.make_workload("c1")
+    s = w.sls
+    ha.set_options(method="jacobi32")
+    try:
+        plan = _pass(w)
+        indptr, indices, data = plan.csc()
+        plan.close()
+    finally:
+        ha.set_options(method="auto")
+    cols = [0, 17, 5000, 10241]
+    wts = O.searchlight_weights_flat(s.indices, s.offsets, s.dists, s.radius, s.nv)
+    ref = O.searchlight_procrustes_csc(w.X, w.Y, s.indices, s.offsets, indptr, indices, weights=wts,
+                                       which_cols=cols, n_jobs=JOBS)
+    for c in cols:
+        sl = slice(indptr[c], indptr[c + 1])
+        assert np.abs(data[sl] - ref[sl]).max() <= TOL_MAX
+        assert np.linalg.norm(data[sl] - ref[sl]) <= TOL_FRO * np.linalg.norm(ref[sl])
+
+
+def test_c1_no_reflection_local():
+    """isReflection = false on 64 searchlights of the 10,242-vertex workload
+    against the oracle (LAPACK + LU sign), with det = +1 for each."""
+    w = synth.make_workload("c1")
+    which = np.arange(0, w.sls.count, 157)[:64]
+    plan = _pass(w, reflection=False, which=which)
+    try:
+        got = plan.local(np.arange(len(which)))
+    finally:
+        plan.close()
+    for g in got:
+        assert abs(np.linalg.det(g.astype(np.float64)) - 1.0) < 1e-3
+    _check_local(got, _oracle_local(w, which, reflection=False), "c1 no reflection")

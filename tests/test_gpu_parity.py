@@ -408,3 +408,41 @@ def test_compute_searchlight_weights_sparse():
     assert np.array_equal(np.concatenate(lists), ref)
     assert mat.shape == (s.count, s.nv)
     assert np.abs(np.asarray(mat.sum(axis=0)).ravel() - 1).max() < 1e-12
+
+
+# ---- hand-derived driver case (row weights, non-symmetric local transforms) ----
+
+@pytest.mark.parametrize("method,tol", [("jacobi64", TOL_64), ("jacobi32", TOL_T), ("polar", TOL_T)])
+def test_driver_row_weights_kat(method, tol):
+    """The known answer of tests/test_oracle_kat.py (derived from
+    csrc/ha_searchlight.c:970-1001,1072-1085), on the GPU dense and CSC entries."""
+    from tests import test_oracle_kat as K
+    ha.set_options(method=method)
+    T = ha.searchlight_procrustes_flat(K.KAT_X, K.KAT_Y, K.KAT_IDX, K.KAT_OFF, K.KAT_DIST, K.KAT_RADIUS)
+    assert np.abs(T - K.KAT_T).max() < tol
+    lists = [K.KAT_IDX[0:2], K.KAT_IDX[2:4]]
+    m = ha.sparse_init(lists, 3)
+    wts = O.searchlight_weights_flat(K.KAT_IDX, K.KAT_OFF, K.KAT_DIST, K.KAT_RADIUS, 3)
+    ha.searchlight_procrustes_csc(K.KAT_X, K.KAT_Y, lists, m, weights=[wts[0:2], wts[2:4]])
+    assert np.abs(m.toarray() - K.KAT_T).max() < tol
+
+
+def test_unrepaired_flags_are_an_error_without_fp64_fallback():
+    """With the FP64 fallback off, a pass that leaves searchlights flagged (here
+    every one: a step budget of 2 stops the iteration early) returns
+    kHaErrorSvd instead of scattering unfinished transforms as a success."""
+    w = synth.make_workload("c0")
+    s = w.sls
+    plan = ha.Plan(w.X.shape[0], s.nv, s.indices, s.offsets, s.dists, s.radius)
+    try:
+        plan.load(w.X, w.Y)
+        ha.set_options(method="polar", fp64_fallback=0, max_polar_steps=2)
+        with pytest.raises(ha.HaError) as ei:
+            plan.execute()
+        assert ei.value.rc == -2
+        ha.set_options(fp64_fallback=1)
+        plan.execute()   # the same budget with the fallback on: repaired by the FP64 path
+        assert plan.stats()["n_fallback"] == s.count
+    finally:
+        ha.set_options(method="auto", fp64_fallback=1, max_polar_steps=28)
+        plan.close()
