@@ -123,6 +123,8 @@ int ha_hip_init(void) {
 	HCK(hipGetDeviceProperties(&prop, 0));
 	g.n_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
 	HCK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+	/* device address of the no-allocation entry point's scratch array (device 0) */
+	KCK(hak_noalloc_attach());
 	g.ready = true;
 	rc = build_schedules();
 	if (rc != kHaSuccess) { ha_hip_cleanup(); return rc; }

@@ -180,6 +180,11 @@ int hak_orth(const float *q, const int64_t *qoff, const int32_t *qld, const int3
              const int32_t *order, int32_t n, int32_t kpmax, int exact, double *out, int32_t *flag,
              double tol, int32_t n_wg, void *stream);
 
+/* ---- no-allocation ha_procrustes (k_noalloc.hip) ---- */
+/* Looks up the device address of the library's static scratch array for the
+ * current device (once per device); 0 or a hipError_t. */
+int hak_noalloc_attach(void);
+
 /* ---- spectral-scaling MLP (k_mlp.hip) ---- */
 /* Y[N][2] = MLP(X[N][12]) with the given parameters and input normalisation. */
 int hak_mlp_forward(const float *params, const float *norm, const float *X, int64_t N, float *Y, void *stream);

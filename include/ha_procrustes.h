@@ -22,6 +22,28 @@ int ha_procrustes(const TMat *X, const TMat *Y, TMat *T, bool isReflection, bool
 /* Replaces ha_procrustes.h:27 / ha_procrustes.c:129-222 (computed in FP64, rounded to FP32). */
 int ha_procrustes_f32(const TMatF *X, const TMatF *Y, TMatF *T, bool isReflection, bool isScaling);
 
+/* ---- no-allocation variant (no counterpart in the reference) ----
+ *
+ * ha_procrustes on device buffers, for callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc. X, Y (M x N) and T (N x N) are device pointers in the
+ * row-major layout of TMat; the work is enqueued on `stream` (NULL: the null
+ * stream) and the call returns without waiting, so the caller synchronizes
+ * before reading T. No hipMalloc, hipFree or allocating library call is made:
+ * the work lives in a static device array of HA_NOALLOC_SCRATCH_BYTES that
+ * each GPU holds until the process ends. Sizes beyond N <= HA_NOALLOC_MAX_N and
+ * M * N <= HA_NOALLOC_MAX_MN return kHaErrorArg. All calls on one GPU share its
+ * array: calls that overlap on one GPU race; calls in order on one stream do
+ * not. ha_hip_init looks up the array for device 0; on another device the
+ * first call does so and must not itself be under graph capture. Same FP64
+ * Jacobi kernel as ha_procrustes, so the same T bit for bit. */
+#define HA_NOALLOC_MAX_N 512
+#define HA_NOALLOC_MAX_MN (1 << 20)
+#define HA_NOALLOC_MAX_DEVICES 16
+#define HA_NOALLOC_SCRATCH_BYTES (27 * (1 << 20) + 64 * 1024)
+int ha_procrustes_noalloc(const double *X, const double *Y, int32_t M, int32_t N, double *T, bool isReflection,
+                          bool isScaling, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
