@@ -39,9 +39,25 @@ WORKLOADS = {
 }
 
 
-def cpu_baseline(w, seconds, threads):
-    """Oracle (numpy/LAPACK FP64, one searchlight per thread) on a bounded sample:
-    random searchlights in batches of `threads` until `seconds` of wall time."""
+def cpu_model():
+    """The host CPU's model name and logical CPU count (from /proc/cpuinfo)."""
+    model, n = "unknown", 0
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                n += 1
+    except OSError:
+        pass
+    return model, n
+
+
+def cpu_baseline(w, seconds, threads, repeats=3):
+    """Oracle (numpy/LAPACK FP64 dgesdd, one searchlight per thread, one BLAS
+    thread each) on bounded samples: `repeats` runs of random searchlights in
+    batches of `threads`, each for about seconds / repeats of wall time; the
+    median rate of the runs (the reference reports medians of repeats,
+    README.md:188). Returns (median rate, per-run rates, searchlights, transforms)."""
     from concurrent.futures import ThreadPoolExecutor
     from oracle import ha_oracle as O
 
@@ -55,17 +71,45 @@ def cpu_baseline(w, seconds, threads):
         return O.procrustes(w.X[:, sl], w.Y[:, sl])
 
     from threadpoolctl import threadpool_limits
-    done, Ts = [], []
+    done, Ts, rates = [], [], []
     # one LAPACK thread per worker: the pool supplies the parallelism
-    with threadpool_limits(limits=1, user_api="blas"):
-        t0 = time.perf_counter()
-        with ThreadPoolExecutor(threads) as ex:
-            while time.perf_counter() - t0 < seconds and len(done) < s.count:
+    with threadpool_limits(limits=1, user_api="blas"), ThreadPoolExecutor(threads) as ex:
+        for _ in range(repeats):
+            n0 = len(done)
+            t0 = time.perf_counter()
+            while time.perf_counter() - t0 < seconds / repeats and len(done) < s.count:
                 batch = perm[len(done):len(done) + threads]
                 Ts.extend(ex.map(one, batch))
                 done.extend(batch)
-        dt = time.perf_counter() - t0
-    return len(done) / dt, np.array(done), Ts
+            rates.append((len(done) - n0) / (time.perf_counter() - t0))
+    return float(np.median(rates)), rates, np.array(done), Ts
+
+
+def end_to_end(w):
+    """The whole call a reference user makes, through the C-ABI, from host
+    arrays to the host CSC transform: searchlight weights
+    (ha_searchlight_weights), the CSC pattern (ha_sparse_init) and
+    ha_searchlight_procrustes (plan, FP64 upload of X and Y, tile choice on a
+    first pass, the pass, download of the CSC data); the reference times this
+    call around hyperalignment_c.py:198-213 (benchmark_neuroboros.py:161-201).
+    Two calls: the first on a fresh library state, the second after it."""
+    s = w.sls
+    lists, dl = s.lists(), s.dist_lists()
+    out = []
+    for _ in range(2):
+        t0 = time.perf_counter()
+        wts = ha.searchlight_weights(lists, dl, s.radius)
+        t1 = time.perf_counter()
+        m = ha.sparse_init(lists, s.nv)
+        t2 = time.perf_counter()
+        ha.searchlight_procrustes_csc(w.X, w.Y, lists, m, weights=wts)
+        t3 = time.perf_counter()
+        out.append({"s": t3 - t0, "weights_s": t1 - t0, "pattern_s": t2 - t1, "procrustes_s": t3 - t2})
+        nnz = m.nnz
+        del m, wts
+    return {"first": out[0], "second": out[1], "nnz": nnz,
+            "includes": "weights, CSC pattern, plan and index build, FP64 upload of X and Y, tile choice, "
+                        "the pass, CSC download (host arrays in and out)"}
 
 
 def comparator(w, args, every):
@@ -115,6 +159,8 @@ def comparator(w, args, every):
     st = plan.stats()
     res["jacobi32"] = {"value": len(which) / dt, "hemisphere_s_est": s.count * dt / len(which),
                        "sample": f"every {every}th searchlight ({len(which)}), one pass",
+                       "note": "unoptimised comparator: scalar one-sided Jacobi, 16 lanes per column "
+                               "pair, columns in HBM, one workgroup barrier per tournament round",
                        "mean_sweeps": st["steps_total"] / max(1, len(which))}
     plan.close()
     print(f"[bench] jacobi32 sample {1e3 * dt:.1f} ms", file=sys.stderr, flush=True)
@@ -167,10 +213,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--method", default="auto", choices=sorted(ha.METHODS))
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=18.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--no-compare", action="store_true", help="skip the plain polar / Jacobi32 comparison")
+    ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end C-ABI call timing")
+    ap.add_argument("--graph", action="store_true",
+                    help="time replays of the captured hipGraph instead of eager passes (the iteration's "
+                         "duration then comes from one eager pass after the timed region)")
     ap.add_argument("--compare-every", type=int, default=40)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (reference for the HBM byte counters)")
@@ -190,7 +240,7 @@ def main():
     print(f"[bench] workload {args.config}: {s.count} searchlights, k {s.sizes.min()}-{s.sizes.max()}, "
           f"n={nt} (generated in {time.perf_counter() - t0:.1f}s)", file=sys.stderr, flush=True)
     ha.init()
-    ha.set_options(method=args.method, tile=args.tile)
+    ha.set_options(method=args.method, tile=args.tile, use_graph=1 if args.graph else 0)
     from hyperalignment_amd import learn
     if os.path.exists(learn.MODEL_PATH):
         learn.install(learn.MODEL_PATH)
@@ -215,6 +265,14 @@ def main():
 
     # dominant kernel: the local-transform kernel (cross-covariance + polar iteration)
     polar_avg = float(np.mean(polar_ms))
+    kernel_timing = "HIP events on the library stream around the iteration, inside the timed passes"
+    if not polar_avg > 0:
+        # a captured pass without its stage events: one eager pass after the timed region
+        ha.set_options(use_graph=0)
+        plan.execute()
+        polar_avg = plan.stats()["kernel_ms_polar_avg"]
+        ha.set_options(use_graph=1 if args.graph else 0)
+        kernel_timing = "HIP events around the iteration of one eager pass after the timed region"
     achieved = st["gemm_flops"] / (polar_avg * 1e-3)
     method = ha.resolved_method(args.method, int(s.sizes.max()))
     tiles = None
@@ -232,7 +290,7 @@ def main():
     roofline = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_BF16 / 1e12, "unit": "TFLOP/s",
                 "frac": achieved / PEAK_BF16, "traffic": pmc_traffic(args.config, method, tiles_used, team),
                 "kernel": kernel,
-                "kernel_ms": polar_avg, "flops_per_launch": st["gemm_flops"]}
+                "kernel_ms": polar_avg, "flops_per_launch": st["gemm_flops"], "kernel_timing": kernel_timing}
     stats_main = st
     # apply and score the transform on the training data (Yhat = X T, vertex-wise r)
     t0 = time.perf_counter()
@@ -243,6 +301,25 @@ def main():
     evaluate = {"ms_apply": ev_st["ms_apply"], "ms_corr": ev_st["ms_corr"],
                 "apply_algorithmic_GBs": ev_bytes / (ev_st["ms_apply"] * 1e-3) / 1e9,
                 "r_mean": float(np.nanmean(r)), "host_s": time.perf_counter() - t0}
+    graph = None
+    if not args.graph:
+        # the same pass captured in a hipGraph and replayed (no per-stage events inside)
+        ha.set_options(use_graph=1)
+        plan.execute()
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            plan.execute()
+            ts.append(time.perf_counter() - t0)
+        ha.set_options(use_graph=0)
+        graph = {"s_per_pass": float(np.median(ts)), "runs": ts,
+                 "gpu_ms_last": plan.stats()["ms_total"]}
+        print(f"[bench] graph replay {1e3 * graph['s_per_pass']:.1f} ms", file=sys.stderr, flush=True)
+    e2e = None
+    if not args.no_e2e:
+        e2e = end_to_end(w)
+        print(f"[bench] end to end {e2e['first']['s']:.2f} s, then {e2e['second']['s']:.2f} s", file=sys.stderr,
+              flush=True)
     compare = None
     if not args.no_compare and method in ("polar", "polar_learned"):
         compare = comparator(w, args, args.compare_every)
@@ -254,12 +331,17 @@ def main():
     cpu = None
     parity = None
     if not args.no_cpu:
-        threads = min(os.cpu_count() or 1, 16)
-        rate, which, Ts = cpu_baseline(w, args.cpu_seconds, threads)
+        avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+        threads = min(avail, 16)
+        rate, rates, which, Ts = cpu_baseline(w, args.cpu_seconds, threads)
+        model, ncpu = cpu_model()
         cpu = {"value": rate, "unit": "searchlights/s", "cores": threads, "kind": "port",
-               "sample": f"{len(which)} random searchlights of {args.config} (about {args.cpu_seconds:.0f} s), "
-                         f"numpy/LAPACK FP64 dgesdd (oracle), one searchlight per thread; hemisphere "
-                         f"estimate {s.count / rate:.1f} s"}
+               "sample": f"median of 3 runs ({', '.join(f'{r:.1f}' for r in rates)} searchlights/s) over "
+                         f"{len(which)} random searchlights of {args.config} (about {args.cpu_seconds:.0f} s in "
+                         f"all), numpy/LAPACK FP64 dgesdd (the oracle), one searchlight per thread and one BLAS "
+                         f"thread each, {threads} threads (of {avail} CPUs available to the process, at most 16 "
+                         f"used; {ncpu} logical CPUs, {model}); hemisphere estimate {s.count / rate:.1f} s",
+               "cpu_model": model, "logical_cpus": ncpu, "cpus_available": avail, "runs": rates}
         loc = plan.local(which)
         errs = [float(np.abs(a - b).max()) for a, b in zip(loc, Ts)]
         fro = [float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(loc, Ts)]
@@ -281,6 +363,8 @@ def main():
         "dtype": "bf16x3" if method in ("polar", "polar_learned") else method,
         "data": "synthetic",
         "config": {"workload": f"{args.config}: {WORKLOADS[args.config]}", "method": method,
+                   "launch": "hipGraph replay (captured on the first pass)" if args.graph else "eager",
+                   "graph_replay": graph,
                    "searchlights": s.count, "hemisphere_s": dt / args.steps,
                    "pattern_ms": st["ms_pattern"], "nnz": st["nnz"], "fp64_fallback": st["n_fallback"],
                    "unconverged": st["n_unconverged"],
@@ -292,6 +376,7 @@ def main():
                    "stage_ms": {k: st[k] for k in ("ms_weights", "ms_crosscov", "ms_polar", "ms_fallback",
                                                    "ms_scatter")},
                    "iteration_tile": tiles, "parity_vs_fp64": parity, "compare": compare,
+                   "end_to_end": e2e,
                    "evaluate": evaluate},
         "roofline": roofline,
         "cpu_baseline": cpu,

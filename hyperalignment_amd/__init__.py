@@ -108,6 +108,7 @@ _SIGS = {
     "ha_sparse_scatter_add": (None, [_P(TSparseCSC), _P(_i32), _P(_i32), _i32, _i32, _P(_d), _P(_d)]),
     "ha_procrustes": (ctypes.c_int, [_P(TMat), _P(TMat), _P(TMat), ctypes.c_bool, ctypes.c_bool]),
     "ha_ridge": (ctypes.c_int, [_P(TMat), _P(TMat), _d, _P(TMat)]),
+    "ha_polar_newton": (ctypes.c_int, [_P(_f), _P(_f), _i32, ctypes.c_bool, ctypes.c_bool, _P(_f), _i32]),
     "ha_ridge_vec": (ctypes.c_int, [_P(TMat), _P(_d), _d, _P(_d)]),
     "ha_searchlight_ridge": (ctypes.c_int, [_P(TMat), _P(TMat), _P(TSearchlights), _P(TSearchlights),
                                             _P(TSparseCSC), _P(_P(_d)), _d]),
@@ -633,6 +634,23 @@ def procrustes(X, Y, reflection=True, scaling=False):
     tt = TMat(_ptr(T, ctypes.c_double), T.shape[0], T.shape[1])
     _check("ha_procrustes", lib().ha_procrustes(ctypes.byref(tx), ctypes.byref(ty), ctypes.byref(tt),
                                                 bool(reflection), bool(scaling)))
+    return T
+
+
+def polar_newton(A, reflection=True, scaling=False, X=None):
+    """ha_polar_newton (csrc/ha_procrustes.c:271-366) on the GPU: the orthogonal
+    polar factor of the square float32 matrix A, with the reference's
+    reflection rule (det < 0: last column negated) and scaling by
+    trace(T^T A) / (sum var(X) * M) when X (M x N) is given."""
+    A = np.ascontiguousarray(A, dtype=np.float32)
+    n = A.shape[0]
+    if A.shape != (n, n):
+        raise ValueError("A must be square")
+    T = np.zeros((n, n), dtype=np.float32)
+    Xf = None if X is None else np.ascontiguousarray(X, dtype=np.float32)
+    m = 0 if Xf is None else Xf.shape[0]
+    _check("ha_polar_newton", lib().ha_polar_newton(_ptr(A, ctypes.c_float), _ptr(T, ctypes.c_float), n,
+                                                    bool(reflection), bool(scaling), _ptr(Xf, ctypes.c_float), m))
     return T
 
 

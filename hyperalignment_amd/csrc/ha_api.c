@@ -426,6 +426,63 @@ int ha_procrustes_f32(const TMatF *X, const TMatF *Y, TMatF *T, bool isReflectio
 	return rc;
 }
 
+/* ---- polar factor of a given matrix (ha_procrustes.c:271-366) ---- */
+
+/* The reference runs the unscaled Newton iteration X <- (X + X^-T)/2 on A in
+ * FP32 (at most 20 steps, relative change below 1e-6); here A goes through the
+ * GPU polar path as a one-searchlight pass with X = I (so the cross-covariance
+ * is A itself, formed with FP32 products) and Y = A: the FP32-product iteration
+ * of the nearly rank-deficient searchlights, from the smallest bucket's
+ * schedule, with the FP64 Jacobi path behind it. Both converge to the
+ * orthogonal polar factor of A. The reference's reflection and scaling rules
+ * follow: det(T) < 0 (LU sign of T) negates T's last column; with isScaling and
+ * X_data, T is multiplied by trace(T^T A) / (sum_j var(X[:, j]) * M). */
+int ha_polar_newton(const float *A, float *T, int32_t N, bool isReflection, bool isScaling, const float *X_data,
+                    int32_t M) {
+	if (!A || !T || N <= 0) return kHaErrorArg;
+	if (isScaling && X_data && M <= 0) return kHaErrorArg;
+	const size_t nn = (size_t)N * N;
+	double *xi = (double *)calloc(nn, sizeof(double)), *ya = (double *)malloc(nn * sizeof(double));
+	int32_t *idx = (int32_t *)malloc((size_t)N * sizeof(int32_t));
+	int32_t off[2] = {0, N}, neg = 0, which = 0;
+	HaHipPlan *p = NULL;
+	int rc = kHaErrorAlloc;
+	if (!xi || !ya || !idx) goto done;
+	for (int32_t i = 0; i < N; ++i) {
+		xi[(size_t)i * N + i] = 1.0;
+		idx[i] = i;
+	}
+	for (size_t e = 0; e < nn; ++e) ya[e] = A[e];
+	rc = ha_hip_plan_create(N, N, idx, off, NULL, 1, 0.0, &p);
+	if (!rc) rc = ha_plan_load_xy(p, xi, ya, false);
+	if (!rc) rc = ha_plan_run(p, true, false, true, kHaHipMethodPolar);
+	if (!rc) rc = ha_hip_plan_download_local(p, &which, 1, T);
+	if (!rc && !isReflection) rc = ha_plan_det_negative(p, &neg);
+	if (rc) goto done;
+	if (neg)
+		for (int32_t i = 0; i < N; ++i) T[(size_t)i * N + (N - 1)] = -T[(size_t)i * N + (N - 1)];
+	if (isScaling && X_data) {
+		double tr = 0.0, var_sum = 0.0;
+		for (size_t e = 0; e < nn; ++e) tr += (double)T[e] * (double)A[e];  /* trace(T^T A) */
+		for (int32_t j = 0; j < N; ++j) {
+			double mean = 0.0, var = 0.0;
+			for (int32_t t = 0; t < M; ++t) mean += X_data[(size_t)t * N + j];
+			mean /= M;
+			for (int32_t t = 0; t < M; ++t) {
+				const double d = X_data[(size_t)t * N + j] - mean;
+				var += d * d;
+			}
+			var_sum += var / M;
+		}
+		const double scale = tr / (var_sum * M);
+		for (size_t e = 0; e < nn; ++e) T[e] = (float)(T[e] * scale);
+	}
+done:
+	ha_hip_plan_destroy(p);
+	free(xi); free(ya); free(idx);
+	return rc;
+}
+
 /* ---- templates ---- */
 
 int ha_searchlight_template_flat(const double *const *dms, int32_t n_subj, int32_t nt, int32_t nv,

@@ -235,6 +235,7 @@ struct HaHipPlan {
 	int32_t *d_which;
 	HaHipStats stats;
 	hipEvent_t ev[8];
+	hipEvent_t ev_graph[2];  /* around a graph launch */
 	bool events;
 	/* polar iteration tiles: per kp bucket, and the k-sorted searchlights of each tile */
 	int32_t bucket_tile[HA_TILE_BUCKETS];
@@ -260,8 +261,10 @@ static void plan_free(HaHipPlan *p) {
 	for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i)
 		if (ptrs[i]) hipFree(ptrs[i]);
 	if (p->gexec) hipGraphExecDestroy(p->gexec);
-	if (p->events)
+	if (p->events) {
 		for (int i = 0; i < 8; ++i) hipEventDestroy(p->ev[i]);
+		for (int i = 0; i < 2; ++i) hipEventDestroy(p->ev_graph[i]);
+	}
 	free(p->h_k);
 	free(p);
 }
@@ -371,6 +374,7 @@ int ha_plan_create_ex(int32_t nt, const HaPlanSetup *su, const int32_t *slx_idx,
 	qsort_r(h_order, (size_t)count, sizeof(int32_t), cmp_k_desc, p->h_k);
 
 	for (int i = 0; i < 8; ++i) HCK(hipEventCreate(&p->ev[i]));
+	for (int i = 0; i < 2; ++i) HCK(hipEventCreate(&p->ev_graph[i]));
 	p->events = true;
 
 	ACK(p->slx, total * sizeof(int32_t));
@@ -1106,6 +1110,31 @@ static int ensure_work64(HaHipPlan *p, int32_t want) {
 	return kHaSuccess;
 }
 
+/* Stage times of the last eager pass from the events enqueue_pass recorded. */
+static void stage_times(HaHipPlan *p, bool is_polar) {
+	float t[5] = {0, 0, 0, 0, 0};
+	for (int i = 0; i < 5; ++i) hipEventElapsedTime(&t[i], p->ev[i], p->ev[i + 1]);
+	float tot = 0, tc = 0, ti = t[1], tk = t[1], to = 0;
+	hipEventElapsedTime(&tot, p->ev[0], p->ev[5]);
+	if (is_polar) {
+		hipEventElapsedTime(&tc, p->ev[1], p->ev[6]);
+		hipEventElapsedTime(&ti, p->ev[6], p->ev[2]);
+		hipEventElapsedTime(&tk, p->ev[6], p->ev[7]);  /* iteration (+ reflection fix) */
+		hipEventElapsedTime(&to, p->ev[7], p->ev[2]);  /* orthogonality check */
+	}
+	p->stats.ms_weights = t[0];
+	p->stats.ms_crosscov = tc;  /* polar path: gather + cross-covariance (+ features) stage */
+	p->stats.ms_polar = ti;     /* local solver (the iteration stage on the polar path) */
+	p->stats.ms_orth = to;
+	p->stats.ms_fallback = t[2];
+	p->stats.ms_scatter = t[4] + t[3];
+	p->stats.ms_total = tot;
+	p->stats.kernel_ms_polar_avg = tk;
+	p->stats.polar_launches =
+	    is_polar ? (p->tile_count[0] > 0) + (p->tile_count[1] > 0) + (p->tile_count[2] > 0) : 1;
+	(void)hipGetLastError();
+}
+
 int ha_plan_run(HaHipPlan *p, bool isReflection, bool isScaling, bool zero_data, int method_in) {
 	int rc = kHaSuccess;
 	uint64_t *clk = NULL;
@@ -1157,26 +1186,7 @@ int ha_plan_run(HaHipPlan *p, bool isReflection, bool isScaling, bool zero_data,
 		rc = enqueue_pass(p, a, method, isReflection, isScaling, zero_data, st, true);
 		if (rc) goto fail;
 		HCK(hipStreamSynchronize(st));
-		float t[5];
-		for (int i = 0; i < 5; ++i) hipEventElapsedTime(&t[i], p->ev[i], p->ev[i + 1]);
-		float tot = 0, tc = 0, ti = t[1], tk = t[1], to = 0;
-		hipEventElapsedTime(&tot, p->ev[0], p->ev[5]);
-		if (is_polar) {
-			hipEventElapsedTime(&tc, p->ev[1], p->ev[6]);
-			hipEventElapsedTime(&ti, p->ev[6], p->ev[2]);
-			hipEventElapsedTime(&tk, p->ev[6], p->ev[7]);  /* iteration (+ reflection fix) */
-			hipEventElapsedTime(&to, p->ev[7], p->ev[2]);  /* orthogonality check */
-		}
-		p->stats.ms_weights = t[0];
-		p->stats.ms_crosscov = tc;  /* polar path: gather + cross-covariance (+ features) stage */
-		p->stats.ms_polar = ti;     /* local solver (the iteration stage on the polar path) */
-		p->stats.ms_orth = to;
-		p->stats.ms_fallback = t[2];
-		p->stats.ms_scatter = t[4] + t[3];
-		p->stats.ms_total = tot;
-		p->stats.kernel_ms_polar_avg = tk;
-		p->stats.polar_launches =
-		    is_polar ? (p->tile_count[0] > 0) + (p->tile_count[1] > 0) + (p->tile_count[2] > 0) : 1;
+		stage_times(p, is_polar);
 		debug_report(p, &a, clk, nclk);
 	} else {
 		/* replay the captured pass when nothing it depends on changed */
@@ -1201,12 +1211,12 @@ int ha_plan_run(HaHipPlan *p, bool isReflection, bool isScaling, bool zero_data,
 			p->gkey.scal = isScaling; p->gkey.zero = zero_data;
 			p->gkey.team = g.opt.team; p->gkey.team_spin = g.opt.team_spin;
 		}
-		HCK(hipEventRecord(p->ev[0], st));
+		HCK(hipEventRecord(p->ev_graph[0], st));
 		HCK(hipGraphLaunch(p->gexec, st));
-		HCK(hipEventRecord(p->ev[5], st));
+		HCK(hipEventRecord(p->ev_graph[1], st));
 		HCK(hipStreamSynchronize(st));
 		float tot = 0;
-		hipEventElapsedTime(&tot, p->ev[0], p->ev[5]);
+		hipEventElapsedTime(&tot, p->ev_graph[0], p->ev_graph[1]);
 		memset(&p->stats.ms_weights, 0, sizeof(double) * 5);  /* ms_weights .. ms_scatter: not split */
 		p->stats.ms_orth = 0;
 		p->stats.ms_total = tot;
@@ -1289,6 +1299,29 @@ int ha_hip_plan_stats(const HaHipPlan *p, HaHipStats *s) {
 	if (!p || !s) return kHaErrorArg;
 	*s = p->stats;
 	return kHaSuccess;
+}
+
+int ha_plan_det_negative(HaHipPlan *p, int32_t *neg) {
+	if (!p || !neg) return kHaErrorArg;
+	if (!p->loaded || !p->q) { ERR("plan has no local transforms yet\n"); return kHaErrorArg; }
+	int rc = kHaSuccess;
+	hipStream_t st = g.stream;
+	int32_t *d = NULL;
+	ACK(d, (size_t)p->count * sizeof(int32_t));
+	{
+		HakLocalArgs a = local_args(p);
+		a.order = NULL;
+		a.n_order = p->count;
+		a.queue = p->queues + 0;
+		a.det_neg = d;
+		KCK(hak_fill_i32(p->queues, 1, 0, st));
+		KCK(hak_detfix(&a, p->work64, p->stride64, p->iota, p->n_slots64, st));
+	}
+	HCK(hipMemcpyAsync(neg, d, (size_t)p->count * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+	HCK(hipStreamSynchronize(st));
+fail:
+	if (d) hipFree(d);
+	return rc;
 }
 
 int ha_hip_plan_orthogonality(HaHipPlan *p, int exact, double *out) {

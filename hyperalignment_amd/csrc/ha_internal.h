@@ -51,6 +51,9 @@ int ha_plan_load_xy(HaHipPlan *p, const double *X, const double *Y, bool col_maj
  * method < 0: use the global options. */
 int ha_plan_run(HaHipPlan *p, bool isReflection, bool isScaling, bool zero_data, int method);
 int ha_plan_download_data(const HaHipPlan *p, double *data);
+/* neg[s] = 1 when the local transform in the plan's store has det < 0 (the LU
+ * sign of k_detfix), 0 otherwise; the store is left as it is. */
+int ha_plan_det_negative(HaHipPlan *p, int32_t *neg);
 /* FP64 local transforms (Jacobi-64 path) of every searchlight, row-major k x k each. */
 int ha_plan_local64(HaHipPlan *p, bool isReflection, bool isScaling, double *out);
 int ha_plan_ridge(HaHipPlan *p, double alpha, bool zero_data, double *betas64);

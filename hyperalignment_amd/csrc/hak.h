@@ -128,12 +128,19 @@ typedef struct {
 	int32_t skip_done;                   /* polar: skip searchlights whose steps[] is already nonzero */
 	int32_t *team_count;                 /* [0] teams whose barrier gave up, [1] searchlights the
 	                                        one-workgroup kernel took over (skip_done); may be NULL */
+	int32_t *det_neg;                    /* hak_detfix: when set, only det_neg[s] = (det Q_s < 0) for
+	                                        every searchlight of the order, flagged or not; Q unchanged */
 } HakLocalArgs;
 
-/* Searchlights the polar path treats as rank deficient: z-scored columns span
- * at most nt - 1 dimensions, so A = X_sl^T Y_sl is singular for k >= nt and,
- * for k = nt - 1, full rank but as ill-conditioned as the samples allow. */
-#define HAK_NEAR_DEFICIENT(k, nt) ((k) >= (nt) - 1)
+/* Searchlights the polar path treats as (nearly) rank deficient: z-scored
+ * columns span at most nt - 1 dimensions, so A = X_sl^T Y_sl is singular for
+ * k >= nt, and its conditioning degrades as c = k / (nt - 1) approaches 1
+ * (for Gaussian columns kappa(X^T X) ~ ((1 + sqrt c) / (1 - sqrt c))^2, about
+ * 1.4e3 at c = 0.9). From c >= 0.9 on, A and the iteration run on FP32
+ * products: bf16x3 left 1.4e-4 of error at n = 16, k = 14 and 15 (c 0.93,
+ * 1.0), FP32 products 3.5e-6. The full-hemisphere workload (c <= 0.884) stays
+ * on bf16x3. */
+#define HAK_NEAR_DEFICIENT(k, nt) (10 * (int64_t)(k) >= 9 * ((int64_t)(nt) - 1))
 
 /* Workgroups per CU of the polar iteration stage (launch bound and slot count). */
 #define HAK_POLAR_OCC 3
@@ -164,7 +171,8 @@ int hak_jacobi64(const HakLocalArgs *a, int want_det_fix, double *work64, int64_
  * searchlight of a->order with flag 0 whose Q has det < 0, Q <- Q - 2 (Q v) v^T
  * with v the smallest right singular vector of the FP64 A (a->sigma_sum, when
  * set, loses 2 sigma_min); a failed Cholesky sets the flag for the FP64 path.
- * work64: n_wg * 4 * stride64 doubles; iota: 0, 1, ..., kpmax - 1. */
+ * work64: n_wg * 4 * stride64 doubles; iota: 0, 1, ..., kpmax - 1. With
+ * a->det_neg set it only records the signs (see HakLocalArgs). */
 int hak_detfix(const HakLocalArgs *a, double *work64, int64_t stride64, const int32_t *iota, int n_wg,
                void *stream);
 

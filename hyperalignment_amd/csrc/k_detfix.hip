@@ -263,13 +263,17 @@ __global__ __launch_bounds__(RT, 4) void k_detfix(HakLocalArgs a, double *work64
     const int it = item_sh;
     if (it >= a.n_order) break;
     const int s = a.order ? a.order[it] : it;
-    if (a.flag[s] != 0) continue;  // the FP64 path (with the fix) takes it
+    if (!a.det_neg && a.flag[s] != 0) continue;  // the FP64 path (with the fix) takes it
     const int o = a.sl_off[s], k = a.sl_off[s + 1] - o, kp = a.qld[s];
     float *Q = a.q + a.qoff[s];
     for (int64_t e = threadIdx.x; e < (int64_t)kp * kp; e += RT) M[e] = Q[e];
     __syncthreads();
     const int sgn = lu_sign(M, k, kp, pmax, pidx, &piv_sh, &sign_sh, reinterpret_cast<float (*)[B + 1]>(&t0[0][0]),
                             reinterpret_cast<float (*)[B + 1]>(&t1[0][0]));
+    if (a.det_neg) {  // signs only
+      if (threadIdx.x == 0) a.det_neg[s] = sgn < 0 ? 1 : 0;
+      continue;
+    }
     if (sgn >= 0) continue;
     // G = A64^T A64 (lower part; identity on the padding) and its Cholesky factor
     crossprod64(a.X64, a.slx_idx + o, a.Y64, a.sly_idx + o, a.nt, k, kp, 0.0, 1.0, A64, false, t0, t1);
@@ -337,7 +341,7 @@ __global__ __launch_bounds__(RT, 4) void k_detfix(HakLocalArgs a, double *work64
 extern "C" int hak_detfix(const HakLocalArgs *a, double *work64, int64_t stride64, const int32_t *iota, int n_wg,
                           void *stream) {
   if (a->n_order <= 0) return 0;
-  if (n_wg <= 0 || !a->flag || !iota) return (int)hipErrorInvalidValue;
+  if (n_wg <= 0 || (!a->flag && !a->det_neg) || !iota) return (int)hipErrorInvalidValue;
   hipLaunchKernelGGL(k_detfix, dim3(n_wg), dim3(RT), 0, (hipStream_t)stream, *a, work64, stride64, iota);
   LAUNCH_RET();
 }

@@ -44,6 +44,22 @@ __device__ void crosscov(GemmSmem<TILE> &sm, const HakLocalArgs &a, int s, int k
   });
 }
 
+// The same cross-covariance with FP32 operands and products (v_mfma_f32_32x32x2_f32),
+// for the nearly rank-deficient searchlights (HAK_NEAR_DEFICIENT): there A is
+// as ill-conditioned as the samples allow, and the bf16x3 products' 2^-16
+// relative error in A reaches the polar factor multiplied by that conditioning.
+// Out of line, so the registers of the bf16x3 kernel stay as they are.
+template <int TILE>
+__device__ __noinline__ void crosscov_f32(GemmSmem<TILE> &sm, const HakLocalArgs &a, int s, int k, int kp,
+                                          float *Ab) {
+  const int o = a.sl_off[s];
+  Opnd oa = opnd_gather(a.X, a.slx_idx + o, a.ld, k);
+  Opnd ob = opnd_gather(a.Y, a.sly_idx + o, a.ld, k);
+  wg_gemm<TILE, OP_F32_GATHER, OP_F32_GATHER, PREC_F32>(sm, oa, ob, kp, a.ld, false, [&](int m, int n, float4 v) {
+    *reinterpret_cast<float4 *>(Ab + m + (int64_t)n * kp) = v;
+  });
+}
+
 // Learned-scaling features of A (12 per searchlight), written when a.features != NULL.
 // fro2 = ||A||_F^2 as frob2 gives it.
 __device__ __noinline__ void features_of(const HakLocalArgs &a, int s, int k, int kp, const float *Ab, float *Pv,
@@ -231,14 +247,17 @@ __global__ __launch_bounds__(NT, 2) void k_crosscov(HakLocalArgs a) {
     // factor is not unique and its smallest singular values are rounding noise
     // (a few 1e-9 of ||A||_F). The plain scale with the schedule of the smallest
     // bucket takes that noise to 1 as well: one orthogonal completion, a valid
-    // Procrustes solution. k = nt - 1 fills the rank the samples allow: A is
-    // full rank but as ill-conditioned as the data get, and takes the same
-    // route (bf16x3 products measured 1.4e-4 from the FP64 factor there, FP32
-    // products stay within the bar). One that does not converge goes to the
-    // FP64 path.
+    // Procrustes solution. Nearly square searchlights (k / (nt - 1) >= 0.9,
+    // HAK_NEAR_DEFICIENT) are full rank but about as ill-conditioned as the
+    // data allow, and take the same route, with A itself from FP32 products
+    // (bf16x3 products measured 1.4e-4 from the FP64 factor there). One that
+    // does not converge goes to the FP64 path.
     const bool deficient = a.rank_check_nt && HAK_NEAR_DEFICIENT(k, a.nt);
     const uint64_t t0 = a.clk ? wall_clock64() : 0;
-    crosscov<TILE>(sm, a, s, k, kp, A);
+    if (deficient)
+      crosscov_f32<TILE>(sm, a, s, k, kp, A);
+    else
+      crosscov<TILE>(sm, a, s, k, kp, A);
     __syncthreads();
     const uint64_t t1 = a.clk ? wall_clock64() : 0;
     const double fro2 = frob2(A, kp, red);
@@ -482,8 +501,8 @@ extern "C" int hak_crosscov(const HakLocalArgs *a, int n_slots, void *stream) {
   LAUNCH_RET();
 }
 
-// prec_f32: FP32 products on 64 x 64 tiles, for rank-deficient searchlights
-// (k >= nt - 1, HAK_NEAR_DEFICIENT): their rounding-noise singular values are raised from about 1e-9
+// prec_f32: FP32 products on 64 x 64 tiles, for (nearly) rank-deficient
+// searchlights (HAK_NEAR_DEFICIENT): their rounding-noise singular values are raised from about 1e-9
 // to 1 over a long schedule, which carries the bf16x3 products' 2^-16 error
 // into the range part of Q (about 5e-5 against 1e-6 with FP32 products).
 extern "C" int hak_polar(const HakLocalArgs *a, void *stream) {

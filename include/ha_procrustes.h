@@ -22,6 +22,18 @@ int ha_procrustes(const TMat *X, const TMat *Y, TMat *T, bool isReflection, bool
 /* Replaces ha_procrustes.h:27 / ha_procrustes.c:129-222 (computed in FP64, rounded to FP32). */
 int ha_procrustes_f32(const TMatF *X, const TMatF *Y, TMatF *T, bool isReflection, bool isScaling);
 
+/* Replaces ha_procrustes.h:44 / ha_procrustes.c:271-366: T (N x N, row-major)
+ * = the orthogonal polar factor of A (N x N, row-major), on the GPU polar
+ * path (FP32 products) instead of the reference's CPU Newton iteration. With
+ * isReflection false and det(T) < 0, T's last column is negated (the
+ * reference's rule for this function, not the rank-1 SVD flip of
+ * ha_procrustes). With isScaling and X_data (M x N, row-major), T is scaled by
+ * trace(T^T A) / (sum_j var(X[:, j]) * M), population variances. Where the
+ * reference's FP32 inverse fails on an exactly singular A (kHaErrorInternal),
+ * this returns one orthogonal polar factor of it. */
+int ha_polar_newton(const float *A, float *T, int32_t N, bool isReflection, bool isScaling, const float *X_data,
+                    int32_t M);
+
 /* ---- no-allocation variant (no counterpart in the reference) ----
  *
  * ha_procrustes on device buffers, for callers that cannot allocate device

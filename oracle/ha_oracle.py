@@ -22,6 +22,9 @@ How each function is pinned (tests/test_oracle.py):
                             random orthogonal perturbations; scaling closed form
   searchlight_procrustes_*  brute-force dense accumulation on tiny inputs and the
                             CSC/dense agreement
+  polar_newton              the reference's own Newton iteration run to convergence
+                            (csrc/ha_procrustes.c:271-306); reflection and scaling
+                            rules checked on constructed cases
   template_procrustes       identical subjects give the input back
                             (csrc/ha_test.c:503-535)
   ridge                     recovers exact betas at alpha 1e-6 (csrc/ha_test.c:336-362);
@@ -164,6 +167,26 @@ def procrustes(Xs, Ys, reflection=True, scaling=False, return_sigma=False):
         T = T * (s.sum() / (var.sum() * n))
     if return_sigma:
         return T, s
+    return T
+
+
+def polar_newton(A, reflection=True, scaling=False, X=None):
+    """ha_polar_newton (csrc/ha_procrustes.c:271-366) at its limit: the Newton
+    iteration X <- (X + X^-T)/2 from A converges to the orthogonal polar factor
+    T = U Vt of A = U s Vt (nonsingular A). Without reflections, det(T) < 0
+    (LU sign of T, :310-326) negates T's last column (:328-331). With
+    scaling and X (M x N) given, T *= trace(T^T A) / (sum_j var(X[:, j]) * M),
+    population variances (:335-362)."""
+    A = np.asarray(A, dtype=np.float64)
+    U, s, Vt = safe_svd(A)
+    T = U @ Vt
+    if not reflection and det_sign(T) < 0.0:
+        T[:, -1] *= -1.0
+    if scaling and X is not None:
+        X = np.asarray(X, dtype=np.float64)
+        M = X.shape[0]
+        var = ((X - X.mean(axis=0)) ** 2).sum(axis=0) / M
+        T = T * (np.trace(T.T @ A) / (var.sum() * M))
     return T
 
 

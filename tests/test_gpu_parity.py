@@ -349,6 +349,10 @@ def test_graph_replay_matches_eager():
         for rep in range(3):
             ha.set_options(use_graph=1)
             plan.execute()
+            st = plan.stats()
+            print(f"[graph] replay {rep}: {st['ms_total']:.2f} ms, iteration {st['kernel_ms_polar_avg']:.2f} ms",
+                  flush=True)
+            assert st["ms_total"] > 0
             got = plan.csc()[2]
             assert np.array_equal(got.view(np.uint64), ref.view(np.uint64)), rep
             ha.set_options(use_graph=0)
@@ -446,3 +450,32 @@ def test_unrepaired_flags_are_an_error_without_fp64_fallback():
     finally:
         ha.set_options(method="auto", fp64_fallback=1, max_polar_steps=28)
         plan.close()
+
+
+# ---- ha_polar_newton (csrc/ha_procrustes.c:271-366) ----
+
+@pytest.mark.parametrize("n", [8, 50, 200])
+def test_polar_newton_matches_oracle(n):
+    """The polar factor of a square FP32 matrix against the oracle's (the limit
+    of the reference's Newton iteration, pinned in tests/test_oracle_kat.py), with
+    the reference's reflection rule (det < 0: last column negated)."""
+    rng = np.random.default_rng(20 + n)
+    A = rng.standard_normal((n, n)).astype(np.float32)
+    if np.linalg.det(A.astype(np.float64)) > 0:
+        A[:, 0] *= -1.0          # det < 0, so the reflection rule acts
+    A64 = A.astype(np.float64)
+    T = ha.polar_newton(A)
+    assert np.abs(T - O.polar_newton(A64)).max() < TOL_T
+    Tn = ha.polar_newton(A, reflection=False)
+    assert np.linalg.det(Tn.astype(np.float64)) > 0
+    assert np.abs(Tn - O.polar_newton(A64, reflection=False)).max() < TOL_T
+
+
+def test_polar_newton_scaling():
+    rng = np.random.default_rng(31)
+    X = rng.standard_normal((60, 12))
+    R = np.linalg.qr(rng.standard_normal((12, 12)))[0]
+    A = (X.T @ (1.5 * X @ R)).astype(np.float32)
+    T = ha.polar_newton(A, scaling=True, X=X)
+    ref = O.polar_newton(A.astype(np.float64), scaling=True, X=X.astype(np.float32).astype(np.float64))
+    assert np.abs(T - ref).max() < TOL_T * max(1.0, np.abs(ref).max())

@@ -113,3 +113,52 @@ def test_template_rotations_closed_form():
     n = S.shape[0]
     tpl = O.template_procrustes(dms, reflection=True, scaling=True, zscore_common=True, level2_iter=2)
     assert np.abs(tpl - S @ Rs[0] / np.sqrt(c / n)).max() < 1e-12
+
+
+def _newton(A, iters=60):
+    """X <- (X + X^-T)/2 from X = A, csrc/ha_procrustes.c:280-306, in FP64."""
+    X = np.array(A, dtype=np.float64)
+    for _ in range(iters):
+        Xn = 0.5 * (X + np.linalg.inv(X).T)
+        if np.linalg.norm(Xn - X) <= 1e-15 * np.linalg.norm(Xn):
+            return Xn
+        X = Xn
+    return X
+
+
+def test_polar_newton_is_the_newton_limit():
+    rng = np.random.default_rng(8)
+    for n in (3, 7, 20):
+        A = rng.standard_normal((n, n))
+        assert np.abs(O.polar_newton(A) - _newton(A)).max() < 1e-12
+
+
+def test_polar_newton_reflection_negates_last_column():
+    """det(A) < 0 and no reflections: the Newton factor with its last column
+    negated (csrc/ha_procrustes.c:328-331), not the SVD rank-1 flip that
+    ha_procrustes applies (csrc/ha_procrustes.c:56-95)."""
+    rng = np.random.default_rng(9)
+    A = rng.standard_normal((6, 6))
+    if np.linalg.det(A) > 0:
+        A[:, 0] *= -1.0
+    T = O.polar_newton(A, reflection=False)
+    ref = _newton(A)
+    ref[:, -1] *= -1.0
+    assert np.abs(T - ref).max() < 1e-12
+    assert np.linalg.det(T) > 0
+    # positive determinant: untouched
+    B = A.copy()
+    B[:, 0] *= -1.0
+    assert np.abs(O.polar_newton(B, reflection=False) - _newton(B)).max() < 1e-12
+
+
+def test_polar_newton_scaling_closed_form():
+    """Y = c X R (R orthogonal, X mean-zero columns): A = X^T Y = c X^T X R, the
+    polar factor is R, trace(R^T A) = c trace(X^T X) = c M sum var(X), so the
+    scaled factor is c R."""
+    rng = np.random.default_rng(10)
+    X = rng.standard_normal((40, 5))
+    X -= X.mean(axis=0)
+    R = np.linalg.qr(rng.standard_normal((5, 5)))[0]
+    A = X.T @ (2.5 * X @ R)
+    assert np.abs(O.polar_newton(A, scaling=True, X=X) - 2.5 * R).max() < 1e-12
