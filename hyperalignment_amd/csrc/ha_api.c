@@ -492,16 +492,12 @@ int ha_searchlight_template_flat(const double *const *dms, int32_t n_subj, int32
 	if (!dms || !sl_indices || !sl_offsets || !tpl_out || n_subj < 2 || nt <= 0 || nv <= 0 || count <= 0 ||
 	    level2_iter < 0)
 		return kHaErrorArg;
-	if (!isReflection) {
-		ERR("the template path always allows reflections (isReflection = false is not supported)\n");
-		return kHaErrorArg;
-	}
 	HaPlanSetup su = {.build_pattern = false, .build_weights_index = true, .index_only = true,
 	                  .nv_x = nv, .nv_y = nv};
 	HaHipPlan *p = NULL;
 	int rc = ha_plan_create_ex(nt, &su, sl_indices, NULL, sl_offsets, sl_dists, count, radius, &p);
 	if (rc) return rc;
-	rc = ha_plan_template(p, dms, n_subj, col_major, isScaling, isZscoreCommon, level2_iter, tpl_out);
+	rc = ha_plan_template(p, dms, n_subj, col_major, isReflection, isScaling, isZscoreCommon, level2_iter, tpl_out);
 	ha_hip_plan_destroy(p);
 	return rc;
 }

@@ -1348,9 +1348,10 @@ fail:
 /* Searchlight Procrustes template over the plan's searchlights (see
  * k_template.hip): subjects dms[s] are nt x nv FP64 (column- or row-major);
  * tpl_out receives the weighted sum of the local templates in the same order.
- * Reflection is always allowed (the reference's template kinds never disable it). */
-int ha_plan_template(HaHipPlan *p, const double *const *dms, int32_t ns, bool col_major, bool isScaling,
-                     bool isZscore, int32_t level2, double *tpl_out) {
+ * Without reflections every Procrustes solve of the chain gets the reflection
+ * fix (det = +1), as ha_template.c:40,80 passes isReflection to ha_procrustes. */
+int ha_plan_template(HaHipPlan *p, const double *const *dms, int32_t ns, bool col_major, bool isReflection,
+                     bool isScaling, bool isZscore, int32_t level2, double *tpl_out) {
 	int rc = kHaSuccess;
 	hipStream_t st = g.stream;
 	const int32_t nt = p->nt, nv = p->nv_x, ld = p->ld, count = p->count;
@@ -1405,6 +1406,7 @@ int ha_plan_template(HaHipPlan *p, const double *const *dms, int32_t ns, bool co
 		t.work = work; t.slot_stride = slot; t.n_slots = n_slots;
 		t.queue = queue; t.status = status;
 		t.zscore = isZscore ? 1 : 0; t.scaling = isScaling ? 1 : 0; t.level2 = level2;
+		t.reflection = isReflection ? 1 : 0;
 		HakLocalArgs pa = local_args(p);
 		KCK(hak_fill_i32(queue, 8, 0, st));
 		HCK(hipEventRecord(p->ev[0], st));

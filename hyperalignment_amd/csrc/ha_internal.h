@@ -62,8 +62,8 @@ int ha_plan_weights(HaHipPlan *p, double *w_flat);
 int ha_plan_pattern(HaHipPlan *p, int32_t **indptr, int32_t **indices, int64_t *nnz);
 int ha_plan_run_features(HaHipPlan *p, float *features, float *sigma_minmax);
 /* Searchlight Procrustes template of ns subjects (nt x nv FP64 each) into tpl_out. */
-int ha_plan_template(HaHipPlan *p, const double *const *dms, int32_t ns, bool col_major, bool isScaling,
-                     bool isZscore, int32_t level2, double *tpl_out);
+int ha_plan_template(HaHipPlan *p, const double *const *dms, int32_t ns, bool col_major, bool isReflection,
+                     bool isScaling, bool isZscore, int32_t level2, double *tpl_out);
 
 void *ha_hip_stream(void);
 int ha_hip_ensure(void);

@@ -242,6 +242,7 @@ typedef struct {
 	int32_t *queue;          /* atomic work counter, zeroed before the launch */
 	int32_t *status;         /* per searchlight: polar steps over all solves, -1 if one failed */
 	int32_t zscore, scaling, level2;
+	int32_t reflection;      /* 0: every solve gets the reflection fix (det Q = +1) */
 } HakTplArgs;
 /* Floats of one template work slot for ns subjects, leading dim ld, kp = kpmax. */
 int64_t hak_template_slot_floats(int32_t ns, int32_t ld, int32_t kpmax);

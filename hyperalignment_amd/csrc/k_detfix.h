@@ -1,0 +1,236 @@
+// k_detfix.h -- device building blocks of the reflection fix (isReflection =
+// false; ha_searchlight.c:612-632, ha_procrustes.c:56-95), shared by the
+// searchlight path's fix (k_detfix.hip) and the template kernel
+// (k_template.hip): the sign of det Q from a blocked FP32 LU, and the smallest
+// eigenvector of an SPD Gram matrix from its FP64 Cholesky factor by block
+// inverse iteration and a Rayleigh-Ritz step (one 256-thread workgroup).
+#ifndef K_DETFIX_H
+#define K_DETFIX_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <math.h>
+#include "k_chol.h"
+
+namespace kdet {
+
+using namespace kchol;
+
+constexpr int NV = 4;      // inverse iteration vectors
+constexpr int NITER = 6;   // inverse iteration steps
+
+__device__ inline double wg_sum(double v, double *red) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  return (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// sign of det of the k x k block of M (FP32, column-major, leading dim kp),
+// destroyed: blocked right-looking LU with partial pivoting (the lowest row
+// index on ties), 32-column panels: each panel factored in place with whole
+// rows swapped as pivots are chosen, the U block beside it solved one column
+// per thread, and the trailing matrix updated once per panel in 32 x 32 LDS
+// tiles. 0 for an exactly singular block.
+__device__ inline int lu_sign(float *M, int k, int kp, float *pmax, int *pidx, int *piv_sh, int *sign_sh,
+                       float (*ls)[B + 1], float (*us)[B + 1]) {
+  if (threadIdx.x == 0) *sign_sh = 1;
+  __syncthreads();
+  for (int j0 = 0; j0 < k; j0 += B) {
+    const int nb = k - j0 < B ? k - j0 : B, je = j0 + nb;
+    // panel: columns j0 .. je-1, rows j0 .. k-1
+    for (int c = j0; c < je; ++c) {
+      float best = -1.f;
+      int bi = c;
+      for (int r = c + threadIdx.x; r < k; r += RT) {
+        const float v = fabsf(M[r + (int64_t)c * kp]);
+        if (v > best) { best = v; bi = r; }
+      }
+      for (int o = 32; o > 0; o >>= 1) {
+        const float ob = __shfl_xor(best, o);
+        const int oi = __shfl_xor(bi, o);
+        if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+      }
+      const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+      if (lane == 0) { pmax[wave] = best; pidx[wave] = bi; }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        float b0 = pmax[0];
+        int i0 = pidx[0];
+        for (int q = 1; q < 4; ++q)
+          if (pmax[q] > b0 || (pmax[q] == b0 && pidx[q] < i0)) { b0 = pmax[q]; i0 = pidx[q]; }
+        *piv_sh = i0;
+        if (i0 != c) *sign_sh = -*sign_sh;
+      }
+      __syncthreads();
+      const int pr = *piv_sh;
+      if (pr != c)
+        for (int j = threadIdx.x; j < k; j += RT) {
+          const float t = M[c + (int64_t)j * kp];
+          M[c + (int64_t)j * kp] = M[pr + (int64_t)j * kp];
+          M[pr + (int64_t)j * kp] = t;
+        }
+      __syncthreads();
+      const float d = M[c + (int64_t)c * kp];
+      if (d == 0.f) {
+        if (threadIdx.x == 0) *sign_sh = 0;
+        __syncthreads();
+        return 0;
+      }
+      if (threadIdx.x == 0 && d < 0.f) *sign_sh = -*sign_sh;
+      // multipliers below the pivot, then the rest of the panel
+      for (int r = c + 1 + threadIdx.x; r < k; r += RT) M[r + (int64_t)c * kp] /= d;
+      __syncthreads();
+      const int nr = k - c - 1, ncol = je - c - 1;
+      for (int64_t e = threadIdx.x; e < (int64_t)nr * ncol; e += RT) {
+        const int r = c + 1 + (int)(e % nr), j = c + 1 + (int)(e / nr);
+        M[r + (int64_t)j * kp] -= M[r + (int64_t)c * kp] * M[c + (int64_t)j * kp];
+      }
+      __syncthreads();
+    }
+    if (je >= k) break;
+    // U block: rows j0 .. je-1, columns je .. k-1, unit lower L11 (one column per thread)
+    for (int j = je + threadIdx.x; j < k; j += RT)
+      for (int i = 1; i < nb; ++i) {
+        float v = M[(j0 + i) + (int64_t)j * kp];
+        for (int p = 0; p < i; ++p) v -= M[(j0 + i) + (int64_t)(j0 + p) * kp] * M[(j0 + p) + (int64_t)j * kp];
+        M[(j0 + i) + (int64_t)j * kp] = v;
+      }
+    __syncthreads();
+    // trailing matrix: M(r, j) -= sum_p L(r, j0 + p) U(j0 + p, j), 32 x 32 tiles
+    const int ri = threadIdx.x & 31, nj = threadIdx.x >> 5;
+    for (int r0 = je; r0 < k; r0 += B)
+      for (int c0 = je; c0 < k; c0 += B) {
+        for (int e = threadIdx.x; e < B * B; e += RT) {
+          const int a = e & 31, b = e >> 5;  // row / column inside the tile, panel index
+          ls[a][b] = (r0 + a < k && b < nb) ? M[(r0 + a) + (int64_t)(j0 + b) * kp] : 0.f;
+          us[b][a] = (c0 + a < k && b < nb) ? M[(j0 + b) + (int64_t)(c0 + a) * kp] : 0.f;
+        }
+        __syncthreads();
+        if (r0 + ri < k) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int j = c0 + nj + 8 * q;
+            if (j < k) {
+              float acc = 0.f;
+#pragma unroll 8
+              for (int p = 0; p < B; ++p) acc += ls[ri][p] * us[p][nj + 8 * q];
+              M[(r0 + ri) + (int64_t)j * kp] -= acc;
+            }
+          }
+        }
+        __syncthreads();
+      }
+  }
+  const int sgn = *sign_sh;
+  __syncthreads();
+  return sgn;
+}
+
+// V <- L^-1 V for the NV columns of V (kp x NV, column-major, leading dim kp),
+// L lower triangular (kp x kp): 32-row blocks, the rows above a block folded in
+// by two threads per (row, column), the block itself solved in LDS.
+__device__ inline void solve_lower_nv(const double *L, double *V, int kp, double (*d)[B + 1], double (*cs)[B + 1],
+                               double (*part)[B + 1]) {
+  const int i = threadIdx.x & 31, q = (threadIdx.x >> 5) & 3, h = threadIdx.x >> 7;
+  for (int j0 = 0; j0 < kp; j0 += B) {
+    double s = 0.0;
+    for (int c = h; c < j0; c += 2) s += L[(j0 + i) + (int64_t)c * kp] * V[c + (int64_t)q * kp];
+    part[h * NV + q][i] = s;
+    for (int e = threadIdx.x; e < B * B; e += RT) d[e & 31][e >> 5] = L[(j0 + (e & 31)) + (int64_t)(j0 + (e >> 5)) * kp];
+    __syncthreads();
+    if (h == 0) cs[i][q] = V[(j0 + i) + (int64_t)q * kp] - (part[q][i] + part[NV + q][i]);
+    __syncthreads();
+    forward_sub(d, cs);
+    if (h == 0) V[(j0 + i) + (int64_t)q * kp] = cs[i][q];
+    __syncthreads();
+  }
+}
+
+// V <- L^-T V: blocks from the bottom, each (row, column) folding in the rows
+// below the block down its own column of L.
+__device__ inline void solve_lower_t_nv(const double *L, double *V, int kp, double (*d)[B + 1], double (*cs)[B + 1],
+                                 double (*part)[B + 1]) {
+  const int i = threadIdx.x & 31, q = (threadIdx.x >> 5) & 3, h = threadIdx.x >> 7;
+  for (int j0 = kp - B; j0 >= 0; j0 -= B) {
+    double s = 0.0;
+    for (int r = j0 + B + h; r < kp; r += 2) s += L[r + (int64_t)(j0 + i) * kp] * V[r + (int64_t)q * kp];
+    part[h * NV + q][i] = s;
+    for (int e = threadIdx.x; e < B * B; e += RT) d[e & 31][e >> 5] = L[(j0 + (e & 31)) + (int64_t)(j0 + (e >> 5)) * kp];
+    __syncthreads();
+    if (h == 0) cs[i][q] = V[(j0 + i) + (int64_t)q * kp] - (part[q][i] + part[NV + q][i]);
+    __syncthreads();
+    backward_sub_t(d, cs);
+    if (h == 0) V[(j0 + i) + (int64_t)q * kp] = cs[i][q];
+    __syncthreads();
+  }
+}
+
+// modified Gram-Schmidt on the NV columns of V (rows < kp)
+__device__ inline void orthonormalize(double *V, int kp, double *red) {
+  for (int q = 0; q < NV; ++q) {
+    for (int p = 0; p < q; ++p) {
+      double s = 0.0;
+      for (int i = threadIdx.x; i < kp; i += RT) s += V[i + (int64_t)p * kp] * V[i + (int64_t)q * kp];
+      s = wg_sum(s, red);
+      for (int i = threadIdx.x; i < kp; i += RT) V[i + (int64_t)q * kp] -= s * V[i + (int64_t)p * kp];
+      __syncthreads();
+    }
+    double s = 0.0;
+    for (int i = threadIdx.x; i < kp; i += RT) s += V[i + (int64_t)q * kp] * V[i + (int64_t)q * kp];
+    s = wg_sum(s, red);
+    const double inv = s > 0.0 ? 1.0 / sqrt(s) : 0.0;
+    for (int i = threadIdx.x; i < kp; i += RT) V[i + (int64_t)q * kp] *= inv;
+    __syncthreads();
+  }
+}
+
+// eigenvector of the smallest eigenvalue of the symmetric NV x NV matrix t
+// (cyclic Jacobi, one thread); returns the eigenvalue
+__device__ inline double small_eig(double (*t)[NV], double *y) {
+  double v[NV][NV];
+  for (int i = 0; i < NV; ++i)
+    for (int j = 0; j < NV; ++j) v[i][j] = i == j ? 1.0 : 0.0;
+  for (int sweep = 0; sweep < 30; ++sweep) {
+    double off = 0.0, tot = 0.0;
+    for (int i = 0; i < NV; ++i)
+      for (int j = 0; j < NV; ++j) {
+        tot += t[i][j] * t[i][j];
+        if (i != j) off += t[i][j] * t[i][j];
+      }
+    if (off <= 1e-30 * tot) break;
+    for (int p = 0; p < NV - 1; ++p)
+      for (int q = p + 1; q < NV; ++q) {
+        if (t[p][q] == 0.0) continue;
+        const double th = (t[q][q] - t[p][p]) / (2.0 * t[p][q]);
+        const double tn = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+        const double c = 1.0 / sqrt(tn * tn + 1.0), s = tn * c;
+        for (int r = 0; r < NV; ++r) {  // columns p, q
+          const double a = t[r][p], b = t[r][q];
+          t[r][p] = c * a - s * b;
+          t[r][q] = s * a + c * b;
+        }
+        for (int r = 0; r < NV; ++r) {  // rows p, q
+          const double a = t[p][r], b = t[q][r];
+          t[p][r] = c * a - s * b;
+          t[q][r] = s * a + c * b;
+        }
+        for (int r = 0; r < NV; ++r) {
+          const double a = v[r][p], b = v[r][q];
+          v[r][p] = c * a - s * b;
+          v[r][q] = s * a + c * b;
+        }
+      }
+  }
+  int m = 0;
+  for (int i = 1; i < NV; ++i)
+    if (t[i][i] < t[m][m]) m = i;
+  for (int i = 0; i < NV; ++i) y[i] = v[i][m];
+  return t[m][m];
+}
+
+}  // namespace kdet
+
+#endif  // K_DETFIX_H

@@ -35,6 +35,7 @@
 #include "hak.h"
 #include "k_gemm.h"
 #include "k_polar.h"
+#include "k_detfix.h"
 
 #define LAUNCH_RET() return (int)hipGetLastError()
 
@@ -92,11 +93,131 @@ __device__ void zscore_cols(float *B, int nt, int ld, int k) {
   __syncthreads();
 }
 
-// out = D_s T with T = procrustes(D_s, R) (reflection allowed, optional
+// LDS of the reflection fix, kept apart from the GEMM staging only by time: the
+// four FP64 32 x 33 tiles alias GemmSmem<TT>, which is idle while the fix runs.
+struct FixLds {
+  float pmax[4];
+  int pidx[4], piv, sign, ok;
+  double tsh[kdet::NV][kdet::NV], ysh[kdet::NV], lam;
+};
+static_assert(sizeof(GemmSmem<TT>) >= 4 * sizeof(double) * kchol::B * (kchol::B + 1), "fix tiles fit the staging");
+
+// Reflection fix of one solve (isReflection = false; ha_procrustes.c:56-95):
+// when det Q < 0, Q <- Q - 2 (Q v) v^T with v the smallest right singular vector
+// of A. The sign comes from an FP32 LU of a copy of Q (|det Q| = 1, so it is
+// robust); v from the FP64 Gram matrix A^T A of the FP32 A (not from Q^T A,
+// whose rounding would reach v amplified by sigma_max over the gap below
+// sigma_min), its Cholesky factor, block inverse iteration and Rayleigh-Ritz,
+// as in k_detfix.hip. Work, all free after polar_solve (Q is in X2): X (an FP32
+// copy of Q for the LU, then the FP64 vectors), G and P (the FP64 Gram matrix).
+// Returns false when the Gram matrix is not positive definite (A singular);
+// *sigma_min receives the smallest singular value removed from trace(Q^T A)
+// twice (0 when nothing is flipped).
+__device__ bool fix_reflection(GemmSmem<TT> &sm, FixLds &fx, const TplSlot &sl, float *q, const float *A, int k,
+                               int kp, double *red, double *sigma_min) {
+  using namespace kdet;
+  double (*t0)[B + 1] = reinterpret_cast<double (*)[B + 1]>(&sm);
+  double (*t1)[B + 1] = t0 + B, (*t2)[B + 1] = t0 + 2 * B, (*t3)[B + 1] = t0 + 3 * B;
+  *sigma_min = 0.0;
+  float *M = sl.w.X;
+  for (int64_t e = threadIdx.x; e < (int64_t)kp * kp; e += NT) M[e] = q[e];
+  __syncthreads();
+  const int sgn = lu_sign(M, k, kp, fx.pmax, fx.pidx, &fx.piv, &fx.sign, reinterpret_cast<float (*)[B + 1]>(t0),
+                          reinterpret_cast<float (*)[B + 1]>(t1));
+  if (sgn >= 0) return true;
+  // G = A^T A in FP64 (lower tiles; identity on the padding diagonal)
+  double *G = reinterpret_cast<double *>(sl.w.G);  // G and P: kp^2 doubles
+  {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = (wave >> 1) * 16, wn = (wave & 1) * 16;
+    for (int tm = 0; tm < kp; tm += B)
+      for (int tn = 0; tn <= tm; tn += B) {
+        f64x4_t acc = {0.0, 0.0, 0.0, 0.0};
+        for (int r0 = 0; r0 < kp; r0 += B) {
+          for (int e = threadIdx.x; e < B * B; e += NT) {
+            const int c = e >> 5, r = e & 31, row = r0 + r;
+            t0[c][r] = (row < k && tm + c < k) ? (double)A[row + (int64_t)(tm + c) * kp] : 0.0;
+            t1[c][r] = (row < k && tn + c < k) ? (double)A[row + (int64_t)(tn + c) * kp] : 0.0;
+          }
+          __syncthreads();
+          acc = mfma32(t0, t1, true, acc);  // G(i, j) += sum_r A(r, tm + i) A(r, tn + j)
+          __syncthreads();
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int m = tm + wm + (lane >> 4) + 4 * r, n = tn + wn + (lane & 15);
+          double v = acc[r];
+          if (m == n && m >= k) v = 1.0;
+          G[m + (int64_t)n * kp] = v;
+        }
+      }
+    __syncthreads();
+  }
+  if (!cholesky64(G, kp, t0, t3, t1, t2, &fx.ok)) return false;
+  // block inverse iteration on NV vectors (padding rows stay zero), then Rayleigh-Ritz
+  double *V = reinterpret_cast<double *>(sl.w.X), *W = V + (int64_t)NV * kp;
+  for (int64_t e = threadIdx.x; e < (int64_t)kp * NV; e += NT) {
+    const int i = (int)(e % kp), p = (int)(e / kp);
+    V[e] = i < k ? sin(0.7548776662466927 * (i + 1) * (p + 1) + 0.5698402909980532 * p) : 0.0;
+  }
+  __syncthreads();
+  orthonormalize(V, kp, red);
+  // the chain's cross-covariances can have clustered smallest singular values
+  // (0.95 apart in the tests): 6 steps left 2e-3 in the template, 24 reach the
+  // FP32 floor (CPU emulation of this chain against the FP64 template)
+  constexpr int NITER_TPL = 24;
+  for (int step = 0; step < NITER_TPL; ++step) {
+    solve_lower_nv(G, V, kp, t0, t3, t1);
+    solve_lower_t_nv(G, V, kp, t0, t3, t1);
+    orthonormalize(V, kp, red);
+  }
+  for (int64_t e = threadIdx.x; e < (int64_t)kp * NV; e += NT) {
+    const int i = (int)(e % kp), p = (int)(e / kp);
+    double acc = 0.0;
+    for (int r = i; r < kp; ++r) acc += G[r + (int64_t)i * kp] * V[r + (int64_t)p * kp];  // L^T(i, r) = L(r, i)
+    W[e] = acc;
+  }
+  __syncthreads();
+  for (int p = 0; p < NV; ++p)
+    for (int r = p; r < NV; ++r) {
+      double acc = 0.0;
+      for (int i = threadIdx.x; i < kp; i += NT) acc += W[i + (int64_t)p * kp] * W[i + (int64_t)r * kp];
+      acc = kdet::wg_sum(acc, red);
+      if (threadIdx.x == 0) fx.tsh[p][r] = fx.tsh[r][p] = acc;
+    }
+  __syncthreads();
+  if (threadIdx.x == 0) fx.lam = small_eig(fx.tsh, fx.ysh);
+  __syncthreads();
+  // v = V y (W's first column), u = Q v (W's second), Q -= 2 u v^T
+  double *vv = W, *uu = W + kp;
+  for (int i = threadIdx.x; i < kp; i += NT) {
+    double acc = 0.0;
+    for (int p = 0; p < NV; ++p) acc += V[i + (int64_t)p * kp] * fx.ysh[p];
+    vv[i] = acc;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kp; i += NT) {
+    double acc = 0.0;
+    for (int j = 0; j < k; ++j) acc += (double)q[i + (int64_t)j * kp] * vv[j];
+    uu[i] = acc;
+  }
+  __syncthreads();
+  for (int64_t e = threadIdx.x; e < (int64_t)kp * kp; e += NT) {
+    const int i = (int)(e % kp), j = (int)(e / kp);
+    q[e] = (float)((double)q[e] - 2.0 * uu[i] * vv[j]);
+  }
+  *sigma_min = sqrt(fmax(fx.lam, 0.0));
+  __syncthreads();
+  return true;
+}
+
+// out = D_s T with T = procrustes(D_s, R) (reflection allowed unless
+// t.reflection is 0, optional
 // scaling sum(sigma) / sum_j ||D_s[:, j] - mean||^2). Returns the polar steps,
 // or -1 if the iteration did not converge (T = I is used then).
-__device__ int solve_one(GemmSmem<TT> &sm, const HakTplArgs &t, const HakLocalArgs &pa, const TplSlot &sl,
-                         int sidx, const float *R, float *out, int k, int kp, double *red) {
+template <bool FIX>
+__device__ int solve_one(GemmSmem<TT> &sm, FixLds &fx, const HakTplArgs &t, const HakLocalArgs &pa,
+                         const TplSlot &sl, int sidx, const float *R, float *out, int k, int kp, double *red) {
   const float *D = sl.dsp + (int64_t)sidx * t.ld * t.kpmax;
   // A = D_s^T R over the time points
   {
@@ -111,7 +232,9 @@ __device__ int solve_one(GemmSmem<TT> &sm, const HakTplArgs &t, const HakLocalAr
   __syncthreads();
   const double fro2 = frob2(sl.w.A, kp, red);
   const float fro = (float)sqrt(fro2);
-  float *q = sl.w.G;  // G is free while the final product runs
+  // Q goes to X2, which polar_solve never uses: its X and G buffers trade roles
+  // every step, so the finishing product may read its X operand from G
+  float *q = sl.w.X2;
   double tr = 0.0;
   int steps = 0;
   if (fro > 0.f && isfinite(fro)) {
@@ -136,6 +259,13 @@ __device__ int solve_one(GemmSmem<TT> &sm, const HakTplArgs &t, const HakLocalAr
     }
   }
   tr = wg_sum_f64(tr, red);
+  if (FIX && steps >= 0) {
+    double smin = 0.0;
+    if (fix_reflection(sm, fx, sl, q, sl.w.A, k, kp, red, &smin))
+      tr -= 2.0 * smin;  // the flipped pair's singular value changes sign in trace(Q^T A)
+    else
+      steps = -1;        // A singular: no smallest pair to flip; the searchlight fails
+  }
   float f = 1.f;
   if (t.scaling && steps >= 0) {
     // sum_j ||D_s[:, j] - mean_j||^2 from the FP32 subject data, in FP64
@@ -173,10 +303,14 @@ __device__ int solve_one(GemmSmem<TT> &sm, const HakTplArgs &t, const HakLocalAr
   return steps;
 }
 
+// FIX: the instance with the reflection fix (t.reflection == 0); the other one
+// is the kernel without it, whose registers it does not touch.
+template <bool FIX>
 __global__ __launch_bounds__(NT, 2) void k_template(HakTplArgs t, HakLocalArgs pa) {
   __shared__ GemmSmem<TT> sm;
   __shared__ double red[4];
   __shared__ int32_t item_sh;
+  __shared__ FixLds fx;
   const TplSlot sl = tpl_slot(t, blockIdx.x);
   for (;;) {
     const int it = next_item(t.queue, &item_sh);
@@ -211,7 +345,7 @@ __global__ __launch_bounds__(NT, 2) void k_template(HakTplArgs t, HakLocalArgs p
     // sequential alignment to the running common space
     for (int sub = 1; sub < ns; ++sub) {
       float *a = sl.al + sub * lkmax;
-      acc_steps(solve_one(sm, t, pa, sl, sub, sl.com, a, k, kp, red));
+      acc_steps(solve_one<FIX>(sm, fx, t, pa, sl, sub, sl.com, a, k, kp, red));
       if (t.zscore) zscore_cols(a, nt, ld, k);
       for (int64_t e = threadIdx.x; e < lk; e += NT) sl.com[e] = (sl.com[e] + a[e]) * 0.5f;
       __syncthreads();
@@ -231,7 +365,7 @@ __global__ __launch_bounds__(NT, 2) void k_template(HakTplArgs t, HakLocalArgs p
         for (int64_t e = threadIdx.x; e < lk; e += NT) sl.com[e] = (sl.sum[e] - a[e]) * inv_n1;
         __syncthreads();
         if (t.zscore) zscore_cols(sl.com, nt, ld, k);
-        acc_steps(solve_one(sm, t, pa, sl, sub, sl.com, a, k, kp, red));
+        acc_steps(solve_one<FIX>(sm, fx, t, pa, sl, sub, sl.com, a, k, kp, red));
       }
     }
     // local template
@@ -280,7 +414,10 @@ extern "C" int64_t hak_template_slot_floats(int32_t ns, int32_t ld, int32_t kpma
 
 extern "C" int hak_template(const HakTplArgs *t, const HakLocalArgs *pa, void *stream) {
   if (t->count <= 0) return 0;
-  hipLaunchKernelGGL(k_template, dim3(t->n_slots), dim3(NT), 0, (hipStream_t)stream, *t, *pa);
+  if (t->reflection)
+    hipLaunchKernelGGL(k_template<false>, dim3(t->n_slots), dim3(NT), 0, (hipStream_t)stream, *t, *pa);
+  else
+    hipLaunchKernelGGL(k_template<true>, dim3(t->n_slots), dim3(NT), 0, (hipStream_t)stream, *t, *pa);
   LAUNCH_RET();
 }
 

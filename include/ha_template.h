@@ -23,7 +23,8 @@ extern "C" {
  * then the sum of the aligned subjects (z-scored per column with the one-pass
  * formula of ha_linalg.c:263-291 when isZscoreCommon, else averaged). Runs as
  * one searchlight holding every column, so nv is meant to be searchlight-sized
- * (up to a few thousand). isReflection = false is rejected with kHaErrorArg. */
+ * (up to a few thousand). isReflection = false gives every Procrustes solve of
+ * the chain det(T) = +1 (ha_template.c:40,80 pass it to ha_procrustes). */
 int ha_template_procrustes(const TMat **dms, int32_t n_subj, bool isReflection, bool isScaling,
                            bool isZscoreCommon, int32_t level2_iter, TMat *tpl_out);
 

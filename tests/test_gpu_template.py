@@ -41,27 +41,61 @@ def test_template_identical_subjects_kat():
     assert ok, err
 
 
+@pytest.mark.parametrize("reflection", [True, False])
 @pytest.mark.parametrize("zscore,level2,scaling", [(True, 2, False), (False, 1, False), (True, 1, True)])
-def test_template_procrustes_matches_oracle(zscore, level2, scaling):
+def test_template_procrustes_matches_oracle(zscore, level2, scaling, reflection):
     rng = np.random.default_rng(5)
     nt, nv, ns = 60, 14, 5
     S = rng.standard_normal((nt, nv))
     dms = []
-    for _ in range(ns):
+    for i in range(ns):
         Q, _ = np.linalg.qr(rng.standard_normal((nv, nv)))
+        if i % 2:
+            Q[:, 0] *= -1.0   # det(Q) alternates, so solves between subjects need the fix
         dms.append(O.zscore_columns(S @ Q + 0.3 * rng.standard_normal((nt, nv))))
-    tpl = ha.template_procrustes(dms, True, scaling, zscore, level2)
-    ref = O.template_procrustes(dms, True, scaling, zscore, level2)
+    tpl = ha.template_procrustes(dms, reflection, scaling, zscore, level2)
+    ref = O.template_procrustes(dms, reflection, scaling, zscore, level2)
     ok, err = err_ok(tpl, ref)
     assert ok, err
+    if not reflection:
+        # the fix acted: the result differs from the template with reflections
+        assert np.abs(tpl - O.template_procrustes(dms, True, scaling, zscore, level2)).max() > 1e-3
 
 
-@pytest.mark.parametrize("weighted", [True, False])
-def test_searchlight_template_small_mesh(weighted):
+@pytest.mark.parametrize("reflection", [True, False])
+def test_template_procrustes_two_tiles(reflection):
+    """70 columns (kp 96: two 64-wide tiles per product, so the finishing product
+    of each polar solve spans several tiles), four subjects, with and without
+    reflections, against the oracle."""
+    rng = np.random.default_rng(9)
+    nt, nv, ns = 200, 70, 4
+    S = rng.standard_normal((nt, nv))
+    dms = []
+    for i in range(ns):
+        Q, _ = np.linalg.qr(rng.standard_normal((nv, nv)))
+        if i % 2:
+            Q[:, 0] *= -1.0
+        dms.append(O.zscore_columns(S @ Q + 0.3 * rng.standard_normal((nt, nv))))
+    tpl = ha.template_procrustes(dms, reflection, False, True, 2)
+    ref = O.template_procrustes(dms, reflection, False, True, 2)
+    ok, err = err_ok(tpl, ref)
+    print(f"[template kp96] reflection {reflection}: max |diff| {err:.2e}", flush=True)
+    assert ok, err
+    if not reflection:
+        assert np.abs(tpl - O.template_procrustes(dms, True, False, True, 2)).max() > 1e-3
+
+
+@pytest.mark.parametrize("weighted,reflection", [(True, True), (False, True), (True, False)])
+def test_searchlight_template_small_mesh(weighted, reflection):
+    """Small mesh (order 3, 30 mm: 642 searchlights of 13-18 vertices), with
+    and without reflections (ha_template.c:40,80 pass isReflection to every
+    Procrustes solve; without them each gets det = +1)."""
     dms, sls = synth.template_subjects(n_subj=4, n=80, order=3, radius=30.0, seed=3)
     d = sls.dists if weighted else None
-    tpl = ha.searchlight_template_flat(dms, sls.indices, sls.offsets, d, sls.radius, level2_iter=2)
-    ref = O.searchlight_template(dms, sls.indices, sls.offsets, d, sls.radius, level2_iter=2)
+    tpl = ha.searchlight_template_flat(dms, sls.indices, sls.offsets, d, sls.radius, reflection=reflection,
+                                       level2_iter=2)
+    ref = O.searchlight_template(dms, sls.indices, sls.offsets, d, sls.radius, reflection=reflection,
+                                 level2_iter=2, n_jobs=8)
     ok, err = err_ok(tpl, ref)
     assert ok, err
 
