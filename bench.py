@@ -210,7 +210,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--method", default="auto", choices=sorted(ha.METHODS))
     ap.add_argument("--cpu-seconds", type=float, default=18.0)
@@ -218,9 +218,9 @@ def main():
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--no-compare", action="store_true", help="skip the plain polar / Jacobi32 comparison")
     ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end C-ABI call timing")
-    ap.add_argument("--graph", action="store_true",
-                    help="time replays of the captured hipGraph instead of eager passes (the iteration's "
-                         "duration then comes from one eager pass after the timed region)")
+    ap.add_argument("--eager", action="store_true",
+                    help="time eager passes instead of replays of the captured hipGraph (the default: the "
+                         "library captures a plan's pass on its second run and replays it)")
     ap.add_argument("--compare-every", type=int, default=40)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (reference for the HBM byte counters)")
@@ -240,7 +240,7 @@ def main():
     print(f"[bench] workload {args.config}: {s.count} searchlights, k {s.sizes.min()}-{s.sizes.max()}, "
           f"n={nt} (generated in {time.perf_counter() - t0:.1f}s)", file=sys.stderr, flush=True)
     ha.init()
-    ha.set_options(method=args.method, tile=args.tile, use_graph=1 if args.graph else 0)
+    ha.set_options(method=args.method, tile=args.tile, use_graph=0 if args.eager else 1)
     from hyperalignment_amd import learn
     if os.path.exists(learn.MODEL_PATH):
         learn.install(learn.MODEL_PATH)
@@ -250,14 +250,21 @@ def main():
           file=sys.stderr, flush=True)
     for i in range(args.warmup):
         plan.execute()
-        print(f"[bench] warmup {i} {plan.stats()['ms_total']:.1f} ms", file=sys.stderr, flush=True)
+        print(f"[bench] warmup {i} {plan.stats()['ms_total']:.1f} ms"
+              f"{' (graph replay)' if plan.stats()['graph_replay'] else ''}", file=sys.stderr, flush=True)
+    if not args.eager and not plan.stats()["graph_replay"]:
+        # the timed passes replay a graph captured outside the timed region
+        plan.execute()
+        print("[bench] capture pass", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    polar_ms = []
+    polar_ms, replays = [], 0
     for i in range(args.steps):
         plan.execute()
         st = plan.stats()
         polar_ms.append(st["kernel_ms_polar_avg"])
-        print(f"[bench] step {i} {st['ms_total']:.1f} ms", file=sys.stderr, flush=True)
+        replays += st["graph_replay"]
+        print(f"[bench] step {i} {st['ms_total']:.1f} ms (iteration {st['kernel_ms_polar_avg']:.1f} ms)",
+              file=sys.stderr, flush=True)
     dt = time.perf_counter() - t0
     st = plan.stats()
     ms_step = 1e3 * dt / args.steps
@@ -265,13 +272,15 @@ def main():
 
     # dominant kernel: the local-transform kernel (cross-covariance + polar iteration)
     polar_avg = float(np.mean(polar_ms))
-    kernel_timing = "HIP events on the library stream around the iteration, inside the timed passes"
-    if not polar_avg > 0:
+    kernel_timing = ("HIP events recorded inside the captured hipGraph on the library stream, around the "
+                     "iteration launches, in every timed replay" if replays else
+                     "HIP events on the library stream around the iteration, inside the timed passes")
+    if not min(polar_ms) > 0:
         # a captured pass without its stage events: one eager pass after the timed region
         ha.set_options(use_graph=0)
         plan.execute()
         polar_avg = plan.stats()["kernel_ms_polar_avg"]
-        ha.set_options(use_graph=1 if args.graph else 0)
+        ha.set_options(use_graph=0 if args.eager else 1)
         kernel_timing = "HIP events around the iteration of one eager pass after the timed region"
     achieved = st["gemm_flops"] / (polar_avg * 1e-3)
     method = ha.resolved_method(args.method, int(s.sizes.max()))
@@ -283,9 +292,12 @@ def main():
         tiles = dict(zip(edges, bt))
         kps = set(((s.sizes + 31) // 32 * 32).tolist())
         tiles_used = "+".join(str(t) for t in sorted({bt[kp_bucket(kp)] for kp in kps}))
-    team = int(ha.get_options().team)
+    team = 0
+    if tiles_used:
+        tn = plan.tuning()
+        team = int(any(tn[kp_bucket(kp)]["teams"] for kp in set(((s.sizes + 31) // 32 * 32).tolist())))
     kernel = "k_polar" + (f"<{tiles_used}>" if tiles_used else "")
-    if team and tiles_used and "128" in tiles_used:
+    if team:
         kernel = "k_polar_grp (teams of 4, 128 x 128 tiles) + " + kernel
     roofline = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_BF16 / 1e12, "unit": "TFLOP/s",
                 "frac": achieved / PEAK_BF16, "traffic": pmc_traffic(args.config, method, tiles_used, team),
@@ -301,20 +313,21 @@ def main():
     evaluate = {"ms_apply": ev_st["ms_apply"], "ms_corr": ev_st["ms_corr"],
                 "apply_algorithmic_GBs": ev_bytes / (ev_st["ms_apply"] * 1e-3) / 1e9,
                 "r_mean": float(np.nanmean(r)), "host_s": time.perf_counter() - t0}
-    graph = None
-    if not args.graph:
-        # the same pass captured in a hipGraph and replayed (no per-stage events inside)
-        ha.set_options(use_graph=1)
+    tuning = plan.tuning()
+    other = None
+    if True:
+        # the same pass launched the other way (eager when the timed passes were graph replays)
+        ha.set_options(use_graph=1 if args.eager else 0)
         plan.execute()
         ts = []
         for _ in range(3):
             t0 = time.perf_counter()
             plan.execute()
             ts.append(time.perf_counter() - t0)
-        ha.set_options(use_graph=0)
-        graph = {"s_per_pass": float(np.median(ts)), "runs": ts,
-                 "gpu_ms_last": plan.stats()["ms_total"]}
-        print(f"[bench] graph replay {1e3 * graph['s_per_pass']:.1f} ms", file=sys.stderr, flush=True)
+        ha.set_options(use_graph=0 if args.eager else 1)
+        other = {"launch": "hipGraph replay" if args.eager else "eager",
+                 "s_per_pass": float(np.median(ts)), "runs": ts, "gpu_ms_last": plan.stats()["ms_total"]}
+        print(f"[bench] {other['launch']} {1e3 * other['s_per_pass']:.1f} ms", file=sys.stderr, flush=True)
     e2e = None
     if not args.no_e2e:
         e2e = end_to_end(w)
@@ -363,8 +376,9 @@ def main():
         "dtype": "bf16x3" if method in ("polar", "polar_learned") else method,
         "data": "synthetic",
         "config": {"workload": f"{args.config}: {WORKLOADS[args.config]}", "method": method,
-                   "launch": "hipGraph replay (captured on the first pass)" if args.graph else "eager",
-                   "graph_replay": graph,
+                   "launch": (f"hipGraph replay (captured in an untimed pass; {replays} of {args.steps} timed passes "
+                              f"were replays)" if not args.eager else "eager"),
+                   "launch_compare": other,
                    "searchlights": s.count, "hemisphere_s": dt / args.steps,
                    "pattern_ms": st["ms_pattern"], "nnz": st["nnz"], "fp64_fallback": st["n_fallback"],
                    "unconverged": st["n_unconverged"],
@@ -375,7 +389,7 @@ def main():
                    "teams": {"gave_up": st["n_team_giveup"], "searchlights_handed_back": st["n_team_handback"]},
                    "stage_ms": {k: st[k] for k in ("ms_weights", "ms_crosscov", "ms_polar", "ms_fallback",
                                                    "ms_scatter")},
-                   "iteration_tile": tiles, "parity_vs_fp64": parity, "compare": compare,
+                   "iteration_tile": tiles, "autotune": tuning, "parity_vs_fp64": parity, "compare": compare,
                    "end_to_end": e2e,
                    "evaluate": evaluate},
         "roofline": roofline,

@@ -74,12 +74,17 @@ class HaHipStats(ctypes.Structure):
                 ("ms_corr", ctypes.c_double), ("n_orth_checked", ctypes.c_int64),
                 ("n_orth_flagged", ctypes.c_int64), ("orth_max", ctypes.c_double),
                 ("n_team_giveup", ctypes.c_int64), ("n_team_handback", ctypes.c_int64),
-                ("ms_orth", ctypes.c_double)]
+                ("ms_orth", ctypes.c_double), ("graph_replay", ctypes.c_int32)]
 
     def as_dict(self):
         d = {name: getattr(self, name) for name, _ in self._fields_}
         d["steps_hist"] = list(self.steps_hist)
         return d
+
+
+class HaHipTuning(ctypes.Structure):
+    _fields_ = [("tile", ctypes.c_int32 * 6), ("team", ctypes.c_int32 * 6), ("sample", ctypes.c_int32 * 6),
+                ("searchlights", ctypes.c_int32 * 6), ("ms", (ctypes.c_float * 3) * 6)]
 
 
 _P = ctypes.POINTER
@@ -130,6 +135,7 @@ _SIGS = {
     "ha_hip_plan_steps": (ctypes.c_int, [ctypes.c_void_p, _P(_i32), _P(_i32)]),
     "ha_hip_plan_predictions": (ctypes.c_int, [ctypes.c_void_p, _P(_f)]),
     "ha_hip_plan_tiles": (ctypes.c_int, [ctypes.c_void_p, _P(_i32)]),
+    "ha_hip_plan_tuning": (ctypes.c_int, [ctypes.c_void_p, _P(HaHipTuning)]),
     "ha_hip_plan_evaluate": (ctypes.c_int, [ctypes.c_void_p, _P(_d), _P(_d), _i32, ctypes.c_bool, _P(_d), _P(_d)]),
     "ha_searchlight_template_flat": (ctypes.c_int, [_P(_P(_d)), _i32, _i32, _i32, _P(_i32), _P(_i32), _P(_d), _i32,
                                                      _d, ctypes.c_bool, ctypes.c_bool, ctypes.c_bool, _i32,
@@ -751,6 +757,24 @@ class Plan:
         t = np.zeros(6, dtype=np.int32)
         _check("ha_hip_plan_tiles", lib().ha_hip_plan_tiles(self.h, _ptr(t, ctypes.c_int32)))
         return t
+
+    def tuning(self):
+        """The autotuner's launch per kp bucket (kp <= 64, 128, 256, 384, 512,
+        larger) and the sample times behind it: a list of dicts with the tile,
+        whether 128 x 128 tiles run in teams, the searchlights of the bucket, the
+        sample timed and its iteration time per candidate (ms; None where not
+        timed)."""
+        t = HaHipTuning()
+        _check("ha_hip_plan_tuning", lib().ha_hip_plan_tuning(self.h, ctypes.byref(t)))
+        edges = ["kp<=64", "kp<=128", "kp<=256", "kp<=384", "kp<=512", "kp>512"]
+        out = []
+        for b in range(6):
+            timed = t.sample[b] > 0
+            out.append({"bucket": edges[b], "searchlights": int(t.searchlights[b]), "tile": int(t.tile[b]),
+                        "teams": bool(t.team[b]), "sample": int(t.sample[b]),
+                        "ms": ({"64": float(t.ms[b][0]), "128": float(t.ms[b][1]),
+                                "128_teams": float(t.ms[b][2]) or None} if timed else None)})
+        return out
 
     def orthogonality(self, exact=True):
         """||Q_s^T Q_s - I||_F of every local transform in FP64 on the GPU (exact:

@@ -43,7 +43,7 @@ static struct {
 		.polar_tol = 1e-2,
 		.plain_ell = 1e-7,
 		.tile = 0,
-		.use_graph = 0,
+		.use_graph = 1,
 		.chunk = 0,
 		.verbose = 0,
 		.learned_margin_scale = 0.15f,
@@ -241,10 +241,17 @@ struct HaHipPlan {
 	int32_t bucket_tile[HA_TILE_BUCKETS];
 	bool tuned;
 	int32_t tuned_opt;  /* the tile option the choice was made under */
-	int32_t *tile_order[3]; int32_t tile_count[3];  /* 64, 128, rank-deficient (FP32 products) */
+	int32_t bucket_team[HA_TILE_BUCKETS];  /* 128-tile buckets: 1 = teams of HAK_GRP workgroups */
+	int32_t tune_sample[HA_TILE_BUCKETS];  /* searchlights timed per bucket (0: not timed) */
+	float tune_ms[HA_TILE_BUCKETS][3];     /* their iteration time: 64, 128 alone, 128 in teams */
+	/* k-sorted searchlights per launch: 64, 128 (teams first), rank-deficient
+	 * (FP32 products), 128 one workgroup each */
+	int32_t *tile_order[HA_TILE_LISTS]; int32_t tile_count[HA_TILE_LISTS];
 	/* captured pass (use_graph) and the arguments it was captured with */
 	hipGraphExec_t gexec;
-	struct { HakLocalArgs a; int method; bool refl, scal, zero; int32_t team; uint32_t team_spin; } gkey;
+	bool gevents;       /* the captured pass records the stage events */
+	int64_t n_passes;   /* polar passes run on this plan */
+	struct { HakLocalArgs a; HaHipOptions opt; int method; bool refl, scal, zero; } gkey;
 	int32_t tuned_team;
 	int32_t tuned_fb;   /* the fp64_fallback option the lists were built under */
 	bool loaded;
@@ -257,7 +264,7 @@ static void plan_free(HaHipPlan *p) {
 	                p->qld, p->qoff, p->q, p->q64, p->X, p->Y, p->X64, p->Y64, p->Xs, p->Ys, p->stage, p->work, p->work64,
 	                p->flag, p->steps, p->queues, p->order, p->sigma_sum, p->var_sum, p->scale,
 	                p->features, p->pred, p->trace, p->sig, p->d_which, p->sl_par, p->tile_order[0],
-	                p->tile_order[1], p->tile_order[2], p->work_ridge, p->grp, p->iota, p->orth};
+	                p->tile_order[1], p->tile_order[2], p->tile_order[3], p->work_ridge, p->grp, p->iota, p->orth};
 	for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i)
 		if (ptrs[i]) hipFree(ptrs[i]);
 	if (p->gexec) hipGraphExecDestroy(p->gexec);
@@ -562,10 +569,17 @@ int ha_check_csc(int32_t ncols, int32_t nrows, const int32_t *indptr, const int3
 	return kHaSuccess;
 }
 
+/* Drops the captured pass: called wherever a buffer or a host-side setting the
+ * pass was captured with is replaced, so that no replay reads freed memory. */
+static void graph_drop(HaHipPlan *p) {
+	if (p->gexec) { hipGraphExecDestroy(p->gexec); p->gexec = NULL; }
+}
+
 int ha_plan_set_pattern(HaHipPlan *p, int32_t ncols, const int32_t *indptr, const int32_t *indices, int64_t nnz,
                         const double *data) {
 	int rc = kHaSuccess;
 	hipStream_t st = g.stream;
+	graph_drop(p);
 	if (ncols > p->nv_y) { ERR("pattern has more columns than Y vertices\n"); return kHaErrorArg; }
 	rc = ha_check_csc(ncols, p->nv_x, indptr, indices, nnz);
 	if (rc) return rc;
@@ -605,6 +619,7 @@ fail:
 
 int ha_plan_set_weights(HaHipPlan *p, const double *w_flat, bool use_computed) {
 	int rc = kHaSuccess;
+	graph_drop(p);
 	if (use_computed) {
 		if (!p->weights_computed) { ERR("plan has no weight index\n"); return kHaErrorArg; }
 		p->weights_none = false;
@@ -774,6 +789,7 @@ static int32_t team_count(const HaHipPlan *p) {
 static int ensure_teams(HaHipPlan *p) {
 	int rc = kHaSuccess;
 	if (!g.opt.team || p->grp || p->n_teams < 0) return kHaSuccess;
+	graph_drop(p);
 	p->n_teams = team_count(p);
 	if (p->n_teams == 0) { p->n_teams = -1; return kHaSuccess; }
 	ACK(p->grp, (size_t)p->n_teams * HAK_GRP_STATE_BYTES);
@@ -781,11 +797,11 @@ fail:
 	return rc;
 }
 
-static int launch_polar128(HaHipPlan *p, HakLocalArgs b, hipStream_t st) {
+static int launch_polar128(HaHipPlan *p, HakLocalArgs b, bool teams, hipStream_t st) {
 	int rc = kHaSuccess;
 	b.tile = 128;
 	b.skip_done = 0;
-	if (g.opt.team && b.n_order > 0) {
+	if (teams && g.opt.team && b.n_order > 0) {
 		if (p->grp) {
 			KCK(hak_fill_i32((int32_t *)p->grp, (int64_t)p->n_teams * (HAK_GRP_STATE_BYTES / 4), 0, st));
 			HakLocalArgs t = b;
@@ -813,21 +829,22 @@ static int kp_bucket(int32_t kp) {
 static int build_tile_orders(HaHipPlan *p) {
 	int rc = kHaSuccess;
 	hipStream_t st = g.stream;
-	int32_t *h = (int32_t *)malloc((size_t)p->count * 3 * sizeof(int32_t));
+	int32_t *h = (int32_t *)malloc((size_t)p->count * HA_TILE_LISTS * sizeof(int32_t));
 	int32_t *ord = (int32_t *)malloc((size_t)p->count * sizeof(int32_t));
 	if (!h || !ord) { free(h); free(ord); return kHaErrorAlloc; }
 	HCK(hipMemcpy(ord, p->order, (size_t)p->count * sizeof(int32_t), hipMemcpyDeviceToHost));
-	int32_t n[3] = {0, 0, 0};
+	int32_t n[HA_TILE_LISTS] = {0, 0, 0, 0};
 	for (int32_t i = 0; i < p->count; ++i) {
 		const int32_t s = ord[i];
+		const int b = kp_bucket((p->h_k[s] + 31) / 32 * 32);
 		/* rank deficient (stage 1 starts them from the smallest bucket): FP32 products */
 		const int t = (g.opt.fp64_fallback && HAK_NEAR_DEFICIENT(p->h_k[s], p->nt))
 		                  ? 2
-		                  : (p->bucket_tile[kp_bucket((p->h_k[s] + 31) / 32 * 32)] == 128 ? 1 : 0);
+		                  : (p->bucket_tile[b] == 128 ? (p->bucket_team[b] ? 1 : 3) : 0);
 		h[(size_t)t * p->count + n[t]++] = s;
 	}
 	p->tuned_fb = g.opt.fp64_fallback;
-	for (int t = 0; t < 3; ++t) {
+	for (int t = 0; t < HA_TILE_LISTS; ++t) {
 		if (!p->tile_order[t]) ACK(p->tile_order[t], (size_t)p->count * sizeof(int32_t));
 		if (n[t]) HCK(hipMemcpyAsync(p->tile_order[t], h + (size_t)t * p->count, (size_t)n[t] * sizeof(int32_t),
 		                             hipMemcpyHostToDevice, st));
@@ -841,9 +858,10 @@ fail:
 }
 
 /* Time of the iteration stage for the searchlights `sel` (device list, n of
- * them) with the given tile; the cross-covariance stage is rerun first because
- * the iteration overwrites A with Q. */
-static int time_tile(HaHipPlan *p, const int32_t *sel, int32_t n, int tile, float *ms) {
+ * them) with the given tile (128: in teams or one workgroup each); the
+ * cross-covariance stage is rerun first because the iteration overwrites A
+ * with Q. */
+static int time_tile(HaHipPlan *p, const int32_t *sel, int32_t n, int tile, bool teams, float *ms) {
 	int rc = kHaSuccess;
 	hipStream_t st = g.stream;
 	HakLocalArgs a = local_args(p);
@@ -861,7 +879,7 @@ static int time_tile(HaHipPlan *p, const int32_t *sel, int32_t n, int tile, floa
 		a.n_slots = p->n_slots_polar;
 		KCK(hak_polar(&a, st));
 	} else {
-		rc = launch_polar128(p, a, st);
+		rc = launch_polar128(p, a, teams, st);
 		if (rc) goto fail;
 	}
 	HCK(hipEventRecord(p->ev[7], st));
@@ -871,17 +889,27 @@ fail:
 	return rc;
 }
 
-/* Picks the iteration tile of every kp bucket. With the tile option set, that
- * tile everywhere; otherwise buckets up to kp 64 and buckets with fewer
- * searchlights than the GPU has iteration slots take 64, and the others are
- * timed with 64 and 128 on an evenly spaced sample of up to two slot-fuls of
- * their searchlights (plain schedule), keeping the faster. Needs loaded data. */
+/* Picks the iteration launch of every kp bucket: 64 x 64 tiles, 128 x 128
+ * tiles with one workgroup per searchlight, or 128 x 128 tiles in teams of
+ * HAK_GRP workgroups. With the tile option set, that tile everywhere (128 in
+ * teams when the team option is on); otherwise buckets up to kp 64 and buckets
+ * with fewer searchlights than the GPU has iteration slots take 64, and the
+ * others are timed with each candidate on an evenly spaced sample of up to two
+ * slot-fuls of their searchlights (plain schedule), keeping the fastest. The
+ * candidates' times stay in the plan (ha_hip_plan_tuning). Needs loaded data. */
 static int tune_tiles(HaHipPlan *p) {
 	int rc = kHaSuccess;
 	int32_t *ord = NULL, *sel = NULL, *d_sel = NULL;
 	rc = ensure_teams(p);
 	if (rc) return rc;
-	for (int b = 0; b < HA_TILE_BUCKETS; ++b) p->bucket_tile[b] = g.opt.tile ? g.opt.tile : 64;
+	graph_drop(p);
+	const bool teams = g.opt.team && p->grp;
+	for (int b = 0; b < HA_TILE_BUCKETS; ++b) {
+		p->bucket_tile[b] = g.opt.tile ? g.opt.tile : 64;
+		p->bucket_team[b] = p->bucket_tile[b] == 128 && teams;
+		p->tune_sample[b] = 0;
+		for (int c = 0; c < 3; ++c) p->tune_ms[b][c] = 0.f;
+	}
 	if (!g.opt.tile) {
 		ord = (int32_t *)malloc((size_t)p->count * sizeof(int32_t));
 		sel = (int32_t *)malloc((size_t)p->count * sizeof(int32_t));
@@ -897,15 +925,22 @@ static int tune_tiles(HaHipPlan *p) {
 			int32_t n = 0;
 			for (int32_t j = 0; j < want && j < nb; ++j) sel[n++] = sel[(int64_t)j * nb / (want < nb ? want : nb)];
 			HCK(hipMemcpy(d_sel, sel, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice));
-			float t64 = 0, t128 = 0, w = 0;
-			rc = time_tile(p, d_sel, n, 64, &w);      /* warm-up */
-			if (!rc) rc = time_tile(p, d_sel, n, 64, &t64);
-			if (!rc) rc = time_tile(p, d_sel, n, 128, &t128);
+			float w = 0, t[3] = {0, 0, 0};
+			rc = time_tile(p, d_sel, n, 64, false, &w);      /* warm-up */
+			if (!rc) rc = time_tile(p, d_sel, n, 64, false, &t[0]);
+			if (!rc) rc = time_tile(p, d_sel, n, 128, false, &t[1]);
+			if (!rc && teams) rc = time_tile(p, d_sel, n, 128, true, &t[2]);
 			if (rc) goto fail;
-			p->bucket_tile[b] = t128 < t64 ? 128 : 64;
+			int best = 0;
+			for (int c = 1; c < (teams ? 3 : 2); ++c)
+				if (t[c] < t[best]) best = c;
+			p->bucket_tile[b] = best ? 128 : 64;
+			p->bucket_team[b] = best == 2;
+			p->tune_sample[b] = n;
+			for (int c = 0; c < 3; ++c) p->tune_ms[b][c] = t[c];
 			if (g.opt.verbose)
 				fprintf(stderr, "ha_hip: tile bucket %d (%d searchlights, sample %d): 64 -> %.2f ms, "
-				        "128 -> %.2f ms\n", b, nb, n, t64, t128);
+				        "128 -> %.2f ms, 128 in teams -> %.2f ms\n", b, nb, n, t[0], t[1], t[2]);
 		}
 	}
 	rc = build_tile_orders(p);
@@ -921,7 +956,7 @@ fail:
 	return rc;
 }
 
-int ha_hip_plan_tiles(HaHipPlan *p, int32_t *bucket_tile) {
+int ha_hip_plan_tiles(HaHipPlan *p, int32_t bucket_tile[6]) {
 	if (!p || !bucket_tile) return kHaErrorArg;
 	if (!p->loaded) { ERR("plan has no data loaded\n"); return kHaErrorArg; }
 	if (!p->tuned || p->tuned_opt != g.opt.tile || p->tuned_team != g.opt.team ||
@@ -930,6 +965,22 @@ int ha_hip_plan_tiles(HaHipPlan *p, int32_t *bucket_tile) {
 		if (rc) return rc;
 	}
 	memcpy(bucket_tile, p->bucket_tile, sizeof(p->bucket_tile));
+	return kHaSuccess;
+}
+
+int ha_hip_plan_tuning(HaHipPlan *p, HaHipTuning *out) {
+	if (!p || !out) return kHaErrorArg;
+	int32_t bt[HA_TILE_BUCKETS];
+	const int rc = ha_hip_plan_tiles(p, bt);
+	if (rc) return rc;
+	memset(out, 0, sizeof(*out));
+	for (int b = 0; b < HA_TILE_BUCKETS; ++b) {
+		out->tile[b] = p->bucket_tile[b];
+		out->team[b] = p->bucket_team[b];
+		out->sample[b] = p->tune_sample[b];
+		for (int c = 0; c < 3; ++c) out->ms[b][c] = p->tune_ms[b][c];
+	}
+	for (int32_t i = 0; i < p->count; ++i) ++out->searchlights[kp_bucket((p->h_k[i] + 31) / 32 * 32)];
 	return kHaSuccess;
 }
 
@@ -987,16 +1038,16 @@ static int enqueue_pass(HaHipPlan *p, HakLocalArgs a, int method, bool isReflect
 		if (timed) HCK(hipEventRecord(p->ev[6], st));
 		/* one persistent launch per tile over its searchlights (k descending), and
 		 * one with FP32 products over the rank-deficient ones */
-		for (int t = 0; t < 3; ++t) {
+		for (int t = 0; t < HA_TILE_LISTS; ++t) {
 			if (!p->tile_count[t]) continue;
 			HakLocalArgs b = a;
-			b.tile = t == 1 ? 128 : 64;
+			b.tile = (t == 1 || t == 3) ? 128 : 64;
 			b.prec_f32 = t == 2;
 			b.order = p->tile_order[t];
 			b.n_order = p->tile_count[t];
-			b.queue = p->queues + (t == 2 ? 7 : 4 + t);
-			if (t == 1) {
-				rc = launch_polar128(p, b, st);
+			b.queue = p->queues + (t == 3 ? 9 : t == 2 ? 7 : 4 + t);
+			if (t == 1 || t == 3) {
+				rc = launch_polar128(p, b, t == 1, st);
 				if (rc) goto fail;
 			} else {
 				b.n_slots = p->n_slots_polar;
@@ -1103,7 +1154,7 @@ static int ensure_work64(HaHipPlan *p, int32_t want) {
 		hipFree(w);
 		return kHaErrorInternal;
 	}
-	if (p->gexec) { hipGraphExecDestroy(p->gexec); p->gexec = NULL; }
+	graph_drop(p);
 	hipFree(p->work64);
 	p->work64 = w;
 	p->n_slots64 = want;
@@ -1131,7 +1182,8 @@ static void stage_times(HaHipPlan *p, bool is_polar) {
 	p->stats.ms_total = tot;
 	p->stats.kernel_ms_polar_avg = tk;
 	p->stats.polar_launches =
-	    is_polar ? (p->tile_count[0] > 0) + (p->tile_count[1] > 0) + (p->tile_count[2] > 0) : 1;
+	    is_polar ? (p->tile_count[0] > 0) + (p->tile_count[1] > 0) + (p->tile_count[2] > 0) + (p->tile_count[3] > 0)
+	             : 1;
 	(void)hipGetLastError();
 }
 
@@ -1168,7 +1220,11 @@ int ha_plan_run(HaHipPlan *p, bool isReflection, bool isScaling, bool zero_data,
 		if (!p->pred) ACK(p->pred, (size_t)p->count * 2 * sizeof(float));
 		a.pred_out = p->pred;
 	}
-	const bool use_graph = g.opt.use_graph != 0;
+	/* graph mode: the first pass of a plan runs eagerly (per-stage debug views
+	 * and the stage events as usual), later ones are captured once and replayed
+	 * while nothing they depend on changes */
+	const bool debug_env = getenv("HA_HIP_TRACE") || getenv("HA_HIP_CLOCK");
+	const bool use_graph = g.opt.use_graph != 0 && p->n_passes > 0 && !debug_env;
 	const int32_t nclk = p->n_slots > p->n_slots_polar ? p->n_slots : p->n_slots_polar;
 	if (!use_graph && is_polar) {
 		if (getenv("HA_HIP_TRACE")) {
@@ -1189,27 +1245,39 @@ int ha_plan_run(HaHipPlan *p, bool isReflection, bool isScaling, bool zero_data,
 		stage_times(p, is_polar);
 		debug_report(p, &a, clk, nclk);
 	} else {
-		/* replay the captured pass when nothing it depends on changed */
+		/* replay the captured pass when nothing it depends on changed: its
+		 * arguments, the options, the method and the flags */
 		const bool same = p->gexec && p->gkey.method == method && p->gkey.refl == isReflection &&
 		                  p->gkey.scal == isScaling && p->gkey.zero == zero_data &&
-		                  p->gkey.team == g.opt.team && p->gkey.team_spin == g.opt.team_spin &&
+		                  memcmp(&p->gkey.opt, &g.opt, sizeof(g.opt)) == 0 &&
 		                  memcmp(&p->gkey.a, &a, sizeof(a)) == 0;
 		if (!same) {
 			if (p->gexec) { hipGraphExecDestroy(p->gexec); p->gexec = NULL; }
-			HCK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-			capturing = true;
-			rc = enqueue_pass(p, a, method, isReflection, isScaling, zero_data, st, false);
-			capturing = false;
-			hipError_t e = hipStreamEndCapture(st, &graph);
-			if (rc) goto fail;
-			if (e != hipSuccess) { ERR("graph capture failed (%s)\n", hipGetErrorString(e)); rc = kHaErrorInternal; goto fail; }
-			HCK(hipGraphInstantiate(&p->gexec, graph, NULL, NULL, 0));
-			HCK(hipGraphDestroy(graph));
-			graph = NULL;
+			/* the stage events are recorded inside the graph; a runtime that
+			 * cannot capture them gets a graph without them */
+			for (int with_events = 1; with_events >= 0 && !p->gexec; --with_events) {
+				HCK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+				capturing = true;
+				rc = enqueue_pass(p, a, method, isReflection, isScaling, zero_data, st, with_events != 0);
+				capturing = false;
+				hipError_t e = hipStreamEndCapture(st, &graph);
+				if (e == hipSuccess && rc == kHaSuccess) {
+					e = hipGraphInstantiate(&p->gexec, graph, NULL, NULL, 0);
+					if (e != hipSuccess) p->gexec = NULL;
+				}
+				if (graph) { hipGraphDestroy(graph); graph = NULL; }
+				(void)hipGetLastError();
+				if (!p->gexec && !with_events) {
+					ERR("graph capture failed (%s)\n", hipGetErrorString(e));
+					if (!rc) rc = kHaErrorInternal;
+					goto fail;
+				}
+				rc = kHaSuccess;
+				p->gevents = with_events != 0;
+			}
 			memset(&p->gkey, 0, sizeof(p->gkey));
-			p->gkey.a = a; p->gkey.method = method; p->gkey.refl = isReflection;
+			p->gkey.a = a; p->gkey.opt = g.opt; p->gkey.method = method; p->gkey.refl = isReflection;
 			p->gkey.scal = isScaling; p->gkey.zero = zero_data;
-			p->gkey.team = g.opt.team; p->gkey.team_spin = g.opt.team_spin;
 		}
 		HCK(hipEventRecord(p->ev_graph[0], st));
 		HCK(hipGraphLaunch(p->gexec, st));
@@ -1217,12 +1285,18 @@ int ha_plan_run(HaHipPlan *p, bool isReflection, bool isScaling, bool zero_data,
 		HCK(hipStreamSynchronize(st));
 		float tot = 0;
 		hipEventElapsedTime(&tot, p->ev_graph[0], p->ev_graph[1]);
-		memset(&p->stats.ms_weights, 0, sizeof(double) * 5);  /* ms_weights .. ms_scatter: not split */
-		p->stats.ms_orth = 0;
+		if (p->gevents) {
+			stage_times(p, is_polar);
+		} else {
+			memset(&p->stats.ms_weights, 0, sizeof(double) * 5);  /* ms_weights .. ms_scatter: not split */
+			p->stats.ms_orth = 0;
+			p->stats.kernel_ms_polar_avg = 0;
+			p->stats.polar_launches = 0;
+		}
 		p->stats.ms_total = tot;
-		p->stats.kernel_ms_polar_avg = 0;
-		p->stats.polar_launches = 0;
 	}
+	++p->n_passes;
+	p->stats.graph_replay = use_graph ? 1 : 0;
 	p->orth_ran = is_polar && orth_check_on(p);
 	stats_update(p, method);
 	if (!g.opt.fp64_fallback && p->stats.n_fallback > 0 && method != kHaHipMethodJacobi64) {

@@ -25,6 +25,7 @@ int ha_polar_schedule(double ell0, int max_steps, float *out, int *growth_len);
 #define HA_SCHED_STEPS 28
 /* kp buckets of the iteration tile choice: kp <= 64, 128, 256, 384, 512, larger */
 #define HA_TILE_BUCKETS 6
+#define HA_TILE_LISTS 4  /* iteration launches: 64, 128 (teams), rank-deficient, 128 alone */
 
 /* Plan options beyond the public ones. */
 typedef struct {

@@ -53,7 +53,9 @@ typedef struct {
 	                            followed by one Newton-Schulz finishing step */
 	double  plain_ell;       /* sigma_min/||A||_F assumed by the plain schedule (default 1e-7) */
 	int32_t tile;            /* GEMM tile edge 64 or 128; 0 = per-bucket autotuned */
-	int32_t use_graph;       /* 1: capture a plan's pass in a hipGraph and replay it */
+	int32_t use_graph;       /* 1 (default): from the second pass of a plan on, the pass is
+	                            captured in a hipGraph once and replayed while its arguments
+	                            and the options stay the same; 0: every pass eager */
 	int32_t chunk;           /* searchlights per polar chunk (0 = auto) */
 	int32_t verbose;         /* 1: timing lines on stderr */
 	float   learned_margin_scale; /* added to the predicted ln(sigma_max/||A||_F) (default 0.15) */
@@ -131,6 +133,8 @@ typedef struct {
 	int64_t n_team_giveup;     /* teams whose barrier wait ran out (their work went back) */
 	int64_t n_team_handback;   /* searchlights the one-workgroup kernel took over from teams */
 	double  ms_orth;           /* the orthogonality check (part of ms_polar) */
+	int32_t graph_replay;      /* 1: the last pass was a replay of the captured hipGraph (its
+	                              stage times come from events recorded inside the graph) */
 } HaHipStats;
 
 /* Uploads the flat searchlight arrays (layout of hyperalignment_c.py:161-169)
@@ -162,6 +166,20 @@ int ha_hip_plan_ridge(HaHipPlan *plan, double alpha);
  * 384, 512, larger): the tile option when set, else timed on a sample of each
  * large bucket on the first polar pass (the plan's data must be loaded). */
 int ha_hip_plan_tiles(HaHipPlan *plan, int32_t bucket_tile[6]);
+
+/* The autotuner's choice per kp bucket (kp <= 64, 128, 256, 384, 512, larger)
+ * and what it measured: for every bucket with at least as many searchlights as
+ * the GPU has iteration slots, the iteration time of an evenly spaced sample
+ * with 64 x 64 tiles, 128 x 128 tiles one workgroup per searchlight, and
+ * 128 x 128 tiles in teams of 4 workgroups of one XCD (0 where not timed). */
+typedef struct {
+	int32_t tile[6];          /* 64 or 128 */
+	int32_t team[6];          /* 1: 128 x 128 tiles in teams */
+	int32_t sample[6];        /* searchlights timed (0: not timed) */
+	int32_t searchlights[6];  /* searchlights of the plan in the bucket */
+	float   ms[6][3];         /* 64, 128 alone, 128 in teams */
+} HaHipTuning;
+int ha_hip_plan_tuning(HaHipPlan *plan, HaHipTuning *out);
 
 /* Applies the plan's CSC transform to new data and scores it, the evaluation of
  * benchmark_neuroboros.py:204-211,229-232: Yhat = X T (X: nt x nv samples, the

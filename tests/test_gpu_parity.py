@@ -337,7 +337,8 @@ def test_rank_deficient_searchlights(method, tol):
 
 def test_graph_replay_matches_eager():
     """A pass captured in a hipGraph and replayed gives the eager result bit for
-    bit on every replay, with eager passes and host reads between replays."""
+    bit on every replay, with eager passes and host reads between replays, and
+    the stage events recorded inside the graph time every replay."""
     w = synth.make_workload("c1")
     s = w.sls
     plan = ha.Plan(w.X.shape[0], s.nv, s.indices, s.offsets, s.dists, s.radius)
@@ -352,14 +353,74 @@ def test_graph_replay_matches_eager():
             st = plan.stats()
             print(f"[graph] replay {rep}: {st['ms_total']:.2f} ms, iteration {st['kernel_ms_polar_avg']:.2f} ms",
                   flush=True)
-            assert st["ms_total"] > 0
+            assert st["graph_replay"] == 1
+            assert st["ms_total"] > 0 and st["kernel_ms_polar_avg"] > 0
+            assert st["ms_crosscov"] > 0 and st["ms_scatter"] > 0
             got = plan.csc()[2]
             assert np.array_equal(got.view(np.uint64), ref.view(np.uint64)), rep
             ha.set_options(use_graph=0)
             plan.execute()
+            assert plan.stats()["graph_replay"] == 0
             assert np.array_equal(plan.csc()[2].view(np.uint64), ref.view(np.uint64))
     finally:
+        ha.set_options(use_graph=1)
+        plan.close()
+
+
+def test_default_passes_replay_after_the_first():
+    """By default the first pass of a plan runs eagerly and the later ones replay
+    the captured graph; a change of options recaptures, and the results stay
+    identical to the eager ones throughout."""
+    w = synth.make_workload("c1")
+    s = w.sls
+    plan = ha.Plan(w.X.shape[0], s.nv, s.indices, s.offsets, s.dists, s.radius)
+    plan.load(w.X, w.Y)
+    try:
+        ha.set_options(method="polar", use_graph=1)
+        plan.execute()
+        assert plan.stats()["graph_replay"] == 0
+        ref = plan.csc()[2].copy()
+        for rep in range(2):
+            plan.execute()
+            assert plan.stats()["graph_replay"] == 1
+            assert np.array_equal(plan.csc()[2].view(np.uint64), ref.view(np.uint64)), rep
+        # scaling is part of the captured pass: a new capture, then replays of it
+        plan.execute(scaling=True)
+        sc = plan.csc()[2].copy()
+        assert not np.array_equal(sc, ref)
+        plan.execute(scaling=True)
+        assert np.array_equal(plan.csc()[2].view(np.uint64), sc.view(np.uint64))
         ha.set_options(use_graph=0)
+        plan.execute(scaling=True)
+        assert np.array_equal(plan.csc()[2].view(np.uint64), sc.view(np.uint64))
+    finally:
+        ha.set_options(use_graph=1)
+        plan.close()
+
+
+def test_autotuner_report():
+    """The tuner's report per kp bucket: every bucket the workload fills gets a
+    launch, timed buckets keep the fastest of the three candidates, and the
+    lists cover every searchlight."""
+    w = synth.make_workload("c1")
+    s = w.sls
+    plan = ha.Plan(w.X.shape[0], s.nv, s.indices, s.offsets, s.dists, s.radius)
+    plan.load(w.X, w.Y)
+    try:
+        ha.set_options(method="polar", tile=0, team=1)
+        tn = plan.tuning()
+        print(tn, flush=True)
+        assert sum(b["searchlights"] for b in tn) == s.count
+        for b in tn:
+            assert b["tile"] in (64, 128)
+            assert not b["teams"] or b["tile"] == 128
+            if b["sample"]:
+                ms = b["ms"]
+                cand = {(64, False): ms["64"], (128, False): ms["128"], (128, True): ms["128_teams"]}
+                chosen = cand[(b["tile"], b["teams"])]
+                assert chosen == min(v for v in cand.values() if v)
+        assert list(plan.tiles()) == [b["tile"] for b in tn]
+    finally:
         plan.close()
 
 
