@@ -136,6 +136,7 @@ _SIGS = {
     "ha_hip_plan_predictions": (ctypes.c_int, [ctypes.c_void_p, _P(_f)]),
     "ha_hip_plan_tiles": (ctypes.c_int, [ctypes.c_void_p, _P(_i32)]),
     "ha_hip_plan_tuning": (ctypes.c_int, [ctypes.c_void_p, _P(HaHipTuning)]),
+    "ha_hip_plan_det_signs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _P(_i32)]),
     "ha_hip_plan_evaluate": (ctypes.c_int, [ctypes.c_void_p, _P(_d), _P(_d), _i32, ctypes.c_bool, _P(_d), _P(_d)]),
     "ha_searchlight_template_flat": (ctypes.c_int, [_P(_P(_d)), _i32, _i32, _i32, _P(_i32), _P(_i32), _P(_d), _i32,
                                                      _d, ctypes.c_bool, ctypes.c_bool, ctypes.c_bool, _i32,
@@ -774,6 +775,15 @@ class Plan:
                         "teams": bool(t.team[b]), "sample": int(t.sample[b]),
                         "ms": ({"64": float(t.ms[b][0]), "128": float(t.ms[b][1]),
                                 "128_teams": float(t.ms[b][2]) or None} if timed else None)})
+        return out
+
+    def det_signs(self, lu_kernel=True):
+        """1 where the last pass's local transform has det Q_s < 0, else 0, per
+        searchlight: from the LU-sign kernel (lu_kernel) or from the LU inside
+        the reflection-fix kernel."""
+        out = np.zeros(self.count, dtype=np.int32)
+        _check("ha_hip_plan_det_signs", lib().ha_hip_plan_det_signs(self.h, 1 if lu_kernel else 0,
+                                                                    _ptr(out, ctypes.c_int32)))
         return out
 
     def orthogonality(self, exact=True):

@@ -225,6 +225,8 @@ struct HaHipPlan {
 	float *sl_par;
 	double *work64; int64_t stride64; int32_t n_slots64;
 	int32_t *iota;   /* 0, 1, ..., kpmax - 1 (the reflection fix's A^T A product) */
+	int32_t *det_neg;  /* reflection fix: per searchlight det Q < 0, from k_lu_sign */
+	int64_t work_floats;  /* size of the work area */
 	double *work_ridge; int32_t n_wg_ridge;  /* ridge: 2 x stride64 doubles per workgroup */
 	void *grp; int32_t n_teams;               /* polar teams (128 x 128 tiles): state per team */
 	int32_t *flag, *steps, *queues, *order;
@@ -264,7 +266,7 @@ static void plan_free(HaHipPlan *p) {
 	                p->qld, p->qoff, p->q, p->q64, p->X, p->Y, p->X64, p->Y64, p->Xs, p->Ys, p->stage, p->work, p->work64,
 	                p->flag, p->steps, p->queues, p->order, p->sigma_sum, p->var_sum, p->scale,
 	                p->features, p->pred, p->trace, p->sig, p->d_which, p->sl_par, p->tile_order[0],
-	                p->tile_order[1], p->tile_order[2], p->tile_order[3], p->work_ridge, p->grp, p->iota, p->orth};
+	                p->tile_order[1], p->tile_order[2], p->tile_order[3], p->work_ridge, p->grp, p->iota, p->orth, p->det_neg};
 	for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i)
 		if (ptrs[i]) hipFree(ptrs[i]);
 	if (p->gexec) hipGraphExecDestroy(p->gexec);
@@ -502,6 +504,7 @@ int ha_plan_create_ex(int32_t nt, const HaPlanSetup *su, const int32_t *slx_idx,
 		const int64_t nbuf = (int64_t)p->n_slots * 5 > (int64_t)p->n_slots_polar * 4
 		                         ? (int64_t)p->n_slots * 5 : (int64_t)p->n_slots_polar * 4;
 		ACK(p->work, (size_t)nbuf * p->slot_stride * sizeof(float));
+		p->work_floats = nbuf * p->slot_stride;
 	}
 	ACK(p->sl_par, (size_t)count * 3 * sizeof(float));
 	p->stride64 = (int64_t)p->kpmax * p->kpmax;
@@ -996,6 +999,21 @@ static int32_t orth_workgroups(const HaHipPlan *p) {
 	return p->count < n ? p->count : n;
 }
 
+/* The LU-sign kernel's slots in the work area (free once the iteration is
+ * done): a copy of Q and the 32 x kpmax U12 block per workgroup. */
+static int64_t lu_slot_floats(const HaHipPlan *p) {
+	return ((int64_t)p->kpmax * p->kpmax + 32 * (int64_t)p->kpmax + 63) / 64 * 64;
+}
+static int32_t lu_workgroups(const HaHipPlan *p) {
+	int64_t n = 2 * (int64_t)g.n_cu;
+	if (n > p->count) n = p->count;
+	if (n > p->work_floats / lu_slot_floats(p)) n = p->work_floats / lu_slot_floats(p);
+	return (int32_t)n;
+}
+static bool lu_kernel_on(const HaHipPlan *p) {
+	return p->det_neg && p->kpmax <= HAK_LU_KPMAX && lu_workgroups(p) > 0;
+}
+
 /* Enqueues the whole pass on st: no allocation, no host synchronisation, so
  * it can be captured into a graph. timed: records the stage events. */
 static int enqueue_pass(HaHipPlan *p, HakLocalArgs a, int method, bool isReflection, bool isScaling,
@@ -1073,6 +1091,13 @@ static int enqueue_pass(HaHipPlan *p, HakLocalArgs a, int method, bool isReflect
 	}
 	if (det_fix && polar_m) {
 		HakLocalArgs b = a;
+		if (lu_kernel_on(p)) {
+			/* the signs first, by the LU kernel, in the idle iteration slots */
+			HakLocalArgs l = a;
+			l.queue = p->queues + 10;
+			KCK(hak_lu_sign(&l, p->work, lu_slot_floats(p), p->kpmax, p->det_neg, lu_workgroups(p), st));
+			b.det_neg_in = p->det_neg;
+		}
 		b.queue = p->queues + 0;
 		KCK(hak_detfix(&b, p->work64, p->stride64, p->iota, p->n_slots64, st));
 	}
@@ -1205,6 +1230,10 @@ int ha_plan_run(HaHipPlan *p, bool isReflection, bool isScaling, bool zero_data,
 	if (!isReflection || method == kHaHipMethodJacobi64) {
 		rc = ensure_work64(p, 4 * g.n_cu);
 		if (rc) return rc;
+	}
+	if (!isReflection && is_polar && !p->det_neg) {
+		graph_drop(p);
+		ACK(p->det_neg, (size_t)p->count * sizeof(int32_t));
 	}
 	if (is_polar && (!p->tuned || p->tuned_opt != g.opt.tile || p->tuned_team != g.opt.team ||
 	                 p->tuned_fb != g.opt.fp64_fallback)) {
@@ -1375,14 +1404,28 @@ int ha_hip_plan_stats(const HaHipPlan *p, HaHipStats *s) {
 	return kHaSuccess;
 }
 
-int ha_plan_det_negative(HaHipPlan *p, int32_t *neg) {
+int ha_plan_det_negative(HaHipPlan *p, int32_t *neg) { return ha_hip_plan_det_signs(p, 0, neg); }
+
+int ha_hip_plan_det_signs(HaHipPlan *p, int lu_kernel, int32_t *neg) {
 	if (!p || !neg) return kHaErrorArg;
 	if (!p->loaded || !p->q) { ERR("plan has no local transforms yet\n"); return kHaErrorArg; }
+	if (lu_kernel && p->kpmax > HAK_LU_KPMAX) { ERR("searchlights too large for the LU kernel\n"); return kHaErrorArg; }
 	int rc = kHaSuccess;
 	hipStream_t st = g.stream;
-	int32_t *d = NULL;
+	int32_t *d = NULL, *zf = NULL;
 	ACK(d, (size_t)p->count * sizeof(int32_t));
-	{
+	if (lu_kernel) {
+		/* every searchlight, flagged or not, as the signs-only mode of k_detfix */
+		ACK(zf, (size_t)p->count * sizeof(int32_t));
+		KCK(hak_fill_i32(zf, p->count, 0, st));
+		HakLocalArgs a = local_args(p);
+		a.order = NULL;
+		a.n_order = p->count;
+		a.queue = p->queues + 10;
+		a.flag = zf;
+		KCK(hak_fill_i32(p->queues + 10, 1, 0, st));
+		KCK(hak_lu_sign(&a, p->work, lu_slot_floats(p), p->kpmax, d, lu_workgroups(p), st));
+	} else {
 		HakLocalArgs a = local_args(p);
 		a.order = NULL;
 		a.n_order = p->count;
@@ -1395,6 +1438,7 @@ int ha_plan_det_negative(HaHipPlan *p, int32_t *neg) {
 	HCK(hipStreamSynchronize(st));
 fail:
 	if (d) hipFree(d);
+	if (zf) hipFree(zf);
 	return rc;
 }
 

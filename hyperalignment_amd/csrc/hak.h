@@ -130,6 +130,8 @@ typedef struct {
 	                                        one-workgroup kernel took over (skip_done); may be NULL */
 	int32_t *det_neg;                    /* hak_detfix: when set, only det_neg[s] = (det Q_s < 0) for
 	                                        every searchlight of the order, flagged or not; Q unchanged */
+	const int32_t *det_neg_in;           /* hak_detfix: the signs hak_lu_sign recorded (1: det Q_s < 0)
+	                                        instead of its own LU (NULL) */
 } HakLocalArgs;
 
 /* Searchlights the polar path treats as (nearly) rank deficient: z-scored
@@ -166,6 +168,16 @@ int hak_jacobi32(const HakLocalArgs *a, void *stream);
  * correction when want_det_fix. work64: n_slots * 4 * stride64 doubles. */
 int hak_jacobi64(const HakLocalArgs *a, int want_det_fix, double *work64, int64_t stride64,
                  int32_t n_slots, int32_t count, void *stream);
+
+/* Sign of det Q_s for every searchlight of a->order with flag 0 (k_detfix.hip):
+ * neg[s] = 1 when det Q_s < 0, else 0 (0 for flagged ones). An FP32 LU with
+ * partial pivoting of a copy of Q_s in the workgroup's slot of `work`
+ * (slot_floats >= kpmax^2 + 32 kpmax floats each, n_wg slots): 32-column
+ * panels factored in LDS, row interchanges kept as a permutation, the trailing
+ * updates on FP32 MFMA. kpmax <= HAK_LU_KPMAX (the panel lives in LDS). */
+#define HAK_LU_KPMAX 1024
+int hak_lu_sign(const HakLocalArgs *a, float *work, int64_t slot_floats, int32_t kpmax, int32_t *neg, int n_wg,
+                void *stream);
 
 /* Reflection fix after the polar iteration (k_detfix.hip): for every
  * searchlight of a->order with flag 0 whose Q has det < 0, Q <- Q - 2 (Q v) v^T

@@ -54,10 +54,15 @@ __global__ __launch_bounds__(RT, 4) void k_detfix(HakLocalArgs a, double *work64
     if (!a.det_neg && a.flag[s] != 0) continue;  // the FP64 path (with the fix) takes it
     const int o = a.sl_off[s], k = a.sl_off[s + 1] - o, kp = a.qld[s];
     float *Q = a.q + a.qoff[s];
-    for (int64_t e = threadIdx.x; e < (int64_t)kp * kp; e += RT) M[e] = Q[e];
-    __syncthreads();
-    const int sgn = lu_sign(M, k, kp, pmax, pidx, &piv_sh, &sign_sh, reinterpret_cast<float (*)[B + 1]>(&t0[0][0]),
-                            reinterpret_cast<float (*)[B + 1]>(&t1[0][0]));
+    int sgn;
+    if (a.det_neg_in) {
+      sgn = a.det_neg_in[s] ? -1 : 1;
+    } else {
+      for (int64_t e = threadIdx.x; e < (int64_t)kp * kp; e += RT) M[e] = Q[e];
+      __syncthreads();
+      sgn = lu_sign(M, k, kp, pmax, pidx, &piv_sh, &sign_sh, reinterpret_cast<float (*)[B + 1]>(&t0[0][0]),
+                    reinterpret_cast<float (*)[B + 1]>(&t1[0][0]));
+    }
     if (a.det_neg) {  // signs only
       if (threadIdx.x == 0) a.det_neg[s] = sgn < 0 ? 1 : 0;
       continue;
@@ -124,7 +129,224 @@ __global__ __launch_bounds__(RT, 4) void k_detfix(HakLocalArgs a, double *work64
   }
 }
 
+typedef float f32x16_lu __attribute__((ext_vector_type(16)));
+constexpr int PW = 32;  // LU panel width
+
+// LDS of k_lu_sign beyond its static part: the panel (kpmax rows of PW + 1
+// floats), the row permutation, and one 32 x 33 staging tile per wave.
+__host__ __device__ constexpr size_t lu_lds_bytes(int kpmax) {
+  return (size_t)kpmax * (PW + 1) * sizeof(float) + (size_t)kpmax * sizeof(int) + 4 * 32 * 33 * sizeof(float);
+}
+
+// sign(det Q_s) of every searchlight of the order with flag 0, one workgroup
+// each from the queue. LU with partial pivoting (largest magnitude, the lowest
+// row on ties) of a copy M of Q in this workgroup's work slot, in FP32 (Q is
+// orthogonal, so |det Q| = 1 and the sign is robust):
+//  * rows are never moved in M: logical row i lives in physical row perm[i],
+//    and every interchange swaps two perm entries and flips the sign;
+//  * each 32-column panel (logical rows j0 .. k-1) is loaded into LDS and
+//    factored there, one column at a time (pivot search, interchange of the
+//    panel rows, multipliers and the rank-1 update of the panel's remaining
+//    columns);
+//  * U12 = L11^-1 A12 is solved one column per thread, into U12T (32 values
+//    per column, contiguous);
+//  * the trailing matrix is updated in place in 32 x 32 tiles, one per wave,
+//    A22 -= L21 U12 as 16 FP32 MFMAs (v_mfma_f32_32x32x2f32) with L21 from the
+//    LDS panel and U12 staged through LDS.
+// det = sign(perm) * prod(pivots). An exactly zero pivot (singular block)
+// records 0 (no fix), like lu_sign.
+__global__ __launch_bounds__(RT, 2) void k_lu_sign(HakLocalArgs a, float *work, int64_t slot_floats, int32_t kpmax,
+                                                int32_t *neg) {
+  extern __shared__ float lds_dyn[];
+  float (*pan)[PW + 1] = reinterpret_cast<float (*)[PW + 1]>(lds_dyn);
+  int *perm = reinterpret_cast<int *>(lds_dyn + (size_t)kpmax * (PW + 1));
+  float (*ub)[32][33] = reinterpret_cast<float (*)[32][33]>(perm + kpmax);
+  __shared__ float pmax[4];
+  __shared__ int pidx[4], piv_sh, sign_sh, item_sh;
+  float *M = work + (int64_t)blockIdx.x * slot_floats;
+  float *U12T = M + (int64_t)kpmax * kpmax;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) item_sh = atomicAdd(a.queue, 1);
+    __syncthreads();
+    const int it = item_sh;
+    if (it >= a.n_order) break;
+    const int s = a.order ? a.order[it] : it;
+    if (a.flag[s] != 0) {
+      if (threadIdx.x == 0) neg[s] = 0;
+      continue;
+    }
+    const int k = a.sl_off[s + 1] - a.sl_off[s], kp = a.qld[s];
+    {
+      const float4 *src = reinterpret_cast<const float4 *>(a.q + a.qoff[s]);
+      float4 *dst = reinterpret_cast<float4 *>(M);
+      const int64_t n4 = (int64_t)kp * kp / 4;
+      for (int64_t e = threadIdx.x; e < n4; e += RT) dst[e] = src[e];
+    }
+    for (int i = threadIdx.x; i < k; i += RT) perm[i] = i;
+    if (threadIdx.x == 0) sign_sh = 1;
+    __syncthreads();
+    for (int j0 = 0; j0 < k; j0 += PW) {
+      const int nb = k - j0 < PW ? k - j0 : PW, m = k - j0, je = j0 + nb;
+      // the panel: logical rows j0 .. k-1 of columns j0 .. je-1
+      for (int e = threadIdx.x; e < m * nb; e += RT) {
+        const int i = e % m, c = e / m;
+        pan[i][c] = M[perm[j0 + i] + (int64_t)(j0 + c) * kp];
+      }
+      __syncthreads();
+      bool singular = false;
+      for (int c = 0; c < nb; ++c) {
+        float best = -1.f;
+        int bi = c;
+        for (int i = c + threadIdx.x; i < m; i += RT) {
+          const float v = fabsf(pan[i][c]);
+          if (v > best) {
+            best = v;
+            bi = i;
+          }
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+          const float ob = __shfl_xor(best, o);
+          const int oi = __shfl_xor(bi, o);
+          if (ob > best || (ob == best && oi < bi)) {
+            best = ob;
+            bi = oi;
+          }
+        }
+        if (lane == 0) {
+          pmax[wave] = best;
+          pidx[wave] = bi;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+          float b0 = pmax[0];
+          int i0 = pidx[0];
+          for (int q = 1; q < 4; ++q)
+            if (pmax[q] > b0 || (pmax[q] == b0 && pidx[q] < i0)) {
+              b0 = pmax[q];
+              i0 = pidx[q];
+            }
+          piv_sh = i0;
+          if (i0 != c) {
+            sign_sh = -sign_sh;
+            const int t = perm[j0 + c];
+            perm[j0 + c] = perm[j0 + i0];
+            perm[j0 + i0] = t;
+          }
+        }
+        __syncthreads();
+        const int pr = piv_sh;
+        if (pr != c && threadIdx.x < nb) {
+          const float t = pan[c][threadIdx.x];
+          pan[c][threadIdx.x] = pan[pr][threadIdx.x];
+          pan[pr][threadIdx.x] = t;
+        }
+        __syncthreads();
+        const float d = pan[c][c];
+        if (d == 0.f) {
+          singular = true;
+          break;
+        }
+        if (threadIdx.x == 0 && d < 0.f) sign_sh = -sign_sh;
+        for (int i = c + 1 + threadIdx.x; i < m; i += RT) {
+          const float l = pan[i][c] / d;
+          pan[i][c] = l;
+          for (int jj = c + 1; jj < nb; ++jj) pan[i][jj] -= l * pan[c][jj];
+        }
+        __syncthreads();
+      }
+      if (singular) {
+        if (threadIdx.x == 0) sign_sh = 0;
+        break;
+      }
+      if (je >= k) break;
+      const int n2 = k - je;
+      // U12 = L11^-1 A12 (unit lower L11 in the panel's first nb rows)
+      for (int jc = threadIdx.x; jc < n2; jc += RT) {
+        const int64_t col = (int64_t)(je + jc) * kp;
+        float u[PW];
+#pragma unroll
+        for (int i = 0; i < PW; ++i) u[i] = i < nb ? M[perm[j0 + i] + col] : 0.f;
+#pragma unroll
+        for (int i = 1; i < PW; ++i) {
+          if (i < nb) {
+            float v = u[i];
+#pragma unroll
+            for (int q = 0; q < i; ++q) v -= pan[i][q] * u[q];
+            u[i] = v;
+          }
+        }
+        float4 *dst = reinterpret_cast<float4 *>(U12T + (int64_t)jc * PW);
+#pragma unroll
+        for (int q = 0; q < PW / 4; ++q) dst[q] = make_float4(u[4 * q], u[4 * q + 1], u[4 * q + 2], u[4 * q + 3]);
+      }
+      __syncthreads();
+      // A22 -= L21 U12, 32 x 32 tiles, one per wave and round
+      const int ntr = (n2 + 31) / 32, ntile = ntr * ntr;
+      const int mrow = lane & 31, h = lane >> 5;
+      for (int base = 0; base < ntile; base += 4) {
+        const int t = base + wave;
+        const bool act = t < ntile;
+        const int r0 = je + 32 * (act ? t / ntr : 0), c0 = je + 32 * (act ? t % ntr : 0);
+        if (act)
+          for (int e = lane; e < 32 * 32; e += 64) {
+            const int n = e >> 5, kk = e & 31;
+            const int col = c0 + n;
+            ub[wave][kk][n] = (col < k && kk < nb) ? U12T[(int64_t)(col - je) * PW + kk] : 0.f;
+          }
+        __syncthreads();
+        if (act) {
+          const int R = r0 + mrow;
+          const bool rv = R < k;
+          const int64_t prow = rv ? perm[R] : 0;
+          f32x16_lu acc;
+#pragma unroll
+          for (int r = 0; r < 16; ++r) {
+            const int C = c0 + (r >> 2) * 8 + h * 4 + (r & 3);
+            acc[r] = (rv && C < k) ? M[prow + (int64_t)C * kp] : 0.f;
+          }
+#pragma unroll
+          for (int st = 0; st < PW / 2; ++st) {
+            const int kk = 2 * st + h;
+            const float av = (rv && kk < nb) ? -pan[R - j0][kk] : 0.f;
+            const float bv = ub[wave][kk][mrow];
+            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(bv, av, acc, 0, 0, 0);
+          }
+#pragma unroll
+          for (int r = 0; r < 16; ++r) {
+            const int C = c0 + (r >> 2) * 8 + h * 4 + (r & 3);
+            if (rv && C < k) M[prow + (int64_t)C * kp] = acc[r];
+          }
+        }
+        __syncthreads();
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) neg[s] = sign_sh < 0 ? 1 : 0;
+  }
+}
+
 }  // namespace
+
+extern "C" int hak_lu_sign(const HakLocalArgs *a, float *work, int64_t slot_floats, int32_t kpmax, int32_t *neg,
+                           int n_wg, void *stream) {
+  if (a->n_order <= 0) return 0;
+  if (n_wg <= 0 || !a->flag || !neg || !work || kpmax <= 0 || kpmax > HAK_LU_KPMAX || kpmax % 32 ||
+      slot_floats < (int64_t)kpmax * kpmax + (int64_t)PW * kpmax || slot_floats % 4)
+    return (int)hipErrorInvalidValue;
+  // the LDS limit is raised once, to the largest panel, on the first (eager) call
+  static bool lds_raised = false;
+  if (!lds_raised) {
+    const hipError_t e = hipFuncSetAttribute((const void *)k_lu_sign, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             (int)lu_lds_bytes(HAK_LU_KPMAX));
+    if (e != hipSuccess) return (int)e;
+    lds_raised = true;
+  }
+  const size_t lds = lu_lds_bytes(kpmax);
+  hipLaunchKernelGGL(k_lu_sign, dim3(n_wg), dim3(RT), lds, (hipStream_t)stream, *a, work, slot_floats, kpmax, neg);
+  LAUNCH_RET();
+}
 
 extern "C" int hak_detfix(const HakLocalArgs *a, double *work64, int64_t stride64, const int32_t *iota, int n_wg,
                           void *stream) {

@@ -285,10 +285,19 @@ __global__ __launch_bounds__(NT, 2) void k_crosscov(HakLocalArgs a) {
         mlp_predict(a, feat_sh, mlp_sh, pred);
         // pred[0] = ln(sigma_max / ||A||_F), pred[1] = log10(sigma_min / sigma_max).
         // scale = predicted sigma_max raised by its margin (never above ||A||_F);
-        // ell = predicted sigma_min / scale lowered by its margin
-        const float lf = fminf(pred[0] + a.mlp_margin_scale, 0.f);
-        scale = fro * expf(lf);
-        ell = exp10f(pred[1] - a.mlp_margin_ell) * expf(pred[0] - lf);
+        // ell = predicted sigma_min / scale lowered by its margin. Outside the
+        // sizes the model was trained on its outputs can be anything, so the
+        // prediction is first held to what A itself proves: sigma_max lies in
+        // [max(power estimate, ||A||_F / sqrt k), ||A||_F] (the power-iteration
+        // estimate of feature 9 is a lower bound), sigma_min <= sigma_max; a
+        // non-finite prediction leaves the plain start.
+        if (isfinite(pred[0]) && isfinite(pred[1])) {
+          const float lo = fmaxf(feat_sh[9], -0.5f * logf((float)k));
+          const float p0 = fminf(fmaxf(pred[0], lo), 0.f);
+          const float lf = fminf(p0 + a.mlp_margin_scale, 0.f);
+          scale = fro * expf(lf);
+          ell = exp10f(fminf(pred[1], 0.f) - a.mlp_margin_ell) * expf(p0 - lf);
+        }
         if (a.pred_out && threadIdx.x == 0) {
           a.pred_out[2 * s] = pred[0];
           a.pred_out[2 * s + 1] = pred[1];

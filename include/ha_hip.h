@@ -181,6 +181,13 @@ typedef struct {
 } HaHipTuning;
 int ha_hip_plan_tuning(HaHipPlan *plan, HaHipTuning *out);
 
+/* Diagnostics of the reflection fix: neg[s] = 1 when the local transform Q_s
+ * of the last pass has det Q_s < 0, else 0, for every searchlight, from the
+ * LU of the reflection-fix kernel (lu_kernel 0) or from the LU-sign kernel the
+ * pass runs before it (lu_kernel 1: panels in LDS, MFMA trailing updates; kp up
+ * to 1024). Q is not changed. */
+int ha_hip_plan_det_signs(HaHipPlan *plan, int lu_kernel, int32_t *neg);
+
 /* Applies the plan's CSC transform to new data and scores it, the evaluation of
  * benchmark_neuroboros.py:204-211,229-232: Yhat = X T (X: nt x nv samples, the
  * plan's nv source vertices; Yhat: nt x nv target vertices) and, when Y and

@@ -287,6 +287,38 @@ def test_c1_polar_columns():
     plan.close()
 
 
+@pytest.mark.parametrize("name", ["c0", "c1"])
+def test_learned_method_outside_its_training_sizes(name):
+    """The learned scaling was trained on searchlights of k 357-441; on smaller
+    ones (k 40-111) its prediction is held to what A proves about sigma_max, so
+    every searchlight still runs the polar iteration (none is left to the FP64
+    path) and the columns match the oracle."""
+    from hyperalignment_amd import learn
+    w = synth.make_workload(name)
+    s = w.sls
+    learn.install(learn.MODEL_PATH)
+    plan = ha.Plan(w.X.shape[0], s.nv, s.indices, s.offsets, s.dists, s.radius)
+    try:
+        plan.load(w.X, w.Y)
+        ha.set_options(method="polar_learned")
+        plan.execute()
+        st = plan.stats()
+        print(f"[learned {name}] fallback {st['n_fallback']}, orth flagged {st['n_orth_flagged']}, "
+              f"mean steps {st['steps_total'] / s.count:.2f}, {st['ms_total']:.1f} ms", flush=True)
+        assert st["n_fallback"] == 0 and st["n_orth_flagged"] == 0 and st["n_unconverged"] == 0
+        indptr, indices, data = plan.csc()
+        cols = [0, 17, s.nv // 2, s.nv - 1]
+        wts = O.searchlight_weights_flat(s.indices, s.offsets, s.dists, s.radius, s.nv)
+        ref = O.searchlight_procrustes_csc(w.X, w.Y, s.indices, s.offsets, indptr, indices, weights=wts,
+                                           which_cols=cols, n_jobs=8)
+        for c in cols:
+            sl = slice(indptr[c], indptr[c + 1])
+            assert np.abs(data[sl] - ref[sl]).max() < TOL_T
+    finally:
+        ha.set_options(method="polar")
+        plan.close()
+
+
 # ---- rank deficiency: parity by invariants (the polar factor is not unique) ----
 
 def _check_rank_deficient(X, Y, T, tol):
