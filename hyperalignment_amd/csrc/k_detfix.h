@@ -130,40 +130,98 @@ __device__ inline int lu_sign(float *M, int k, int kp, float *pmax, int *pidx, i
 }
 
 // V <- L^-1 V for the NV columns of V (kp x NV, column-major, leading dim kp),
-// L lower triangular (kp x kp): 32-row blocks, the rows above a block folded in
-// by two threads per (row, column), the block itself solved in LDS.
+// L lower triangular (kp x kp): 32-row blocks. The rows above a block are
+// folded in by all 256 threads: thread (row i = t % 32, range h = t / 32) sums
+// L(j0 + i, c) V(c, :) over c = h, h + 8, ... < j0 (the 32 rows of one c are
+// one coalesced load), the eight partial sums meet in LDS, and the block itself
+// is solved in LDS.
 __device__ inline void solve_lower_nv(const double *L, double *V, int kp, double (*d)[B + 1], double (*cs)[B + 1],
                                double (*part)[B + 1]) {
-  const int i = threadIdx.x & 31, q = (threadIdx.x >> 5) & 3, h = threadIdx.x >> 7;
+  const int i = threadIdx.x & 31, h = threadIdx.x >> 5;
   for (int j0 = 0; j0 < kp; j0 += B) {
-    double s = 0.0;
-    for (int c = h; c < j0; c += 2) s += L[(j0 + i) + (int64_t)c * kp] * V[c + (int64_t)q * kp];
-    part[h * NV + q][i] = s;
+    double sq[NV] = {0.0, 0.0, 0.0, 0.0};
+    const double *Lr = L + (j0 + i);
+#pragma unroll 4
+    for (int c = h; c < j0; c += 8) {
+      const double l = Lr[(int64_t)c * kp];
+#pragma unroll
+      for (int q = 0; q < NV; ++q) sq[q] += l * V[c + (int64_t)q * kp];
+    }
+#pragma unroll
+    for (int q = 0; q < NV; ++q) part[h * NV + q][i] = sq[q];
     for (int e = threadIdx.x; e < B * B; e += RT) d[e & 31][e >> 5] = L[(j0 + (e & 31)) + (int64_t)(j0 + (e >> 5)) * kp];
     __syncthreads();
-    if (h == 0) cs[i][q] = V[(j0 + i) + (int64_t)q * kp] - (part[q][i] + part[NV + q][i]);
+    if (threadIdx.x < B * NV) {
+      const int q = threadIdx.x >> 5;
+      double t = V[(j0 + i) + (int64_t)q * kp];
+      for (int hh = 0; hh < 8; ++hh) t -= part[hh * NV + q][i];
+      cs[i][q] = t;
+    }
     __syncthreads();
     forward_sub(d, cs);
-    if (h == 0) V[(j0 + i) + (int64_t)q * kp] = cs[i][q];
+    if (threadIdx.x < B * NV) V[(j0 + i) + (int64_t)(threadIdx.x >> 5) * kp] = cs[i][threadIdx.x >> 5];
     __syncthreads();
   }
 }
 
-// V <- L^-T V: blocks from the bottom, each (row, column) folding in the rows
-// below the block down its own column of L.
+// V <- L^-T V: blocks from the bottom. The rows below a block are folded in
+// down the block's own columns of L: thread (column i = t / 8, offset h = t % 8)
+// sums L(r, j0 + i) V(r, :) over r = j0 + 32 + h, + 8, ... (eight consecutive r
+// of one column are one 64-byte load), the partial sums meet in LDS.
 __device__ inline void solve_lower_t_nv(const double *L, double *V, int kp, double (*d)[B + 1], double (*cs)[B + 1],
                                  double (*part)[B + 1]) {
-  const int i = threadIdx.x & 31, q = (threadIdx.x >> 5) & 3, h = threadIdx.x >> 7;
+  const int h = threadIdx.x & 7, ic = threadIdx.x >> 3;
+  const int i = threadIdx.x & 31;
   for (int j0 = kp - B; j0 >= 0; j0 -= B) {
-    double s = 0.0;
-    for (int r = j0 + B + h; r < kp; r += 2) s += L[r + (int64_t)(j0 + i) * kp] * V[r + (int64_t)q * kp];
-    part[h * NV + q][i] = s;
+    double sq[NV] = {0.0, 0.0, 0.0, 0.0};
+    const double *Lc = L + (int64_t)(j0 + ic) * kp;
+#pragma unroll 4
+    for (int r = j0 + B + h; r < kp; r += 8) {
+      const double l = Lc[r];
+#pragma unroll
+      for (int q = 0; q < NV; ++q) sq[q] += l * V[r + (int64_t)q * kp];
+    }
+#pragma unroll
+    for (int q = 0; q < NV; ++q) part[h * NV + q][ic] = sq[q];
     for (int e = threadIdx.x; e < B * B; e += RT) d[e & 31][e >> 5] = L[(j0 + (e & 31)) + (int64_t)(j0 + (e >> 5)) * kp];
     __syncthreads();
-    if (h == 0) cs[i][q] = V[(j0 + i) + (int64_t)q * kp] - (part[q][i] + part[NV + q][i]);
+    if (threadIdx.x < B * NV) {
+      const int q = threadIdx.x >> 5;
+      double t = V[(j0 + i) + (int64_t)q * kp];
+      for (int hh = 0; hh < 8; ++hh) t -= part[hh * NV + q][i];
+      cs[i][q] = t;
+    }
     __syncthreads();
     backward_sub_t(d, cs);
-    if (h == 0) V[(j0 + i) + (int64_t)q * kp] = cs[i][q];
+    if (threadIdx.x < B * NV) V[(j0 + i) + (int64_t)(threadIdx.x >> 5) * kp] = cs[i][threadIdx.x >> 5];
+    __syncthreads();
+  }
+}
+
+// W = L^T V (kp x NV) for L lower triangular: W(i, :) = sum over r >= i of
+// L(r, i) V(r, :), 32 rows of W at a time, summed down the columns of L as in
+// solve_lower_t_nv (thread: column i0 + t / 8, rows r = i + t % 8, + 8, ...).
+__device__ inline void lower_t_times(const double *L, const double *V, double *W, int kp, double (*part)[B + 1]) {
+  const int h = threadIdx.x & 7, ic = threadIdx.x >> 3;
+  for (int i0 = 0; i0 < kp; i0 += B) {
+    double sq[NV] = {0.0, 0.0, 0.0, 0.0};
+    const int ci = i0 + ic;
+    const double *Lc = L + (int64_t)ci * kp;
+#pragma unroll 4
+    for (int r = ci + h; r < kp; r += 8) {
+      const double l = Lc[r];
+#pragma unroll
+      for (int q = 0; q < NV; ++q) sq[q] += l * V[r + (int64_t)q * kp];
+    }
+#pragma unroll
+    for (int q = 0; q < NV; ++q) part[h * NV + q][ic] = sq[q];
+    __syncthreads();
+    if (threadIdx.x < B * NV) {
+      const int ii = threadIdx.x & 31, q = threadIdx.x >> 5;
+      double t = 0.0;
+      for (int hh = 0; hh < 8; ++hh) t += part[hh * NV + q][ii];
+      W[(i0 + ii) + (int64_t)q * kp] = t;
+    }
     __syncthreads();
   }
 }

@@ -89,13 +89,7 @@ __global__ __launch_bounds__(RT, 4) void k_detfix(HakLocalArgs a, double *work64
     }
     // Rayleigh-Ritz: T = (L^T V)^T (L^T V)
     double *W = V + (int64_t)NV * kp;
-    for (int64_t e = threadIdx.x; e < (int64_t)kp * NV; e += RT) {
-      const int i = (int)(e % kp), q = (int)(e / kp);
-      double acc = 0.0;
-      for (int r = i; r < kp; ++r) acc += G[r + (int64_t)i * kp] * V[r + (int64_t)q * kp];  // L^T(i, r) = L(r, i)
-      W[e] = acc;
-    }
-    __syncthreads();
+    lower_t_times(G, V, W, kp, t1);
     for (int p = 0; p < NV; ++p)
       for (int q = p; q < NV; ++q) {
         double acc = 0.0;
@@ -133,141 +127,216 @@ typedef float f32x16_lu __attribute__((ext_vector_type(16)));
 constexpr int PW = 32;  // LU panel width
 
 // LDS of k_lu_sign beyond its static part: the panel (kpmax rows of PW + 1
-// floats), the row permutation, and one 32 x 33 staging tile per wave.
+// floats), the row permutation and one 32 x 33 staging tile per wave.
 __host__ __device__ constexpr size_t lu_lds_bytes(int kpmax) {
   return (size_t)kpmax * (PW + 1) * sizeof(float) + (size_t)kpmax * sizeof(int) + 4 * 32 * 33 * sizeof(float);
 }
 
+// One column step of the register-panel LU of k_lu_sign (column C of the
+// panel, a compile-time constant so that the panel rows stay in registers):
+// pivot search, interchange through LDS, multipliers and rank-1 update of this
+// thread's rows. Two barriers; a singular pivot sets `singular` (uniformly).
+template <int XS>
+struct LuCol {
+  float (&r)[XS][PW];
+  int tid, lane, wave, nb, m, j0;
+  bool singular;
+  float *pmax;
+  int *pidx;
+  float *prow, *crow;
+  int *perm;
+  int *sign_sh;
+};
+
+template <int C, int XS>
+__device__ __forceinline__ void lu_column(LuCol<XS> &L) {
+  if (C >= L.nb || L.singular) return;
+  const int tid = L.tid;
+  float best = -1.f;
+  int bi = C;
+#pragma unroll
+  for (int x = 0; x < XS; ++x) {
+    const int i = tid + RT * x;
+    const float v = fabsf(L.r[x][C]);
+    if (i >= C && i < L.m && v > best) {
+      best = v;
+      bi = i;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const float ob = __shfl_xor(best, o);
+    const int oi = __shfl_xor(bi, o);
+    if (ob > best || (ob == best && oi < bi)) {
+      best = ob;
+      bi = oi;
+    }
+  }
+  if (L.lane == 0) {
+    L.pmax[L.wave] = best;
+    L.pidx[L.wave] = bi;
+  }
+  __syncthreads();
+  // every thread picks the same winner
+  float b0 = L.pmax[0];
+  int pr = L.pidx[0];
+#pragma unroll
+  for (int q = 1; q < 4; ++q)
+    if (L.pmax[q] > b0 || (L.pmax[q] == b0 && L.pidx[q] < pr)) {
+      b0 = L.pmax[q];
+      pr = L.pidx[q];
+    }
+  // the pivot row to prow (from its owner), row C to crow (thread C, slot 0)
+  const int tp = pr % RT, xp = pr / RT;
+  if (tid == tp) {
+#pragma unroll
+    for (int x = 0; x < XS; ++x)
+      if (x == xp) {
+#pragma unroll
+        for (int j = 0; j < PW; ++j) L.prow[j] = L.r[x][j];
+      }
+  }
+  if (tid == C && pr != C) {
+#pragma unroll
+    for (int j = 0; j < PW; ++j) L.crow[j] = L.r[0][j];
+  }
+  if (tid == 0 && pr != C) {
+    *L.sign_sh = -*L.sign_sh;
+    const int t = L.perm[L.j0 + C];
+    L.perm[L.j0 + C] = L.perm[L.j0 + pr];
+    L.perm[L.j0 + pr] = t;
+  }
+  __syncthreads();
+  if (pr != C) {
+    if (tid == C) {
+#pragma unroll
+      for (int j = 0; j < PW; ++j) L.r[0][j] = L.prow[j];
+    }
+    if (tid == tp) {
+#pragma unroll
+      for (int x = 0; x < XS; ++x)
+        if (x == xp) {
+#pragma unroll
+          for (int j = 0; j < PW; ++j) L.r[x][j] = L.crow[j];
+        }
+    }
+  }
+  const float d = L.prow[C];
+  if (d == 0.f) {
+    L.singular = true;
+    return;
+  }
+  if (tid == 0 && d < 0.f) *L.sign_sh = -*L.sign_sh;
+  // the rows below: thread t keeps its rows t, t + RT, ... for every column
+#pragma unroll
+  for (int x = 0; x < XS; ++x) {
+    const int i = tid + RT * x;
+    if (i > C && i < L.m) {
+      const float l = L.r[x][C] / d;
+      L.r[x][C] = l;
+#pragma unroll
+      for (int j = C + 1; j < PW; ++j) L.r[x][j] -= l * L.prow[j];
+    }
+  }
+}
+
+template <int XS, int C = 0>
+__device__ __forceinline__ void lu_columns(LuCol<XS> &L) {
+  if constexpr (C < PW) {
+    lu_column<C, XS>(L);
+    lu_columns<XS, C + 1>(L);
+  }
+}
+
 // sign(det Q_s) of every searchlight of the order with flag 0, one workgroup
 // each from the queue. LU with partial pivoting (largest magnitude, the lowest
-// row on ties) of a copy M of Q in this workgroup's work slot, in FP32 (Q is
-// orthogonal, so |det Q| = 1 and the sign is robust):
+// row on ties) of Q in FP32 (Q is orthogonal, so |det Q| = 1 and the sign is
+// robust), Q left unchanged: the first panel's updates go to M in this
+// workgroup's work slot, and the later panels work on M:
 //  * rows are never moved in M: logical row i lives in physical row perm[i],
 //    and every interchange swaps two perm entries and flips the sign;
-//  * each 32-column panel (logical rows j0 .. k-1) is loaded into LDS and
-//    factored there, one column at a time (pivot search, interchange of the
-//    panel rows, multipliers and the rank-1 update of the panel's remaining
-//    columns);
+//  * each 32-column panel (logical rows j0 .. k-1) is factored in registers:
+//    thread t holds the panel rows t, t + 256, ... (XS of them). Per column,
+//    the waves' pivot candidates meet in LDS and every thread picks the same
+//    winner; the owners of the pivot row and of row c swap them through LDS,
+//    where the pivot row stays for the rank-1 update of every thread's rows:
+//    two barriers per column. The factored panel then goes to LDS;
 //  * U12 = L11^-1 A12 is solved one column per thread, into U12T (32 values
 //    per column, contiguous);
-//  * the trailing matrix is updated in place in 32 x 32 tiles, one per wave,
-//    A22 -= L21 U12 as 16 FP32 MFMAs (v_mfma_f32_32x32x2f32) with L21 from the
-//    LDS panel and U12 staged through LDS.
+//  * the trailing matrix is updated in place in 32 x 32 tiles, each wave on
+//    its own tiles (column by column) without block barriers: A22 -= L21 U12
+//    as 16 FP32 MFMAs (v_mfma_f32_32x32x2f32), L21 from the LDS panel, the U12
+//    block staged by the wave in its own LDS tile once per tile column, and the
+//    next tile's A22 loaded while the current one's MFMAs run.
 // det = sign(perm) * prod(pivots). An exactly zero pivot (singular block)
-// records 0 (no fix), like lu_sign.
+// records 0 (no fix), like lu_sign. XS * 256 >= kpmax.
+template <int XS>
 __global__ __launch_bounds__(RT, 2) void k_lu_sign(HakLocalArgs a, float *work, int64_t slot_floats, int32_t kpmax,
                                                 int32_t *neg) {
   extern __shared__ float lds_dyn[];
   float (*pan)[PW + 1] = reinterpret_cast<float (*)[PW + 1]>(lds_dyn);
   int *perm = reinterpret_cast<int *>(lds_dyn + (size_t)kpmax * (PW + 1));
-  float (*ub)[32][33] = reinterpret_cast<float (*)[32][33]>(perm + kpmax);
+  float (*ubs)[32][33] = reinterpret_cast<float (*)[32][33]>(perm + kpmax);
   __shared__ float pmax[4];
-  __shared__ int pidx[4], piv_sh, sign_sh, item_sh;
+  __shared__ int pidx[4], sign_sh, item_sh;
+  __shared__ float prow[PW], crow[PW];
   float *M = work + (int64_t)blockIdx.x * slot_floats;
   float *U12T = M + (int64_t)kpmax * kpmax;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   for (;;) {
     __syncthreads();
-    if (threadIdx.x == 0) item_sh = atomicAdd(a.queue, 1);
+    if (tid == 0) item_sh = atomicAdd(a.queue, 1);
     __syncthreads();
     const int it = item_sh;
     if (it >= a.n_order) break;
     const int s = a.order ? a.order[it] : it;
     if (a.flag[s] != 0) {
-      if (threadIdx.x == 0) neg[s] = 0;
+      if (tid == 0) neg[s] = 0;
       continue;
     }
     const int k = a.sl_off[s + 1] - a.sl_off[s], kp = a.qld[s];
-    {
-      const float4 *src = reinterpret_cast<const float4 *>(a.q + a.qoff[s]);
-      float4 *dst = reinterpret_cast<float4 *>(M);
-      const int64_t n4 = (int64_t)kp * kp / 4;
-      for (int64_t e = threadIdx.x; e < n4; e += RT) dst[e] = src[e];
-    }
-    for (int i = threadIdx.x; i < k; i += RT) perm[i] = i;
-    if (threadIdx.x == 0) sign_sh = 1;
+    // Q itself is only read: the first panel, its U12 and its trailing update
+    // read Q, and the trailing update writes M, which every later phase reads
+    const float *Qs = a.q + a.qoff[s];
+    for (int i = tid; i < k; i += RT) perm[i] = i;
+    if (tid == 0) sign_sh = 1;
     __syncthreads();
-    for (int j0 = 0; j0 < k; j0 += PW) {
+    bool singular = false;
+    for (int j0 = 0; j0 < k && !singular; j0 += PW) {
       const int nb = k - j0 < PW ? k - j0 : PW, m = k - j0, je = j0 + nb;
-      // the panel: logical rows j0 .. k-1 of columns j0 .. je-1
-      for (int e = threadIdx.x; e < m * nb; e += RT) {
-        const int i = e % m, c = e / m;
-        pan[i][c] = M[perm[j0 + i] + (int64_t)(j0 + c) * kp];
+      const float *Ms = j0 == 0 ? Qs : M;
+      // the panel rows of this thread: relative rows tid + RT x
+      float r[XS][PW];
+#pragma unroll
+      for (int x = 0; x < XS; ++x) {
+        const int i = tid + RT * x;
+        const int64_t prw = i < m ? perm[j0 + i] : 0;
+#pragma unroll
+        for (int c = 0; c < PW; ++c) r[x][c] = (i < m && c < nb) ? Ms[prw + (int64_t)(j0 + c) * kp] : 0.f;
+      }
+      {
+        LuCol<XS> col{r, tid, lane, wave, nb, m, j0, false, pmax, pidx, prow, crow, perm, &sign_sh};
+        lu_columns<XS>(col);
+        singular = col.singular;
+      }
+      if (singular) break;
+      // the factored panel (L11 \ U11 over L21) to LDS
+#pragma unroll
+      for (int x = 0; x < XS; ++x) {
+        const int i = tid + RT * x;
+        if (i < m) {
+#pragma unroll
+          for (int c = 0; c < PW; ++c) pan[i][c] = r[x][c];
+        }
       }
       __syncthreads();
-      bool singular = false;
-      for (int c = 0; c < nb; ++c) {
-        float best = -1.f;
-        int bi = c;
-        for (int i = c + threadIdx.x; i < m; i += RT) {
-          const float v = fabsf(pan[i][c]);
-          if (v > best) {
-            best = v;
-            bi = i;
-          }
-        }
-        for (int o = 32; o > 0; o >>= 1) {
-          const float ob = __shfl_xor(best, o);
-          const int oi = __shfl_xor(bi, o);
-          if (ob > best || (ob == best && oi < bi)) {
-            best = ob;
-            bi = oi;
-          }
-        }
-        if (lane == 0) {
-          pmax[wave] = best;
-          pidx[wave] = bi;
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-          float b0 = pmax[0];
-          int i0 = pidx[0];
-          for (int q = 1; q < 4; ++q)
-            if (pmax[q] > b0 || (pmax[q] == b0 && pidx[q] < i0)) {
-              b0 = pmax[q];
-              i0 = pidx[q];
-            }
-          piv_sh = i0;
-          if (i0 != c) {
-            sign_sh = -sign_sh;
-            const int t = perm[j0 + c];
-            perm[j0 + c] = perm[j0 + i0];
-            perm[j0 + i0] = t;
-          }
-        }
-        __syncthreads();
-        const int pr = piv_sh;
-        if (pr != c && threadIdx.x < nb) {
-          const float t = pan[c][threadIdx.x];
-          pan[c][threadIdx.x] = pan[pr][threadIdx.x];
-          pan[pr][threadIdx.x] = t;
-        }
-        __syncthreads();
-        const float d = pan[c][c];
-        if (d == 0.f) {
-          singular = true;
-          break;
-        }
-        if (threadIdx.x == 0 && d < 0.f) sign_sh = -sign_sh;
-        for (int i = c + 1 + threadIdx.x; i < m; i += RT) {
-          const float l = pan[i][c] / d;
-          pan[i][c] = l;
-          for (int jj = c + 1; jj < nb; ++jj) pan[i][jj] -= l * pan[c][jj];
-        }
-        __syncthreads();
-      }
-      if (singular) {
-        if (threadIdx.x == 0) sign_sh = 0;
-        break;
-      }
       if (je >= k) break;
       const int n2 = k - je;
       // U12 = L11^-1 A12 (unit lower L11 in the panel's first nb rows)
-      for (int jc = threadIdx.x; jc < n2; jc += RT) {
+      for (int jc = tid; jc < n2; jc += RT) {
         const int64_t col = (int64_t)(je + jc) * kp;
         float u[PW];
 #pragma unroll
-        for (int i = 0; i < PW; ++i) u[i] = i < nb ? M[perm[j0 + i] + col] : 0.f;
+        for (int i = 0; i < PW; ++i) u[i] = i < nb ? Ms[perm[j0 + i] + col] : 0.f;
 #pragma unroll
         for (int i = 1; i < PW; ++i) {
           if (i < nb) {
@@ -282,48 +351,62 @@ __global__ __launch_bounds__(RT, 2) void k_lu_sign(HakLocalArgs a, float *work, 
         for (int q = 0; q < PW / 4; ++q) dst[q] = make_float4(u[4 * q], u[4 * q + 1], u[4 * q + 2], u[4 * q + 3]);
       }
       __syncthreads();
-      // A22 -= L21 U12, 32 x 32 tiles, one per wave and round
+      // A22 -= L21 U12 in 32 x 32 tiles: wave w takes tiles w, w + 4, ... in
+      // column-major order (tile t: row block t % ntr, column block t / ntr)
       const int ntr = (n2 + 31) / 32, ntile = ntr * ntr;
       const int mrow = lane & 31, h = lane >> 5;
-      for (int base = 0; base < ntile; base += 4) {
-        const int t = base + wave;
-        const bool act = t < ntile;
-        const int r0 = je + 32 * (act ? t / ntr : 0), c0 = je + 32 * (act ? t % ntr : 0);
-        if (act)
-          for (int e = lane; e < 32 * 32; e += 64) {
-            const int n = e >> 5, kk = e & 31;
-            const int col = c0 + n;
-            ub[wave][kk][n] = (col < k && kk < nb) ? U12T[(int64_t)(col - je) * PW + kk] : 0.f;
-          }
-        __syncthreads();
-        if (act) {
-          const int R = r0 + mrow;
-          const bool rv = R < k;
-          const int64_t prow = rv ? perm[R] : 0;
-          f32x16_lu acc;
+      float (*ub)[33] = ubs[wave];
+      auto tile_load = [&](int t, f32x16_lu &acc) {
+        const int R = je + 32 * (t % ntr) + mrow, c0 = je + 32 * (t / ntr);
+        const bool rv = R < k;
+        const int64_t prw = rv ? perm[R] : 0;
 #pragma unroll
-          for (int r = 0; r < 16; ++r) {
-            const int C = c0 + (r >> 2) * 8 + h * 4 + (r & 3);
-            acc[r] = (rv && C < k) ? M[prow + (int64_t)C * kp] : 0.f;
-          }
+        for (int r2 = 0; r2 < 16; ++r2) {
+          const int C = c0 + (r2 >> 2) * 8 + h * 4 + (r2 & 3);
+          acc[r2] = (rv && C < k) ? Ms[prw + (int64_t)C * kp] : 0.f;
+        }
+      };
+      f32x16_lu accA, accB;
+      int cur_tj = -1;
+      if (wave < ntile) tile_load(wave, accA);
+      for (int t = wave; t < ntile; t += 4) {
+        const int tj = t / ntr, c0 = je + 32 * tj;
+        const int R = je + 32 * (t % ntr) + mrow;
+        const bool rv = R < k;
+        if (tj != cur_tj) {
+          // the U12 block of this tile column into the wave's own LDS tile; only
+          // this wave writes and reads it, and a wave's LDS accesses complete
+          // in order
+          __builtin_amdgcn_wave_barrier();
 #pragma unroll
-          for (int st = 0; st < PW / 2; ++st) {
-            const int kk = 2 * st + h;
-            const float av = (rv && kk < nb) ? -pan[R - j0][kk] : 0.f;
-            const float bv = ub[wave][kk][mrow];
-            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(bv, av, acc, 0, 0, 0);
+          for (int x = 0; x < 16; ++x) {
+            const int e = lane + 64 * x, n = e >> 5, kk = e & 31, col = c0 + n;
+            ub[kk][n] = (col < k && kk < nb) ? U12T[(int64_t)(col - je) * PW + kk] : 0.f;
           }
+          __builtin_amdgcn_wave_barrier();
+          cur_tj = tj;
+        }
+        if (t + 4 < ntile) tile_load(t + 4, accB);
 #pragma unroll
-          for (int r = 0; r < 16; ++r) {
-            const int C = c0 + (r >> 2) * 8 + h * 4 + (r & 3);
-            if (rv && C < k) M[prow + (int64_t)C * kp] = acc[r];
+        for (int st = 0; st < PW / 2; ++st) {
+          const int kk = 2 * st + h;
+          const float av = (rv && kk < nb) ? -pan[R - j0][kk] : 0.f;
+          accA = __builtin_amdgcn_mfma_f32_32x32x2f32(ub[kk][mrow], av, accA, 0, 0, 0);
+        }
+        if (rv) {
+          const int64_t prw = perm[R];
+#pragma unroll
+          for (int r2 = 0; r2 < 16; ++r2) {
+            const int C = c0 + (r2 >> 2) * 8 + h * 4 + (r2 & 3);
+            if (C < k) M[prw + (int64_t)C * kp] = accA[r2];
           }
         }
-        __syncthreads();
+        accA = accB;
       }
+      __syncthreads();
     }
     __syncthreads();
-    if (threadIdx.x == 0) neg[s] = sign_sh < 0 ? 1 : 0;
+    if (tid == 0) neg[s] = (singular || sign_sh >= 0) ? 0 : 1;
   }
 }
 
@@ -338,13 +421,21 @@ extern "C" int hak_lu_sign(const HakLocalArgs *a, float *work, int64_t slot_floa
   // the LDS limit is raised once, to the largest panel, on the first (eager) call
   static bool lds_raised = false;
   if (!lds_raised) {
-    const hipError_t e = hipFuncSetAttribute((const void *)k_lu_sign, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                             (int)lu_lds_bytes(HAK_LU_KPMAX));
+    hipError_t e = hipFuncSetAttribute((const void *)k_lu_sign<2>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lu_lds_bytes(2 * RT));
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute((const void *)k_lu_sign<4>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lu_lds_bytes(HAK_LU_KPMAX));
     if (e != hipSuccess) return (int)e;
     lds_raised = true;
   }
   const size_t lds = lu_lds_bytes(kpmax);
-  hipLaunchKernelGGL(k_lu_sign, dim3(n_wg), dim3(RT), lds, (hipStream_t)stream, *a, work, slot_floats, kpmax, neg);
+  if (kpmax <= 2 * RT)
+    hipLaunchKernelGGL(k_lu_sign<2>, dim3(n_wg), dim3(RT), lds, (hipStream_t)stream, *a, work, slot_floats, kpmax,
+                       neg);
+  else
+    hipLaunchKernelGGL(k_lu_sign<4>, dim3(n_wg), dim3(RT), lds, (hipStream_t)stream, *a, work, slot_floats, kpmax,
+                       neg);
   LAUNCH_RET();
 }
 

@@ -171,13 +171,7 @@ __device__ bool fix_reflection(GemmSmem<TT> &sm, FixLds &fx, const TplSlot &sl, 
     solve_lower_t_nv(G, V, kp, t0, t3, t1);
     orthonormalize(V, kp, red);
   }
-  for (int64_t e = threadIdx.x; e < (int64_t)kp * NV; e += NT) {
-    const int i = (int)(e % kp), p = (int)(e / kp);
-    double acc = 0.0;
-    for (int r = i; r < kp; ++r) acc += G[r + (int64_t)i * kp] * V[r + (int64_t)p * kp];  // L^T(i, r) = L(r, i)
-    W[e] = acc;
-  }
-  __syncthreads();
+  lower_t_times(G, V, W, kp, t1);
   for (int p = 0; p < NV; ++p)
     for (int r = p; r < NV; ++r) {
       double acc = 0.0;
