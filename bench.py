@@ -10,9 +10,13 @@ scatter into the CSC transform. Prints one JSON line (searchlights/s).
 
 import os
 
-# the CPU baseline times one LAPACK call per worker thread
+# the CPU baseline runs one searchlight per OpenMP thread, each with a
+# single-threaded BLAS, threads bound to consecutive cores (the OpenMP runtime
+# reads these when it starts)
 for _v in ("OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS", "OMP_NUM_THREADS_BLAS"):
     os.environ.setdefault(_v, "1")
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
 
 import argparse
 import json
@@ -39,48 +43,55 @@ WORKLOADS = {
 }
 
 
-def cpu_model():
-    """The host CPU's model name and logical CPU count (from /proc/cpuinfo)."""
-    model, n = "unknown", 0
+def cpu_topology():
+    """The host CPU's model name, logical CPUs and physical cores (distinct
+    (physical id, core id) pairs of /proc/cpuinfo)."""
+    model, logical, cores = "unknown", 0, set()
+    phys = core = None
     try:
         for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                model = line.split(":", 1)[1].strip()
-                n += 1
+            key, _, val = line.partition(":")
+            key, val = key.strip(), val.strip()
+            if key == "model name":
+                model = val
+                logical += 1
+            elif key == "physical id":
+                phys = val
+            elif key == "core id":
+                core = val
+            elif not key and phys is not None:
+                cores.add((phys, core))
+                phys = core = None
+        if phys is not None:
+            cores.add((phys, core))
     except OSError:
         pass
-    return model, n
+    return model, logical, (len(cores) or logical)
 
 
 def cpu_baseline(w, seconds, threads, repeats=3):
-    """Oracle (numpy/LAPACK FP64 dgesdd, one searchlight per thread, one BLAS
-    thread each) on bounded samples: `repeats` runs of random searchlights in
-    batches of `threads`, each for about seconds / repeats of wall time; the
-    median rate of the runs (the reference reports medians of repeats,
-    README.md:188). Returns (median rate, per-run rates, searchlights, transforms)."""
-    from concurrent.futures import ThreadPoolExecutor
-    from oracle import ha_oracle as O
-
+    """The C/OpenMP restatement of the reference's CPU FP64 path
+    (oracle/cpu_ref.c: ha_searchlight.c:950-1106 with the local solve of
+    ha_searchlight.c:578-659, LAPACK dgesdd / dgesvd and BLAS dgemm, one
+    searchlight per thread) on bounded samples: `repeats` runs over random
+    searchlights in batches of 4 per thread, each for about seconds / repeats
+    of wall time; the median rate of the runs (the reference reports medians of
+    repeats, README.md:188). Returns (median rate, per-run rates, searchlights,
+    transforms)."""
+    from oracle import cpu_ref
     s = w.sls
-    lists = s.lists()
     rng = np.random.default_rng(0)
     perm = rng.permutation(s.count)
-
-    def one(i):
-        sl = lists[i]
-        return O.procrustes(w.X[:, sl], w.Y[:, sl])
-
-    from threadpoolctl import threadpool_limits
     done, Ts, rates = [], [], []
-    # one LAPACK thread per worker: the pool supplies the parallelism
-    with threadpool_limits(limits=1, user_api="blas"), ThreadPoolExecutor(threads) as ex:
+    from threadpoolctl import threadpool_limits
+    with threadpool_limits(limits=1, user_api="blas"):
         for _ in range(repeats):
             n0 = len(done)
             t0 = time.perf_counter()
             while time.perf_counter() - t0 < seconds / repeats and len(done) < s.count:
-                batch = perm[len(done):len(done) + threads]
-                Ts.extend(ex.map(one, batch))
-                done.extend(batch)
+                batch = perm[len(done):len(done) + 4 * threads]
+                Ts.extend(cpu_ref.procrustes_searchlights(w.X, w.Y, s.indices, s.offsets, batch, threads))
+                done.extend(batch.tolist())
             rates.append((len(done) - n0) / (time.perf_counter() - t0))
     return float(np.median(rates)), rates, np.array(done), Ts
 
@@ -345,16 +356,20 @@ def main():
     parity = None
     if not args.no_cpu:
         avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
-        threads = min(avail, 16)
+        model, ncpu, ncores = cpu_topology()
+        # physical cores of the CPUs this process may use
+        threads = max(1, min(avail, ncores, avail * ncores // max(1, ncpu)))
         rate, rates, which, Ts = cpu_baseline(w, args.cpu_seconds, threads)
-        model, ncpu = cpu_model()
         cpu = {"value": rate, "unit": "searchlights/s", "cores": threads, "kind": "port",
                "sample": f"median of 3 runs ({', '.join(f'{r:.1f}' for r in rates)} searchlights/s) over "
                          f"{len(which)} random searchlights of {args.config} (about {args.cpu_seconds:.0f} s in "
-                         f"all), numpy/LAPACK FP64 dgesdd (the oracle), one searchlight per thread and one BLAS "
-                         f"thread each, {threads} threads (of {avail} CPUs available to the process, at most 16 "
-                         f"used; {ncpu} logical CPUs, {model}); hemisphere estimate {s.count / rate:.1f} s",
-               "cpu_model": model, "logical_cpus": ncpu, "cpus_available": avail, "runs": rates}
+                         f"all): the C/OpenMP restatement of the reference's CPU FP64 path (oracle/cpu_ref.c: "
+                         f"gather, dgemm, dgesdd, U Vt), one searchlight per OpenMP thread with a single-threaded "
+                         f"BLAS, {threads} threads bound to cores ({avail} CPUs available to the process; "
+                         f"{ncpu} logical CPUs, {ncores} physical cores, {model}); hemisphere estimate "
+                         f"{s.count / rate:.1f} s",
+               "cpu_model": model, "logical_cpus": ncpu, "physical_cores": ncores, "cpus_available": avail,
+               "runs": rates}
         loc = plan.local(which)
         errs = [float(np.abs(a - b).max()) for a, b in zip(loc, Ts)]
         fro = [float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(loc, Ts)]

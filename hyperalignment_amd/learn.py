@@ -124,7 +124,7 @@ def collect(seeds, ns, noises, sample, rng_seed=0):
     return np.concatenate(X), np.concatenate(Y), meta
 
 
-def fit(X, Y, epochs=40, batch=4096, lr=2e-3, w=(10.0, 1.0), seed=0):
+def fit(X, Y, epochs=60, batch=4096, lr=2e-3, w=(10.0, 1.0), seed=0):
     mean = X.mean(axis=0)
     std = X.std(axis=0)
     std[std < 1e-6] = 1.0
@@ -242,8 +242,9 @@ def main(argv=None):
     t = sub.add_parser("train")
     t.add_argument("--out", default="gpurun_out/polar_mlp.npz")
     t.add_argument("--seeds", type=int, default=8)
-    t.add_argument("--sample", type=int, default=2800)
-    t.add_argument("--epochs", type=int, default=40)
+    # the committed model (model/polar_mlp.npz, its info record) was trained with these
+    t.add_argument("--sample", type=int, default=3000)
+    t.add_argument("--epochs", type=int, default=60)
     e = sub.add_parser("eval")
     e.add_argument("--model", default=MODEL_PATH)
     e.add_argument("--seeds", type=int, default=2)
