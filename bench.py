@@ -11,12 +11,9 @@ scatter into the CSC transform. Prints one JSON line (searchlights/s).
 import os
 
 # the CPU baseline runs one searchlight per OpenMP thread, each with a
-# single-threaded BLAS, threads bound to consecutive cores (the OpenMP runtime
-# reads these when it starts)
+# single-threaded BLAS (oracle/cpu_ref.c binds its threads to the process's CPUs)
 for _v in ("OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS", "OMP_NUM_THREADS_BLAS"):
     os.environ.setdefault(_v, "1")
-os.environ.setdefault("OMP_PROC_BIND", "close")
-os.environ.setdefault("OMP_PLACES", "cores")
 
 import argparse
 import json
@@ -69,7 +66,7 @@ def cpu_topology():
     return model, logical, (len(cores) or logical)
 
 
-def cpu_baseline(w, seconds, threads, repeats=3):
+def cpu_baseline(w, seconds, threads, repeats=3, reflection=True, cpus=None):
     """The C/OpenMP restatement of the reference's CPU FP64 path
     (oracle/cpu_ref.c: ha_searchlight.c:950-1106 with the local solve of
     ha_searchlight.c:578-659, LAPACK dgesdd / dgesvd and BLAS dgemm, one
@@ -90,7 +87,8 @@ def cpu_baseline(w, seconds, threads, repeats=3):
             t0 = time.perf_counter()
             while time.perf_counter() - t0 < seconds / repeats and len(done) < s.count:
                 batch = perm[len(done):len(done) + 4 * threads]
-                Ts.extend(cpu_ref.procrustes_searchlights(w.X, w.Y, s.indices, s.offsets, batch, threads))
+                Ts.extend(cpu_ref.procrustes_searchlights(w.X, w.Y, s.indices, s.offsets, batch, threads,
+                                                          reflection=reflection, cpus=cpus))
                 done.extend(batch.tolist())
             rates.append((len(done) - n0) / (time.perf_counter() - t0))
     return float(np.median(rates)), rates, np.array(done), Ts
@@ -233,9 +231,12 @@ def main():
                     help="time eager passes instead of replays of the captured hipGraph (the default: the "
                          "library captures a plan's pass on its second run and replays it)")
     ap.add_argument("--compare-every", type=int, default=40)
+    ap.add_argument("--no-reflection", action="store_true",
+                    help="time passes with isReflection = false (the det(Q) < 0 fix, ha_searchlight.c:612-632)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (reference for the HBM byte counters)")
     args = ap.parse_args()
+    refl = not args.no_reflection
     if args.copy_calibration:
         copy_calibration()
         return
@@ -260,17 +261,17 @@ def main():
     print(f"[bench] plan ready: nnz {plan.nnz()}, pattern {plan.stats()['ms_pattern']:.1f} ms",
           file=sys.stderr, flush=True)
     for i in range(args.warmup):
-        plan.execute()
+        plan.execute(reflection=refl)
         print(f"[bench] warmup {i} {plan.stats()['ms_total']:.1f} ms"
               f"{' (graph replay)' if plan.stats()['graph_replay'] else ''}", file=sys.stderr, flush=True)
     if not args.eager and not plan.stats()["graph_replay"]:
         # the timed passes replay a graph captured outside the timed region
-        plan.execute()
+        plan.execute(reflection=refl)
         print("[bench] capture pass", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     polar_ms, replays = [], 0
     for i in range(args.steps):
-        plan.execute()
+        plan.execute(reflection=refl)
         st = plan.stats()
         polar_ms.append(st["kernel_ms_polar_avg"])
         replays += st["graph_replay"]
@@ -289,7 +290,7 @@ def main():
     if not min(polar_ms) > 0:
         # a captured pass without its stage events: one eager pass after the timed region
         ha.set_options(use_graph=0)
-        plan.execute()
+        plan.execute(reflection=refl)
         polar_avg = plan.stats()["kernel_ms_polar_avg"]
         ha.set_options(use_graph=0 if args.eager else 1)
         kernel_timing = "HIP events around the iteration of one eager pass after the timed region"
@@ -329,11 +330,11 @@ def main():
     if True:
         # the same pass launched the other way (eager when the timed passes were graph replays)
         ha.set_options(use_graph=1 if args.eager else 0)
-        plan.execute()
+        plan.execute(reflection=refl)
         ts = []
         for _ in range(3):
             t0 = time.perf_counter()
-            plan.execute()
+            plan.execute(reflection=refl)
             ts.append(time.perf_counter() - t0)
         ha.set_options(use_graph=0 if args.eager else 1)
         other = {"launch": "hipGraph replay" if args.eager else "eager",
@@ -355,17 +356,20 @@ def main():
     cpu = None
     parity = None
     if not args.no_cpu:
+        from oracle import cpu_ref
         avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
         model, ncpu, ncores = cpu_topology()
-        # physical cores of the CPUs this process may use
-        threads = max(1, min(avail, ncores, avail * ncores // max(1, ncpu)))
-        rate, rates, which, Ts = cpu_baseline(w, args.cpu_seconds, threads)
+        # one thread per physical core this process may use, each bound to it
+        cpus = cpu_ref.core_cpus()
+        threads = len(cpus)
+        rate, rates, which, Ts = cpu_baseline(w, args.cpu_seconds, threads, reflection=refl, cpus=cpus)
         cpu = {"value": rate, "unit": "searchlights/s", "cores": threads, "kind": "port",
                "sample": f"median of 3 runs ({', '.join(f'{r:.1f}' for r in rates)} searchlights/s) over "
                          f"{len(which)} random searchlights of {args.config} (about {args.cpu_seconds:.0f} s in "
                          f"all): the C/OpenMP restatement of the reference's CPU FP64 path (oracle/cpu_ref.c: "
                          f"gather, dgemm, dgesdd, U Vt), one searchlight per OpenMP thread with a single-threaded "
-                         f"BLAS, {threads} threads bound to cores ({avail} CPUs available to the process; "
+                         f"BLAS, {threads} threads each bound to one CPU of a distinct core ({avail} CPUs available to the "
+                         f"process; "
                          f"{ncpu} logical CPUs, {ncores} physical cores, {model}); hemisphere estimate "
                          f"{s.count / rate:.1f} s",
                "cpu_model": model, "logical_cpus": ncpu, "physical_cores": ncores, "cpus_available": avail,
@@ -391,6 +395,7 @@ def main():
         "dtype": "bf16x3" if method in ("polar", "polar_learned") else method,
         "data": "synthetic",
         "config": {"workload": f"{args.config}: {WORKLOADS[args.config]}", "method": method,
+                   "isReflection": refl,
                    "launch": (f"hipGraph replay (captured in an untimed pass; {replays} of {args.steps} timed passes "
                               f"were replays)" if not args.eager else "eager"),
                    "launch_compare": other,

@@ -18,6 +18,8 @@
  * (the bench hands over the ones scipy exports), so this file needs no BLAS
  * headers or libraries at build time.
  */
+#define _GNU_SOURCE
+#include <sched.h>
 #include <stdlib.h>
 #include <string.h>
 #include <omp.h>
@@ -111,13 +113,15 @@ static int local_solve(Ws *w, int nt, int k, int reflection, dgemm_fn dgemm, dge
 
 /* Local transforms of the searchlights which[0 .. n_which-1] (flat lists idx,
  * off as hyperalignment_c.py:161-169) on n_threads OpenMP threads, dynamic
- * schedule, one searchlight at a time per thread. X, Y: nt x nv FP64,
+ * schedule, one searchlight at a time per thread. Thread t is bound to cpus[t]
+ * (the caller's choice, one CPU per physical core; ncpus 0: no binding); the
+ * calling thread gets its own affinity back afterwards. X, Y: nt x nv FP64,
  * column-major. T_out (may be NULL): the k x k column-major transforms one
  * after the other in the order of which. Returns 0, -1 (allocation) or -2 (an
  * SVD that both drivers failed). */
 int cpu_ref_procrustes(void *gemm, void *gesdd, void *gesvd, void *getrf, void *ger, const double *X,
                        const double *Y, int nt, const int *idx, const int *off, const int *which, int n_which,
-                       int reflection, int n_threads, double *T_out) {
+                       int reflection, int n_threads, const int *cpus, int ncpus, double *T_out) {
 	dgemm_fn dgemm = (dgemm_fn)gemm;
 	dgesdd_fn dgesdd = (dgesdd_fn)gesdd;
 	dgesvd_fn dgesvd = (dgesvd_fn)gesvd;
@@ -134,8 +138,16 @@ int cpu_ref_procrustes(void *gemm, void *gesdd, void *gesvd, void *getrf, void *
 		acc += (size_t)k * k;
 	}
 	int err = 0;
+	cpu_set_t own;
+	const int have_own = sched_getaffinity(0, sizeof(own), &own) == 0;
 #pragma omp parallel num_threads(n_threads)
 	{
+		if (ncpus > 0) {
+			cpu_set_t one;
+			CPU_ZERO(&one);
+			CPU_SET(cpus[omp_get_thread_num() % ncpus], &one);
+			sched_setaffinity(0, sizeof(one), &one);
+		}
 		Ws w;
 		int ok = ws_alloc(&w, nt, kmax, dgesdd, dgesvd) == 0;
 		if (!ok) {
@@ -160,6 +172,7 @@ int cpu_ref_procrustes(void *gemm, void *gesdd, void *gesvd, void *getrf, void *
 		}
 		ws_free(&w);
 	}
+	if (ncpus > 0 && have_own) sched_setaffinity(0, sizeof(own), &own);
 	free(tpos);
 	return err;
 }

@@ -37,14 +37,35 @@ def lib():
         L.cpu_ref_procrustes.restype = ctypes.c_int
         P = ctypes.c_void_p
         L.cpu_ref_procrustes.argtypes = [P, P, P, P, P, P, P, ctypes.c_int, P, P, P, ctypes.c_int, ctypes.c_int,
-                                         ctypes.c_int, P]
+                                         ctypes.c_int, P, ctypes.c_int, P]
         _lib = L
     return _lib
 
 
-def procrustes_searchlights(X, Y, indices, offsets, which, threads, reflection=True):
+def core_cpus():
+    """One CPU of each physical core this process may run on (from the
+    affinity mask and /sys topology; CPUs without topology count as cores)."""
+    import glob
+    avail = sorted(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else list(range(os.cpu_count() or 1))
+    seen, out = set(), []
+    for c in avail:
+        key = None
+        try:
+            pkg = open(f"/sys/devices/system/cpu/cpu{c}/topology/physical_package_id").read().strip()
+            core = open(f"/sys/devices/system/cpu/cpu{c}/topology/core_id").read().strip()
+            key = (pkg, core)
+        except OSError:
+            key = ("cpu", c)
+        if key not in seen:
+            seen.add(key)
+            out.append(c)
+    return out
+
+
+def procrustes_searchlights(X, Y, indices, offsets, which, threads, reflection=True, cpus=None):
     """Local transforms (k x k each) of the searchlights `which`, computed by
-    the C restatement on `threads` OpenMP threads; X, Y: nt x nv FP64."""
+    the C restatement on `threads` OpenMP threads, thread t bound to cpus[t]
+    (None: unbound); X, Y: nt x nv FP64."""
     import scipy.linalg.cython_blas as cb
     import scipy.linalg.cython_lapack as cl
     Xf = np.asfortranarray(X, dtype=np.float64)
@@ -55,10 +76,11 @@ def procrustes_searchlights(X, Y, indices, offsets, which, threads, reflection=T
     sizes = (off[wh + 1] - off[wh]).astype(np.int64)
     out = np.zeros(int((sizes ** 2).sum()), dtype=np.float64)
     ptr = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    cp = np.ascontiguousarray(cpus if cpus is not None else [0], dtype=np.int32)
     rc = lib().cpu_ref_procrustes(
         _fn_ptr(cb, "dgemm"), _fn_ptr(cl, "dgesdd"), _fn_ptr(cl, "dgesvd"), _fn_ptr(cl, "dgetrf"),
         _fn_ptr(cb, "dger"), ptr(Xf), ptr(Yf), int(X.shape[0]), ptr(idx), ptr(off), ptr(wh), len(wh),
-        1 if reflection else 0, int(threads), ptr(out))
+        1 if reflection else 0, int(threads), ptr(cp), len(cpus) if cpus is not None else 0, ptr(out))
     if rc != 0:
         raise RuntimeError(f"cpu_ref_procrustes failed ({rc})")
     res, o = [], 0
