@@ -12,8 +12,8 @@ compared):
   * Q^T Q = I for every one of the 40,962 local transforms: the exact FP64
     ||Q^T Q - I||_F of each is at most 1e-3 (the pass's own check sends larger
     ones to the FP64 path; typical values are about 1e-4).
-Also here: one cell of the learned-scaling evaluation grid (seed 101, n = 500,
-noise 1.0) against the oracle, reflection-free local transforms on 64
+Also here: the held-out cells of the learned-scaling evaluation grid at n = 500
+(seeds 100 and 101, noise 0.1, 0.3, 1.0) against the oracle, reflection-free local transforms on 64
 searchlights against LAPACK + LU sign, and the FP32 Jacobi comparator on
 sampled columns of the 10,242-vertex workload.
 """
@@ -160,19 +160,30 @@ def test_c2_no_reflection_local():
     _check_local(got, ref, "c2 no reflection")
 
 
-def test_c3_cell_seed101_noise1():
-    """One held-out cell of the learned-scaling evaluation grid (seed 101,
-    n = 500, noise 1.0) on the full hemisphere, against the oracle."""
-    w = synth.make_workload("c2", seed=101, n=500, noise=1.0)
+@pytest.mark.parametrize("seed", [100, 101])
+@pytest.mark.parametrize("noise", [0.1, 0.3, 1.0])
+def test_c3_cell(seed, noise):
+    """The held-out cells of the learned-scaling evaluation grid at n = 500
+    (seeds 100 and 101, noise 0.1, 0.3 and 1.0) on the full hemisphere: the
+    learned method against the oracle on sampled local transforms, and its mean
+    step count against the plain iteration's on the same data."""
+    w = synth.make_workload("c2", seed=seed, n=500, noise=noise)
     plan = _pass(w)
     try:
         st = plan.stats()
-        which = np.sort(np.random.default_rng(3).choice(w.sls.count, N_LOCAL, replace=False))
+        learned = st["steps_total"] / w.sls.count
+        assert st["n_fallback"] == 0
+        which = np.sort(np.random.default_rng(3).choice(w.sls.count, 64, replace=False))
         got = plan.local(which)
-        print(f"[c3 cell] mean steps {st['steps_total'] / w.sls.count:.2f}, fp64 {st['n_fallback']}", flush=True)
+        ha.set_options(method="polar")
+        plan.execute()
+        plain = plan.stats()["steps_total"] / w.sls.count
     finally:
+        ha.set_options(method="auto")
         plan.close()
-    _check_local(got, _oracle_local(w, which), "c3 seed 101 noise 1.0")
+    print(f"[c3 cell seed {seed} noise {noise}] mean steps learned {learned:.2f}, plain {plain:.2f}", flush=True)
+    assert learned < plain
+    _check_local(got, _oracle_local(w, which), f"c3 seed {seed} noise {noise}")
 
 
 def test_c1_jacobi32_columns():
