@@ -120,6 +120,8 @@ _SIGS = {
     "ha_hip_plan_ridge": (ctypes.c_int, [ctypes.c_void_p, _d]),
     "ha_mesh_searchlights": (ctypes.c_int, [_P(_d), _i32, _P(_i32), _i32, _d, _P(_i32), _i32, _P(_P(_i32)),
                                             _P(_P(_i32)), _P(_P(_d))]),
+    "ha_mesh_searchlights_geodesic": (ctypes.c_int, [_P(_d), _i32, _P(_i32), _i32, _d, _P(_i32), _i32,
+                                                     _P(_P(_i32)), _P(_P(_i32)), _P(_P(_d))]),
     "ha_mesh_free": (None, [ctypes.c_void_p]),
     "ha_hip_plan_create": (ctypes.c_int, [_i32, _i32, _P(_i32), _P(_i32), _P(_d), _i32, _d,
                                           _P(ctypes.c_void_p)]),
@@ -504,11 +506,12 @@ def searchlight_ridge(X, Y, sls, sls_Y=None, mat0=None, alpha=1e3, weights=None)
     return searchlight_ridge_csc(X, Y, sls, mat, sls_Y, weights, alpha)
 
 
-def mesh_searchlights_flat(coords, faces, radius, centers=None):
-    """ha_mesh_searchlights: searchlights of `radius` along the mesh edges
-    (Dijkstra) around each centre (every vertex by default), as the flat arrays
-    the searchlight entry points take: (indices int32, offsets int32, dists float64),
-    each searchlight sorted by vertex id."""
+def mesh_searchlights_flat(coords, faces, radius, centers=None, geodesic=True):
+    """Searchlights of `radius` around each centre (every vertex by default), as
+    the flat arrays the searchlight entry points take: (indices int32, offsets
+    int32, dists float64), each searchlight sorted by vertex id. geodesic:
+    distances across the faces (ha_mesh_searchlights_geodesic, fast marching);
+    else along the mesh edges (ha_mesh_searchlights, Dijkstra)."""
     V = np.ascontiguousarray(coords, dtype=np.float64)
     F = np.ascontiguousarray(faces, dtype=np.int32)
     if V.ndim != 2 or V.shape[1] != 3 or F.ndim != 2 or F.shape[1] != 3:
@@ -518,7 +521,8 @@ def mesh_searchlights_flat(coords, faces, radius, centers=None):
     pd = ctypes.POINTER(ctypes.c_double)()
     nc = V.shape[0] if C is None else len(C)
     L = lib()
-    _check("ha_mesh_searchlights", L.ha_mesh_searchlights(
+    name = "ha_mesh_searchlights_geodesic" if geodesic else "ha_mesh_searchlights"
+    _check(name, getattr(L, name)(
         _ptr(V, ctypes.c_double), V.shape[0], _ptr(F, ctypes.c_int32), F.shape[0], float(radius),
         _ptr(C, ctypes.c_int32), 0 if C is None else len(C), ctypes.byref(pi), ctypes.byref(po), ctypes.byref(pd)))
     try:
@@ -532,12 +536,13 @@ def mesh_searchlights_flat(coords, faces, radius, centers=None):
     return idx, off, dists
 
 
-def mesh_searchlights(coords, faces, radius, centers=None, return_dists=False):
+def mesh_searchlights(coords, faces, radius, centers=None, return_dists=False, geodesic=True):
     """Searchlights on a triangle mesh in the form neuroboros' nb.sls returns
     them (benchmark_neuroboros.py:153): a list of vertex-id arrays, and with
-    return_dists the matching distance arrays. Distances run along the mesh
-    edges (Dijkstra); a searchlight holds the vertices closer than radius."""
-    idx, off, d = mesh_searchlights_flat(coords, faces, radius, centers)
+    return_dists the matching distance arrays. Distances are geodesic across
+    the faces (fast marching), or along the mesh edges with geodesic=False
+    (Dijkstra); a searchlight holds the vertices closer than radius."""
+    idx, off, d = mesh_searchlights_flat(coords, faces, radius, centers, geodesic)
     sls = [idx[off[i]:off[i + 1]] for i in range(len(off) - 1)]
     if not return_dists:
         return sls

@@ -6,9 +6,17 @@
  * each centre vertex, the vertices within radius mm along the cortical
  * surface and their distances, which then feed the weights
  * (ha_searchlight.c:36-92) and the searchlight passes. This builds the same
- * lists from any triangle mesh: distances along the mesh edges (Dijkstra, edge
- * length = Euclidean distance of its end points), the vertices with
- * distance < radius (the centre included, at 0), sorted by vertex id.
+ * lists from any triangle mesh, the vertices with distance < radius (the centre
+ * included, at 0), sorted by vertex id, with two distances:
+ *  - along the mesh edges (Dijkstra, edge length = Euclidean distance of its
+ *    end points: ha_mesh_searchlights);
+ *  - geodesic across the faces (ha_mesh_searchlights_geodesic): fast marching
+ *    with the planar unfolding update. When a vertex is accepted, each face
+ *    around it updates its other two vertices from the edge and, where the
+ *    face's third vertex is accepted as well, from the virtual point source at
+ *    the two accepted distances on the far side of their edge (exact on a flat
+ *    mesh whose shortest paths cross the faces, since a straight ray into a
+ *    vertex enters through one edge of a face around it).
  *
  * One centre per thread: a binary heap and a distance array per thread; the
  * distances reached are reset after each centre, so a centre costs its own
@@ -83,9 +91,50 @@ static int cmp_i32(const void *a, const void *b) {
 
 void ha_mesh_free(void *p) { free(p); }
 
+static double dist3(const double *p, const double *q) {
+	const double dx = p[0] - q[0], dy = p[1] - q[1], dz = p[2] - q[2];
+	return sqrt(dx * dx + dy * dy + dz * dz);
+}
+
+/* Distance at pc from the wavefront that reached pa at da and pb at db, in the
+ * plane of the face (pa, pb, pc) unfolded about its edge ab: the virtual
+ * source s lies at distances da and db from a and b, on the side of ab away
+ * from c; the value is |s c| when the segment s-c crosses the edge ab, else
+ * infinity (the edge updates cover that case). */
+static double unfold_update(const double *pa, const double *pb, const double *pc, double da, double db) {
+	const double c = dist3(pa, pb), b = dist3(pa, pc), a = dist3(pb, pc);
+	if (!(c > 0)) return INFINITY;
+	const double cx = (b * b - a * a + c * c) / (2.0 * c), cy2 = b * b - cx * cx;
+	const double sx = (da * da - db * db + c * c) / (2.0 * c), sy2 = da * da - sx * sx;
+	if (!(cy2 > 0) || sy2 < 0) return INFINITY;
+	const double cy = sqrt(cy2), sy = -sqrt(sy2);
+	const double x0 = sx + (cx - sx) * (-sy) / (cy - sy);  /* where s-c crosses the edge's line */
+	if (x0 < 0.0 || x0 > c) return INFINITY;
+	const double ex = cx - sx, ey = cy - sy;
+	return sqrt(ex * ex + ey * ey);
+}
+
+static int mesh_searchlights_impl(const double *coords, int32_t nv, const int32_t *faces, int32_t nf, double radius,
+                                  const int32_t *centers, int32_t n_centers, int32_t **sl_indices,
+                                  int32_t **sl_offsets, double **sl_dists, int geodesic);
+
 int ha_mesh_searchlights(const double *coords, int32_t nv, const int32_t *faces, int32_t nf, double radius,
                          const int32_t *centers, int32_t n_centers, int32_t **sl_indices, int32_t **sl_offsets,
                          double **sl_dists) {
+	return mesh_searchlights_impl(coords, nv, faces, nf, radius, centers, n_centers, sl_indices, sl_offsets,
+	                              sl_dists, 0);
+}
+
+int ha_mesh_searchlights_geodesic(const double *coords, int32_t nv, const int32_t *faces, int32_t nf,
+                                  double radius, const int32_t *centers, int32_t n_centers, int32_t **sl_indices,
+                                  int32_t **sl_offsets, double **sl_dists) {
+	return mesh_searchlights_impl(coords, nv, faces, nf, radius, centers, n_centers, sl_indices, sl_offsets,
+	                              sl_dists, 1);
+}
+
+static int mesh_searchlights_impl(const double *coords, int32_t nv, const int32_t *faces, int32_t nf, double radius,
+                                  const int32_t *centers, int32_t n_centers, int32_t **sl_indices,
+                                  int32_t **sl_offsets, double **sl_dists, int geodesic) {
 	if (!coords || !faces || nv <= 0 || nf <= 0 || !(radius > 0) || !isfinite(radius) || !sl_indices ||
 	    !sl_offsets || !sl_dists) {
 		ERR("invalid mesh searchlight arguments\n");
@@ -104,6 +153,8 @@ int ha_mesh_searchlights(const double *coords, int32_t nv, const int32_t *faces,
 	int64_t *adj_ptr = (int64_t *)calloc((size_t)nv + 1, sizeof(int64_t));
 	int32_t *adj = NULL;
 	double *wgt = NULL;
+	int64_t *vf_ptr = NULL;  /* faces around each vertex (geodesic) */
+	int32_t *vf = NULL;
 	Hit **lists = (Hit **)calloc((size_t)nc, sizeof(Hit *));
 	int32_t *counts = (int32_t *)calloc((size_t)nc, sizeof(int32_t));
 	if (!adj_ptr || !lists || !counts) { rc = kHaErrorAlloc; goto done; }
@@ -154,6 +205,19 @@ int ha_mesh_searchlights(const double *coords, int32_t nv, const int32_t *faces,
 			wgt[i] = sqrt(dx * dx + dy * dy + dz * dz);
 		}
 
+	if (geodesic) {
+		vf_ptr = (int64_t *)calloc((size_t)nv + 1, sizeof(int64_t));
+		vf = (int32_t *)malloc((size_t)3 * nf * sizeof(int32_t));
+		int64_t *fill = (int64_t *)malloc((size_t)nv * sizeof(int64_t));
+		if (!vf_ptr || !vf || !fill) { free(fill); rc = kHaErrorAlloc; goto done; }
+		for (int64_t i = 0; i < 3 * (int64_t)nf; ++i) ++vf_ptr[faces[i] + 1];
+		for (int32_t v = 0; v < nv; ++v) vf_ptr[v + 1] += vf_ptr[v];
+		memcpy(fill, vf_ptr, (size_t)nv * sizeof(int64_t));
+		for (int64_t f = 0; f < nf; ++f)
+			for (int e = 0; e < 3; ++e) vf[fill[faces[3 * f + e]]++] = (int32_t)f;
+		free(fill);
+	}
+
 	int failed = 0;
 #pragma omp parallel num_threads(ha_host_threads())
 	{
@@ -190,13 +254,41 @@ int ha_mesh_searchlights(const double *coords, int32_t nv, const int32_t *faces,
 					out[ns].v = it.v;
 					out[ns].d = it.d;
 					++ns;
-					for (int64_t i = adj_ptr[it.v]; i < adj_ptr[it.v + 1]; ++i) {
-						const int32_t u = adj[i];
-						const double nd = it.d + wgt[i];
-						if (nd < radius && nd < dist[u]) {
-							if (dist[u] == INFINITY) touched[nt++] = u;
-							dist[u] = nd;
-							if (heap_push(&hp, nd, u)) { bad = 1; break; }
+					if (!geodesic) {
+						for (int64_t i = adj_ptr[it.v]; i < adj_ptr[it.v + 1]; ++i) {
+							const int32_t u = adj[i];
+							const double nd = it.d + wgt[i];
+							if (nd < radius && nd < dist[u]) {
+								if (dist[u] == INFINITY) touched[nt++] = u;
+								dist[u] = nd;
+								if (heap_push(&hp, nd, u)) { bad = 1; break; }
+							}
+						}
+						continue;
+					}
+					/* fast marching: every face around the accepted vertex updates
+					 * its two other vertices */
+					const double *pv = coords + 3 * (int64_t)it.v;
+					for (int64_t i = vf_ptr[it.v]; i < vf_ptr[it.v + 1] && !bad; ++i) {
+						const int32_t *fc = faces + 3 * (int64_t)vf[i];
+						int32_t o[2], no = 0;
+						for (int e = 0; e < 3; ++e)
+							if (fc[e] != it.v && no < 2) o[no++] = fc[e];
+						if (no != 2) continue;  /* degenerate face */
+						for (int side = 0; side < 2; ++side) {
+							const int32_t u = o[side], w = o[1 - side];
+							if (done_v[u]) continue;
+							const double *pu = coords + 3 * (int64_t)u;
+							double nd = it.d + dist3(pv, pu);
+							if (done_v[w]) {
+								const double t = unfold_update(pv, coords + 3 * (int64_t)w, pu, it.d, dist[w]);
+								if (t < nd) nd = t;
+							}
+							if (nd < radius && nd < dist[u]) {
+								if (dist[u] == INFINITY) touched[nt++] = u;
+								dist[u] = nd;
+								if (heap_push(&hp, nd, u)) { bad = 1; break; }
+							}
 						}
 					}
 				}
@@ -234,6 +326,6 @@ int ha_mesh_searchlights(const double *coords, int32_t nv, const int32_t *faces,
 	}
 done:
 	if (lists) for (int32_t i = 0; i < nc; ++i) free(lists[i]);
-	free(lists); free(counts); free(adj_ptr); free(adj); free(wgt);
+	free(lists); free(counts); free(adj_ptr); free(adj); free(wgt); free(vf_ptr); free(vf);
 	return rc;
 }

@@ -27,6 +27,16 @@ extern "C" {
 int ha_mesh_searchlights(const double *coords, int32_t nv, const int32_t *faces, int32_t nf, double radius,
                          const int32_t *centers, int32_t n_centers, int32_t **sl_indices, int32_t **sl_offsets,
                          double **sl_dists);
+/* The same lists with geodesic distances across the faces instead of along the
+ * edges: fast marching with the planar unfolding update (a face whose two
+ * other vertices are accepted gives the third the distance from the virtual
+ * point source at their two distances, when the straight ray crosses their
+ * edge; every edge also gives its Dijkstra update). Exact on flat meshes whose
+ * geodesics cross the faces; on a sphere within a small fraction of the
+ * great-circle distance (tests/test_mesh.py). */
+int ha_mesh_searchlights_geodesic(const double *coords, int32_t nv, const int32_t *faces, int32_t nf,
+                                  double radius, const int32_t *centers, int32_t n_centers, int32_t **sl_indices,
+                                  int32_t **sl_offsets, double **sl_dists);
 void ha_mesh_free(void *p);
 
 #ifdef __cplusplus

@@ -1,6 +1,7 @@
 """Searchlight Procrustes on searchlights built from a triangle mesh
-(ha_mesh_searchlights: edge-path distances, as neuroboros' nb.sls provides them
-to benchmark_neuroboros.py:153), against the CPU FP64 oracle. Tolerance: max-abs
+(ha_mesh_searchlights_geodesic: geodesic distances across the faces, the lists
+neuroboros' nb.sls provides to benchmark_neuroboros.py:153), against the CPU
+FP64 oracle. Tolerance: max-abs
 1e-4 on T, the bar of the other polar tests."""
 
 import numpy as np

@@ -1,11 +1,18 @@
-"""Searchlights on a triangle mesh (ha_mesh_searchlights, host code: no GPU).
+"""Searchlights on a triangle mesh (host code: no GPU).
 
 The reference takes real-mesh searchlights from neuroboros
-(benchmark_neuroboros.py:153), which is not in this image; the builder is
-checked against a full Dijkstra in the oracle, a closed form on a triangulated
-grid, symmetry, the triangle inequality, and the weight invariant of the
-reference's own searchlight tests (tests/test_searchlight.py: weights summed
-per vertex are 1). Distances to 1e-12 (sums of edge lengths in path order).
+(benchmark_neuroboros.py:153), which is not in this image. Two distances:
+  * along the edges (ha_mesh_searchlights): checked against a full Dijkstra
+    in the oracle, a closed form on a triangulated grid, symmetry and the
+    triangle inequality, to 1e-12 (sums of edge lengths in path order);
+  * geodesic across the faces (ha_mesh_searchlights_geodesic, fast marching):
+    on a flat triangulated grid every distance is the Euclidean one to 1e-12,
+    and on the icosphere every distance lies within 0.1% below the
+    great-circle distance (the polyhedron is inscribed in the sphere), so the
+    searchlights equal the great-circle ones except at vertices within 0.1% of
+    the radius.
+Both keep the weight invariant of the reference's own searchlight tests
+(tests/test_searchlight.py: weights summed per vertex are 1).
 """
 
 import base64
@@ -42,7 +49,7 @@ def grid_dist(i, j):
 
 def test_matches_oracle_dijkstra_icosphere():
     v, f = synth.icosphere_mesh(2)
-    sls, ds = ha.mesh_searchlights(v, f, 40.0, return_dists=True)
+    sls, ds = ha.mesh_searchlights(v, f, 40.0, return_dists=True, geodesic=False)
     osl, ods = O.mesh_searchlights(v, f, 40.0)
     assert len(sls) == len(osl) == v.shape[0]
     for a, b, da, db in zip(sls, osl, ds, ods):
@@ -53,13 +60,13 @@ def test_matches_oracle_dijkstra_icosphere():
 
 def test_grid_closed_form():
     coords, faces = grid_mesh(9)
-    sls, ds = ha.mesh_searchlights(coords, faces, 100.0, centers=[0], return_dists=True)
+    sls, ds = ha.mesh_searchlights(coords, faces, 100.0, centers=[0], return_dists=True, geodesic=False)
     sl, d = sls[0], ds[0]
     assert len(sl) == 81
     assert np.abs(d - grid_dist(sl // 9, sl % 9)).max() < 1e-12
     # strictly below the radius: the centre, 4 axis neighbours (1) and 2 diagonal
     # ones (sqrt 2); every other point is at 2 or more
-    sls, ds = ha.mesh_searchlights(coords, faces, 2.0, centers=[40], return_dists=True)
+    sls, ds = ha.mesh_searchlights(coords, faces, 2.0, centers=[40], return_dists=True, geodesic=False)
     sl, d = sls[0], ds[0]
     assert len(sl) == 7
     assert np.abs(d - grid_dist(sl // 9 - 4, sl % 9 - 4)).max() < 1e-12
@@ -67,7 +74,7 @@ def test_grid_closed_form():
 
 def test_symmetric_and_triangle_inequality():
     v, f = synth.icosphere_mesh(3)
-    idx, off, d = ha.mesh_searchlights_flat(v, f, 25.0)
+    idx, off, d = ha.mesh_searchlights_flat(v, f, 25.0, geodesic=False)
     pairs = {}
     for c in range(len(off) - 1):
         for j in range(off[c], off[c + 1]):
@@ -79,15 +86,54 @@ def test_symmetric_and_triangle_inequality():
             assert dd == 0.0
 
 
-def test_sparse_centers_match_full():
+@pytest.mark.parametrize("geodesic", [True, False])
+def test_sparse_centers_match_full(geodesic):
     """Centres taken from a subset (the reference's center_space case,
     tests/test_searchlight.py:33-43) give the same lists as from every vertex."""
     v, f = synth.icosphere_mesh(3)
-    full = ha.mesh_searchlights(v, f, 20.0)
+    full = ha.mesh_searchlights(v, f, 20.0, geodesic=geodesic)
     centers = np.arange(0, v.shape[0], 7)
-    sub = ha.mesh_searchlights(v, f, 20.0, centers=centers)
+    sub = ha.mesh_searchlights(v, f, 20.0, centers=centers, geodesic=geodesic)
     for c, sl in zip(centers, sub):
         assert np.array_equal(sl, full[c])
+
+
+@pytest.mark.parametrize("center", [0, 40, 44])
+def test_geodesic_flat_grid_is_euclidean(center):
+    """On a flat triangulated grid the geodesic distances are the Euclidean
+    ones: every shortest path is a straight ray across the faces, which the
+    unfolding update follows exactly (the edge distances are up to 8% longer)."""
+    coords, faces = grid_mesh(9)
+    sls, ds = ha.mesh_searchlights(coords, faces, 100.0, centers=[center], return_dists=True)
+    sl, d = sls[0], ds[0]
+    assert len(sl) == 81
+    assert np.abs(d - np.linalg.norm(coords[sl] - coords[center], axis=1)).max() < 1e-12
+    # a radius that cuts through the grid keeps exactly the vertices strictly inside it
+    sls = ha.mesh_searchlights(coords, faces, 2.5, centers=[40])
+    inside = np.nonzero(np.linalg.norm(coords - coords[40], axis=1) < 2.5)[0]
+    assert np.array_equal(sls[0], inside)
+
+
+def test_geodesic_icosphere_matches_great_circle():
+    """On the 10,242-vertex icosphere (radius 100 mm) the geodesic distances lie
+    within 0.1% below the great-circle ones, and the 20 mm searchlights equal
+    the great-circle searchlights of the synthetic workloads except at
+    vertices within 0.1% of the radius (edge distances: 5-24% longer)."""
+    w = synth.make_workload("c1")
+    s = w.sls
+    v, f = synth.icosphere_mesh(5)
+    R = np.linalg.norm(v, axis=1).mean()
+    sls, ds = ha.mesh_searchlights(v, f, s.radius, return_dists=True)
+    for c, (sl, d) in enumerate(zip(sls, ds)):
+        # (arccos near 1 resolves angles to about 1e-8 rad: 1e-5 mm of slack)
+        gc = R * np.arccos(np.clip(v[sl] @ v[c] / R ** 2, -1.0, 1.0))
+        assert np.all(d <= gc + 1e-5)
+        assert np.all(d >= gc * (1 - 1e-3) - 1e-5)
+    glists = s.lists()
+    for c in range(0, len(sls), 7):
+        for u in set(sls[c].tolist()) ^ set(glists[c].tolist()):
+            gc = R * np.arccos(np.clip(v[u] @ v[c] / R ** 2, -1.0, 1.0))
+            assert abs(gc - s.radius) < 1e-3 * s.radius
 
 
 @pytest.mark.parametrize("uniform", [True, False])
