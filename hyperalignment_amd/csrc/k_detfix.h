@@ -129,19 +129,59 @@ __device__ inline int lu_sign(float *M, int k, int kp, float *pmax, int *pidx, i
   return sgn;
 }
 
-// V <- L^-1 V for the NV columns of V (kp x NV, column-major, leading dim kp),
-// L lower triangular (kp x kp): 32-row blocks. The rows above a block are
-// folded in by all 256 threads: thread (row i = t % 32, range h = t / 32) sums
-// L(j0 + i, c) V(c, :) over c = h, h + 8, ... < j0 (the 32 rows of one c are
-// one coalesced load), the eight partial sums meet in LDS, and the block itself
-// is solved in LDS.
-__device__ inline void solve_lower_nv(const double *L, double *V, int kp, double (*d)[B + 1], double (*cs)[B + 1],
-                               double (*part)[B + 1]) {
+// v on lane l, read by every lane of the wave (l uniform).
+__device__ __forceinline__ double lane_f64(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __hiloint2double(hi, lo);
+}
+
+// Wave 0 applies d^-1 (TR: d^-T) to rows 0..31 of the NV vectors at vb
+// (leading dim kp), less the eight partial sums in part (part[h NV + q][row]),
+// d the 32 x 32 lower triangular diagonal block. Lane l holds row r = l % 32 of
+// vectors l / 32 and l / 32 + 2; the rows are solved in order (TR: from the
+// last), the solved value of each reaching the other lanes by readlane, so the
+// block needs no barrier.
+template <bool TR>
+__device__ __forceinline__ void block_solve(const double (*d)[B + 1], const double (*part)[B + 1], double *vb,
+                                            int kp) {
+  const int r = threadIdx.x & 31, qa = (threadIdx.x >> 5) & 1;
+  double t0 = vb[r + (int64_t)qa * kp], t1 = vb[r + (int64_t)(qa + 2) * kp];
+#pragma unroll
+  for (int hh = 0; hh < 8; ++hh) {
+    t0 -= part[hh * NV + qa][r];
+    t1 -= part[hh * NV + qa + 2][r];
+  }
+  const double dinv = 1.0 / d[r][r];
+#pragma unroll
+  for (int s = 0; s < B; ++s) {
+    const int ii = TR ? B - 1 - s : s;
+    const double y0 = t0 * dinv, y1 = t1 * dinv;  // final on the lanes of row ii
+    const double a0 = lane_f64(y0, ii), b0 = lane_f64(y0, B + ii);
+    const double a1 = lane_f64(y1, ii), b1 = lane_f64(y1, B + ii);
+    const double x0 = qa ? b0 : a0, x1 = qa ? b1 : a1;
+    const bool below = TR ? r < ii : r > ii;
+    const double l = below ? (TR ? d[ii][r] : d[r][ii]) : 0.0;
+    t0 = r == ii ? x0 : t0 - l * x0;
+    t1 = r == ii ? x1 : t1 - l * x1;
+  }
+  vb[r + (int64_t)qa * kp] = t0;
+  vb[r + (int64_t)(qa + 2) * kp] = t1;
+}
+
+// V <- L^-1 V for the NV columns of V (kp x NV, column-major, leading dim kp;
+// in LDS where the caller has room), L lower triangular (kp x kp): 32-row
+// blocks. The rows above a block are folded in by all 256 threads: thread
+// (row i = t % 32, range h = t / 32) sums L(j0 + i, c) V(c, :) over c = h,
+// h + 8, ... < j0 (the 32 rows of one c are one coalesced load, eight of them in
+// flight), the eight partial sums meet in LDS, and wave 0 solves the block.
+__device__ inline void solve_lower_nv(const double *L, double *V, int kp, double (*d)[B + 1], double (*part)[B + 1]) {
   const int i = threadIdx.x & 31, h = threadIdx.x >> 5;
   for (int j0 = 0; j0 < kp; j0 += B) {
     double sq[NV] = {0.0, 0.0, 0.0, 0.0};
     const double *Lr = L + (j0 + i);
-#pragma unroll 4
+#pragma unroll 8
     for (int c = h; c < j0; c += 8) {
       const double l = Lr[(int64_t)c * kp];
 #pragma unroll
@@ -151,15 +191,7 @@ __device__ inline void solve_lower_nv(const double *L, double *V, int kp, double
     for (int q = 0; q < NV; ++q) part[h * NV + q][i] = sq[q];
     for (int e = threadIdx.x; e < B * B; e += RT) d[e & 31][e >> 5] = L[(j0 + (e & 31)) + (int64_t)(j0 + (e >> 5)) * kp];
     __syncthreads();
-    if (threadIdx.x < B * NV) {
-      const int q = threadIdx.x >> 5;
-      double t = V[(j0 + i) + (int64_t)q * kp];
-      for (int hh = 0; hh < 8; ++hh) t -= part[hh * NV + q][i];
-      cs[i][q] = t;
-    }
-    __syncthreads();
-    forward_sub(d, cs);
-    if (threadIdx.x < B * NV) V[(j0 + i) + (int64_t)(threadIdx.x >> 5) * kp] = cs[i][threadIdx.x >> 5];
+    if (threadIdx.x < 64) block_solve<false>(d, part, V + j0, kp);
     __syncthreads();
   }
 }
@@ -168,14 +200,13 @@ __device__ inline void solve_lower_nv(const double *L, double *V, int kp, double
 // down the block's own columns of L: thread (column i = t / 8, offset h = t % 8)
 // sums L(r, j0 + i) V(r, :) over r = j0 + 32 + h, + 8, ... (eight consecutive r
 // of one column are one 64-byte load), the partial sums meet in LDS.
-__device__ inline void solve_lower_t_nv(const double *L, double *V, int kp, double (*d)[B + 1], double (*cs)[B + 1],
-                                 double (*part)[B + 1]) {
+__device__ inline void solve_lower_t_nv(const double *L, double *V, int kp, double (*d)[B + 1],
+                                        double (*part)[B + 1]) {
   const int h = threadIdx.x & 7, ic = threadIdx.x >> 3;
-  const int i = threadIdx.x & 31;
   for (int j0 = kp - B; j0 >= 0; j0 -= B) {
     double sq[NV] = {0.0, 0.0, 0.0, 0.0};
     const double *Lc = L + (int64_t)(j0 + ic) * kp;
-#pragma unroll 4
+#pragma unroll 8
     for (int r = j0 + B + h; r < kp; r += 8) {
       const double l = Lc[r];
 #pragma unroll
@@ -185,15 +216,7 @@ __device__ inline void solve_lower_t_nv(const double *L, double *V, int kp, doub
     for (int q = 0; q < NV; ++q) part[h * NV + q][ic] = sq[q];
     for (int e = threadIdx.x; e < B * B; e += RT) d[e & 31][e >> 5] = L[(j0 + (e & 31)) + (int64_t)(j0 + (e >> 5)) * kp];
     __syncthreads();
-    if (threadIdx.x < B * NV) {
-      const int q = threadIdx.x >> 5;
-      double t = V[(j0 + i) + (int64_t)q * kp];
-      for (int hh = 0; hh < 8; ++hh) t -= part[hh * NV + q][i];
-      cs[i][q] = t;
-    }
-    __syncthreads();
-    backward_sub_t(d, cs);
-    if (threadIdx.x < B * NV) V[(j0 + i) + (int64_t)(threadIdx.x >> 5) * kp] = cs[i][threadIdx.x >> 5];
+    if (threadIdx.x < 64) block_solve<true>(d, part, V + j0, kp);
     __syncthreads();
   }
 }
@@ -224,6 +247,43 @@ __device__ inline void lower_t_times(const double *L, const double *V, double *W
     }
     __syncthreads();
   }
+}
+
+// tsh = W^T W for the NV columns of W (kp x NV): its ten distinct entries in
+// one pass, summed over each wave by shuffles and over the four waves in LDS
+// (red10, 40 doubles) in a fixed order; ends with a barrier.
+__device__ inline void gram_nv(const double *W, int kp, double *red10, double (*tsh)[NV]) {
+  constexpr int NE = NV * (NV + 1) / 2;
+  double sm[NE];
+#pragma unroll
+  for (int e = 0; e < NE; ++e) sm[e] = 0.0;
+  for (int i = threadIdx.x; i < kp; i += RT) {
+    double w[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) w[q] = W[i + (int64_t)q * kp];
+    int e = 0;
+#pragma unroll
+    for (int p = 0; p < NV; ++p)
+#pragma unroll
+      for (int q = p; q < NV; ++q) sm[e++] += w[p] * w[q];
+  }
+#pragma unroll
+  for (int e = 0; e < NE; ++e)
+    for (int o = 32; o > 0; o >>= 1) sm[e] += __shfl_xor(sm[e], o);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int e = 0; e < NE; ++e) red10[wave * NE + e] = sm[e];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int e = 0;
+    for (int p = 0; p < NV; ++p)
+      for (int q = p; q < NV; ++q, ++e) {
+        const double v = (red10[e] + red10[NE + e]) + (red10[2 * NE + e] + red10[3 * NE + e]);
+        tsh[p][q] = tsh[q][p] = v;
+      }
+  }
+  __syncthreads();
 }
 
 // modified Gram-Schmidt on the NV columns of V (rows < kp)

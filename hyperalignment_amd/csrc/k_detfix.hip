@@ -33,17 +33,79 @@ using namespace kchol;
 
 using namespace kdet;
 
+typedef __attribute__((address_space(3))) double lds_f64;
+
+// The vector phase of the fix for one searchlight whose Cholesky factor L of
+// G = A64^T A64 is in G: block inverse iteration on NV vectors V, Rayleigh-Ritz,
+// then Q <- Q - 2 (Q v) v^T. V lives at Vp (LDS, or global for large kp), W =
+// L^T V at Wg (global), v and u at vs and us (kp doubles each; beside V, or
+// over W, which is dead by then). Returns the smallest eigenvalue of G
+// (sigma_min^2). Not inlined, so that its registers do not crowd the
+// products' loops in the kernel; the LDS arguments keep their address space.
+template <typename VP>
+__device__ __noinline__ double fix_vector(const double *G, float *Q, int k, int kp, VP *Vp, double *Wg, VP *vsp,
+                                          VP *usp, lds_f64 *dp, lds_f64 *partp, lds_f64 *redp, lds_f64 *red10p,
+                                          lds_f64 *tshp, lds_f64 *yshp, lds_f64 *lamp) {
+  double *V = (double *)Vp, *vs = (double *)vsp, *us = (double *)usp;
+  double (*d)[B + 1] = (double (*)[B + 1])(double *)dp, (*part)[B + 1] = (double (*)[B + 1])(double *)partp;
+  double *red = (double *)redp, *red10 = (double *)red10p, *ysh = (double *)yshp, *lam_sh = (double *)lamp;
+  double (*tsh)[NV] = (double (*)[NV])(double *)tshp;
+  for (int64_t e = threadIdx.x; e < (int64_t)kp * NV; e += RT) {
+    const int i = (int)(e % kp), q = (int)(e / kp);
+    V[e] = i < k ? sin(0.7548776662466927 * (i + 1) * (q + 1) + 0.5698402909980532 * q) : 0.0;
+  }
+  __syncthreads();
+  orthonormalize(V, kp, red);
+  for (int step = 0; step < NITER; ++step) {
+    solve_lower_nv(G, V, kp, d, part);
+    solve_lower_t_nv(G, V, kp, d, part);
+    orthonormalize(V, kp, red);
+  }
+  // Rayleigh-Ritz: T = (L^T V)^T (L^T V), its ten distinct entries summed in one pass
+  lower_t_times(G, V, Wg, kp, part);
+  gram_nv(Wg, kp, red10, tsh);
+  if (threadIdx.x == 0) *lam_sh = small_eig(tsh, ysh);
+  __syncthreads();
+  // v = V y, u = Q v, Q -= 2 u v^T
+  for (int i = threadIdx.x; i < kp; i += RT) {
+    double acc = 0.0;
+#pragma unroll
+    for (int q = 0; q < NV; ++q) acc += V[i + (int64_t)q * kp] * ysh[q];
+    vs[i] = acc;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kp; i += RT) {
+    double acc = 0.0;
+#pragma unroll 8
+    for (int j = 0; j < k; ++j) acc += (double)Q[i + (int64_t)j * kp] * vs[j];
+    us[i] = acc;
+  }
+  __syncthreads();
+  for (int64_t e = threadIdx.x; e < (int64_t)kp * kp; e += RT) {
+    const int i = (int)(e % kp), j = (int)(e / kp);
+    Q[e] = (float)((double)Q[e] - 2.0 * us[i] * vs[j]);
+  }
+  const double lam = *lam_sh;
+  __syncthreads();
+  return lam;
+}
+
 __global__ __launch_bounds__(RT, 4) void k_detfix(HakLocalArgs a, double *work64, int64_t stride64,
                                                const int32_t *iota) {
-  __shared__ double t0[B][B + 1], t1[B][B + 1], t2[B][B + 1], t3[B][B + 1];
-  __shared__ double red[4];
+  // t[0..3]: the FP64 tiles of the products and the Cholesky factorization;
+  // in the vector phase t[0] and t[1] are the solves' diagonal block and
+  // partial sums, and t[2..3] hold V (kp <= 528; then v and u go to t[0..1])
+  __shared__ double tl[4][B][B + 1];
+  __shared__ double red[4], red10[4 * 10];
   __shared__ float pmax[4];
   __shared__ int pidx[4], piv_sh, sign_sh, ok_sh, item_sh;
   __shared__ double tsh[NV][NV], ysh[NV], lam_sh;
+  double (*t0)[B + 1] = tl[0], (*t1)[B + 1] = tl[1], (*t2)[B + 1] = tl[2], (*t3)[B + 1] = tl[3];
+  constexpr int VCAP = 2 * B * (B + 1);  // doubles of t[2..3]
   double *A64 = work64 + (int64_t)blockIdx.x * 4 * stride64;
   double *G = A64 + stride64;
   float *M = reinterpret_cast<float *>(A64 + 2 * stride64);
-  double *V = A64 + 3 * stride64;  // kp x NV, then W = L^T V (kp x NV)
+  double *Vg = A64 + 3 * stride64;  // V when it does not fit in LDS, then W = L^T V (kp x NV each)
   for (;;) {
     __syncthreads();
     if (threadIdx.x == 0) item_sh = atomicAdd(a.queue, 1);
@@ -75,51 +137,15 @@ __global__ __launch_bounds__(RT, 4) void k_detfix(HakLocalArgs a, double *work64
       if (threadIdx.x == 0) a.flag[s] = 1;
       continue;
     }
-    // block inverse iteration; the padding rows stay zero (G is block diagonal)
-    for (int64_t e = threadIdx.x; e < (int64_t)kp * NV; e += RT) {
-      const int i = (int)(e % kp), q = (int)(e / kp);
-      V[e] = i < k ? sin(0.7548776662466927 * (i + 1) * (q + 1) + 0.5698402909980532 * q) : 0.0;
-    }
-    __syncthreads();
-    orthonormalize(V, kp, red);
-    for (int step = 0; step < NITER; ++step) {
-      solve_lower_nv(G, V, kp, t0, t3, t1);
-      solve_lower_t_nv(G, V, kp, t0, t3, t1);
-      orthonormalize(V, kp, red);
-    }
-    // Rayleigh-Ritz: T = (L^T V)^T (L^T V)
-    double *W = V + (int64_t)NV * kp;
-    lower_t_times(G, V, W, kp, t1);
-    for (int p = 0; p < NV; ++p)
-      for (int q = p; q < NV; ++q) {
-        double acc = 0.0;
-        for (int i = threadIdx.x; i < kp; i += RT) acc += W[i + (int64_t)p * kp] * W[i + (int64_t)q * kp];
-        acc = wg_sum(acc, red);
-        if (threadIdx.x == 0) tsh[p][q] = tsh[q][p] = acc;
-      }
-    __syncthreads();
-    if (threadIdx.x == 0) lam_sh = small_eig(tsh, ysh);
-    __syncthreads();
-    // v = V y (into W's first column), u = Q v (into W's second), Q -= 2 u v^T
-    double *vv = W, *uu = W + kp;
-    for (int i = threadIdx.x; i < kp; i += RT) {
-      double acc = 0.0;
-      for (int q = 0; q < NV; ++q) acc += V[i + (int64_t)q * kp] * ysh[q];
-      vv[i] = acc;
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < kp; i += RT) {
-      double acc = 0.0;
-      for (int j = 0; j < k; ++j) acc += (double)Q[i + (int64_t)j * kp] * vv[j];
-      uu[i] = acc;
-    }
-    __syncthreads();
-    for (int64_t e = threadIdx.x; e < (int64_t)kp * kp; e += RT) {
-      const int i = (int)(e % kp), j = (int)(e / kp);
-      Q[e] = (float)((double)Q[e] - 2.0 * uu[i] * vv[j]);
-    }
-    if (threadIdx.x == 0 && a.sigma_sum) a.sigma_sum[s] -= 2.0 * sqrt(fmax(lam_sh, 0.0));
-    __syncthreads();
+    // the vector phase; the padding rows of V stay zero (G is block diagonal)
+    double *W = Vg + (int64_t)NV * kp;
+    lds_f64 *l0 = (lds_f64 *)&t0[0][0], *l1 = (lds_f64 *)&t1[0][0], *lr = (lds_f64 *)red,
+            *lr10 = (lds_f64 *)red10, *lt = (lds_f64 *)&tsh[0][0], *ly = (lds_f64 *)ysh, *ll = (lds_f64 *)&lam_sh;
+    const double lam = kp * NV <= VCAP
+                           ? fix_vector<lds_f64>(G, Q, k, kp, (lds_f64 *)&t2[0][0], W, l0, l1, l0, l1, lr, lr10, lt,
+                                                 ly, ll)
+                           : fix_vector<double>(G, Q, k, kp, Vg, W, W, W + kp, l0, l1, lr, lr10, lt, ly, ll);
+    if (threadIdx.x == 0 && a.sigma_sum) a.sigma_sum[s] -= 2.0 * sqrt(fmax(lam, 0.0));
   }
 }
 

@@ -167,8 +167,8 @@ __device__ bool fix_reflection(GemmSmem<TT> &sm, FixLds &fx, const TplSlot &sl, 
   // FP32 floor (CPU emulation of this chain against the FP64 template)
   constexpr int NITER_TPL = 24;
   for (int step = 0; step < NITER_TPL; ++step) {
-    solve_lower_nv(G, V, kp, t0, t3, t1);
-    solve_lower_t_nv(G, V, kp, t0, t3, t1);
+    solve_lower_nv(G, V, kp, t0, t1);
+    solve_lower_t_nv(G, V, kp, t0, t1);
     orthonormalize(V, kp, red);
   }
   lower_t_times(G, V, W, kp, t1);
