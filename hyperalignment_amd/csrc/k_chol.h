@@ -91,32 +91,49 @@ static __device__ void crossprod64(const double *__restrict__ P, const int32_t *
   __syncthreads();
 }
 
-// cs(:, n) <- D^-1 cs(:, n) for the 32 right-hand sides n, D = d lower
-// triangular (one thread per right-hand side).
-static __device__ __forceinline__ void forward_sub(double (*d)[B + 1], double (*cs)[B + 1]) {
-  if (threadIdx.x < B) {
-    const int n = threadIdx.x;
-    for (int i = 0; i < B; ++i) {
-      double v = cs[i][n];
-      for (int q = 0; q < i; ++q) v -= d[i][q] * cs[q][n];
-      cs[i][n] = v / d[i][i];
+// v on lane l, read by every lane of the wave (l uniform).
+static __device__ __forceinline__ double lane_f64(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __hiloint2double(hi, lo);
+}
+
+// cs(:, n) <- D^-1 cs(:, n) (TR: D^-T cs(:, n)) for the 32 right-hand sides n,
+// D = d lower triangular. Wave w takes the right-hand sides 8w .. 8w + 7, lane l
+// row r = l % 32 of four of them (8w + 4 (l / 32) + 0..3). The rows are solved
+// in order (TR: from the last), each solved row reaching the other lanes by
+// readlane, so no step waits on LDS or a barrier. Ends with a barrier.
+template <bool TR>
+static __device__ __forceinline__ void subst32(const double (*d)[B + 1], double (*cs)[B + 1]) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r = lane & 31, hi = lane >> 5, n0 = wave * 8 + hi * 4;
+  double t[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) t[j] = cs[r][n0 + j];
+  const double dinv = 1.0 / d[r][r];
+#pragma unroll
+  for (int s = 0; s < B; ++s) {
+    const int ii = TR ? B - 1 - s : s;
+    const bool below = TR ? r < ii : r > ii;
+    const double l = below ? (TR ? d[ii][r] : d[r][ii]) : 0.0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const double y = t[j] * dinv;  // final on the lanes of row ii
+      const double a = lane_f64(y, ii), b = lane_f64(y, B + ii);
+      const double x = hi ? b : a;
+      t[j] = r == ii ? x : t[j] - l * x;
     }
   }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) cs[r][n0 + j] = t[j];
   __syncthreads();
 }
 
-// cs(:, n) <- D^-T cs(:, n), D = d lower triangular.
-static __device__ __forceinline__ void backward_sub_t(double (*d)[B + 1], double (*cs)[B + 1]) {
-  if (threadIdx.x < B) {
-    const int n = threadIdx.x;
-    for (int i = B - 1; i >= 0; --i) {
-      double v = cs[i][n];
-      for (int q = i + 1; q < B; ++q) v -= d[q][i] * cs[q][n];  // D^T(i, q) = D(q, i)
-      cs[i][n] = v / d[i][i];
-    }
-  }
-  __syncthreads();
-}
+static_assert(RT == 4 * 64 && B == 32, "subst32: four waves of eight right-hand sides");
+
+static __device__ __forceinline__ void forward_sub(double (*d)[B + 1], double (*cs)[B + 1]) { subst32<false>(d, cs); }
+static __device__ __forceinline__ void backward_sub_t(double (*d)[B + 1], double (*cs)[B + 1]) { subst32<true>(d, cs); }
 
 // In-place lower Cholesky of the kp x kp block G (kp a multiple of 32). Returns
 // false if a pivot is not positive (G not positive definite). Left-looking by

@@ -129,14 +129,6 @@ __device__ inline int lu_sign(float *M, int k, int kp, float *pmax, int *pidx, i
   return sgn;
 }
 
-// v on lane l, read by every lane of the wave (l uniform).
-__device__ __forceinline__ double lane_f64(double v, int l) {
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_readlane((int)b, l);
-  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
-  return __hiloint2double(hi, lo);
-}
-
 // Wave 0 applies d^-1 (TR: d^-T) to rows 0..31 of the NV vectors at vb
 // (leading dim kp), less the eight partial sums in part (part[h NV + q][row]),
 // d the 32 x 32 lower triangular diagonal block. Lane l holds row r = l % 32 of
