@@ -14,17 +14,20 @@
 //                                        ha_linalg.c:263-291; without z-scoring
 //                                        the plain averages of the C code)
 //
-// Every Procrustes runs on the MFMA engine with FP32 products
-// (v_mfma_f32_32x32x2_f32): A = D_s^T R (K = time points), the scheduled
-// polar iteration (k_polar.h) and the product D_s Q. The chain is what needs
-// them: 47 dependent solves on cross-covariances with sigma_min / sigma_max
-// near 1e-4 (20 mm, n = 400) amplify bf16x3 rounding about a hundredfold.
+// Every Procrustes forms A = D_s^T R (K = time points) and its polar factor
+// in FP64 on v_mfma_f64_16x16x4f64 (k_polar64.h), and the product D_s Q on the
+// MFMA engine with FP32 products (v_mfma_f32_32x32x2_f32). The chain's
+// cross-covariances reach sigma_min / sigma_max of 1e-8 (20 mm, n = 400), where
+// an FP32 iteration loses the smallest singular pair: one local template then
+// came out 0.9 from the FP64 reference, and FP32 products of A alone left 1e-4.
 // Work slot layout (FP32 words, column-major, leading dim ld = roundup(nt, 32),
 // kp = roundup(k, 32), padding rows/columns exactly zero):
 //   dsp[ns]   subject columns, FP32 ld x kp each
 //   al[ns]    aligned subjects, FP32 ld x kp
 //   com, sum  FP32 ld x kp; one more ld x kp block is spare
-//   A, X, X2, G, P: the polar iteration's kp x kp buffers (kpmax x kpmax each)
+//   A, X, X2, G, P: FP32 kp x kp buffers (kpmax x kpmax each): Q in X2, the
+//             reflection fix's work in X, G, P
+//   A64, X64, G64, P64: the FP64 iteration's kp x kp buffers
 // The local templates (nt x k per searchlight) are combined per vertex by
 // k_template_gather with the searchlight weights (searchlight.py:215-236:
 // tpl[:, sl] += local * w), in searchlight order, in FP64.
@@ -36,6 +39,7 @@
 #include "k_gemm.h"
 #include "k_polar.h"
 #include "k_detfix.h"
+#include "k_polar64.h"
 
 #define LAUNCH_RET() return (int)hipGetLastError()
 
@@ -48,6 +52,7 @@ struct TplSlot {
   float *al;       // subject s at al + s ld kpmax
   float *com, *sum;
   WorkSlots w;
+  double *A64, *X64, *G64, *P64;
 };
 
 __device__ __forceinline__ TplSlot tpl_slot(const HakTplArgs &t, int slot) {
@@ -64,6 +69,11 @@ __device__ __forceinline__ TplSlot tpl_slot(const HakTplArgs &t, int slot) {
   s.w.X2 = pw + 2 * kk;
   s.w.G = pw + 3 * kk;
   s.w.P = pw + 4 * kk;
+  double *p64 = reinterpret_cast<double *>(pw + 5 * kk);  // 5 kk is even: 8-byte aligned
+  s.A64 = p64;
+  s.X64 = p64 + kk;
+  s.G64 = p64 + 2 * kk;
+  s.P64 = p64 + 3 * kk;
   return s;
 }
 
@@ -105,15 +115,16 @@ static_assert(sizeof(GemmSmem<TT>) >= 4 * sizeof(double) * kchol::B * (kchol::B 
 // Reflection fix of one solve (isReflection = false; ha_procrustes.c:56-95):
 // when det Q < 0, Q <- Q - 2 (Q v) v^T with v the smallest right singular vector
 // of A. The sign comes from an FP32 LU of a copy of Q (|det Q| = 1, so it is
-// robust); v from the FP64 Gram matrix A^T A of the FP32 A (not from Q^T A,
+// robust); v from the FP64 Gram matrix A^T A of the FP64 A (not from Q^T A,
 // whose rounding would reach v amplified by sigma_max over the gap below
 // sigma_min), its Cholesky factor, block inverse iteration and Rayleigh-Ritz,
-// as in k_detfix.hip. Work, all free after polar_solve (Q is in X2): X (an FP32
+// as in k_detfix.hip. Work, all free after the iteration (Q is in X2): X (an FP32
 // copy of Q for the LU, then the FP64 vectors), G and P (the FP64 Gram matrix).
 // Returns false when the Gram matrix is not positive definite (A singular);
 // *sigma_min receives the smallest singular value removed from trace(Q^T A)
 // twice (0 when nothing is flipped).
-__device__ bool fix_reflection(GemmSmem<TT> &sm, FixLds &fx, const TplSlot &sl, float *q, const float *A, int k,
+template <typename TA>
+__device__ bool fix_reflection(GemmSmem<TT> &sm, FixLds &fx, const TplSlot &sl, float *q, const TA *A, int k,
                                int kp, double *red, double *sigma_min) {
   using namespace kdet;
   double (*t0)[B + 1] = reinterpret_cast<double (*)[B + 1]>(&sm);
@@ -205,49 +216,43 @@ __device__ bool fix_reflection(GemmSmem<TT> &sm, FixLds &fx, const TplSlot &sl, 
   return true;
 }
 
+// A64 = D^T R (kp x kp, zero outside the k x k block) from the FP32 columns of
+// D and R (ld x kp each, leading dim ld, K = ld with zero padding rows), in
+// FP64. Not inlined, like the iteration: their registers stay out of the
+// kernel's.
+__device__ __noinline__ void cross64(const float *D, const float *R, int ld, int k, int kp, double *A64,
+                                     kp64::lds_f64 *stg) {
+  double *as = (double *)stg, *bs = (double *)stg + kp64::STG;
+  kp64::gemm64<true>(D, ld, R, ld, kp, kp, ld, false, as, bs, [&](int m, int n, double v) {
+    A64[m + (int64_t)n * kp] = (m < k && n < k) ? v : 0.0;
+  });
+}
+
 // out = D_s T with T = procrustes(D_s, R) (reflection allowed unless
 // t.reflection is 0, optional
 // scaling sum(sigma) / sum_j ||D_s[:, j] - mean||^2). Returns the polar steps,
 // or -1 if the iteration did not converge (T = I is used then).
 template <bool FIX>
 __device__ int solve_one(GemmSmem<TT> &sm, FixLds &fx, const HakTplArgs &t, const HakLocalArgs &pa,
-                         const TplSlot &sl, int sidx, const float *R, float *out, int k, int kp, double *red) {
+                         const TplSlot &sl, int sidx, const float *R, float *out, int k, int kp, double *red,
+                         kp64::lds_f64 *stg) {
   const float *D = sl.dsp + (int64_t)sidx * t.ld * t.kpmax;
-  // A = D_s^T R over the time points
-  {
-    const Opnd od = opnd_f32(D, OP_F32_GATHER, t.ld, k);
-    const Opnd orr = opnd_f32(R, OP_F32_GATHER, t.ld, k);
-    float *A = sl.w.A;
-    wg_gemm_mn<TT, OP_F32_GATHER, OP_F32_GATHER, PREC_F32>(sm, od, orr, kp, kp, t.ld, false,
-                                                          [&](int m, int n, float4 v) {
-                                                            *reinterpret_cast<float4 *>(A + m + n * kp) = v;
-                                                          });
-  }
-  __syncthreads();
-  const double fro2 = frob2(sl.w.A, kp, red);
-  const float fro = (float)sqrt(fro2);
-  // Q goes to X2, which polar_solve never uses: its X and G buffers trade roles
-  // every step, so the finishing product may read its X operand from G
+  // A = D_s^T R over the time points in FP64 (the data are FP32 values), and
+  // its polar factor Q by the FP64 iteration; Q goes to X2 in FP32 for the
+  // product D_s Q, and trace(Q^T A) in FP64 for the scaling
+  const kp64::Sched sc{pa.sched, pa.sched_len, pa.max_sched, pa.n_buckets, pa.max_steps, pa.bucket_per_decade,
+                       pa.plain_ell};
+  cross64(D, R, t.ld, k, kp, sl.A64, stg);
+  int steps = kp64::polar64(sc, sl.A64, sl.X64, sl.G64, sl.P64, k, kp, stg, (kp64::lds_f64 *)red);
   float *q = sl.w.X2;
   double tr = 0.0;
-  int steps = 0;
-  if (fro > 0.f && isfinite(fro)) {
-    const float *A = sl.w.A;
-    SoloTeam team;
-    const PolarOut po = polar_solve<TT, PREC_F32>(sm, pa, sl.w, k, kp, fro, fro, pa.plain_ell, false, red,
-                                                  nullptr, team, [&](int m, int n, float4 v) {
-                                                    const float4 o = *reinterpret_cast<const float4 *>(A + m + n * kp);
-                                                    *reinterpret_cast<float4 *>(q + m + n * kp) = v;
-                                                    tr += (double)v.x * o.x + (double)v.y * o.y + (double)v.z * o.z +
-                                                          (double)v.w * o.w;
-                                                  });
-    steps = po.done ? po.steps : -1;
-  } else {
-    steps = -1;
-  }
-  if (steps < 0) {
-    // no polar factor (zero or non-finite A, or no convergence): identity
-    for (int64_t e = threadIdx.x; e < (int64_t)kp * kp; e += NT) {
+  for (int64_t e = threadIdx.x; e < (int64_t)kp * kp; e += NT) {
+    if (steps >= 0) {
+      const double v = sl.X64[e];
+      q[e] = (float)v;
+      tr += v * sl.A64[e];
+    } else {
+      // no polar factor (zero or non-finite A, or no convergence): identity
       const int m = (int)(e % kp), n = (int)(e / kp);
       q[e] = (m == n && m < k) ? 1.f : 0.f;
     }
@@ -255,7 +260,7 @@ __device__ int solve_one(GemmSmem<TT> &sm, FixLds &fx, const HakTplArgs &t, cons
   tr = wg_sum_f64(tr, red);
   if (FIX && steps >= 0) {
     double smin = 0.0;
-    if (fix_reflection(sm, fx, sl, q, sl.w.A, k, kp, red, &smin))
+    if (fix_reflection(sm, fx, sl, q, sl.A64, k, kp, red, &smin))
       tr -= 2.0 * smin;  // the flipped pair's singular value changes sign in trace(Q^T A)
     else
       steps = -1;        // A singular: no smallest pair to flip; the searchlight fails
@@ -305,6 +310,8 @@ __global__ __launch_bounds__(NT, 2) void k_template(HakTplArgs t, HakLocalArgs p
   __shared__ double red[4];
   __shared__ int32_t item_sh;
   __shared__ FixLds fx;
+  __shared__ double stg64[2 * kp64::STG];  // the FP64 products' staging
+  kp64::lds_f64 *stg = (kp64::lds_f64 *)stg64;
   const TplSlot sl = tpl_slot(t, blockIdx.x);
   for (;;) {
     const int it = next_item(t.queue, &item_sh);
@@ -339,7 +346,7 @@ __global__ __launch_bounds__(NT, 2) void k_template(HakTplArgs t, HakLocalArgs p
     // sequential alignment to the running common space
     for (int sub = 1; sub < ns; ++sub) {
       float *a = sl.al + sub * lkmax;
-      acc_steps(solve_one<FIX>(sm, fx, t, pa, sl, sub, sl.com, a, k, kp, red));
+      acc_steps(solve_one<FIX>(sm, fx, t, pa, sl, sub, sl.com, a, k, kp, red, stg));
       if (t.zscore) zscore_cols(a, nt, ld, k);
       for (int64_t e = threadIdx.x; e < lk; e += NT) sl.com[e] = (sl.com[e] + a[e]) * 0.5f;
       __syncthreads();
@@ -359,7 +366,7 @@ __global__ __launch_bounds__(NT, 2) void k_template(HakTplArgs t, HakLocalArgs p
         for (int64_t e = threadIdx.x; e < lk; e += NT) sl.com[e] = (sl.sum[e] - a[e]) * inv_n1;
         __syncthreads();
         if (t.zscore) zscore_cols(sl.com, nt, ld, k);
-        acc_steps(solve_one<FIX>(sm, fx, t, pa, sl, sub, sl.com, a, k, kp, red));
+        acc_steps(solve_one<FIX>(sm, fx, t, pa, sl, sub, sl.com, a, k, kp, red, stg));
       }
     }
     // local template
@@ -403,7 +410,7 @@ __global__ void k_template_gather(const float *__restrict__ loc, const int64_t *
 
 extern "C" int64_t hak_template_slot_floats(int32_t ns, int32_t ld, int32_t kpmax) {
   const int64_t lk = (int64_t)ld * kpmax;
-  return 2 * (int64_t)ns * lk + 3 * lk + 5 * (int64_t)kpmax * kpmax;
+  return 2 * (int64_t)ns * lk + 3 * lk + (5 + 8) * (int64_t)kpmax * kpmax;  // FP32 buffers, then 4 FP64 ones
 }
 
 extern "C" int hak_template(const HakTplArgs *t, const HakLocalArgs *pa, void *stream) {
