@@ -6,6 +6,11 @@ covariances, local polar factors (plus FP64 fallback), and the deterministic
 scatter into the CSC transform. Prints one JSON line (searchlights/s).
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2] [--method auto]
+
+Two more modes time the paths beside it, one JSON line each: --mode ridge (the
+searchlight ridge pass, ha_searchlight_ridge, on the same workload) and --mode
+template (the searchlight template of config 4: 16 subjects, order-5 icosphere,
+n = 400, 20 mm, level2 = 2, through ha_searchlight_template_flat).
 """
 
 import os
@@ -29,6 +34,7 @@ import hyperalignment_amd as ha  # noqa: E402
 from hyperalignment_amd import synth  # noqa: E402
 
 PEAK_BF16 = 2.5e15
+PEAK_FP64 = 78.6e12
 PEAK_HBM = 8.0e12
 
 WORKLOADS = {
@@ -215,6 +221,134 @@ def kp_bucket(kp):
     return 5
 
 
+def one_blas_thread():
+    """numpy's BLAS limited to one thread (whatever the environment sets): the
+    single-core CPU samples below are many small factorizations, which a
+    multithreaded BLAS slows down."""
+    from threadpoolctl import threadpool_limits
+    return threadpool_limits(limits=1, user_api="blas")
+
+
+def bench_ridge(args):
+    """The searchlight ridge pass (ha_searchlight_ridge, ha_searchlight.c:1378-1429): per
+    searchlight betas = (X_sl^T X_sl + alpha I)^-1 X_sl^T Y_sl in FP64, scattered like the
+    Procrustes transforms. Roofline: the solve kernel's 3 n k^2 + 7 k^3 / 3 FP64 flops per
+    searchlight against the FP64 MFMA peak."""
+    w = synth.make_workload(args.config)
+    s = w.sls
+    nt = w.X.shape[0]
+    ha.init()
+    plan = ha.Plan(nt, s.nv, s.indices, s.offsets, s.dists, s.radius)
+    plan.load(w.X, w.Y)
+    for i in range(args.warmup):
+        plan.ridge(args.alpha)
+        print(f"[bench] ridge warmup {i} {plan.stats()['ms_total']:.1f} ms", file=sys.stderr, flush=True)
+    solve_ms = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        plan.ridge(args.alpha)
+        st = plan.stats()
+        solve_ms.append(st["ms_polar"])
+        print(f"[bench] ridge step {i} {st['ms_total']:.1f} ms (solve {st['ms_polar']:.1f} ms)", file=sys.stderr,
+              flush=True)
+    dt = time.perf_counter() - t0
+    k = s.sizes.astype(np.float64)
+    flops = float((3.0 * nt * k * k + 7.0 * k ** 3 / 3.0).sum())
+    kms = float(np.mean(solve_ms))
+    achieved = flops / (kms * 1e-3)
+    roofline = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+                "frac": achieved / PEAK_FP64, "traffic": None, "kernel": "k_ridge64", "kernel_ms": kms,
+                "flops_per_launch": flops,
+                "kernel_timing": "HIP events on the library stream around the solve launch of every timed pass"}
+    cpu = parity = None
+    if not args.no_cpu:
+        from oracle import ha_oracle as O
+        rng = np.random.default_rng(11)
+        order = rng.permutation(s.count)
+        lists = s.lists()
+        done, betas = [], []
+        with one_blas_thread():
+            t0 = time.perf_counter()
+            while time.perf_counter() - t0 < args.cpu_seconds / 2 and len(done) < s.count:
+                c = int(order[len(done)])
+                betas.append(O.ridge(w.X[:, lists[c]], w.Y[:, lists[c]], args.alpha))
+                done.append(c)
+            rate = len(done) / (time.perf_counter() - t0)
+        cpu = {"value": rate, "unit": "searchlights/s", "cores": 1, "kind": "port",
+               "sample": f"{len(done)} random searchlights of {args.config}: the oracle's FP64 ridge through the "
+                         f"SVD of X_sl (ha_ridge.c:13-54, ridge.py:7-12), numpy with a single-threaded BLAS; "
+                         f"hemisphere estimate {s.count / rate:.0f} s"}
+        loc = plan.local(np.array(done, dtype=np.int32))
+        errs = [float(np.abs(a - b).max() / max(1e-30, np.abs(b).max())) for a, b in zip(loc, betas)]
+        parity = {"local_max_rel": max(errs), "local_median_rel": float(np.median(errs)), "searchlights": len(errs),
+                  "note": "max |GPU betas (stored FP32) - oracle| over the largest |beta| of each searchlight"}
+    out = {"metric": "searchlights/s (full-hemisphere searchlight ridge pass)", "value": s.count * args.steps / dt,
+           "unit": "searchlights/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": 1e3 * dt / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"{args.config}: {WORKLOADS[args.config]}", "alpha": args.alpha,
+                      "stage_ms": {"ms_weights": st["ms_weights"], "ms_solve": st["ms_polar"],
+                                   "ms_scatter": st["ms_scatter"]},
+                      "parity_vs_fp64": parity},
+           "roofline": roofline, "cpu_baseline": cpu}
+    print(json.dumps(out))
+    plan.close()
+
+
+def bench_template(args):
+    """The searchlight template of config 4 (ha_searchlight_template_flat; ha_template.c:14-114
+    per searchlight, combined as searchlight.py:215-236): 16 subjects, each call uploads them,
+    runs the template kernel and the FP64 gather, and downloads the template."""
+    dms, sls = synth.template_subjects(n_subj=16, n=400, order=5, radius=20.0, seed=0)
+    ha.init()
+
+    def run():
+        return ha.searchlight_template_flat(dms, sls.indices, sls.offsets, sls.dists, sls.radius, level2_iter=2)
+
+    for i in range(args.warmup):
+        t0 = time.perf_counter()
+        run()
+        print(f"[bench] template warmup {i} {time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        tpl = run()
+    dt = time.perf_counter() - t0
+    cpu = None
+    if not args.no_cpu:
+        from oracle import ha_oracle as O
+        lists = sls.lists()
+        rng = np.random.default_rng(12)
+        order = rng.permutation(sls.count)
+        n = 0
+        with one_blas_thread():
+            t0 = time.perf_counter()
+            while time.perf_counter() - t0 < args.cpu_seconds / 2 and n < sls.count:
+                c = lists[int(order[n])]
+                O.template_procrustes([d[:, c] for d in dms], level2_iter=2)
+                n += 1
+            rate = n / (time.perf_counter() - t0)
+        cpu = {"value": rate, "unit": "searchlights/s", "cores": 1, "kind": "port",
+               "sample": f"{n} random searchlights' local templates (the oracle's FP64 chain with LAPACK SVDs, "
+                         f"ha_template.c:14-114), numpy with a single-threaded BLAS; whole template estimate "
+                         f"{sls.count / rate:.0f} s"}
+    out = {"metric": "searchlights/s (config-4 searchlight template, 47 Procrustes solves per searchlight)",
+           "value": sls.count * args.steps / dt, "unit": "searchlights/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": 1e3 * dt / args.steps, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64 (A and polar factors) + f32 (storage, D Q)",
+           "data": "synthetic",
+           "config": {"workload": "config 4: 16 subjects, order-5 icosphere (10,242 vertices), n = 400, 20 mm "
+                                  "searchlights, level2 = 2, z-scored",
+                      "timed": "each step a whole ha_searchlight_template_flat call (host data in, template out)",
+                      "finite": bool(np.all(np.isfinite(tpl))),
+                      "parity_vs_fp64": "tests/test_gpu_template.py (16 columns within 1.2e-7 of the largest "
+                                        "magnitude)"},
+           "roofline": None,
+           "roofline_note": "not reported: the FP64 iteration's flops follow each solve's step count, which the "
+                            "flat call does not return; the kernel time is in the rocprof summary",
+           "cpu_baseline": cpu}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -233,6 +367,9 @@ def main():
     ap.add_argument("--compare-every", type=int, default=40)
     ap.add_argument("--no-reflection", action="store_true",
                     help="time passes with isReflection = false (the det(Q) < 0 fix, ha_searchlight.c:612-632)")
+    ap.add_argument("--mode", default="procrustes", choices=["procrustes", "ridge", "template"],
+                    help="the searchlight Procrustes pass (the headline), the ridge pass, or the config-4 template")
+    ap.add_argument("--alpha", type=float, default=1.0, help="ridge penalty (--mode ridge)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (reference for the HBM byte counters)")
     args = ap.parse_args()
@@ -244,6 +381,12 @@ def main():
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               f"(the searchlight pass shards by owner-computes column blocks, not run here)", file=sys.stderr)
         sys.exit(2)
+    if args.mode == "ridge":
+        bench_ridge(args)
+        return
+    if args.mode == "template":
+        bench_template(args)
+        return
 
     t0 = time.perf_counter()
     w = synth.make_workload(args.config)
@@ -362,7 +505,8 @@ def main():
         # one thread per physical core this process may use, each bound to it
         cpus = cpu_ref.core_cpus()
         threads = len(cpus)
-        rate, rates, which, Ts = cpu_baseline(w, args.cpu_seconds, threads, reflection=refl, cpus=cpus)
+        with one_blas_thread():  # one searchlight per OpenMP thread, each with a single-threaded BLAS
+            rate, rates, which, Ts = cpu_baseline(w, args.cpu_seconds, threads, reflection=refl, cpus=cpus)
         cpu = {"value": rate, "unit": "searchlights/s", "cores": threads, "kind": "port",
                "sample": f"median of 3 runs ({', '.join(f'{r:.1f}' for r in rates)} searchlights/s) over "
                          f"{len(which)} random searchlights of {args.config} (about {args.cpu_seconds:.0f} s in "
