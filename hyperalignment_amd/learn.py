@@ -21,7 +21,7 @@ Workload grid (config 3): the full-hemisphere mesh (order-6 icosphere, 20 mm
 searchlights) with seeds 0-7, n_samples in {200, 300, 500} and noise sigma in
 {0.1, 0.3, 1.0}; evaluation on held-out seeds.
 
-  python -m hyperalignment_amd.learn train --out gpurun_out/polar_mlp.npz
+  python -m hyperalignment_amd.learn train --out polar_mlp.npz
   python -m hyperalignment_amd.learn eval --model hyperalignment_amd/model/polar_mlp.npz
 """
 
@@ -240,7 +240,7 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
     t = sub.add_parser("train")
-    t.add_argument("--out", default="gpurun_out/polar_mlp.npz")
+    t.add_argument("--out", default="polar_mlp.npz")
     t.add_argument("--seeds", type=int, default=8)
     # the committed model (model/polar_mlp.npz, its info record) was trained with these
     t.add_argument("--sample", type=int, default=3000)
