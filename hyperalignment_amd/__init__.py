@@ -288,7 +288,7 @@ def searchlight_procrustes(X, Y, sls, dists, radius, backend="hip", reflection=T
     accumulated in place (the C entry adds into T_out).
     """
     if backend not in BACKENDS:
-        raise ValueError(f"Unknown backend '{backend}'. Choose from: {list(BACKENDS)}")
+        raise ValueError(f"backend {backend!r} is not one of {sorted(BACKENDS)}")
     idx, offsets, d = _flat(sls, dists)
     return searchlight_procrustes_flat(X, Y, idx, offsets, d, radius, backend=backend, reflection=reflection,
                                        scaling=scaling, n_jobs=n_jobs, T=T)

@@ -343,8 +343,7 @@ static int pick_tile(int kpmax) {
 int ha_plan_create_ex(int32_t nt, const HaPlanSetup *su, const int32_t *slx_idx, const int32_t *sly_idx,
                       const int32_t *sl_off, const double *sl_dists, int32_t count, double radius,
                       HaHipPlan **out) {
-	int rc = ha_hip_ensure();
-	if (rc) return rc;
+	/* the arguments first (host only), so that they are reported as such with or without a device */
 	if (!out || !slx_idx || !sl_off || count <= 0 || nt <= 0 || su->nv_x <= 0 || su->nv_y <= 0) {
 		ERR("invalid plan arguments\n");
 		return kHaErrorArg;
@@ -357,6 +356,8 @@ int ha_plan_create_ex(int32_t nt, const HaPlanSetup *su, const int32_t *slx_idx,
 		if (slx_idx[k] < 0 || slx_idx[k] >= su->nv_x) { ERR("vertex index %d out of range\n", slx_idx[k]); return kHaErrorArg; }
 		if (sly_idx && (sly_idx[k] < 0 || sly_idx[k] >= su->nv_y)) { ERR("vertex index %d out of range\n", sly_idx[k]); return kHaErrorArg; }
 	}
+	int rc = ha_hip_ensure();
+	if (rc) return rc;
 	HaHipPlan *p = (HaHipPlan *)calloc(1, sizeof(HaHipPlan));
 	if (!p) return kHaErrorAlloc;
 	hipStream_t st = g.stream;

@@ -1,7 +1,8 @@
-"""A caller's CSC pattern is checked on the host before anything reaches the
-GPU: a malformed indptr or row list makes the CSC entry points return
-kHaErrorArg (-3) instead of letting the scatter write out of bounds. No GPU
-is needed: the check runs before any device call."""
+"""A caller's CSC pattern and searchlight lists are checked on the host before
+anything reaches the GPU: a malformed indptr or row list, an empty searchlight
+or a vertex index out of range makes the entry points return kHaErrorArg (-3)
+instead of letting a kernel read or write out of bounds. No GPU is needed: the
+checks run before any device call."""
 
 import types
 
@@ -55,3 +56,37 @@ def test_good_pattern_passes_the_check():
         ha.searchlight_procrustes_csc(X, Y, [[0, 1, 2]], m)
     except ha.HaError as e:
         assert e.rc != -3
+
+
+EMPTY = [np.array([0, 1], dtype=np.int32), np.array([], dtype=np.int32), np.array([2], dtype=np.int32)]
+
+
+def test_empty_searchlight_is_rejected_dense():
+    """An empty searchlight (size 0) is an argument error of the dense entry
+    point (ha_searchlight_procrustes_dense), reported before any device call."""
+    X = np.random.default_rng(1).standard_normal((8, 3))
+    with pytest.raises(ha.HaError) as ei:
+        ha.searchlight_procrustes(X, X.copy(), EMPTY, None, 0.0)
+    assert ei.value.rc == -3
+
+
+@pytest.mark.parametrize("entry", ["procrustes", "ridge"])
+def test_empty_searchlight_is_rejected_csc(entry):
+    X = np.random.default_rng(1).standard_normal((8, 3))
+    fn = ha.searchlight_procrustes_csc if entry == "procrustes" else ha.searchlight_ridge_csc
+    with pytest.raises(ha.HaError) as ei:
+        fn(X, X.copy(), [[0, 1, 2], []], _mat(*GOOD, 3, 3))
+    assert ei.value.rc == -3
+
+
+def test_vertex_index_out_of_range_is_rejected():
+    X = np.random.default_rng(1).standard_normal((8, 3))
+    with pytest.raises(ha.HaError) as ei:
+        ha.searchlight_procrustes(X, X.copy(), [np.array([0, 3], dtype=np.int32)], None, 0.0)
+    assert ei.value.rc == -3
+
+
+def test_unknown_backend_is_a_value_error():
+    X = np.zeros((4, 2))
+    with pytest.raises(ValueError):
+        ha.searchlight_procrustes(X, X, [np.array([0, 1])], None, 0.0, backend="cuda9")
