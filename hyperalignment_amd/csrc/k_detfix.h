@@ -298,8 +298,15 @@ __device__ inline void orthonormalize(double *V, int kp, double *red) {
 }
 
 // eigenvector of the smallest eigenvalue of the symmetric NV x NV matrix t
-// (cyclic Jacobi, one thread); returns the eigenvalue
-__device__ inline double small_eig(double (*t)[NV], double *y) {
+// (cyclic Jacobi, one thread) among its first nvalid rows and columns; the
+// others (vectors that k < NV dimensions cannot hold: the block iteration
+// leaves them zero or rounding noise in the same span, so they would carry a
+// zero eigenvalue) are set aside and get y = 0. Returns the eigenvalue.
+__device__ inline double small_eig(double (*t)[NV], double *y, int nvalid) {
+  for (int i = nvalid; i < NV; ++i)
+    for (int j = 0; j < NV; ++j) {
+      t[i][j] = t[j][i] = 0.0;
+    }
   double v[NV][NV];
   for (int i = 0; i < NV; ++i)
     for (int j = 0; j < NV; ++j) v[i][j] = i == j ? 1.0 : 0.0;
@@ -334,10 +341,12 @@ __device__ inline double small_eig(double (*t)[NV], double *y) {
         }
       }
   }
+  // the set-aside rows stay decoupled (their off-diagonal entries are zero, so
+  // no rotation touches them), and the minimum runs over the valid ones
   int m = 0;
-  for (int i = 1; i < NV; ++i)
+  for (int i = 1; i < nvalid; ++i)
     if (t[i][i] < t[m][m]) m = i;
-  for (int i = 0; i < NV; ++i) y[i] = v[i][m];
+  for (int i = 0; i < NV; ++i) y[i] = i < nvalid ? v[i][m] : 0.0;
   return t[m][m];
 }
 

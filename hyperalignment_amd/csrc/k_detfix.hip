@@ -64,7 +64,7 @@ __device__ __noinline__ double fix_vector(const double *G, float *Q, int k, int 
   // Rayleigh-Ritz: T = (L^T V)^T (L^T V), its ten distinct entries summed in one pass
   lower_t_times(G, V, Wg, kp, part);
   gram_nv(Wg, kp, red10, tsh);
-  if (threadIdx.x == 0) *lam_sh = small_eig(tsh, ysh);
+  if (threadIdx.x == 0) *lam_sh = small_eig(tsh, ysh, k < NV ? k : NV);
   __syncthreads();
   // v = V y, u = Q v, Q -= 2 u v^T
   for (int i = threadIdx.x; i < kp; i += RT) {

@@ -191,7 +191,7 @@ __device__ bool fix_reflection(GemmSmem<TT> &sm, FixLds &fx, const TplSlot &sl, 
       if (threadIdx.x == 0) fx.tsh[p][r] = fx.tsh[r][p] = acc;
     }
   __syncthreads();
-  if (threadIdx.x == 0) fx.lam = small_eig(fx.tsh, fx.ysh);
+  if (threadIdx.x == 0) fx.lam = small_eig(fx.tsh, fx.ysh, k < NV ? k : NV);
   __syncthreads();
   // v = V y (W's first column), u = Q v (W's second), Q -= 2 u v^T
   double *vv = W, *uu = W + kp;

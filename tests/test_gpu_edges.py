@@ -1,0 +1,78 @@
+"""Ragged searchlight sizes on the GPU against the FP64 oracle.
+
+One pass over searchlights of 1, 2 and 3 vertices and of sizes on both sides
+of every tile edge the kernels use (32, 64, 128; kp = roundup(k, 32)), with
+distance weights and overlapping lists, through the dense entry point
+(ha_searchlight_procrustes_dense), for each method and with and without
+reflections (ha_searchlight.c:950-1106, :612-632). A one-vertex searchlight's
+transform is sign(x^T y) (+1 without reflections).
+"""
+
+import numpy as np
+import pytest
+
+import hyperalignment_amd as ha
+from oracle import ha_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [1, 2, 3, 31, 32, 33, 63, 64, 65, 95, 127, 128, 129, 160, 200]
+TOL = {"auto": 1e-4, "polar": 1e-4, "jacobi32": 1e-4, "jacobi64": 1e-6}
+
+
+@pytest.fixture(scope="module", autouse=True)
+def hip():
+    ha.init()
+    yield
+    ha.set_options(method="auto")
+
+
+@pytest.fixture(scope="module")
+def case():
+    rng = np.random.default_rng(31)
+    nv, nt = 420, 300
+    X = rng.standard_normal((nt, nv))
+    # Y a noisy rotation of X in random blocks, so the local problems are neither
+    # trivial nor all alike
+    Y = X @ np.linalg.qr(rng.standard_normal((nv, nv)))[0] * 0.3 + rng.standard_normal((nt, nv))
+    lists, dists = [], []
+    for k in SIZES + SIZES[::-1]:
+        c = int(rng.integers(nv))
+        rest = rng.choice(np.setdiff1d(np.arange(nv), [c]), k - 1, replace=False)
+        sl = np.concatenate([[c], rest]).astype(np.int32)
+        lists.append(sl)
+        d = np.concatenate([[0.0], rng.uniform(0.5, 10.0, k - 1)])
+        dists.append(d)
+    sizes = np.array([len(l) for l in lists])
+    offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    idx = np.concatenate(lists).astype(np.int32)
+    d = np.concatenate(dists)
+    ref = {r: O.searchlight_procrustes_dense(X, Y, idx, offsets, d, 10.0, reflection=r) for r in (True, False)}
+    return X, Y, idx, offsets, d, ref
+
+
+@pytest.mark.parametrize("reflection", [True, False])
+@pytest.mark.parametrize("method", ["auto", "polar", "jacobi32", "jacobi64"])
+def test_ragged_sizes_match_the_oracle(case, method, reflection):
+    X, Y, idx, offsets, d, ref = case
+    ha.set_options(method=method)
+    T = ha.searchlight_procrustes_flat(X, Y, idx, offsets, d, 10.0, reflection=reflection)
+    err = np.abs(T - ref[reflection]).max()
+    print(f"[edges] {method} reflection={reflection}: max |T - T_fp64| {err:.2e}", flush=True)
+    assert err < TOL[method], err
+
+
+def test_one_vertex_searchlights(case):
+    """k = 1: the local transform is sign(x^T y) with reflections and +1
+    without, so T at the centre is that sign times the centre's weight sum."""
+    rng = np.random.default_rng(5)
+    nt, nv = 50, 6
+    X = rng.standard_normal((nt, nv))
+    Y = X * np.array([1.0, -1.0, 2.0, -0.5, 1.0, -3.0])
+    idx = np.arange(nv, dtype=np.int32)
+    offsets = np.arange(nv + 1, dtype=np.int32)
+    ha.set_options(method="auto")
+    for reflection in (True, False):
+        T = ha.searchlight_procrustes_flat(X, Y, idx, offsets, None, 0.0, reflection=reflection)
+        want = np.diag(np.sign(np.sum(X * Y, axis=0)) if reflection else np.ones(nv))
+        assert np.abs(T - want).max() < 1e-6, (reflection, np.diag(T))

@@ -157,3 +157,24 @@ def test_template_c4_sampled_columns():
     print(f"[c4] oracle over {len(which)} searchlights {time.time() - t0:.1f} s: max-abs {err:.3e}, "
           f"relative to max |value| ({mag:.2f}) {err / mag:.3e}", file=sys.stderr, flush=True)
     assert err < TOL_C4 * max(1.0, mag), err
+
+
+@pytest.mark.parametrize("nv", [1, 2, 3, 5])
+def test_template_few_columns_without_reflections(nv):
+    """Fewer columns than the reflection fix's four iteration vectors: its
+    Rayleigh-Ritz step then uses only the first k of them (the others hold no
+    direction of their own), and every solve still ends with det = +1."""
+    rng = np.random.default_rng(40 + nv)
+    nt, ns = 30, 4
+    S = rng.standard_normal((nt, nv))
+    dms = []
+    for i in range(ns):
+        Q, _ = np.linalg.qr(rng.standard_normal((nv, nv)))
+        if i % 2:
+            Q[:, 0] *= -1.0
+        dms.append(O.zscore_columns(S @ Q + 0.3 * rng.standard_normal((nt, nv))))
+    tpl = ha.template_procrustes(dms, False, False, True, 2)
+    ref = O.template_procrustes(dms, False, False, True, 2)
+    ok, err = err_ok(tpl, ref)
+    print(f"[template nv {nv}] without reflections: max |diff| {err:.2e}", flush=True)
+    assert ok, err
