@@ -76,3 +76,32 @@ def test_one_vertex_searchlights(case):
         T = ha.searchlight_procrustes_flat(X, Y, idx, offsets, None, 0.0, reflection=reflection)
         want = np.diag(np.sign(np.sum(X * Y, axis=0)) if reflection else np.ones(nv))
         assert np.abs(T - want).max() < 1e-6, (reflection, np.diag(T))
+
+
+@pytest.fixture(scope="module")
+def large():
+    rng = np.random.default_rng(77)
+    nt, nv = 1300, 1200
+    X = rng.standard_normal((nt, nv))
+    Y = 0.3 * X[:, rng.permutation(nv)] + rng.standard_normal((nt, nv))
+    lists = [np.sort(rng.choice(nv, k, replace=False)).astype(np.int32) for k in (520, 600, 1030, 1100)]
+    sizes = np.array([len(l) for l in lists])
+    offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    idx = np.concatenate(lists)
+    ref = {r: O.searchlight_procrustes_dense(X, Y, idx, offsets, None, 0.0, reflection=r) for r in (True, False)}
+    return X, Y, idx, offsets, ref
+
+
+@pytest.mark.parametrize("method,reflection", [("auto", True), ("auto", False), ("polar", True), ("polar", False),
+                                               ("jacobi64", False)])  # (FP64 Jacobi at k = 1,100: 33 s a pass)
+def test_large_searchlights(large, method, reflection):
+    """Full-rank searchlights above 512 vertices (kp 544-1120): without
+    reflections the fix's iteration vectors no longer fit in LDS (kp > 528)
+    and, above kp = 1024, the sign of det Q comes from the LU inside the fix
+    kernel instead of the LU kernel."""
+    X, Y, idx, offsets, ref = large
+    ha.set_options(method=method)
+    T = ha.searchlight_procrustes_flat(X, Y, idx, offsets, None, 0.0, reflection=reflection)
+    err = np.abs(T - ref[reflection]).max()
+    print(f"[edges large] {method} reflection={reflection}: max |T - T_fp64| {err:.2e}", flush=True)
+    assert err < TOL[method], err
