@@ -105,3 +105,32 @@ def test_large_searchlights(large, method, reflection):
     err = np.abs(T - ref[reflection]).max()
     print(f"[edges large] {method} reflection={reflection}: max |T - T_fp64| {err:.2e}", flush=True)
     assert err < TOL[method], err
+
+
+@pytest.mark.parametrize("reflection", [True, False])
+def test_ragged_sizes_with_scaling(case, reflection):
+    """isScaling (ha_searchlight.c:641-653): each transform times the sum of its
+    singular values (less twice the smallest when the fix flipped it) over the
+    searchlight's summed column variances, on the ragged sizes."""
+    X, Y, idx, offsets, d, _ = case
+    ref = O.searchlight_procrustes_dense(X, Y, idx, offsets, d, 10.0, reflection=reflection, scaling=True)
+    ha.set_options(method="auto")
+    T = ha.searchlight_procrustes_flat(X, Y, idx, offsets, d, 10.0, reflection=reflection, scaling=True)
+    err = np.abs(T - ref).max() / max(1.0, np.abs(ref).max())
+    print(f"[edges scaling] reflection={reflection}: max |T - T_fp64| / max |T| {err:.2e}", flush=True)
+    assert err < 1e-4, err
+
+
+def test_ridge_ragged_sizes(case):
+    """The searchlight ridge (ha_searchlight.c:1378-1429) on the ragged sizes:
+    FP64 Cholesky and triangular solves whose 32 x 32 in-block substitutions
+    run on four waves, for 1-200 vertices, against the oracle's SVD form."""
+    X, Y, idx, offsets, _, _ = case
+    lists = [idx[offsets[s]:offsets[s + 1]] for s in range(len(offsets) - 1)]
+    nv = X.shape[1]
+    m = ha.sparse_init(lists, nv)
+    ha.searchlight_ridge_csc(X, Y, lists, m, alpha=10.0)
+    ref = O.searchlight_ridge_csc(X, Y, idx, offsets, m.indptr, m.indices, 10.0)
+    err = np.abs(m.data - ref).max() / np.abs(ref).max()
+    print(f"[edges ridge] max |B - B_fp64| / max |B| {err:.2e}", flush=True)
+    assert err < 1e-6, err
