@@ -430,6 +430,33 @@ def test_default_passes_replay_after_the_first():
         plan.close()
 
 
+def test_graph_replay_without_reflections():
+    """The pass without reflections (the LU-sign and reflection-fix kernels,
+    their queues and the det sign buffer) captured and replayed gives the
+    eager result bit for bit, also when passes with reflections come between
+    (each change of the flag captures anew)."""
+    w = synth.make_workload("c1")
+    s = w.sls
+    plan = ha.Plan(w.X.shape[0], s.nv, s.indices, s.offsets, s.dists, s.radius)
+    plan.load(w.X, w.Y)
+    try:
+        ha.set_options(method="polar", use_graph=0)
+        plan.execute(reflection=False)
+        ref_f = plan.csc()[2].copy()
+        plan.execute(reflection=True)
+        ref_t = plan.csc()[2].copy()
+        assert not np.array_equal(ref_f, ref_t)
+        ha.set_options(use_graph=1)
+        for refl, ref in ((False, ref_f), (False, ref_f), (True, ref_t), (False, ref_f), (False, ref_f)):
+            plan.execute(reflection=refl)
+            got = plan.csc()[2]
+            print(f"[graph refl] reflection={refl} replay={plan.stats()['graph_replay']}", flush=True)
+            assert np.array_equal(got.view(np.uint64), ref.view(np.uint64)), refl
+    finally:
+        ha.set_options(use_graph=1)
+        plan.close()
+
+
 def test_autotuner_report():
     """The tuner's report per kp bucket: every bucket the workload fills gets a
     launch, timed buckets keep the fastest of the three candidates, and the
