@@ -82,11 +82,13 @@ struct Chunk {
 // epi(m, n, value) once per element (m + 16 consecutive lanes: consecutive m).
 // A(m, k) = a[m + k lda] (TA = false) or a[k + m lda] (TA = true); B(k, n) =
 // b[k + n ldb]. sym (M == N, C symmetric): only the tiles and 16 x 16 blocks on
-// and above the diagonal are formed, and the others delivered from them. as,
-// bs: STG doubles of LDS each. Ends with a barrier.
+// and above the diagonal are formed, and the others delivered from them. same:
+// B(k, n) = A(n, k) (B = A^T: X^T X, or G G with G symmetric), so a diagonal
+// tile's B chunk is its A chunk, staged once. as, bs: STG doubles of LDS each.
+// Ends with a barrier.
 template <bool TA, typename SA, typename SB, class Epi>
 __device__ __forceinline__ void gemm64(const SA *a, int lda, const SB *b, int ldb, int M, int N, int K, bool sym,
-                                       double *as, double *bs, Epi epi) {
+                                       bool same, double *as, double *bs, Epi epi) {
   a = uni(a);
   b = uni(b);
   lda = uni(lda);
@@ -118,15 +120,17 @@ __device__ __forceinline__ void gemm64(const SA *a, int lda, const SB *b, int ld
         for (int j = 0; j < 4; ++j) acc[i][j] = f64x4_t{0.0, 0.0, 0.0, 0.0};
       Chunk<TA, SA> ca;
       Chunk<true, SB> cb;
+      const bool one = same && tm == tn;  // the B chunk is the A chunk
+      const double *bsrc = one ? as : bs;
       ca.load(a, lda, tm, 0, M, K);
-      cb.load(b, ldb, tn, 0, N, K);
+      if (!one) cb.load(b, ldb, tn, 0, N, K);
       for (int k0 = 0; k0 < K; k0 += KCH) {
         ca.store(as);
-        cb.store(bs);
+        if (!one) cb.store(bs);
         __syncthreads();
         if (k0 + KCH < K) {
           ca.load(a, lda, tm, k0 + KCH, M, K);
-          cb.load(b, ldb, tn, k0 + KCH, N, K);
+          if (!one) cb.load(b, ldb, tn, k0 + KCH, N, K);
         }
 #pragma unroll
         for (int s = 0; s < KCH / 4; ++s) {
@@ -134,7 +138,7 @@ __device__ __forceinline__ void gemm64(const SA *a, int lda, const SB *b, int ld
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
             fa[i] = as[(4 * s + lk) * LSTR + (wr + 2 * i) * 16 + li];
-            fb[i] = bs[(4 * s + lk) * LSTR + (wc + 2 * i) * 16 + li];
+            fb[i] = bsrc[(4 * s + lk) * LSTR + (wc + 2 * i) * 16 + li];
           }
           // operands swapped: lane l then holds C(m0 + l % 16, n0 + l / 16 + 4 r)
 #pragma unroll
@@ -219,7 +223,7 @@ __device__ __noinline__ int polar64(Sched sc, const double *A, double *X, double
   for (int step = 0; step < sc.max_steps; ++step) {
     // G = X^T X and r = ||G - I||_F over the k x k block
     double r2 = 0.0;
-    gemm64<true>(Xb, kp, Xb, kp, kp, kp, kp, true, as, bs, [&](int m, int n, double v) {
+    gemm64<true>(Xb, kp, Xb, kp, kp, kp, kp, true, true, as, bs, [&](int m, int n, double v) {
       Gb[m + (int64_t)n * kp] = v;
       if (m < k && n < k) {
         const double d = v - (m == n ? 1.0 : 0.0);
@@ -249,13 +253,13 @@ __device__ __noinline__ int polar64(Sched sc, const double *A, double *X, double
     }
     // P = ca I + cb G + cc G^2 on the k x k block, zero outside
     const double *Gr = Gb;
-    gemm64<false>(Gr, kp, Gr, kp, kp, kp, kp, true, as, bs, [&](int m, int n, double v) {
+    gemm64<false>(Gr, kp, Gr, kp, kp, kp, kp, true, true, as, bs, [&](int m, int n, double v) {
       const int64_t i = m + (int64_t)n * kp;
       P[i] = (m < k && n < k) ? (m == n ? ca : 0.0) + cb * Gr[i] + cc * v : 0.0;
     });
     // X P goes over G (dead after P), and the two trade roles
     double *Xn = Gb;
-    gemm64<false>(Xb, kp, P, kp, kp, kp, kp, false, as, bs,
+    gemm64<false>(Xb, kp, P, kp, kp, kp, kp, false, false, as, bs,
                   [&](int m, int n, double v) { Xn[m + (int64_t)n * kp] = v; });
     Gb = Xb;
     Xb = Xn;

@@ -223,7 +223,7 @@ __device__ bool fix_reflection(GemmSmem<TT> &sm, FixLds &fx, const TplSlot &sl, 
 __device__ __noinline__ void cross64(const float *D, const float *R, int ld, int k, int kp, double *A64,
                                      kp64::lds_f64 *stg) {
   double *as = (double *)stg, *bs = (double *)stg + kp64::STG;
-  kp64::gemm64<true>(D, ld, R, ld, kp, kp, ld, false, as, bs, [&](int m, int n, double v) {
+  kp64::gemm64<true>(D, ld, R, ld, kp, kp, ld, false, false, as, bs, [&](int m, int n, double v) {
     A64[m + (int64_t)n * kp] = (m < k && n < k) ? v : 0.0;
   });
 }
