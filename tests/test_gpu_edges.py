@@ -134,3 +134,19 @@ def test_ridge_ragged_sizes(case):
     err = np.abs(m.data - ref).max() / np.abs(ref).max()
     print(f"[edges ridge] max |B - B_fp64| / max |B| {err:.2e}", flush=True)
     assert err < 1e-6, err
+
+
+@pytest.mark.parametrize("scale", [1e-25, 1e-12, 1e12, 1e25])
+def test_data_scale_outside_fp32(case, scale):
+    """The transform does not depend on the data's scale (polar(c A) = polar(A)).
+    At 1e-25 or 1e25 the cross-covariance (about scale^2) lies outside FP32's
+    range, so the FP32/bf16x3 path cannot form it; those searchlights must reach
+    the FP64 path instead of returning garbage, as the reference's FP64 path
+    handles them."""
+    X, Y, idx, offsets, d, ref = case
+    ha.set_options(method="auto")
+    T = ha.searchlight_procrustes_flat(X * scale, Y * scale, idx, offsets, d, 10.0)
+    err = np.abs(T - ref[True]).max()
+    print(f"[edges scale] {scale:g}: max |T - T_fp64| {err:.2e}", flush=True)
+    assert np.all(np.isfinite(T))
+    assert err < 1e-4, err
