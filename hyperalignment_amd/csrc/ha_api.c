@@ -179,6 +179,8 @@ int ha_searchlight_procrustes_hip(const double *X_data, const double *Y_data, in
                                   bool col_major) {
 	if (!X_data || !Y_data || !sl_indices || !sl_offsets || !T_out) return kHaErrorArg;
 	if (nt <= 0 || nv <= 0 || count <= 0) return kHaErrorArg;
+	if (ha_check_finite(X_data, (int64_t)nt * nv, "X") || ha_check_finite(Y_data, (int64_t)nt * nv, "Y"))
+		return kHaErrorArg;
 	HaHipPlan *p = NULL;
 	int rc = ha_hip_plan_create(nt, nv, sl_indices, sl_offsets, sl_dists, count, radius, &p);
 	if (rc) return rc;
@@ -237,6 +239,9 @@ int ha_searchlight_procrustes_hip_csc(const TMat *X, const TMat *Y, const TSearc
 	if (sy->count != sls_X->count) return kHaErrorArg;
 	if (mat->cols > Y->cols) { ERR("CSC pattern has more columns than Y vertices\n"); return kHaErrorArg; }
 	if (ha_check_csc(mat->cols, X->cols, mat->indptr, mat->indices, mat->nnz)) return kHaErrorArg;
+	if (ha_check_finite(X->data, (int64_t)X->rows * X->cols, "X") ||
+	    ha_check_finite(Y->data, (int64_t)Y->rows * Y->cols, "Y"))
+		return kHaErrorArg;
 	for (int32_t s = 0; s < sls_X->count; ++s)
 		if (sy->sizes[s] != sls_X->sizes[s]) {
 			ERR("searchlight %d has %d X and %d Y vertices; this backend needs equal sizes\n", s,
@@ -300,6 +305,9 @@ int ha_searchlight_ridge(const TMat *X, const TMat *Y, const TSearchlights *sls_
 	if (sy->count != sls_X->count) return kHaErrorArg;
 	if (mat->cols > Y->cols) { ERR("CSC pattern has more columns than Y vertices\n"); return kHaErrorArg; }
 	if (ha_check_csc(mat->cols, X->cols, mat->indptr, mat->indices, mat->nnz)) return kHaErrorArg;
+	if (ha_check_finite(X->data, (int64_t)X->rows * X->cols, "X") ||
+	    ha_check_finite(Y->data, (int64_t)Y->rows * Y->cols, "Y"))
+		return kHaErrorArg;
 	for (int32_t s = 0; s < sls_X->count; ++s)
 		if (sy->sizes[s] != sls_X->sizes[s]) {
 			ERR("searchlight %d has %d X and %d Y vertices; this backend needs equal sizes\n", s,
@@ -492,6 +500,8 @@ int ha_searchlight_template_flat(const double *const *dms, int32_t n_subj, int32
 	if (!dms || !sl_indices || !sl_offsets || !tpl_out || n_subj < 2 || nt <= 0 || nv <= 0 || count <= 0 ||
 	    level2_iter < 0)
 		return kHaErrorArg;
+	for (int32_t s = 0; s < n_subj; ++s)
+		if (!dms[s] || ha_check_finite(dms[s], (int64_t)nt * nv, "a subject")) return kHaErrorArg;
 	HaPlanSetup su = {.build_pattern = false, .build_weights_index = true, .index_only = true,
 	                  .nv_x = nv, .nv_y = nv};
 	HaHipPlan *p = NULL;

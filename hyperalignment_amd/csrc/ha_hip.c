@@ -81,6 +81,20 @@ int ha_host_threads(void) {
 }
 
 void ha_set_num_threads(int n) { g.host_threads = n > 0 ? n : 0; }
+
+/* kHaErrorArg unless every one of the n values is finite (a host scan before
+ * any upload: NaN or Inf in X or Y would otherwise come back as NaN transforms,
+ * or as the FP64 path's best effort, with no error). */
+int ha_check_finite(const double *A, int64_t n, const char *what) {
+	int bad = 0;
+#pragma omp parallel for schedule(static) reduction(|| : bad) num_threads(ha_host_threads())
+	for (int64_t i = 0; i < n; ++i) bad = bad || !isfinite(A[i]);
+	if (bad) {
+		ERR("%s holds a value that is not finite\n", what);
+		return kHaErrorArg;
+	}
+	return kHaSuccess;
+}
 int ha_get_num_threads(void) { return ha_host_threads(); }
 
 static int build_schedules(void) {
@@ -668,6 +682,11 @@ int ha_plan_load_xy(HaHipPlan *p, const double *X, const double *Y, bool col_maj
 }
 
 int ha_hip_plan_load(HaHipPlan *plan, const double *X, const double *Y, bool col_major) {
+	/* the entry points check their data before they make a plan; a plan's own load checks here */
+	if (!plan || !X || !Y) return kHaErrorArg;
+	if (ha_check_finite(X, (int64_t)plan->nt * plan->nv_x, "X") ||
+	    ha_check_finite(Y, (int64_t)plan->nt * plan->nv_y, "Y"))
+		return kHaErrorArg;
 	return ha_plan_load_xy(plan, X, Y, col_major);
 }
 
@@ -1479,8 +1498,9 @@ int ha_plan_template(HaHipPlan *p, const double *const *dms, int32_t ns, bool co
 	int32_t *order = NULL, *status = NULL, *queue = NULL, *h_order = NULL, *h_status = NULL;
 	int64_t *loc_off = NULL, *h_loc_off = NULL;
 	if (!p->weights_computed || ns < 2 || level2 < 0) { ERR("template needs >= 2 subjects\n"); return kHaErrorArg; }
-	for (int32_t s = 0; s < ns; ++s)
+	for (int32_t s = 0; s < ns; ++s) {
 		if (!dms[s]) { ERR("subject %d has no data\n", s); return kHaErrorArg; }
+	}
 	h_order = (int32_t *)malloc((size_t)count * sizeof(int32_t));
 	h_status = (int32_t *)malloc((size_t)count * sizeof(int32_t));
 	h_loc_off = (int64_t *)malloc((size_t)(count + 1) * sizeof(int64_t));

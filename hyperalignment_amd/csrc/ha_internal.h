@@ -69,6 +69,7 @@ int ha_plan_template(HaHipPlan *p, const double *const *dms, int32_t ns, bool co
 void *ha_hip_stream(void);
 int ha_hip_ensure(void);
 int ha_host_threads(void);
+int ha_check_finite(const double *A, int64_t n, const char *what);
 
 #ifdef __cplusplus
 }

@@ -90,3 +90,25 @@ def test_unknown_backend_is_a_value_error():
     X = np.zeros((4, 2))
     with pytest.raises(ValueError):
         ha.searchlight_procrustes(X, X, [np.array([0, 1])], None, 0.0, backend="cuda9")
+
+
+@pytest.mark.parametrize("bad", [np.nan, np.inf, -np.inf])
+@pytest.mark.parametrize("entry", ["dense", "csc", "ridge", "template"])
+def test_non_finite_data_is_rejected(entry, bad):
+    """NaN or Inf in X, Y or a subject is an argument error of the searchlight
+    entry points, found by a host scan before anything reaches the GPU."""
+    rng = np.random.default_rng(2)
+    X = rng.standard_normal((8, 3))
+    Y = rng.standard_normal((8, 3))
+    Y[5, 1] = bad
+    lists = [np.array([0, 1, 2], dtype=np.int32)]
+    with pytest.raises(ha.HaError) as ei:
+        if entry == "dense":
+            ha.searchlight_procrustes(X, Y, lists, None, 0.0)
+        elif entry == "csc":
+            ha.searchlight_procrustes_csc(X, Y, [[0, 1, 2]], _mat(*GOOD, 3, 3))
+        elif entry == "ridge":
+            ha.searchlight_ridge_csc(X, Y, [[0, 1, 2]], _mat(*GOOD, 3, 3))
+        else:
+            ha.searchlight_template_flat([X, Y], lists[0], np.array([0, 3], dtype=np.int32), None, 0.0)
+    assert ei.value.rc == -3

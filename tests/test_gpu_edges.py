@@ -150,3 +150,22 @@ def test_data_scale_outside_fp32(case, scale):
     print(f"[edges scale] {scale:g}: max |T - T_fp64| {err:.2e}", flush=True)
     assert np.all(np.isfinite(T))
     assert err < 1e-4, err
+
+
+def test_plan_load_rejects_non_finite_data(case):
+    """A plan's load (ha_hip_plan_load) rejects NaN or Inf with kHaErrorArg and
+    leaves the plan usable for finite data."""
+    X, Y, idx, offsets, d, ref = case
+    plan = ha.Plan(X.shape[0], X.shape[1], idx, offsets, d, 10.0)
+    try:
+        Yb = Y.copy()
+        Yb[3, 7] = np.nan
+        with pytest.raises(ha.HaError) as ei:
+            plan.load(X, Yb)
+        assert ei.value.rc == -3
+        plan.load(X, Y)
+        ha.set_options(method="auto")
+        plan.execute()
+        assert plan.stats()["n_fallback"] >= 0
+    finally:
+        plan.close()
