@@ -753,9 +753,66 @@ extern "C" int hak_jacobi32(const HakLocalArgs *a, void *stream) {
   LAUNCH_RET();
 }
 
+// sign(det M) of the k x k matrix M (column-major, leading dimension kp) by LU
+// with partial pivoting of a copy in L; the pivot is the first row of largest
+// |value|, as dgetrf takes it. For the orthogonal factors U and V of the FP64
+// path |det| = 1, so every pivot is far from zero and the sign is robust.
+__device__ int lu_det_sign64(const double *M, double *L, int k, int kp, double *pmax_sh, int32_t *pidx_sh,
+                             int32_t *piv_sh, int32_t *sign_sh) {
+  for (int64_t e = threadIdx.x; e < (int64_t)kp * kp; e += NT) L[e] = M[e];
+  if (threadIdx.x == 0) *sign_sh = 1;
+  __syncthreads();
+  for (int c = 0; c < k; ++c) {
+    // pivot search in column c, rows c..k-1
+    double best = -1.0;
+    int bi = c;
+    for (int r = c + threadIdx.x; r < k; r += NT) {
+      double v = fabs(L[r + (int64_t)c * kp]);
+      if (v > best) { best = v; bi = r; }
+    }
+    for (int o2 = 32; o2 > 0; o2 >>= 1) {
+      double ob = __shfl_xor(best, o2);
+      int oi = __shfl_xor(bi, o2);
+      if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) { pmax_sh[wave] = best; pidx_sh[wave] = bi; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double b0 = pmax_sh[0];
+      int i0 = pidx_sh[0];
+      for (int q = 1; q < 4; ++q)
+        if (pmax_sh[q] > b0 || (pmax_sh[q] == b0 && pidx_sh[q] < i0)) { b0 = pmax_sh[q]; i0 = pidx_sh[q]; }
+      *piv_sh = i0;
+      if (i0 != c) *sign_sh = -*sign_sh;
+    }
+    __syncthreads();
+    const int pr = *piv_sh;
+    if (pr != c)
+      for (int j = threadIdx.x; j < k; j += NT) {
+        double t = L[c + (int64_t)j * kp];
+        L[c + (int64_t)j * kp] = L[pr + (int64_t)j * kp];
+        L[pr + (int64_t)j * kp] = t;
+      }
+    __syncthreads();
+    const double d = L[c + (int64_t)c * kp];
+    if (threadIdx.x == 0 && d < 0.0) *sign_sh = -*sign_sh;
+    if (d != 0.0) {
+      for (int64_t e = threadIdx.x; e < (int64_t)(k - c - 1) * (k - c - 1); e += NT) {
+        int r = c + 1 + (int)(e % (k - c - 1)), j = c + 1 + (int)(e / (k - c - 1));
+        L[r + (int64_t)j * kp] -= L[r + (int64_t)c * kp] / d * L[c + (int64_t)j * kp];
+      }
+    }
+    __syncthreads();
+  }
+  const int sg = *sign_sh;
+  __syncthreads();
+  return sg;
+}
+
 // FP64 path: one workgroup per flagged searchlight (all of them when
-// a.flag == NULL). Work per slot: A, V, LU scratch (3 * slot_stride64 doubles)
-// and sig (kp doubles, inside the LU scratch tail).
+// a.flag == NULL). Work per slot (4 * stride64 doubles, stride64 = kpmax^2):
+// A (then U), V, the LU copy of the sign test, and sig (kp doubles).
 __global__ __launch_bounds__(NT) void k_jacobi64(HakLocalArgs a, int want_det_fix, double *work64,
                                                  int64_t stride64, int32_t count) {
   __shared__ double red[4];
@@ -805,55 +862,6 @@ __global__ __launch_bounds__(NT) void k_jacobi64(HakLocalArgs a, int want_det_fi
         }
       }
     __syncthreads();
-    if (want_det_fix) {
-      // sign(det Q) = sign(det A) for nonsingular A: LU with partial pivoting on a copy
-      for (int64_t e = threadIdx.x; e < (int64_t)kp * kp; e += NT) Lm[e] = Am[e];
-      if (threadIdx.x == 0) sign_sh = 1;
-      __syncthreads();
-      for (int c = 0; c < k; ++c) {
-        // pivot search in column c, rows c..k-1
-        double best = -1.0;
-        int bi = c;
-        for (int r = c + threadIdx.x; r < k; r += NT) {
-          double v = fabs(Lm[r + (int64_t)c * kp]);
-          if (v > best) { best = v; bi = r; }
-        }
-        for (int o2 = 32; o2 > 0; o2 >>= 1) {
-          double ob = __shfl_xor(best, o2);
-          int oi = __shfl_xor(bi, o2);
-          if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
-        }
-        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-        if (lane == 0) { pmax_sh[wave] = best; pidx_sh[wave] = bi; }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-          double b0 = pmax_sh[0];
-          int i0 = pidx_sh[0];
-          for (int q = 1; q < 4; ++q)
-            if (pmax_sh[q] > b0 || (pmax_sh[q] == b0 && pidx_sh[q] < i0)) { b0 = pmax_sh[q]; i0 = pidx_sh[q]; }
-          piv_sh = i0;
-          if (i0 != c) sign_sh = -sign_sh;
-        }
-        __syncthreads();
-        const int pr = piv_sh;
-        if (pr != c)
-          for (int j = threadIdx.x; j < k; j += NT) {
-            double t = Lm[c + (int64_t)j * kp];
-            Lm[c + (int64_t)j * kp] = Lm[pr + (int64_t)j * kp];
-            Lm[pr + (int64_t)j * kp] = t;
-          }
-        __syncthreads();
-        const double d = Lm[c + (int64_t)c * kp];
-        if (threadIdx.x == 0 && d < 0.0) sign_sh = -sign_sh;
-        if (d != 0.0) {
-          for (int64_t e = threadIdx.x; e < (int64_t)(k - c - 1) * (k - c - 1); e += NT) {
-            int r = c + 1 + (int)(e % (k - c - 1)), j = c + 1 + (int)(e / (k - c - 1));
-            Lm[r + (int64_t)j * kp] -= Lm[r + (int64_t)c * kp] / d * Lm[c + (int64_t)j * kp];
-          }
-        }
-        __syncthreads();
-      }
-    }
     int sweeps = jacobi_sweeps<double>(Am, Vm, k, kp, &flag_sh);
     // U columns, rank completion for zero singular values
     if (threadIdx.x == 0) rank_sh = 0;
@@ -924,13 +932,20 @@ __global__ __launch_bounds__(NT) void k_jacobi64(HakLocalArgs a, int want_det_fi
       if (threadIdx.x == 0) sig[j] = 0.0;
       __syncthreads();
     }
-    // reflection fix: if det(Q) < 0, flip the pair of the smallest singular value
+    // reflection fix (ha_searchlight.c:611-631): if det(Q) < 0, flip the pair
+    // of the smallest singular value. sign(det Q) = sign(det U) sign(det V),
+    // from the finished factors as the reference takes it from Q = U V^T: for
+    // a rank-deficient A (k >= nt) det A is rounding noise and the completed
+    // columns of U decide the sign.
     int jmin = 0;
+    bool flip = false;
     if (want_det_fix) {
       for (int j = 1; j < k; ++j)
         if (sig[j] <= sig[jmin]) jmin = j;
+      const int su = lu_det_sign64(Am, Lm, k, kp, pmax_sh, pidx_sh, &piv_sh, &sign_sh);
+      const int sv = lu_det_sign64(Vm, Lm, k, kp, pmax_sh, pidx_sh, &piv_sh, &sign_sh);
+      flip = su * sv < 0;
     }
-    const bool flip = want_det_fix && sign_sh < 0;
     float *Q = a.q + a.qoff[s];
     double *Q64 = a.q64 ? a.q64 + a.qoff[s] : nullptr;
     compose_uvt<double, float>(Am, Vm, k, kp, Q, Q64, xs64, ys64, flip ? jmin : -1);

@@ -169,3 +169,55 @@ def test_plan_load_rejects_non_finite_data(case):
         assert plan.stats()["n_fallback"] >= 0
     finally:
         plan.close()
+
+
+@pytest.mark.parametrize("method", ["auto", "polar"])
+def test_rank_deficient_without_reflections(method):
+    """k >= n_samples with isReflection = false: the fix's Cholesky of A^T A
+    fails (A is singular), so these searchlights take the FP64 path, which
+    applies the fix itself. The optimal rotation is not unique here, so the
+    checks are its invariants: Q^T Q = I, det Q = +1, and the residual
+    ||X Q - Y|| of the oracle's rotation (LAPACK + LU sign)."""
+    rng = np.random.default_rng(13)
+    nv, nt = 60, 10
+    X = rng.standard_normal((nt, nv))
+    Y = rng.standard_normal((nt, nv))
+    lists = [np.sort(rng.choice(nv, k, replace=False)).astype(np.int32) for k in (12, 16, 16, 24, 40)]
+    sizes = np.array([len(l) for l in lists])
+    offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    idx = np.concatenate(lists)
+    plan = ha.Plan(nt, nv, idx, offsets, None, 0.0)
+    try:
+        plan.load(X, Y)
+        ha.set_options(method=method)
+        plan.execute(reflection=False)
+        for l, T in zip(lists, plan.local(np.arange(len(lists)))):
+            T = T.astype(np.float64)
+            k = len(l)
+            assert np.abs(T.T @ T - np.eye(k)).max() < 1e-4
+            assert np.linalg.det(T) > 0
+            To = O.procrustes(X[:, l], Y[:, l], reflection=False)
+            r, ro = np.linalg.norm(X[:, l] @ T - Y[:, l]), np.linalg.norm(X[:, l] @ To - Y[:, l])
+            print(f"[edges rank] {method} k {k}: residual {r:.6f} vs {ro:.6f}", flush=True)
+            assert abs(r - ro) <= 1e-4 * max(1.0, ro)
+    finally:
+        plan.close()
+
+
+@pytest.mark.parametrize("reflection", [True, False])
+def test_single_procrustes_fewer_samples_than_columns(reflection):
+    """ha_procrustes (FP64 Jacobi) with M < N: A = X^T Y is singular, the
+    transform is one of many optimal ones, and without reflections its det is
+    +1 (the sign of det U V^T, ha_searchlight.c:611-631, not of the singular
+    A). Seeds 0-5 hold both signs of the oracle's det with reflections."""
+    for seed in range(6):
+        rng = np.random.default_rng(100 + seed)
+        X = rng.standard_normal((10, 16))
+        Y = rng.standard_normal((10, 16))
+        T = ha.procrustes(X, Y, reflection=reflection)
+        To = O.procrustes(X, Y, reflection=reflection)
+        assert np.abs(T.T @ T - np.eye(16)).max() < 1e-10
+        if not reflection:
+            assert np.linalg.det(T) > 0
+        r, ro = np.linalg.norm(X @ T - Y), np.linalg.norm(X @ To - Y)
+        assert abs(r - ro) <= 1e-10 * max(1.0, ro), (seed, r, ro)
