@@ -20,6 +20,9 @@
 // cross-covariances reach sigma_min / sigma_max of 1e-8 (20 mm, n = 400), where
 // an FP32 iteration loses the smallest singular pair: one local template then
 // came out 0.9 from the FP64 reference, and FP32 products of A alone left 1e-4.
+// A solve the iteration does not finish (a zero or singular A: k >= nt; or
+// sigma_min / sigma_max below its schedules' 1e-9) takes the FP64 one-sided
+// Jacobi SVD of k_jacobi.h, which completes U over zero singular values.
 // Work slot layout (FP32 words, column-major, leading dim ld = roundup(nt, 32),
 // kp = roundup(k, 32), padding rows/columns exactly zero):
 //   dsp[ns]   subject columns, FP32 ld x kp each
@@ -40,6 +43,7 @@
 #include "k_polar.h"
 #include "k_detfix.h"
 #include "k_polar64.h"
+#include "k_jacobi.h"
 
 #define LAUNCH_RET() return (int)hipGetLastError()
 
@@ -228,6 +232,52 @@ __device__ __noinline__ void cross64(const float *D, const float *R, int ld, int
   });
 }
 
+// The FP64 Jacobi solve (k_jacobi.h) of one Procrustes problem of the chain,
+// for an A the iteration does not finish: its schedules reach sigma_min /
+// sigma_max of about 1e-9, and a singular A (k >= nt z-scored samples leave
+// rank <= nt - 1; the FP64 A's rounding-level singular values are about 1e-16
+// of ||A||) lies far below that; also for an A whose reflection fix found
+// A^T A singular. Q = U V^T with U completed over the zero singular values
+// (D_s Q does not depend on the completion when the null space of A is that
+// of D_s, as for z-scored data of full rank nt - 1), and Q = I for A = 0, as
+// LAPACK's SVD gives. Without reflections det Q = +1. A64 is copied to X64
+// (then U), G64 holds V, P64 the LU work and A64 the singular values. Returns
+// (sweeps, or -1 with Q = I for a non-finite A; sum(sigma) less twice a flipped
+// one, the trace term of the scaling). Not inlined, like the iteration.
+__device__ __noinline__ double2 solve_svd64(double *A64, double *X64, double *G64, double *P64, float *q, int k,
+                                            int kp, int fix, kp64::lds_f64 *stg, kp64::lds_f64 *redp) {
+  A64 = kp64::uni(A64);
+  X64 = kp64::uni(X64);
+  G64 = kp64::uni(G64);
+  P64 = kp64::uni(P64);
+  q = kp64::uni(q);
+  k = kp64::uni(k);
+  kp = kp64::uni(kp);
+  fix = kp64::uni(fix);
+  double *red = (double *)redp;
+  const int64_t kk2 = (int64_t)kp * kp;
+  double f2 = 0.0;
+  for (int64_t e = threadIdx.x; e < kk2; e += NT) {
+    const double v = A64[e];
+    X64[e] = v;
+    f2 += v * v;
+  }
+  f2 = wg_sum_f64(f2, red);
+  if (!isfinite(f2)) {
+    for (int64_t e = threadIdx.x; e < kk2; e += NT) {
+      const int m = (int)(e % kp), n = (int)(e / kp);
+      q[e] = (m == n && m < k) ? 1.f : 0.f;
+    }
+    __syncthreads();
+    return make_double2(-1.0, 0.0);
+  }
+  double ss = 0.0;
+  const int sw = kjac::svd_polar64(X64, G64, P64, A64, k, kp, fix != 0, q, nullptr,
+                                   *reinterpret_cast<kjac::SvdLds *>((double *)stg), red, &ss);
+  return make_double2(sw > 0 ? (double)sw : -1.0, ss);
+}
+static_assert(sizeof(kjac::SvdLds) <= 2 * kp64::STG * sizeof(double), "the Jacobi solve's LDS fits the staging");
+
 // out = D_s T with T = procrustes(D_s, R) (reflection allowed unless
 // t.reflection is 0, optional
 // scaling sum(sigma) / sum_j ||D_s[:, j] - mean||^2). Returns the polar steps,
@@ -246,24 +296,27 @@ __device__ int solve_one(GemmSmem<TT> &sm, FixLds &fx, const HakTplArgs &t, cons
   int steps = kp64::polar64(sc, sl.A64, sl.X64, sl.G64, sl.P64, k, kp, stg, (kp64::lds_f64 *)red);
   float *q = sl.w.X2;
   double tr = 0.0;
-  for (int64_t e = threadIdx.x; e < (int64_t)kp * kp; e += NT) {
-    if (steps >= 0) {
+  if (steps >= 0) {
+    for (int64_t e = threadIdx.x; e < (int64_t)kp * kp; e += NT) {
       const double v = sl.X64[e];
       q[e] = (float)v;
       tr += v * sl.A64[e];
-    } else {
-      // no polar factor (zero or non-finite A, or no convergence): identity
-      const int m = (int)(e % kp), n = (int)(e / kp);
-      q[e] = (m == n && m < k) ? 1.f : 0.f;
+    }
+    tr = wg_sum_f64(tr, red);
+    if (FIX) {
+      double smin = 0.0;
+      if (fix_reflection(sm, fx, sl, q, sl.A64, k, kp, red, &smin))
+        tr -= 2.0 * smin;  // the flipped pair's singular value changes sign in trace(Q^T A)
+      else
+        steps = -1;        // A^T A not positive definite: the Jacobi solve below
     }
   }
-  tr = wg_sum_f64(tr, red);
-  if (FIX && steps >= 0) {
-    double smin = 0.0;
-    if (fix_reflection(sm, fx, sl, q, sl.A64, k, kp, red, &smin))
-      tr -= 2.0 * smin;  // the flipped pair's singular value changes sign in trace(Q^T A)
-    else
-      steps = -1;        // A singular: no smallest pair to flip; the searchlight fails
+  if (steps < 0) {
+    // zero, singular or too ill-conditioned for the iteration's schedules
+    const double2 r = solve_svd64(sl.A64, sl.X64, sl.G64, sl.P64, q, k, kp, FIX ? 1 : 0, stg,
+                                  (kp64::lds_f64 *)red);
+    steps = (int)r.x;
+    tr = r.y;
   }
   float f = 1.f;
   if (t.scaling && steps >= 0) {

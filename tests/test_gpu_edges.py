@@ -221,3 +221,24 @@ def test_single_procrustes_fewer_samples_than_columns(reflection):
             assert np.linalg.det(T) > 0
         r, ro = np.linalg.norm(X @ T - Y), np.linalg.norm(X @ To - Y)
         assert abs(r - ro) <= 1e-10 * max(1.0, ro), (seed, r, ro)
+
+
+@pytest.mark.parametrize("reflection", [True, False])
+def test_zero_cross_covariance_gives_identity(reflection):
+    """A searchlight over all-zero columns has A = 0: LAPACK's SVD of the zero
+    matrix gives U = V = I, so the reference's T is the identity; the FP64
+    path returns it (every singular value zero: no completion)."""
+    rng = np.random.default_rng(8)
+    nt, nv = 40, 12
+    X = rng.standard_normal((nt, nv))
+    Y = rng.standard_normal((nt, nv))
+    X[:, :5] = 0.0
+    idx = np.array([0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11], dtype=np.int32)
+    offsets = np.array([0, 5, 12], dtype=np.int32)
+    ha.set_options(method="auto")
+    T = ha.searchlight_procrustes_flat(X, Y, idx, offsets, None, 0.0, reflection=reflection)
+    ref = O.searchlight_procrustes_dense(X, Y, idx, offsets, None, 0.0, reflection=reflection)
+    assert np.abs(T[:5, :5] - np.eye(5)).max() < 1e-12
+    err = np.abs(T - ref).max()
+    print(f"[edges zero A] reflection={reflection}: max |T - T_fp64| {err:.2e}", flush=True)
+    assert err < 1e-4, err

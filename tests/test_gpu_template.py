@@ -178,3 +178,42 @@ def test_template_few_columns_without_reflections(nv):
     ok, err = err_ok(tpl, ref)
     print(f"[template nv {nv}] without reflections: max |diff| {err:.2e}", flush=True)
     assert ok, err
+
+
+@pytest.mark.parametrize("reflection", [True, False])
+@pytest.mark.parametrize("zscore,level2", [(True, 2), (False, 1)])
+def test_template_fewer_samples_than_columns(reflection, zscore, level2):
+    """nt = 10 samples, 14 columns: every cross-covariance of the chain is
+    singular (rank <= nt - 1), below what the polar iteration's schedules
+    reach, so each solve takes the FP64 Jacobi SVD, which completes U over the
+    zero singular values. With z-scored subjects the null space of A is that
+    of D_s, so D_s Q (and the template) does not depend on the completion and
+    matches the oracle's LAPACK solve; without reflections the flipped pair is
+    a zero one and changes nothing either."""
+    rng = np.random.default_rng(21)
+    nt, nv, ns = 10, 14, 4
+    S = rng.standard_normal((nt, nv))
+    dms = [O.zscore_columns(S @ np.linalg.qr(rng.standard_normal((nv, nv)))[0] + 0.3 * rng.standard_normal((nt, nv)))
+           for _ in range(ns)]
+    tpl = ha.template_procrustes(dms, reflection, False, zscore, level2)
+    ref = O.template_procrustes(dms, reflection, False, zscore, level2)
+    ok, err = err_ok(tpl, ref)
+    print(f"[template nt < nv] reflection {reflection} zscore {zscore}: max |diff| {err:.2e}", flush=True)
+    assert ok, err
+
+
+@pytest.mark.parametrize("reflection", [True, False])
+def test_searchlight_template_few_samples(reflection):
+    """Small mesh (order 3, 30 mm: searchlights of 13-18 vertices) with 15
+    samples, so some searchlights have k >= nt (singular chains, the Jacobi
+    solve) and some k < nt (the polar iteration), against the oracle."""
+    dms, sls = synth.template_subjects(n_subj=4, n=15, order=3, radius=30.0, seed=4)
+    sizes = np.diff(sls.offsets)
+    assert (sizes >= 15).any() and (sizes < 15).any()
+    tpl = ha.searchlight_template_flat(dms, sls.indices, sls.offsets, sls.dists, sls.radius, reflection=reflection,
+                                       level2_iter=2)
+    ref = O.searchlight_template(dms, sls.indices, sls.offsets, sls.dists, sls.radius, reflection=reflection,
+                                 level2_iter=2, n_jobs=8)
+    ok, err = err_ok(tpl, ref)
+    print(f"[template mesh nt 15] reflection {reflection}: max |diff| {err:.2e}", flush=True)
+    assert ok, err
