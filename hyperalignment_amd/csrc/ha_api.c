@@ -356,35 +356,51 @@ done:
 /* X (M x N) and Y (M x P) are padded with zero columns to K = max(N, P) and
  * solved as one K-vertex searchlight: for alpha > 0 the padded system is block
  * diagonal, diag(X^T X + alpha I, alpha I), so betas[:N, :P] are unchanged. */
+/* One M x N by M x P problem as ceil(P / N) searchlights of N vertices: each
+ * takes every column of X and the next N columns of Y (the last ones padded
+ * with zero columns, whose betas are zero), so the systems stay N x N (or M x M
+ * in the dual form when M < N) and alpha = 0 needs only X of full rank, as the
+ * reference's SVD form (ha_ridge.c:13-54). */
 int ha_ridge(const TMat *X, const TMat *Y, double alpha, TMat *betas) {
 	if (!X || !Y || !betas || !X->data || !Y->data || !betas->data) return kHaErrorArg;
 	const int32_t M = X->rows, N = X->cols, P = Y->cols;
 	if (M <= 0 || N <= 0 || P <= 0 || Y->rows != M || betas->rows != N || betas->cols != P) return kHaErrorArg;
-	const int32_t K = N > P ? N : P;
-	double *xp = (double *)calloc((size_t)M * K, sizeof(double));
-	double *yp = (double *)calloc((size_t)M * K, sizeof(double));
-	double *bk = (double *)malloc((size_t)K * K * sizeof(double));
-	int32_t *idx = (int32_t *)malloc((size_t)K * sizeof(int32_t));
-	int32_t off[2] = {0, K};
+	if (ha_check_finite(X->data, (int64_t)M * N, "X") || ha_check_finite(Y->data, (int64_t)M * P, "Y"))
+		return kHaErrorArg;
+	const int32_t nch = (P + N - 1) / N;
+	if ((int64_t)nch * N > INT32_MAX) return kHaErrorArg;
+	const int32_t PY = nch * N;
+	double *yp = (double *)calloc((size_t)M * PY, sizeof(double));
+	double *bk = (double *)malloc((size_t)nch * N * N * sizeof(double));
+	int32_t *ix = (int32_t *)malloc((size_t)PY * sizeof(int32_t));
+	int32_t *iy = (int32_t *)malloc((size_t)PY * sizeof(int32_t));
+	int32_t *off = (int32_t *)malloc((size_t)(nch + 1) * sizeof(int32_t));
 	HaHipPlan *p = NULL;
 	int rc = kHaErrorAlloc;
-	if (!xp || !yp || !bk || !idx) goto done;
-	for (int32_t t = 0; t < M; ++t) {
-		memcpy(xp + (int64_t)t * K, X->data + (int64_t)t * N, (size_t)N * sizeof(double));
-		memcpy(yp + (int64_t)t * K, Y->data + (int64_t)t * P, (size_t)P * sizeof(double));
+	if (!yp || !bk || !ix || !iy || !off) goto done;
+	for (int32_t t = 0; t < M; ++t)
+		memcpy(yp + (int64_t)t * PY, Y->data + (int64_t)t * P, (size_t)P * sizeof(double));
+	for (int32_t c = 0; c < nch; ++c) {
+		off[c] = c * N;
+		for (int32_t i = 0; i < N; ++i) {
+			ix[c * N + i] = i;
+			iy[c * N + i] = c * N + i;
+		}
 	}
-	for (int32_t i = 0; i < K; ++i) idx[i] = i;
+	off[nch] = PY;
 	HaPlanSetup su = {.build_pattern = false, .build_weights_index = false, .index_only = false,
-	                  .nv_x = K, .nv_y = K};
-	rc = ha_plan_create_ex(M, &su, idx, NULL, off, NULL, 1, 0.0, &p);
-	if (!rc) rc = ha_plan_load_xy(p, xp, yp, false);
+	                  .nv_x = N, .nv_y = PY};
+	rc = ha_plan_create_ex(M, &su, ix, iy, off, NULL, nch, 0.0, &p);
+	if (!rc) rc = ha_plan_load_xy(p, X->data, yp, false);
 	if (!rc) rc = ha_plan_ridge(p, alpha, false, bk);
 	if (!rc)
-		for (int32_t r = 0; r < N; ++r)
-			memcpy(betas->data + (int64_t)r * P, bk + (int64_t)r * K, (size_t)P * sizeof(double));
+		for (int32_t c = 0; c < nch; ++c)
+			for (int32_t r = 0; r < N; ++r)
+				for (int32_t j = 0; j < N && c * N + j < P; ++j)
+					betas->data[(int64_t)r * P + c * N + j] = bk[((int64_t)c * N + r) * N + j];
 done:
 	ha_hip_plan_destroy(p);
-	free(xp); free(yp); free(bk); free(idx);
+	free(yp); free(bk); free(ix); free(iy); free(off);
 	return rc;
 }
 
@@ -399,6 +415,10 @@ int ha_ridge_vec(const TMat *X, const double *y, double alpha, double *betas_out
 int ha_procrustes(const TMat *X, const TMat *Y, TMat *T, bool isReflection, bool isScaling) {
 	if (!X || !Y || !T || !X->data || !Y->data || !T->data) return kHaErrorArg;
 	if (X->rows != Y->rows || X->cols != Y->cols || T->rows != X->cols || T->cols != X->cols) return kHaErrorArg;
+	if (X->rows <= 0 || X->cols <= 0) return kHaErrorArg;
+	if (ha_check_finite(X->data, (int64_t)X->rows * X->cols, "X") ||
+	    ha_check_finite(Y->data, (int64_t)Y->rows * Y->cols, "Y"))
+		return kHaErrorArg;
 	const int32_t N = X->cols;
 	int32_t *idx = (int32_t *)malloc((size_t)N * sizeof(int32_t));
 	int32_t off[2] = {0, N};

@@ -1782,7 +1782,16 @@ int ha_plan_ridge(HaHipPlan *p, double alpha, bool zero_data, double *betas64) {
 		rc = kHaErrorInternal;
 		goto fail;
 	}
-	KCK(hak_ridge64(&a, alpha, p->work_ridge, p->stride64, p->n_wg_ridge, p->count, st));
+	{
+		/* the order is k-descending: the searchlights with fewer time points
+		 * than vertices (the dual form, nt x nt systems) come first */
+		int32_t n_dual = 0;
+		for (int32_t s = 0; s < p->count; ++s) n_dual += p->h_k[s] > p->nt;
+		HakLocalArgs b = a;
+		b.queue = p->queues + 2;
+		KCK(hak_ridge64(&b, alpha, p->work_ridge, p->stride64, p->n_wg_ridge, 0, n_dual, 1, st));
+		KCK(hak_ridge64(&a, alpha, p->work_ridge, p->stride64, p->n_wg_ridge, n_dual, p->count, 0, st));
+	}
 	HCK(hipEventRecord(p->ev[2], st));
 	if (!betas64)
 		KCK(hak_scatter(p->slx, p->off, p->sl_of, p->occ_ptr, p->occ, p->ncols, p->indptr64, p->indices, p->data,
@@ -1804,11 +1813,14 @@ int ha_plan_ridge(HaHipPlan *p, double alpha, bool zero_data, double *betas64) {
 		p->stats.nnz = p->nnz;
 		/* algorithmic flops 3 n k^2 + 7 k^3 / 3: X^T X (symmetric, n k^2), X^T Y
 		 * (2 n k^2), the Cholesky (k^3 / 3) and two triangular solves with k
-		 * right-hand sides (2 k^3) */
+		 * right-hand sides (2 k^3); dual form (n < k): X X^T (k n^2), the
+		 * Cholesky (n^3 / 3), two solves with k right-hand sides (2 k n^2) and
+		 * X^T Z (2 n k^2) */
 		double flops = 0;
 		for (int32_t s = 0; s < p->count; ++s) {
-			const double k = p->h_k[s];
-			flops += 3.0 * p->nt * k * k + 7.0 * k * k * k / 3.0;
+			const double k = p->h_k[s], n = p->nt;
+			flops += k > n ? 3.0 * k * n * n + n * n * n / 3.0 + 2.0 * n * k * k
+			               : 3.0 * n * k * k + 7.0 * k * k * k / 3.0;
 		}
 		p->stats.gemm_flops = flops;
 	}
@@ -1821,9 +1833,9 @@ int ha_plan_ridge(HaHipPlan *p, double alpha, bool zero_data, double *betas64) {
 			if (fl[s]) { if (first < 0) first = s; ++bad; }
 		p->stats.n_fallback = bad;
 		if (bad) {
-			ERR("ridge: X_sl^T X_sl + alpha I is not positive definite for %d searchlight(s) (first %d, alpha %g); "
-			    "alpha must be > 0 where a searchlight has fewer time points than vertices or a constant column\n",
-			    bad, first, alpha);
+			ERR("ridge: X_sl^T X_sl + alpha I (X_sl X_sl^T + alpha I where a searchlight has fewer time points "
+			    "than vertices) is not positive definite for %d searchlight(s) (first %d, alpha %g); alpha = 0 "
+			    "needs X_sl of full rank\n", bad, first, alpha);
 			rc = kHaErrorSvd;
 			goto fail;
 		}

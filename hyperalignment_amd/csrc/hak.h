@@ -227,12 +227,13 @@ int hak_scale(const double *sigma_sum, const double *var_sum, int32_t nt, int32_
 
 /* ---- searchlight ridge (k_ridge.hip) ---- */
 #define HAK_RIDGE_OCC 4  /* ridge workgroups per CU (33.8 KB LDS, at most 128 VGPRs each) */
-/* Per searchlight (order, count): betas = (X^T X + alpha I)^-1 X^T Y in FP64
- * into the Q store (FP32, and q64 when set); flag[s] = 1 if X^T X + alpha I is
- * not positive definite. n_wg workgroups; work64: n_wg * 2 * stride64 doubles
- * (stride64 >= kpmax^2). */
-int hak_ridge64(const HakLocalArgs *a, double alpha, double *work64, int64_t stride64, int n_wg, int32_t count,
-                void *stream);
+/* Per searchlight of a.order[begin .. end-1]: betas = (X^T X + alpha I)^-1 X^T Y
+ * in FP64 into the Q store (FP32, and q64 when set); flag[s] = 1 if the system
+ * is not positive definite. dual: searchlights with nt < k, solved as
+ * X^T (X X^T + alpha I)^-1 Y. n_wg workgroups; work64: n_wg * 2 * stride64
+ * doubles (stride64 >= kpmax^2). */
+int hak_ridge64(const HakLocalArgs *a, double alpha, double *work64, int64_t stride64, int n_wg, int32_t begin,
+                int32_t end, int dual, void *stream);
 
 /* ---- applying and scoring a transform (k_apply.hip) ---- */
 /* Y = X T: X nt x nv FP64 column-major (leading dim nt), T CSC with ncols

@@ -36,12 +36,14 @@ static __device__ __forceinline__ f64x4_t mfma32(const double (*xa)[B + 1], cons
   return acc;
 }
 
-// out[m + n kp] = sum_t P[cols_p[m]] Q[cols_q[n]] over t < nt (m, n < k; zero
-// elsewhere) + diag on m == n < k and 1 on m == n >= k.
+// out[m + n kp] = sum_t P[cols_p[m] nt + t] Q[cols_q[n] nt + t] over t < nt
+// (m, n < k; zero elsewhere) + diag on m == n < k and pad_diag on m == n >= k.
+// QDENSE: Q's columns in order with leading dimension ldq (cols_q unused).
 // lower: only the tiles on and below the diagonal (the Cholesky reads no others).
+template <bool QDENSE = false>
 static __device__ void crossprod64(const double *__restrict__ P, const int32_t *cols_p, const double *__restrict__ Q,
                             const int32_t *cols_q, int nt, int k, int kp, double diag, double pad_diag, double *out,
-                            bool lower, double (*xs)[B + 1], double (*ys)[B + 1]) {
+                            bool lower, double (*xs)[B + 1], double (*ys)[B + 1], int64_t ldq = 0) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = (wave >> 1) * 16, wn = (wave & 1) * 16;
   for (int tm = 0; tm < kp; tm += B)
@@ -57,7 +59,10 @@ static __device__ void crossprod64(const double *__restrict__ P, const int32_t *
       for (int u = 0; u < U; ++u) {
         const int c = (threadIdx.x + u * RT) >> 5;
         ox[u] = tm + c < k ? (int64_t)cols_p[tm + c] * nt : -1;
-        oy[u] = tn + c < k ? (int64_t)cols_q[tn + c] * nt : -1;
+        if constexpr (QDENSE)
+          oy[u] = tn + c < k ? (int64_t)(tn + c) * ldq : -1;
+        else
+          oy[u] = tn + c < k ? (int64_t)cols_q[tn + c] * nt : -1;
       }
       auto load = [&](int t0) {
 #pragma unroll
@@ -86,6 +91,54 @@ static __device__ void crossprod64(const double *__restrict__ P, const int32_t *
         double v = acc[r];
         if (m == n) v += (m < k) ? diag : pad_diag;
         out[m + (int64_t)n * kp] = v;
+      }
+    }
+  __syncthreads();
+}
+
+// The row Gram matrix of the nt x k block X_sl = P[:, cols] (column j at
+// P + cols[j] nt): out[m + n nq] = sum_{j < k} X_sl(m, j) X_sl(n, j) for m, n
+// < nt, + diag on m == n < nt and pad_diag on m == n >= nt, over the tiles on
+// and below the diagonal of the nq x nq result (nq = roundup(nt, 32)). Each
+// chunk stages 32 columns of the block, read down their rows (coalesced).
+static __device__ void gram_rows64(const double *__restrict__ P, const int32_t *cols, int nt, int k, int nq,
+                                   double diag, double pad_diag, double *out, double (*xs)[B + 1],
+                                   double (*ys)[B + 1]) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = (wave >> 1) * 16, wn = (wave & 1) * 16;
+  constexpr int U = B * B / RT;
+  for (int tm = 0; tm < nq; tm += B)
+    for (int tn = 0; tn <= tm; tn += B) {
+      f64x4_t acc = {0.0, 0.0, 0.0, 0.0};
+      double px[U], py[U];
+      auto load = [&](int j0) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int e = threadIdx.x + u * RT, r = e & 31, j = j0 + (e >> 5);
+          const int64_t base = j < k ? (int64_t)cols[j] * nt : -1;
+          px[u] = (base >= 0 && tm + r < nt) ? P[base + tm + r] : 0.0;
+          py[u] = (base >= 0 && tn + r < nt) ? P[base + tn + r] : 0.0;
+        }
+      };
+      load(0);
+      for (int j0 = 0; j0 < k; j0 += B) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int e = threadIdx.x + u * RT;
+          xs[e & 31][e >> 5] = px[u];  // (row, column of the chunk)
+          ys[e & 31][e >> 5] = py[u];
+        }
+        __syncthreads();
+        if (j0 + B < k) load(j0 + B);
+        acc = mfma32(xs, ys, true, acc);
+        __syncthreads();
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = tm + wm + (lane >> 4) + 4 * r, n = tn + wn + (lane & 15);
+        double v = acc[r];
+        if (m == n) v += (m < nt) ? diag : pad_diag;
+        out[m + (int64_t)n * nq] = v;
       }
     }
   __syncthreads();
@@ -222,7 +275,8 @@ static __device__ bool cholesky64(double *G, int kp, double (*d)[B + 1], double 
 }
 
 // C <- L^-1 C (trans false) or C <- L^-T C (trans true), L lower triangular
-// and C kp x kp (kp a multiple of 32), left-looking by 32 x 32 blocks: block
+// kp x kp and C kp x nc (both multiples of 32; nc = 0: kp) with leading
+// dimension ldc (0: kp), left-looking by 32 x 32 blocks: block
 // (r0, n0) of the solution is X = D^-1 (C(r0, n0) - sum_j S(j) X(j, n0)) with
 // D the diagonal block of L (D^T when trans), S(j) = L(r0, j) over the blocks j
 // before r0 (L(j, r0)^T over the blocks after it when trans). The sum runs on
@@ -230,8 +284,10 @@ static __device__ bool cholesky64(double *G, int kp, double (*d)[B + 1], double 
 // current one's MFMAs run; blocks go in the order that leaves every X(j, n0)
 // final before it is read, and X overwrites C.
 static __device__ void trsm_ll(const double *L, double *C, int kp, bool trans, double (*d)[B + 1], double (*cs)[B + 1],
-                               double (*as)[B + 1], double (*bs)[B + 1]) {
+                               double (*as)[B + 1], double (*bs)[B + 1], int nc = 0, int64_t ldc = 0) {
   constexpr int U = B * B / RT;
+  const int ncol = nc ? nc : kp;       // C: kp x ncol (a multiple of 32), leading dimension ldc
+  const int64_t lc = ldc ? ldc : kp;
   const int nb = kp / B;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = (wave >> 1) * 16, wn = (wave & 1) * 16;
@@ -239,7 +295,7 @@ static __device__ void trsm_ll(const double *L, double *C, int kp, bool trans, d
     const int r0 = (trans ? nb - 1 - bi : bi) * B;
     for (int e = threadIdx.x; e < B * B; e += RT) d[e & 31][e >> 5] = L[(r0 + (e & 31)) + (int64_t)(r0 + (e >> 5)) * kp];
     const int jbeg = trans ? r0 + B : 0, jend = trans ? kp : r0;
-    for (int n0 = 0; n0 < kp; n0 += B) {
+    for (int n0 = 0; n0 < ncol; n0 += B) {
       f64x4_t acc = {0.0, 0.0, 0.0, 0.0};
       double pa[U], pb[U];
       auto load = [&](int j0) {
@@ -248,7 +304,7 @@ static __device__ void trsm_ll(const double *L, double *C, int kp, bool trans, d
           const int e = threadIdx.x + u * RT, lo = e & 31, hi = e >> 5;
           // A(i, c): L(r0 + i, j0 + c), or L(j0 + c, r0 + i) when trans; both read down a column
           pa[u] = trans ? L[(j0 + lo) + (int64_t)(r0 + hi) * kp] : L[(r0 + lo) + (int64_t)(j0 + hi) * kp];
-          pb[u] = C[(j0 + lo) + (int64_t)(n0 + hi) * kp];  // B(c, n) = X(j0 + c, n0 + n)
+          pb[u] = C[(j0 + lo) + (int64_t)(n0 + hi) * lc];  // B(c, n) = X(j0 + c, n0 + n)
         }
       };
       if (jbeg < jend) load(jbeg);
@@ -270,14 +326,14 @@ static __device__ void trsm_ll(const double *L, double *C, int kp, bool trans, d
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int i = wm + (lane >> 4) + 4 * r, n = wn + (lane & 15);
-        cs[i][n] = C[(r0 + i) + (int64_t)(n0 + n) * kp] - acc[r];
+        cs[i][n] = C[(r0 + i) + (int64_t)(n0 + n) * lc] - acc[r];
       }
       __syncthreads();
       if (trans)
         backward_sub_t(d, cs);
       else
         forward_sub(d, cs);
-      for (int e = threadIdx.x; e < B * B; e += RT) C[(r0 + (e & 31)) + (int64_t)(n0 + (e >> 5)) * kp] = cs[e & 31][e >> 5];
+      for (int e = threadIdx.x; e < B * B; e += RT) C[(r0 + (e & 31)) + (int64_t)(n0 + (e >> 5)) * lc] = cs[e & 31][e >> 5];
       __syncthreads();
     }
   }

@@ -8,7 +8,10 @@
 // FP64 products over the time points), a blocked Cholesky G = L L^T in place
 // (32-column panels: unblocked factor of the diagonal block in LDS, the panel
 // below solved 32 rows at a time in LDS, 32 x 32-tiled update of the trailing
-// matrix), then C <- L^-T L^-1 C by left-looking blocked substitution. Padding rows and
+// matrix), then C <- L^-T L^-1 C by left-looking blocked substitution. With
+// fewer time points than vertices (nt < k) the kernel of k_ridge_dual.hip
+// solves the dual form X_sl^T (X_sl X_sl^T + alpha I)^-1 Y_sl on the nt x nt
+// row Gram matrix with the same steps. Padding rows and
 // columns k..kp-1 carry an identity diagonal, so the padded system stays SPD
 // and its solution is zero there. The betas (k x k) go to the searchlight's
 // place in the Q store (FP32; FP64 too when a.q64 is set) for the
@@ -35,15 +38,15 @@ __device__ __forceinline__ int next_sl(int32_t *queue, int32_t *sh) {
 }
 
 
-__global__ __launch_bounds__(RT, HAK_RIDGE_OCC) void k_ridge64(HakLocalArgs a, double alpha, double *work64, int64_t stride64,
-                                                int32_t count) {
+__global__ __launch_bounds__(RT, HAK_RIDGE_OCC) void k_ridge64(HakLocalArgs a, double alpha, double *work64,
+                                                               int64_t stride64, int32_t begin, int32_t end) {
   __shared__ double t0[B][B + 1], t1[B][B + 1], t2[B][B + 1], t3[B][B + 1];
   __shared__ int32_t item_sh, ok_sh;
   double *Gm = work64 + (int64_t)blockIdx.x * 2 * stride64;
   double *Cm = Gm + stride64;
   for (;;) {
-    const int it = next_sl(a.queue, &item_sh);
-    if (it >= count) break;
+    const int it = begin + next_sl(a.queue, &item_sh);
+    if (it >= end) break;
     const int s = a.order ? a.order[it] : it;
     const int o = a.sl_off[s];
     const int k = a.sl_off[s + 1] - o;
@@ -70,9 +73,14 @@ __global__ __launch_bounds__(RT, HAK_RIDGE_OCC) void k_ridge64(HakLocalArgs a, d
 
 }  // namespace
 
+extern "C" int hak_ridge64_dual(const HakLocalArgs *a, double alpha, double *work64, int64_t stride64, int n_wg,
+                                int32_t begin, int32_t end, void *stream);
+
 extern "C" int hak_ridge64(const HakLocalArgs *a, double alpha, double *work64, int64_t stride64, int n_wg,
-                           int32_t count, void *stream) {
-  if (count <= 0) return 0;
-  hipLaunchKernelGGL(k_ridge64, dim3(n_wg), dim3(RT), 0, (hipStream_t)stream, *a, alpha, work64, stride64, count);
+                           int32_t begin, int32_t end, int dual, void *stream) {
+  if (end <= begin) return 0;
+  if (dual) return hak_ridge64_dual(a, alpha, work64, stride64, n_wg, begin, end, stream);
+  hipLaunchKernelGGL(k_ridge64, dim3(n_wg), dim3(RT), 0, (hipStream_t)stream, *a, alpha, work64, stride64, begin,
+                     end);
   LAUNCH_RET();
 }

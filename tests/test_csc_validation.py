@@ -93,10 +93,11 @@ def test_unknown_backend_is_a_value_error():
 
 
 @pytest.mark.parametrize("bad", [np.nan, np.inf, -np.inf])
-@pytest.mark.parametrize("entry", ["dense", "csc", "ridge", "template"])
+@pytest.mark.parametrize("entry", ["dense", "csc", "ridge", "template", "ha_procrustes", "ha_ridge"])
 def test_non_finite_data_is_rejected(entry, bad):
     """NaN or Inf in X, Y or a subject is an argument error of the searchlight
-    entry points, found by a host scan before anything reaches the GPU."""
+    entry points and of the single-problem ha_procrustes and ha_ridge, found by
+    a host scan before anything reaches the GPU."""
     rng = np.random.default_rng(2)
     X = rng.standard_normal((8, 3))
     Y = rng.standard_normal((8, 3))
@@ -109,6 +110,10 @@ def test_non_finite_data_is_rejected(entry, bad):
             ha.searchlight_procrustes_csc(X, Y, [[0, 1, 2]], _mat(*GOOD, 3, 3))
         elif entry == "ridge":
             ha.searchlight_ridge_csc(X, Y, [[0, 1, 2]], _mat(*GOOD, 3, 3))
-        else:
+        elif entry == "template":
             ha.searchlight_template_flat([X, Y], lists[0], np.array([0, 3], dtype=np.int32), None, 0.0)
+        elif entry == "ha_procrustes":
+            ha.procrustes(X, Y)
+        else:
+            ha.ridge(X, Y, 1.0)
     assert ei.value.rc == -3

@@ -58,10 +58,32 @@ def test_ridge_vec():
     assert rel_err(ha.ridge(X, y, 4.0), O.ridge(X, y, 4.0)) < REL_64
 
 
+@pytest.mark.parametrize("M,N,P", [(10, 40, 40), (10, 40, 7), (60, 20, 45), (25, 25, 3)])
+def test_ridge_alpha_zero_matches_the_svd_form(M, N, P):
+    """alpha = 0 (ha_ridge.c:13-54: d = 1 / s over the thin SVD): with X of full
+    rank the betas are the least-squares solution, the minimum-norm one when M
+    < N, which the GPU takes from the dual form X^T (X X^T)^-1 Y. P > N runs as
+    ceil(P / N) N-column problems, so the padding columns never enter X."""
+    rng = np.random.default_rng(M * 131 + N + P)
+    X = rng.standard_normal((M, N))
+    Y = rng.standard_normal((M, P))
+    got = ha.ridge(X, Y, 0.0)
+    ref = O.ridge(X, Y, 0.0)
+    print(f"[ridge alpha 0] M={M} N={N} P={P}: rel {rel_err(got, ref):.2e}", flush=True)
+    assert rel_err(got, ref) < REL_64
+
+
 def test_ridge_not_positive_definite_is_an_error():
-    """alpha = 0 with fewer samples than features: X^T X is singular."""
+    """alpha = 0 without full rank: a zero column (M >= N, X^T X singular) or a
+    zero row (M < N, X X^T singular) leaves an exactly zero pivot, where the
+    reference's d = 1 / s divides by a rounding-level singular value."""
     rng = np.random.default_rng(22)
+    X = rng.standard_normal((40, 10))
+    X[:, 3] = 0.0
+    with pytest.raises(ha.HaError):
+        ha.ridge(X, rng.standard_normal((40, 10)), 0.0)
     X = rng.standard_normal((10, 40))
+    X[4, :] = 0.0
     with pytest.raises(ha.HaError):
         ha.ridge(X, rng.standard_normal((10, 40)), 0.0)
     with pytest.raises(ha.HaError):
@@ -135,3 +157,24 @@ def test_plan_ridge_c1():
                                   indptr, indices, 1e3, weights=wts, which_cols=set(cols.tolist()))
     sel = np.concatenate([np.arange(indptr[c], indptr[c + 1]) for c in cols])
     assert rel_err(data[sel], ref[sel]) < REL_SL
+
+
+@pytest.mark.parametrize("alpha", [0.0, 5.0])
+def test_searchlight_ridge_dual_form(alpha):
+    """Searchlights on both sides of nt = 24 (12-70 vertices) in one pass: those
+    with k > nt run the dual-form kernel (an nt x nt system), the others the
+    k x k one, against the oracle's SVD form. Non-centred data of full row
+    rank, so alpha = 0 has the minimum-norm solution everywhere it is dual."""
+    rng = np.random.default_rng(77)
+    nt, nv = 24, 90
+    X = rng.standard_normal((nt, nv)) + 0.5
+    Y = rng.standard_normal((nt, nv))
+    sizes = [12, 23, 70, 25, 24, 40, 64, 65, 33] if alpha else [25, 33, 40, 64, 65, 70]
+    lists = [np.sort(rng.choice(nv, k, replace=False)).astype(np.int32) for k in sizes]
+    m = ha.sparse_init(lists, nv)
+    ha.searchlight_ridge_csc(X, Y, lists, m, alpha=alpha)
+    idx = np.concatenate(lists)
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    ref = O.searchlight_ridge_csc(X, Y, idx, off, m.indptr, m.indices, alpha)
+    print(f"[ridge dual] alpha {alpha}: rel {rel_err(m.data, ref):.2e}", flush=True)
+    assert rel_err(m.data, ref) < REL_SL
