@@ -293,7 +293,9 @@ __device__ int solve_one(GemmSmem<TT> &sm, FixLds &fx, const HakTplArgs &t, cons
   const kp64::Sched sc{pa.sched, pa.sched_len, pa.max_sched, pa.n_buckets, pa.max_steps, pa.bucket_per_decade,
                        pa.plain_ell};
   cross64(D, R, t.ld, k, kp, sl.A64, stg);
-  int steps = kp64::polar64(sc, sl.A64, sl.X64, sl.G64, sl.P64, k, kp, stg, (kp64::lds_f64 *)red);
+  // k > nt: A = D_s^T R has rank <= nt < k, so the iteration cannot finish;
+  // such a solve goes to the Jacobi solve directly
+  int steps = k > t.nt ? -1 : kp64::polar64(sc, sl.A64, sl.X64, sl.G64, sl.P64, k, kp, stg, (kp64::lds_f64 *)red);
   float *q = sl.w.X2;
   double tr = 0.0;
   if (steps >= 0) {
