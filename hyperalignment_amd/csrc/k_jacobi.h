@@ -34,13 +34,23 @@ __device__ __forceinline__ T grp16_sum(T v) {
 
 // Rotation threshold |a_p . a_q| > tol * |a_p| |a_q| with tol = 4 eps sqrt(k):
 // the rounding level of the computed dot products, so the last sweep finds
-// nothing left to rotate instead of chasing noise.
+// nothing left to rotate instead of chasing noise. A column of rounding noise
+// (a zero singular value) never meets the test: its rotations shrink it
+// towards underflow, far below the rank cut smax k cut of the completion
+// (cut: 2.3e-16 in FP64, 1.2e-7 in FP32), which replaces it; the sweeps then
+// end at the limit with the other columns converged.
 template <typename T>
 struct JacobiTol;
 template <>
-struct JacobiTol<float> { static constexpr double eps = 1.19e-7; static constexpr int sweeps = 30; };
+struct JacobiTol<float> {
+  static constexpr double eps = 1.19e-7, cut = 1.2e-7;
+  static constexpr int sweeps = 30;
+};
 template <>
-struct JacobiTol<double> { static constexpr double eps = 2.22e-16; static constexpr int sweeps = 30; };
+struct JacobiTol<double> {
+  static constexpr double eps = 2.22e-16, cut = 2.3e-16;
+  static constexpr int sweeps = 30;
+};
 
 // Orthogonalises the k columns of Am (col-major, ld kp) while accumulating the
 // rotations in Vm (starts as identity). Returns the number of sweeps used
@@ -216,7 +226,8 @@ struct SvdLds {
 // the zero matrix (T = I). With det_fix (isReflection = false), det Q < 0
 // flips the pair of the smallest singular value (ha_searchlight.c:611-631),
 // the sign from LUs of U and V. Lm: kp x kp work (det_fix only); sig: k
-// doubles. Returns the sweeps (negative if the sweeps did not converge);
+// doubles. Returns the sweeps (negative if the sweeps reached their limit,
+// which a singular A does: its noise columns never meet the rotation test);
 // *sigma_sum = sum(sigma) less twice the flipped one.
 __device__ inline int svd_polar64(double *Am, double *Vm, double *Lm, double *sig, int k, int kp, bool det_fix,
                                   float *Q, double *Q64, SvdLds &sv, double *red, double *sigma_sum) {
@@ -235,7 +246,7 @@ __device__ inline int svd_polar64(double *Am, double *Vm, double *Lm, double *si
   __syncthreads();
   double smax = 0;
   for (int j = 0; j < k; ++j) smax = fmax(smax, sig[j]);
-  const double cut = smax * k * 2.3e-16;
+  const double cut = smax * k * JacobiTol<double>::cut;
   for (int64_t e = threadIdx.x; e < (int64_t)k * kp; e += NT) {
     const int j = (int)(e / kp), m = (int)(e % kp);
     Am[e] = smax == 0.0 ? (m == j ? 1.0 : 0.0) : (sig[j] > cut) ? Am[e] / sig[j] : 0.0;

@@ -545,7 +545,7 @@ __device__ double jacobi_compose(T *Am, const T *Vm, int k, int kp, TO *Q, int q
   }
   __syncthreads();
   for (int j = 0; j < k; ++j) smax = sig[j] > smax ? sig[j] : smax;
-  const T cut = smax * (T)k * (sizeof(T) == 8 ? (T)2.3e-16 : (T)1.2e-7);
+  const T cut = smax * (T)k * (T)JacobiTol<T>::cut;
   for (int j = threadIdx.x; j < k; j += NT) {
     if (sig[j] <= cut) atomicExch(rank_sh, 1);
   }

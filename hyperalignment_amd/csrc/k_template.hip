@@ -242,8 +242,8 @@ __device__ __noinline__ void cross64(const float *D, const float *R, int ld, int
 // of D_s, as for z-scored data of full rank nt - 1), and Q = I for A = 0, as
 // LAPACK's SVD gives. Without reflections det Q = +1. A64 is copied to X64
 // (then U), G64 holds V, P64 the LU work and A64 the singular values. Returns
-// (sweeps, or -1 with Q = I for a non-finite A; sum(sigma) less twice a flipped
-// one, the trace term of the scaling). Not inlined, like the iteration.
+// (the sweeps, or -1 with Q = I for a non-finite A; sum(sigma) less twice a
+// flipped one, the trace term of the scaling). Not inlined, like the iteration.
 __device__ __noinline__ double2 solve_svd64(double *A64, double *X64, double *G64, double *P64, float *q, int k,
                                             int kp, int fix, kp64::lds_f64 *stg, kp64::lds_f64 *redp) {
   A64 = kp64::uni(A64);
@@ -274,7 +274,10 @@ __device__ __noinline__ double2 solve_svd64(double *A64, double *X64, double *G6
   double ss = 0.0;
   const int sw = kjac::svd_polar64(X64, G64, P64, A64, k, kp, fix != 0, q, nullptr,
                                    *reinterpret_cast<kjac::SvdLds *>((double *)stg), red, &ss);
-  return make_double2(sw > 0 ? (double)sw : -1.0, ss);
+  // sweeps at their limit (sw < 0) are the singular case: the noise columns
+  // were shrunk below the rank cut and completed, as in the searchlight FP64
+  // path, which takes the same result
+  return make_double2((double)(sw > 0 ? sw : -sw), ss);
 }
 static_assert(sizeof(kjac::SvdLds) <= 2 * kp64::STG * sizeof(double), "the Jacobi solve's LDS fits the staging");
 

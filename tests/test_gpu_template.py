@@ -217,3 +217,24 @@ def test_searchlight_template_few_samples(reflection):
     ok, err = err_ok(tpl, ref)
     print(f"[template mesh nt 15] reflection {reflection}: max |diff| {err:.2e}", flush=True)
     assert ok, err
+
+
+@pytest.mark.parametrize("reflection", [True, False])
+def test_template_subject_with_zero_column(reflection):
+    """A subject with an all-zero column (a vertex without signal): its
+    cross-covariances have an exactly zero singular value, which the FP64
+    iteration cannot finish, so its solves take the Jacobi solve. The zero
+    column is in the null space of D_s, so D_s Q does not depend on the
+    completion, and the template matches the oracle (the reference's SVD
+    gives a valid T there too, where this path used to report kHaErrorSvd)."""
+    rng = np.random.default_rng(33)
+    nt, nv, ns = 50, 12, 4
+    S = rng.standard_normal((nt, nv))
+    dms = [O.zscore_columns(S @ np.linalg.qr(rng.standard_normal((nv, nv)))[0] + 0.3 * rng.standard_normal((nt, nv)))
+           for _ in range(ns)]
+    dms[2][:, 5] = 0.0
+    tpl = ha.template_procrustes(dms, reflection, False, True, 2)
+    ref = O.template_procrustes(dms, reflection, False, True, 2)
+    ok, err = err_ok(tpl, ref)
+    print(f"[template zero column] reflection {reflection}: max |diff| {err:.2e}", flush=True)
+    assert ok, err
