@@ -242,3 +242,41 @@ def test_zero_cross_covariance_gives_identity(reflection):
     err = np.abs(T - ref).max()
     print(f"[edges zero A] reflection={reflection}: max |T - T_fp64| {err:.2e}", flush=True)
     assert err < 1e-4, err
+
+
+@pytest.mark.parametrize("reflection", [True, False])
+@pytest.mark.parametrize("method", ["auto", "polar", "jacobi64"])
+def test_searchlight_with_a_zero_column(method, reflection):
+    """A vertex without signal (an all-zero column of X) inside full-rank
+    searchlights: A has an exactly zero singular value, which the polar
+    iteration cannot finish, so the FP64 path solves these searchlights (its
+    noise column is completed). The transform is then not unique in that
+    direction, so the checks are the invariants: Q^T Q = I, the oracle's
+    residual, and det Q = +1 without reflections."""
+    rng = np.random.default_rng(55)
+    nt, nv = 60, 40
+    X = rng.standard_normal((nt, nv))
+    Y = 0.5 * X @ np.linalg.qr(rng.standard_normal((nv, nv)))[0] + rng.standard_normal((nt, nv))
+    X[:, 7] = 0.0
+    lists = [np.sort(np.concatenate([[7], rng.choice(np.setdiff1d(np.arange(nv), [7]), k - 1, replace=False)]))
+             .astype(np.int32) for k in (5, 20, 33)]
+    sizes = np.array([len(l) for l in lists])
+    offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    plan = ha.Plan(nt, nv, np.concatenate(lists), offsets, None, 0.0)
+    try:
+        plan.load(X, Y)
+        ha.set_options(method=method)
+        plan.execute(reflection=reflection)
+        for l, T in zip(lists, plan.local(np.arange(len(lists)))):
+            T = T.astype(np.float64)
+            k = len(l)
+            assert np.abs(T.T @ T - np.eye(k)).max() < 1e-4
+            if not reflection:
+                assert np.linalg.det(T) > 0
+            To = O.procrustes(X[:, l], Y[:, l], reflection=reflection)
+            r, ro = np.linalg.norm(X[:, l] @ T - Y[:, l]), np.linalg.norm(X[:, l] @ To - Y[:, l])
+            print(f"[edges zero column] {method} reflection={reflection} k {k}: residual {r:.6f} vs {ro:.6f}",
+                  flush=True)
+            assert abs(r - ro) <= 1e-4 * max(1.0, ro)
+    finally:
+        plan.close()
