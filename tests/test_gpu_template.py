@@ -238,3 +238,20 @@ def test_template_subject_with_zero_column(reflection):
     ok, err = err_ok(tpl, ref)
     print(f"[template zero column] reflection {reflection}: max |diff| {err:.2e}", flush=True)
     assert ok, err
+
+
+@pytest.mark.parametrize("reflection", [True, False])
+def test_template_fewer_samples_than_columns_with_scaling(reflection):
+    """isScaling on singular chains (nt = 10, 14 columns): the Jacobi solve's
+    sum of singular values (less twice a flipped one, zero here) scales each
+    D_s Q as the reference's sum(s) / sum_j var, against the oracle."""
+    rng = np.random.default_rng(23)
+    nt, nv, ns = 10, 14, 4
+    S = rng.standard_normal((nt, nv))
+    dms = [O.zscore_columns(S @ np.linalg.qr(rng.standard_normal((nv, nv)))[0] + 0.3 * rng.standard_normal((nt, nv)))
+           for _ in range(ns)]
+    tpl = ha.template_procrustes(dms, reflection, True, True, 1)
+    ref = O.template_procrustes(dms, reflection, True, True, 1)
+    ok, err = err_ok(tpl, ref)
+    print(f"[template nt < nv, scaling] reflection {reflection}: max |diff| {err:.2e}", flush=True)
+    assert ok, err
