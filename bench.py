@@ -9,7 +9,7 @@ scatter into the CSC transform. Prints one JSON line (searchlights/s).
 
 Two more modes time the paths beside it, one JSON line each: --mode ridge (the
 searchlight ridge pass, ha_searchlight_ridge, on the same workload) and --mode
-template (the searchlight template of config 4: 16 subjects, order-5 icosphere,
+template (the 16-subject searchlight template: order-5 icosphere,
 n = 400, 20 mm, level2 = 2, through ha_searchlight_template_flat).
 """
 
@@ -296,7 +296,7 @@ def bench_ridge(args):
 
 
 def bench_template(args):
-    """The searchlight template of config 4 (ha_searchlight_template_flat; ha_template.c:14-114
+    """The 16-subject searchlight template (ha_searchlight_template_flat; ha_template.c:14-114
     per searchlight, combined as searchlight.py:215-236): 16 subjects, each call uploads them,
     runs the template kernel and the FP64 gather, and downloads the template."""
     dms, sls = synth.template_subjects(n_subj=16, n=400, order=5, radius=20.0, seed=0)
@@ -331,12 +331,12 @@ def bench_template(args):
                "sample": f"{n} random searchlights' local templates (the oracle's FP64 chain with LAPACK SVDs, "
                          f"ha_template.c:14-114), numpy with a single-threaded BLAS; whole template estimate "
                          f"{sls.count / rate:.0f} s"}
-    out = {"metric": "searchlights/s (config-4 searchlight template, 47 Procrustes solves per searchlight)",
+    out = {"metric": "searchlights/s (16-subject searchlight template, 47 Procrustes solves per searchlight)",
            "value": sls.count * args.steps / dt, "unit": "searchlights/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": 1e3 * dt / args.steps, "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f64 (A and polar factors) + f32 (storage, D Q)",
            "data": "synthetic",
-           "config": {"workload": "config 4: 16 subjects, order-5 icosphere (10,242 vertices), n = 400, 20 mm "
+           "config": {"workload": "template: 16 subjects, order-5 icosphere (10,242 vertices), n = 400, 20 mm "
                                   "searchlights, level2 = 2, z-scored",
                       "timed": "each step a whole ha_searchlight_template_flat call (host data in, template out)",
                       "finite": bool(np.all(np.isfinite(tpl))),
@@ -368,7 +368,7 @@ def main():
     ap.add_argument("--no-reflection", action="store_true",
                     help="time passes with isReflection = false (the det(Q) < 0 fix, ha_searchlight.c:612-632)")
     ap.add_argument("--mode", default="procrustes", choices=["procrustes", "ridge", "template"],
-                    help="the searchlight Procrustes pass (the headline), the ridge pass, or the config-4 template")
+                    help="the searchlight Procrustes pass (the headline), the ridge pass, or the 16-subject template")
     ap.add_argument("--alpha", type=float, default=1.0, help="ridge penalty (--mode ridge)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (reference for the HBM byte counters)")

@@ -59,7 +59,8 @@ class HaHipOptions(ctypes.Structure):
                 ("plain_ell", ctypes.c_double), ("tile", ctypes.c_int32), ("use_graph", ctypes.c_int32),
                 ("chunk", ctypes.c_int32), ("verbose", ctypes.c_int32),
                 ("learned_margin_scale", ctypes.c_float), ("learned_margin_ell", ctypes.c_float),
-                ("team", ctypes.c_int32), ("team_spin", ctypes.c_uint32), ("orth_tol", ctypes.c_double)]
+                ("team", ctypes.c_int32), ("team_spin", ctypes.c_uint32), ("orth_tol", ctypes.c_double),
+                ("template_ell", ctypes.c_double)]
 
 
 class HaHipStats(ctypes.Structure):

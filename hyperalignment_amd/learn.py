@@ -17,7 +17,7 @@ ell too small costs about half a step per decade, one too large a restart of the
 schedule, and a scale below sigma_max makes the attempt diverge, after which the
 kernel redoes the searchlight as the plain iteration.
 
-Workload grid (config 3): the full-hemisphere mesh (order-6 icosphere, 20 mm
+Workload grid: the full-hemisphere mesh (order-6 icosphere, 20 mm
 searchlights) with seeds 0-7, n_samples in {200, 300, 500} and noise sigma in
 {0.1, 0.3, 1.0}; evaluation on held-out seeds.
 

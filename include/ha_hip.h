@@ -70,6 +70,9 @@ typedef struct {
 	                            ||Q^T Q - I||_F (four-probe estimate) is checked after the
 	                            iteration; above orth_tol it goes to the FP64 path (default
 	                            1e-3; 0 switches the check off) */
+	double  template_ell;    /* sigma_min/||A||_F assumed by the template's FP64 schedule
+	                            (default 3e-4; smaller ones are reached by its restarts,
+	                            and a solve the iteration cannot finish goes to FP64 Jacobi) */
 } HaHipOptions;
 
 void ha_hip_get_options(HaHipOptions *opt);

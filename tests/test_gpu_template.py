@@ -123,7 +123,7 @@ def test_template_rotations_closed_form(zscore):
     assert ok, err
 
 
-# Config 4's local cross-covariances are ill-conditioned (sigma_min / sigma_max
+# The 16-subject workload's local cross-covariances are ill-conditioned (sigma_min / sigma_max
 # about 2.5e-4 at 20 mm, n = 400, and down to 1.8e-8 in a few solves), and each
 # searchlight chains 47 Procrustes solves. With FP32 cross-covariances and
 # polar iterations one searchlight's local template was 0.89 off, and these 16
@@ -134,7 +134,7 @@ TOL_C4 = 1e-4
 
 
 def test_template_c4_sampled_columns():
-    """Config 4 (16 subjects, order-5 mesh, n=400, 20 mm, level2 = 2): 16
+    """The 16-subject workload (order-5 mesh, n=400, 20 mm, level2 = 2): 16
     template columns (vertices of one searchlight) against the oracle over
     every searchlight holding any of them."""
     import sys

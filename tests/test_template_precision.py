@@ -1,8 +1,9 @@
 """Why the template kernel forms A and the polar factor in FP64 (host emulation, no GPU).
 
 The template chain (ha_template.c:14-114) runs 47 dependent Procrustes solves per
-searchlight at config 4. Most of its cross-covariances have sigma_min / sigma_max
-around 2.5e-4, but some reach 1e-8: in searchlight 2435 of the config-4 data one
+searchlight on the 16-subject workload (order-5 mesh, n = 400, 20 mm). Most of its
+cross-covariances have sigma_min / sigma_max around 2.5e-4, but some reach 1e-8: in
+searchlight 2435 of that data one
 subject's A has 1.8e-8. An FP32 polar iteration cannot resolve that smallest
 singular pair, and the local template then leaves the FP64 reference by O(1).
 
