@@ -5,6 +5,8 @@ import ctypes
 import os
 import re
 
+import pytest
+
 import hyperalignment_amd as ha
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -37,3 +39,16 @@ def test_library_exports_every_declared_function():
     lib = ctypes.CDLL(ha.LIB_PATH)
     missing = [n for n in sorted(declared()) if not hasattr(lib, n)]
     assert not missing, missing
+
+
+def test_options_struct_matches_the_library():
+    """The ctypes mirror of HaHipOptions (include/ha_hip.h) has the library's
+    layout: the defaults of its first and last fields come back as the C
+    initialiser sets them, and an invalid value of the last field is rejected
+    by ha_hip_set_options (no device is needed for either call)."""
+    o = ha.get_options()
+    assert o.method == 0 and o.fp64_fallback == 1 and o.max_polar_steps == 28
+    assert o.plain_ell == 1e-7 and o.orth_tol == 1e-3 and o.template_ell == 3e-4
+    with pytest.raises(ha.HaError):
+        ha.set_options(template_ell=0.0)
+    assert ha.get_options().template_ell == 3e-4
