@@ -310,9 +310,22 @@ def bench_template(args):
         run()
         print(f"[bench] template warmup {i} {time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
+    kms, flops = [], []
     for i in range(args.steps):
         tpl = run()
+        st = ha.last_template_stats()
+        kms.append(st.ms_polar)
+        flops.append(st.gemm_flops)
     dt = time.perf_counter() - t0
+    kms_avg, flops_avg = float(np.mean(kms)), float(np.mean(flops))
+    achieved = flops_avg / (kms_avg * 1e-3)
+    roofline = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+                "frac": achieved / PEAK_FP64, "traffic": None, "kernel": "k_template<false>", "kernel_ms": kms_avg,
+                "flops_per_launch": flops_avg,
+                "flops": "FP64 algorithmic: 2 n k^2 per solve (A = D_s^T R) and 4 k^3 per polar step "
+                         "(X^T X and G^2 symmetric, X P), from each searchlight's step count; the FP32 "
+                         "products D_s Q are not counted",
+                "kernel_timing": "HIP events on the library stream around the template launch of every timed call"}
     cpu = None
     if not args.no_cpu:
         from oracle import ha_oracle as O
@@ -342,9 +355,7 @@ def bench_template(args):
                       "finite": bool(np.all(np.isfinite(tpl))),
                       "parity_vs_fp64": "tests/test_gpu_template.py (16 columns within 1.2e-7 of the largest "
                                         "magnitude)"},
-           "roofline": None,
-           "roofline_note": "not reported: the FP64 iteration's flops follow each solve's step count, which the "
-                            "flat call does not return; the kernel time is in the rocprof summary",
+           "roofline": roofline,
            "cpu_baseline": cpu}
     print(json.dumps(out))
 

@@ -342,6 +342,14 @@ def searchlight_template_flat(dms, idx, offsets, dists, radius, reflection=True,
     return np.ascontiguousarray(out)
 
 
+def last_template_stats():
+    """ha_hip_last_template_stats: the last searchlight_template_flat call's
+    kernel time (ms_polar), polar steps (steps_total) and FP64 flops (gemm_flops)."""
+    s = HaHipStats()
+    _check("ha_hip_last_template_stats", lib().ha_hip_last_template_stats(ctypes.byref(s)))
+    return s
+
+
 def template_procrustes(dms, reflection=True, scaling=False, zscore_common=True, level2_iter=1):
     """ha_template_procrustes on the GPU (csrc/ha_template.h:18-22): the
     Procrustes template of whole (nt, nv) subject matrices."""

@@ -511,6 +511,16 @@ done:
 	return rc;
 }
 
+/* The statistics of the last ha_searchlight_template_flat call (its plan is
+ * gone when the call returns): kernel time, step counts and FP64 flops. */
+static HaHipStats g_last_template_stats;
+
+int ha_hip_last_template_stats(HaHipStats *stats) {
+	if (!stats) return kHaErrorArg;
+	*stats = g_last_template_stats;
+	return kHaSuccess;
+}
+
 /* ---- templates ---- */
 
 int ha_searchlight_template_flat(const double *const *dms, int32_t n_subj, int32_t nt, int32_t nv,
@@ -528,6 +538,7 @@ int ha_searchlight_template_flat(const double *const *dms, int32_t n_subj, int32
 	int rc = ha_plan_create_ex(nt, &su, sl_indices, NULL, sl_offsets, sl_dists, count, radius, &p);
 	if (rc) return rc;
 	rc = ha_plan_template(p, dms, n_subj, col_major, isReflection, isScaling, isZscoreCommon, level2_iter, tpl_out);
+	ha_hip_plan_stats(p, &g_last_template_stats);
 	ha_hip_plan_destroy(p);
 	return rc;
 }

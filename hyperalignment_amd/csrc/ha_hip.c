@@ -1580,6 +1580,18 @@ int ha_plan_template(HaHipPlan *p, const double *const *dms, int32_t ns, bool co
 		}
 		p->stats.n_unconverged = failed;
 		p->stats.steps_total = steps;
+		/* FP64 algorithmic flops: per solve A = D_s^T R (2 n k^2), per polar step
+		 * X^T X and G^2 (symmetric, k^3 each) and X P (2 k^3); a Jacobi solve
+		 * counts its sweeps as steps */
+		{
+			const double n_solves = (double)(ns - 1) + (double)level2 * ns;
+			double flops = 0;
+			for (int32_t s = 0; s < count; ++s) {
+				const double k = p->h_k[s];
+				flops += n_solves * 2.0 * nt * k * k + (h_status[s] > 0 ? (double)h_status[s] : 0.0) * 4.0 * k * k * k;
+			}
+			p->stats.gemm_flops = flops;
+		}
 		if (failed) {
 			ERR("template: %lld searchlights had a local Procrustes that did not converge\n", (long long)failed);
 			rc = kHaErrorSvd;

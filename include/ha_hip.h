@@ -208,6 +208,11 @@ int ha_hip_plan_download(const HaHipPlan *plan, int32_t *indptr, int32_t *indice
 int ha_hip_plan_download_local(const HaHipPlan *plan, const int32_t *which, int32_t n_which,
                                float *out);
 int ha_hip_plan_stats(const HaHipPlan *plan, HaHipStats *stats);
+/* No counterpart in the reference: the statistics of the last
+ * ha_searchlight_template_flat call (ms_polar: the template kernel's time;
+ * steps_total: the polar steps of its solves; gemm_flops: their FP64
+ * algorithmic flops), for callers that time the template. */
+int ha_hip_last_template_stats(HaHipStats *stats);
 /* FP64 orthogonality of every local transform in the plan's store (after a
  * pass): out[s] = ||Q_s^T Q_s - I||_F, from every entry of the Gram matrix
  * when exact, else the four-probe estimate the pass itself uses. */

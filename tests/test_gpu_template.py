@@ -255,3 +255,18 @@ def test_template_fewer_samples_than_columns_with_scaling(reflection):
     ok, err = err_ok(tpl, ref)
     print(f"[template nt < nv, scaling] reflection {reflection}: max |diff| {err:.2e}", flush=True)
     assert ok, err
+
+
+def test_last_template_stats():
+    """ha_hip_last_template_stats reports the last flat call: its kernel time,
+    the polar steps of its solves (at least one per solve) and their FP64 flops
+    (at least the cross-covariances, 2 n k^2 per solve)."""
+    dms, sls = synth.template_subjects(n_subj=4, n=80, order=3, radius=30.0, seed=3)
+    ha.searchlight_template_flat(dms, sls.indices, sls.offsets, sls.dists, sls.radius, level2_iter=2)
+    st = ha.last_template_stats()
+    k = np.diff(sls.offsets).astype(np.float64)
+    n_solves = 3 + 2 * 4
+    assert st.ms_polar > 0
+    assert st.steps_total >= n_solves * sls.count
+    assert st.gemm_flops > (n_solves * 2.0 * 80 * k * k).sum()
+    assert st.n_unconverged == 0
